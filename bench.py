@@ -1,0 +1,176 @@
+"""Benchmark: build + traversal throughput of the yggdrax tree pipeline on one MI355X.
+
+Workload (one "step"): 2^24 clustered float32 points (64 Plummer clusters,
+scale radii 10^U(-3,-1), Dirichlet(0.5) weights, 10 % uniform background,
+lognormal masses; PCG64 seed 3), resident in HBM; build_octree(leaf_size=64)
+-> compute_tree_geometry -> build_interactions_and_neighbors(theta=0.5,
+mac_type="dehnen") with no capacities (count pass + exact-length fill), all
+outputs in the reference's layout (int64 indices). Metric: Mpts/s =
+N / wall time per step.
+
+Prints ONE JSON line. The roofline object is for the dominant kernel (the
+walk's generation kernel); its algorithmic bytes use the per-pair model
+2 x 32 B node records + 8 B queue read per evaluated pair, 8 B per pushed
+child pair and 4 B per emitted list entry (an atomic slot plus the entry in
+the fill pass). The CPU baseline is the oracle (single thread) on a 2^20
+sample from the same generator.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mpts/s build+traversal, 2^24 pts, 1 GPU; learned/batched vs plain speedup"
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_pipeline(ygg, pos, mass, leaf, theta):
+    tree, ps, _, _ = ygg.build_octree(pos, mass, leaf_size=leaf, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=theta, mac_type="dehnen")
+    return tree, geom, inter, nbrs
+
+
+def stage_times(ygg, torch, pos, mass, leaf, theta):
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    tree, ps, _, _ = ygg.build_octree(pos, mass, leaf_size=leaf, return_reordered=True)
+    ev[1].record()
+    geom = ygg.compute_tree_geometry(tree, ps)
+    ev[2].record()
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=theta, mac_type="dehnen")
+    ev[3].record()
+    torch.cuda.synchronize()
+    return dict(build_ms=ev[0].elapsed_time(ev[1]), geometry_ms=ev[1].elapsed_time(ev[2]),
+                traversal_ms=ev[2].elapsed_time(ev[3]),
+                far_entries=int(inter.sources.numel()), near_entries=int(nbrs.neighbors.numel()),
+                num_nodes=int(tree.num_nodes), num_levels=int(tree.num_levels))
+
+
+def walk_roofline(ygg, torch, pos, mass, leaf, theta):
+    from yggdrax_amd import interactions as I
+    I.WALK_TIMING["enabled"] = True
+    I.WALK_TIMING["log"].clear()
+    try:
+        run_pipeline(ygg, pos, mass, leaf, theta)
+        torch.cuda.synchronize()
+    finally:
+        I.WALK_TIMING["enabled"] = False
+    total_ms = 0.0
+    total_bytes = 0.0
+    launches = 0
+    pairs_total = 0
+    for rec in I.WALK_TIMING["log"]:
+        acc, near, ref = rec["decisions"]
+        pairs = acc + near + ref
+        pushes = sum(rec["gen_sizes"][1:])
+        entries = 2 * (acc + near)
+        total_bytes += pairs * (2 * 32 + 8) + pushes * 8 + entries * 4
+        total_ms += sum(rec["gen_ms"])
+        launches += sum(1 for v in rec["gen_sizes"][:-1] if v > 0)
+        pairs_total += pairs
+    avg_ms = total_ms / max(1, launches)
+    avg_bytes = total_bytes / max(1, launches)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_walk_gen", launches=launches, avg_launch_ms=round(avg_ms, 5),
+                bytes_per_launch=round(avg_bytes), pairs_per_s=round(pairs_total / (total_ms * 1e-3)),
+                bytes_per_pair=round(total_bytes / max(1, pairs_total), 2))
+
+
+def cpu_baseline(sample_n, leaf, theta):
+    import oracle
+    from yggdrax_amd import inputs
+    pos, mass = inputs.clustered(sample_n, 3)
+    t0 = time.perf_counter()
+    topo = oracle.build_octree(pos, mass, leaf_size=leaf)
+    geom = oracle.compute_tree_geometry(topo, topo["positions_sorted"])
+    from oracle import oracle as O
+    O.run_walk_with_retries(topo, geom, theta, "dehnen")
+    dt = time.perf_counter() - t0
+    return dict(value=round(sample_n / dt / 1e6, 4), unit="Mpts/s", cores=1, kind="port",
+                sample=f"oracle build_octree+geometry+count walk on {sample_n} points of the "
+                       f"same clustered generator (seed 3), {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--log2n", type=int, default=24)
+    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--cpu-sample-log2n", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import yggdrax_amd as ygg
+    from yggdrax_amd import inputs
+
+    n = 1 << args.log2n
+    _log(f"generating clustered input n={n}")
+    pos_np, mass_np = inputs.clustered(n, 3)
+    pos = torch.from_numpy(pos_np).cuda()
+    mass = torch.from_numpy(mass_np).cuda()
+    torch.cuda.synchronize()
+    for w in range(args.warmup):
+        run_pipeline(ygg, pos, mass, args.leaf, args.theta)
+        torch.cuda.synchronize()
+        _log(f"warmup {w} done")
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for s in range(args.steps):
+        run_pipeline(ygg, pos, mass, args.leaf, args.theta)
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(end) / args.steps
+    _log(f"timed {args.steps} steps: {ms:.3f} ms/step (wall {wall / args.steps * 1e3:.3f} ms)")
+    stages = stage_times(ygg, torch, pos, mass, args.leaf, args.theta)
+    _log(f"stages {stages}")
+    roof = walk_roofline(ygg, torch, pos, mass, args.leaf, args.theta)
+    _log(f"roofline {roof}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(1 << args.cpu_sample_log2n, args.leaf, args.theta)
+        _log(f"cpu baseline {cpu}")
+    value = n / (ms * 1e-3) / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"clustered 2^{args.log2n} points (64 Plummer clusters + 10% "
+                               f"uniform background, seed 3), build_octree leaf {args.leaf}, "
+                               f"dehnen theta {args.theta}, count pass + exact fill, int64 outputs",
+                   "parallelism": "single GPU"},
+        "stages": stages, "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,244 @@
+/* C ABI of the yggdrax MI355X tree pipeline (libyggdrax_amd.so).
+ *
+ * Every entry point takes caller-owned device pointers, plain sizes, a
+ * caller-provided device workspace (size from the matching *_workspace_bytes
+ * query) and a hipStream_t passed as void*. No entry point allocates device
+ * memory or synchronises the stream; all launch asynchronously on `stream`.
+ * Return value: YGG_OK or one of the YGG_ERR_* codes.
+ *
+ * Device indices are int32 (node ids, particle positions < 2^31); the host
+ * wrappers widen them to the index dtype of the reference (int64 unless
+ * YGGDRAX_INDEX_PRECISION=int32). Citations name the reference function each
+ * entry point replaces (files under TobiBu/yggdrax, path:line).
+ */
+#ifndef YGGDRAX_AMD_H
+#define YGGDRAX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YGG_OK 0
+#define YGG_ERR_ARG 1
+#define YGG_ERR_HIP 2
+#define YGG_ERR_WORKSPACE 3
+#define YGG_ERR_UNSUPPORTED 4
+
+/* ---- primitives ---------------------------------------------------------- */
+
+/* Stable LSD radix sort of (key, u32 value) pairs over key bits
+ * [begin_bit, end_bit), 8 bits per pass. vals_in may be NULL (values become
+ * 0..n-1). keys_in must differ from keys_out. Replaces the XLA sorts behind
+ * jnp.argsort(stable=True) (yggdrax/_tree_impl.py:401, yggdrax/octree.py:259,
+ * yggdrax/_interactions_impl.py:1082). */
+size_t ygg_sort_workspace_bytes(int64_t n, int key_bytes, int begin_bit, int end_bit);
+int ygg_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                       size_t ws_bytes, void* stream);
+int ygg_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                       size_t ws_bytes, void* stream);
+
+/* ---- bounds and Morton codes -------------------------------------------- */
+
+/* infer_bounds (yggdrax/bounds.py:11-33): bounds6 = {lo.x, lo.y, lo.z,
+ * hi.x, hi.y, hi.z} of positions (n x 3 float32, row-major), padded by
+ * max(5 % of the span, 1e-6). */
+size_t ygg_bounds_workspace_bytes(void);
+int ygg_infer_bounds(const float* pos, int64_t n, float* bounds6, void* ws, size_t ws_bytes,
+                     void* stream);
+
+/* morton_encode_impl (yggdrax/morton.py:43-77): 21-bit-per-axis codes. */
+int ygg_morton_encode(const float* pos, int64_t n, const float* bounds6, uint64_t* codes,
+                      void* stream);
+
+/* Morton encode fused with the digit histograms, then a stable sort by code
+ * (yggdrax/tree.py:399-404, yggdrax/_tree_impl.py:397-402): sorted_codes[n]
+ * and perm[n] (particle index of each sorted position). */
+size_t ygg_morton_sort_workspace_bytes(int64_t n);
+int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t* sorted_codes,
+                    uint32_t* perm, void* ws, size_t ws_bytes, void* stream);
+
+/* Gathers of the Morton-sorted particles (return_reordered of
+ * yggdrax/_tree_impl.py:1045-1050): pos_sorted[i] = pos[perm[i]],
+ * mass_sorted[i] = mass[perm[i]] (mass may be NULL), inverse[perm[i]] = i. */
+int ygg_reorder(const float* pos, const float* mass, const uint32_t* perm, int64_t n,
+                float* pos_sorted, float* mass_sorted, int32_t* inverse, void* stream);
+
+/* ---- radix (LBVH) topology ---------------------------------------------- */
+
+/* Output arrays of the adaptive radix build for L = ceil(n/leaf_size) leaves
+ * and T = 2L-1 nodes (internal nodes 0..L-2, leaf j is node L-1+j). */
+typedef struct YggRadixTopology {
+  int32_t* parent;         /* [T], -1 for the root */
+  int32_t* left_child;     /* [L-1] */
+  int32_t* right_child;    /* [L-1] */
+  uint8_t* left_is_leaf;   /* [L-1] */
+  uint8_t* right_is_leaf;  /* [L-1] */
+  int32_t* node_ranges;    /* [T][2], inclusive particle range */
+  int32_t* node_level;     /* [T] */
+  int32_t* level_offsets;  /* [65] */
+  int32_t* nodes_by_level; /* [T] */
+  uint64_t* leaf_codes;    /* [L] */
+  int32_t* scalars;        /* [8]: num_levels, max_level, deep_levels,
+                              malformed, roots, root */
+} YggRadixTopology;
+
+/* _build_tree_from_leaf_partitions (yggdrax/_tree_impl.py:1009-1355) for the
+ * fixed-count leaf partition of build_tree / build_octree's adaptive mode
+ * (yggdrax/_tree_impl.py:405-406, yggdrax/tree.py:409-410). If
+ * scalars[2] (deep_levels) is set on return, node levels exceeded 255 and
+ * ygg_levels_pointer_doubling must be run before node_level, level_offsets
+ * and nodes_by_level are valid. */
+size_t ygg_radix_build_workspace_bytes(int64_t n, int64_t leaf_size);
+int ygg_radix_build(const uint64_t* sorted_codes, int64_t n, int64_t leaf_size,
+                    const YggRadixTopology* out, void* ws, size_t ws_bytes, void* stream);
+
+/* Exact bounded pointer-doubling levels (yggdrax/_tree_impl.py:1272-1316)
+ * for trees whose depth exceeds the fast path's walk (duplicate leaf
+ * codes). Rewrites node_level, level_offsets, nodes_by_level, scalars[0..1]. */
+int ygg_levels_pointer_doubling(const YggRadixTopology* topo, int64_t num_nodes, void* ws,
+                                size_t ws_bytes, void* stream);
+
+/* ---- explicit octree metadata -------------------------------------------- */
+
+typedef struct YggOctreeMeta {
+  int32_t* oct_parent;         /* [M], M = max(1, 2L-1) */
+  int32_t* oct_children;       /* [M][8] */
+  int32_t* oct_child_counts;   /* [M] */
+  uint8_t* oct_child_mask;     /* [M][8] */
+  uint8_t* oct_valid_mask;     /* [M] */
+  uint64_t* oct_node_codes;    /* [M] */
+  int32_t* oct_node_depths;    /* [M] */
+  int32_t* oct_node_ranges;    /* [M][2] */
+  int32_t* oct_nodes_by_level; /* [M] */
+  int32_t* oct_level_offsets;  /* [23] */
+  uint8_t* oct_leaf_mask;      /* [M] */
+  int32_t* oct_leaf_nodes;     /* [M], -1 padded */
+  int32_t* radix_node_to_oct;  /* [T]; radix_leaf_to_oct = tail L entries */
+  int32_t* leaf_indices;       /* [L] native (octree) leaf order, node ids */
+  int32_t* leaf_position;      /* [T] native row of a leaf node, else -1 */
+  int32_t* scalars;            /* [4]: oct_num_levels, num_valid, num_leaf */
+} YggOctreeMeta;
+
+/* build_explicit_octree_metadata + augment_radix_topology_with_octree
+ * (yggdrax/octree.py:204-419, 479-574) and the octree leaf ordering of
+ * _resolve_leaf_ordering (yggdrax/_interactions_impl.py:1071-1103). */
+size_t ygg_octree_meta_workspace_bytes(int64_t num_leaves);
+int ygg_octree_meta(const uint64_t* sorted_codes, const int32_t* node_ranges, int64_t num_leaves,
+                    const YggOctreeMeta* out, void* ws, size_t ws_bytes, void* stream);
+
+
+/* ---- geometry ------------------------------------------------------------ */
+
+/* compute_tree_geometry (yggdrax/geometry.py:18-75,
+ * yggdrax/_geometry_impl.py:176-338): per-node AABB centre, half extent
+ * (clamped to >= 1e-6), bounding-sphere radius and max half extent from the
+ * Morton-sorted positions. level_sync = 0 uses the bottom-up merge (valid for
+ * well-formed trees, YggRadixTopology.scalars[3] == 0); level_sync = 1 runs
+ * the reference's level-by-level pass using node_level and max_level. */
+size_t ygg_geometry_workspace_bytes(int64_t num_nodes);
+int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int32_t* left,
+                 const int32_t* right, const int32_t* parent, const int32_t* node_level,
+                 int64_t num_nodes, int64_t leaf_cap, int level_sync, int32_t max_level,
+                 float* center, float* half_extent, float* radius, float* max_extent, void* ws,
+                 size_t ws_bytes, void* stream);
+
+/* Particle-order leaf table of a radix tree (_resolve_leaf_ordering,
+ * yggdrax/_interactions_impl.py:1071-1103 without octree metadata). */
+int ygg_radix_leaf_order(int64_t num_leaves, int32_t* leaf_indices, int32_t* leaf_position,
+                         void* stream);
+
+/* ---- dual-tree traversal -------------------------------------------------- */
+
+#define YGG_MAC_BH 0
+#define YGG_MAC_DEHNEN 1
+#define YGG_MAC_ENGBLOM 2
+
+#define YGG_WALK_COUNT 0   /* counts only (the count pass)                     */
+#define YGG_WALK_FIXED 1   /* rows of fixed capacity: row r at r * cap          */
+#define YGG_WALK_OFFSETS 2 /* rows at row_start[r] with capacity row_cap[r]     */
+
+/* Node records for the walk: 32 bytes per node (centre, MAC extent of
+ * _build_mac_extents, yggdrax/_interactions_impl.py:993-1061, children,
+ * native leaf row). */
+int ygg_walk_prepare_nodes(const float* center, const float* radius, const float* max_extent,
+                           const int32_t* left, const int32_t* right,
+                           const int32_t* leaf_position, int64_t num_nodes, int mac_type,
+                           float dehnen_radius_scale, void* nodes_out, void* stream);
+
+typedef struct YggWalkParams {
+  int32_t num_nodes, num_internal, num_leaves, root;
+  int32_t mac_type, mode, collect_far, collect_near;
+  float theta_sq, theta_root; /* float32(theta)^2 and its sqrt (engblom) */
+  int32_t far_cap, near_cap;  /* fixed row capacities (YGG_WALK_FIXED) */
+  int64_t queue_cap;          /* pair-queue (wavefront) capacity */
+  int32_t max_generations, pad0;
+  const void* nodes;          /* [num_nodes] records from ygg_walk_prepare_nodes */
+  uint64_t* wf0;              /* [queue_cap] wavefront ping buffer */
+  uint64_t* wf1;              /* [queue_cap] wavefront pong buffer */
+  uint32_t* gen_sizes;        /* [max_generations + 1] raw push counts */
+  uint32_t* flags;            /* [4]: queue, far, near overflow */
+  uint64_t* decisions;        /* [3]: accept, near, refine */
+  int32_t* far_counts;        /* [num_nodes] */
+  int32_t* near_counts;       /* [num_leaves], native leaf rows */
+  const int64_t* far_row_start;
+  const int32_t* far_row_cap;
+  int32_t* far_entries;
+  const int64_t* near_row_start;
+  const int32_t* near_row_cap;
+  int32_t* near_entries;
+} YggWalkParams;
+
+/* One fixed-capacity attempt of _dual_tree_walk_impl
+ * (yggdrax/_interactions_impl.py:1131-1785), generation by generation from
+ * the pair (root, root). Blocks on `stream` between batches of generations
+ * to read the wavefront sizes into host_status (pinned host memory of
+ * max_generations + 5 words: gen sizes, then the 4 flags). If gen_ms is not
+ * NULL (host array of max_generations floats, max_generations <= 1024), the
+ * duration of every generation's kernel is measured with HIP events
+ * recorded on `stream` around the launch. */
+int ygg_walk_bfs(const YggWalkParams* params, uint32_t* host_status, int32_t* num_generations,
+                 float* gen_ms, void* stream);
+
+/* Exclusive prefix sum of int32 counts (optionally gathered through
+ * `gather`) into int64 offsets[n + 1]. */
+size_t ygg_scan_workspace_bytes(int64_t n);
+int ygg_exclusive_scan_i32(const int32_t* counts, const int32_t* gather, int64_t* offsets,
+                           int64_t n, void* ws, size_t ws_bytes, void* stream);
+
+/* row_start[order[r]] = offsets[r] (order NULL: identity). */
+int ygg_rows_from_offsets(const int32_t* order, const int64_t* offsets, int64_t n,
+                          int64_t* row_start, void* stream);
+
+/* counts[i] = min(counts[i], caps ? caps[i] : fixed_cap). */
+int ygg_cap_counts(int32_t* counts, int64_t n, int32_t fixed_cap, const int32_t* caps,
+                   void* stream);
+
+/* Far list in level-major CSR (_result_to_interactions,
+ * yggdrax/_interactions_impl.py:3561-3637): rows taken in nodes_by_level
+ * order from entries (row r at row_start[r], or r * fixed_cap when
+ * row_start is NULL). Writes sources/targets/target_levels[F] and
+ * node_offsets[T + 1] as int64 (out_i64) or int32. */
+int ygg_far_output(int64_t num_nodes, const int32_t* nodes_by_level, const int64_t* lm_offsets,
+                   const int32_t* counts, int32_t fixed_cap, const int64_t* row_start,
+                   const int32_t* entries, const int32_t* node_level, int out_i64,
+                   void* sources, void* targets, void* target_levels, void* node_offsets,
+                   void* stream);
+
+/* Near list CSR in native leaf order plus the dense -1 padded
+ * neighbor_leaf_positions[L][max_count] (_result_to_neighbors,
+ * yggdrax/_interactions_impl.py:3640-3837). */
+int ygg_near_output(int64_t num_leaves, const int64_t* offsets, const int32_t* counts,
+                    int32_t fixed_cap, const int64_t* row_start, const int32_t* entries,
+                    const int32_t* leaf_position, int64_t max_count, int out_i64,
+                    void* neighbors, void* neighbor_leaf_positions, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YGGDRAX_AMD_H */

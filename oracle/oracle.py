@@ -1,0 +1,497 @@
+"""Python driver of the CPU oracle (test infrastructure only).
+
+Wraps ``libygg_oracle.so`` (a C++ restatement of the reference algorithm)
+and restates, in plain Python/numpy, the reference's host-side pieces:
+topology assembly (yggdrax/tree.py:375-423, 1482-1549), MAC extents
+(yggdrax/_interactions_impl.py:993-1061), leaf ordering (1071-1103), the
+capacity/retry driver (3840-4440) and output shaping (3561-3837).
+Outputs are dicts of numpy arrays whose keys are the reference's NamedTuple
+field names; indices are int64 as under the reference's default
+INDEX_DTYPE.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libygg_oracle.so")
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        import subprocess
+
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return ctypes.CDLL(_LIB_PATH)
+
+
+lib = _load()
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int64
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+lib.ygo_walk_run.restype = ctypes.c_void_p
+lib.ygo_walk_run.argtypes = [_I, _I, _P, _P, _P, _P, _P, _I, ctypes.c_float,
+                             ctypes.c_int, _I, _I, _I, ctypes.c_int, ctypes.c_int]
+lib.ygo_walk_sizes.argtypes = [_P, _P]
+lib.ygo_walk_copy.argtypes = [_P, _P, _P, _P, _P, _P]
+lib.ygo_walk_free.argtypes = [_P]
+lib.ygo_geometry.restype = ctypes.c_int
+
+MAC_CODES = {"bh": 0, "dehnen": 1, "engblom": 2}
+
+
+def infer_bounds(positions):
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    lib.ygo_infer_bounds(_ptr(pos), _I(pos.shape[0]), _ptr(lo), _ptr(hi))
+    return lo, hi
+
+
+def morton_encode(positions, bounds):
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    lo = np.ascontiguousarray(bounds[0], dtype=np.float32)
+    hi = np.ascontiguousarray(bounds[1], dtype=np.float32)
+    codes = np.zeros(pos.shape[0], np.uint64)
+    lib.ygo_morton_encode(_ptr(pos), _I(pos.shape[0]), _ptr(lo), _ptr(hi), _ptr(codes))
+    return codes
+
+
+def stable_argsort_u64(keys):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    perm = np.zeros(keys.shape[0], np.int64)
+    lib.ygo_stable_argsort_u64(_ptr(keys), _I(keys.shape[0]), _ptr(perm))
+    return perm
+
+
+def _radix_from_sorted(sorted_codes, leaf_size):
+    n = sorted_codes.shape[0]
+    L = (n + leaf_size - 1) // leaf_size
+    T = 2 * L - 1
+    NI = L - 1
+    out = dict(
+        parent=np.zeros(T, np.int64),
+        left_child=np.zeros(NI, np.int64),
+        right_child=np.zeros(NI, np.int64),
+        left_is_leaf=np.zeros(NI, np.uint8),
+        right_is_leaf=np.zeros(NI, np.uint8),
+        node_ranges=np.zeros((T, 2), np.int64),
+        node_level=np.zeros(T, np.int64),
+        level_offsets=np.zeros(65, np.int64),
+        nodes_by_level=np.zeros(T, np.int64),
+        leaf_codes=np.zeros(L, np.uint64),
+    )
+    scal = np.zeros(4, np.int64)
+    sc = np.ascontiguousarray(sorted_codes, dtype=np.uint64)
+    lib.ygo_radix_build(
+        _ptr(sc), _I(n), _I(leaf_size), _ptr(out["parent"]), _ptr(out["left_child"]),
+        _ptr(out["right_child"]), _ptr(out["left_is_leaf"]), _ptr(out["right_is_leaf"]),
+        _ptr(out["node_ranges"]), _ptr(out["node_level"]), _ptr(out["level_offsets"]),
+        _ptr(out["nodes_by_level"]), _ptr(scal), _ptr(out["leaf_codes"]))
+    out["left_is_leaf"] = out["left_is_leaf"].astype(bool)
+    out["right_is_leaf"] = out["right_is_leaf"].astype(bool)
+    out["num_levels"] = int(scal[0])
+    out["num_internal_nodes"] = NI
+    out["num_particles"] = n
+    out["leaf_depths"] = np.full(L, -1, np.int64)
+    out["use_morton_geometry"] = False
+    out["leaf_size"] = int(leaf_size)
+    return out
+
+
+def build_tree(positions, masses=None, bounds=None, *, leaf_size=8,
+               lexsort=False):
+    """tree.py:1413-1478 -> _tree_impl.py:355-420 (radix adaptive build)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    n = pos.shape[0]
+    if n < 1:
+        raise ValueError("Need at least one particle")
+    if leaf_size < 1:
+        raise ValueError("leaf_size must be >= 1")
+    b = infer_bounds(pos) if bounds is None else (
+        np.asarray(bounds[0], np.float32), np.asarray(bounds[1], np.float32))
+    codes = morton_encode(pos, b)
+    perm = stable_argsort_u64(codes)
+    sorted_codes = codes[perm]
+    topo = _radix_from_sorted(sorted_codes, leaf_size)
+    topo["particle_indices"] = perm
+    topo["morton_codes"] = sorted_codes
+    topo["bounds_min"] = b[0].copy()
+    topo["bounds_max"] = b[1].copy()
+    topo["positions_sorted"] = pos[perm]
+    if masses is not None:
+        topo["masses_sorted"] = np.asarray(masses, np.float32)[perm]
+    inv = np.zeros(n, np.int64)
+    inv[perm] = np.arange(n)
+    topo["inverse_permutation"] = inv
+    return topo
+
+
+def octree_metadata(topo):
+    """octree.py:479-574 on a radix topology with adaptive leaves."""
+    T = topo["parent"].shape[0]
+    L = (T + 1) // 2
+    M = max(1, 2 * L - 1)
+    out = dict(
+        oct_parent=np.zeros(M, np.int64),
+        oct_children=np.zeros((M, 8), np.int64),
+        oct_child_counts=np.zeros(M, np.int64),
+        oct_child_mask=np.zeros((M, 8), np.uint8),
+        oct_valid_mask=np.zeros(M, np.uint8),
+        oct_node_codes=np.zeros(M, np.uint64),
+        oct_node_depths=np.zeros(M, np.int64),
+        oct_node_ranges=np.zeros((M, 2), np.int64),
+        oct_nodes_by_level=np.zeros(M, np.int64),
+        oct_level_offsets=np.zeros(23, np.int64),
+        oct_leaf_mask=np.zeros(M, np.uint8),
+        oct_leaf_nodes=np.zeros(M, np.int64),
+        radix_node_to_oct=np.zeros(T, np.int64),
+        radix_leaf_to_oct=np.zeros(L, np.int64),
+    )
+    resolved_codes = np.zeros(L, np.uint64)
+    resolved_depths = np.zeros(L, np.int64)
+    scal = np.zeros(4, np.int64)
+    sc = np.ascontiguousarray(topo["morton_codes"], np.uint64)
+    nr = np.ascontiguousarray(topo["node_ranges"], np.int64)
+    lib.ygo_octree_meta(
+        _ptr(sc), _ptr(nr), _I(L), _ptr(out["oct_parent"]), _ptr(out["oct_children"]),
+        _ptr(out["oct_child_counts"]), _ptr(out["oct_child_mask"]), _ptr(out["oct_valid_mask"]),
+        _ptr(out["oct_node_codes"]), _ptr(out["oct_node_depths"]), _ptr(out["oct_node_ranges"]),
+        _ptr(out["oct_nodes_by_level"]), _ptr(out["oct_level_offsets"]), _ptr(scal),
+        _ptr(out["oct_leaf_mask"]), _ptr(out["oct_leaf_nodes"]), _ptr(out["radix_node_to_oct"]),
+        _ptr(out["radix_leaf_to_oct"]), _ptr(resolved_codes), _ptr(resolved_depths))
+    for k in ("oct_child_mask", "oct_valid_mask", "oct_leaf_mask"):
+        out[k] = out[k].astype(bool)
+    out["oct_num_levels"] = int(scal[0])
+    out["_resolved_codes"] = resolved_codes
+    out["_resolved_depths"] = resolved_depths
+    return out
+
+
+def build_octree(positions, masses=None, bounds=None, *, leaf_size=8):
+    """tree.py:1482-1549 -> _build_octree_result (375-423) + octree.py:553-574.
+
+    The octree path sorts with lexsort((orig_idx, codes)), which is the same
+    stable order as the radix path's stable argsort.
+    """
+    topo = build_tree(positions, masses, bounds, leaf_size=leaf_size)
+    topo.update(octree_metadata(topo))
+    return topo
+
+
+def compute_tree_geometry(topo, positions_sorted, max_leaf_size=None):
+    """geometry.py:18-75 -> _geometry_impl.py:176-338."""
+    pos = np.ascontiguousarray(positions_sorted, dtype=np.float32)
+    T = topo["parent"].shape[0]
+    cap = max_leaf_size if max_leaf_size is not None else topo.get("leaf_size")
+    if cap is None:
+        cap = int((topo["node_ranges"][:, 1] - topo["node_ranges"][:, 0]).max()) + 1
+    center = np.zeros((T, 3), np.float32)
+    half = np.zeros((T, 3), np.float32)
+    radius = np.zeros(T, np.float32)
+    maxext = np.zeros(T, np.float32)
+    NI = (T - 1) // 2
+    left = np.ascontiguousarray(topo["left_child"], np.int64) if NI else np.zeros(1, np.int64)
+    right = np.ascontiguousarray(topo["right_child"], np.int64) if NI else np.zeros(1, np.int64)
+    st = lib.ygo_geometry(_ptr(pos), _ptr(np.ascontiguousarray(topo["node_ranges"], np.int64)),
+                          _ptr(left), _ptr(right),
+                          _ptr(np.ascontiguousarray(topo["parent"], np.int64)), _I(T), _I(int(cap)),
+                          _ptr(center), _ptr(half), _ptr(radius), _ptr(maxext))
+    if st != 0:
+        raise RuntimeError("geometry depth pointer doubling did not converge")
+    return dict(center=center, half_extent=half, radius=radius, max_extent=maxext)
+
+
+def mac_extents(geom, mac_type, dehnen_radius_scale=1.0):
+    """_build_mac_extents (993-1061). Zero-extent propagation is the identity
+    here because every half extent is clamped to >= 1e-6."""
+    if mac_type in ("dehnen", "engblom"):
+        e = np.asarray(geom["radius"], np.float32)
+    else:
+        e = np.asarray(geom["max_extent"], np.float32)
+    if mac_type == "dehnen":
+        e = (np.float32(dehnen_radius_scale) * e).astype(np.float32)
+    return np.ascontiguousarray(e, np.float32)
+
+
+def leaf_ordering(topo):
+    """_resolve_leaf_ordering (1071-1103)."""
+    T = topo["parent"].shape[0]
+    NI = (T - 1) // 2
+    L = T - NI
+    po = np.arange(NI, T, dtype=np.int64)
+    native = np.arange(L, dtype=np.int64)
+    if "radix_leaf_to_oct" in topo:
+        native = np.argsort(np.asarray(topo["radix_leaf_to_oct"], np.int64), kind="stable")
+    leaf_indices = po[native]
+    p2n = np.zeros(L, np.int64)
+    p2n[native] = np.arange(L)
+    leaf_pos = np.full(T, -1, np.int64)
+    leaf_pos[leaf_indices] = np.arange(L)
+    return leaf_indices, leaf_pos, po, p2n
+
+
+def walk(topo, geom, theta, mac_type="bh", dehnen_radius_scale=1.0, *,
+         queue_cap, far_cap, near_cap, collect_far=True, collect_near=True):
+    """One fixed-capacity attempt of _dual_tree_walk_impl (1131-1905)."""
+    T = topo["parent"].shape[0]
+    NI = (T - 1) // 2
+    L = T - NI
+    leaf_indices, leaf_pos, _, _ = leaf_ordering(topo)
+    ext = mac_extents(geom, mac_type, dehnen_radius_scale)
+    center = np.ascontiguousarray(geom["center"], np.float32)
+    root = int(np.argmin(topo["parent"]))
+    left = np.ascontiguousarray(topo["left_child"], np.int64) if NI else np.zeros(1, np.int64)
+    right = np.ascontiguousarray(topo["right_child"], np.int64) if NI else np.zeros(1, np.int64)
+    h = lib.ygo_walk_run(_I(T), _I(NI), _ptr(left), _ptr(right), _ptr(center), _ptr(ext),
+                         _ptr(leaf_pos), _I(root), ctypes.c_float(theta),
+                         ctypes.c_int(MAC_CODES[mac_type]), _I(int(queue_cap)), _I(int(far_cap)),
+                         _I(int(near_cap)), ctypes.c_int(int(collect_far)),
+                         ctypes.c_int(int(collect_near)))
+    try:
+        sizes = np.zeros(9, np.int64)
+        lib.ygo_walk_sizes(ctypes.c_void_p(h), _ptr(sizes))
+        fc = np.zeros(T, np.int64)
+        nc = np.zeros(L, np.int64)
+        fe = np.zeros(max(1, sizes[0]), np.int64)
+        ne = np.zeros(max(1, sizes[1]), np.int64)
+        gs = np.zeros(max(1, sizes[2]), np.int64)
+        lib.ygo_walk_copy(ctypes.c_void_p(h), _ptr(fc), _ptr(fe), _ptr(nc), _ptr(ne), _ptr(gs))
+    finally:
+        lib.ygo_walk_free(ctypes.c_void_p(h))
+    return dict(far_counts=fc, far_entries=fe[: sizes[0]], near_counts=nc,
+                near_entries=ne[: sizes[1]], gen_sizes=gs[: sizes[2]],
+                queue_overflow=bool(sizes[3]), far_overflow=bool(sizes[4]),
+                near_overflow=bool(sizes[5]), accept_decisions=int(sizes[6]),
+                near_decisions=int(sizes[7]), refine_decisions=int(sizes[8]),
+                leaf_indices=leaf_indices, leaf_position=leaf_pos)
+
+
+# --- capacity ladders (_interactions_impl.py:296-316, 451-563) -------------
+
+def _queue_candidates_bounded(max_capacity, process_block):
+    qmax = max(4, int(max_capacity))
+    base = min(max(1024, int(process_block) * 16), qmax)
+    c = [base]
+    while c[-1] < qmax:
+        nxt = min(qmax, c[-1] * 2)
+        if nxt == c[-1]:
+            break
+        c.append(nxt)
+    return c
+
+
+def _auto_pair_queue_candidates(total_nodes, num_internal):
+    conservative = 128 * max(4, total_nodes)
+    start = min(max(4096, num_internal * 2), conservative)
+    if start >= conservative:
+        return [int(conservative)]
+    c = []
+    cap = start
+    while cap < conservative:
+        c.append(int(cap))
+        cap = max(cap * 2, cap + 1024)
+    c.append(int(conservative))
+    return list(dict.fromkeys(c))
+
+
+def _interaction_capacity_candidates(requested, total_nodes):
+    if requested is not None:
+        if requested < 1:
+            raise ValueError("max_interactions_per_node must be >= 1")
+        return [int(requested)]
+    scaled = min(max(total_nodes // 4, 2048), 16384)
+    p2 = 1 << (max(1, scaled) - 1).bit_length()
+    return [int(min(16384, max(2048, p2)))]
+
+
+def _neighbor_capacity_candidates(requested, num_leaves):
+    base = max(1, int(requested))
+    hard = max(1, ((1 << 31) - 1) // max(1, int(num_leaves)))
+    base = min(base, hard)
+    c = [base]
+    while c[-1] < hard:
+        nxt = min(hard, max(c[-1] * 2, c[-1] + 1024))
+        if nxt == c[-1]:
+            break
+        c.append(nxt)
+    return list(dict.fromkeys(c))
+
+
+QUEUE_ERR_USER = "Pair queue capacity exceeded; increase max_pair_queue and rebuild."
+QUEUE_ERR_AUTO = ("Pair queue capacity exceeded; pass a larger max_pair_queue "
+                  "value to override auto-scaling.")
+FAR_ERR = "Interaction list capacity exceeded; increase max_interactions_per_node and rebuild."
+NEAR_ERR = "Neighbor list capacity exceeded; increase max_neighbors_per_leaf and rebuild."
+
+
+def run_walk_with_retries(topo, geom, theta, mac_type="bh", dehnen_radius_scale=1.0, *,
+                          max_interactions_per_node=None, max_neighbors_per_leaf=2048,
+                          max_pair_queue=None, traversal_config=None, events=None):
+    """_run_dual_tree_walk_raw (3840-4440), eager mode, default pair policy."""
+    T = topo["parent"].shape[0]
+    NI = (T - 1) // 2
+    L = T - NI
+    if dehnen_radius_scale <= 0.0:
+        raise ValueError("dehnen_radius_scale must be > 0")
+    kw = dict(theta=theta, mac_type=mac_type, dehnen_radius_scale=dehnen_radius_scale)
+    cfg = traversal_config
+    if cfg is not None:
+        qmax = max(4, int(cfg.max_pair_queue))
+        inter = [int(cfg.max_interactions_per_node)]
+        nbr = _neighbor_capacity_candidates(int(cfg.max_neighbors_per_leaf), L)
+        user_q = True
+        qc = _queue_candidates_bounded(qmax, int(cfg.process_block))
+        for cap in _auto_pair_queue_candidates(T, NI):
+            if cap > qc[-1]:
+                qc.append(cap)
+        qc = list(dict.fromkeys(qc))
+    else:
+        user_q = max_pair_queue is not None
+        qc = [max(4, int(max_pair_queue))] if user_q else _auto_pair_queue_candidates(T, NI)
+        inter = _interaction_capacity_candidates(max_interactions_per_node, T)
+        nbr = _neighbor_capacity_candidates(max_neighbors_per_leaf, L)
+    use_count = max_interactions_per_node is None and max_pair_queue is None and cfg is None
+    qerr = QUEUE_ERR_USER if user_q else QUEUE_ERR_AUTO
+
+    if use_count:
+        cap = max(T * 128, _auto_pair_queue_candidates(T, NI)[-1])
+        cnt = walk(topo, geom, queue_cap=cap, far_cap=1 << 62, near_cap=1 << 62, **kw)
+        if cnt["queue_overflow"]:
+            raise RuntimeError("Count-pass traversal wavefront overflowed its conservative buffer.")
+        if int(cnt["near_counts"].max(initial=0)) > max(1, ((1 << 31) - 1) // max(1, L)):
+            raise RuntimeError("Suggested per-leaf neighbor capacity is too large to safely "
+                               "allocate on the host (would overflow int32). Enable local "
+                               "refinement or pass explicit capacities "
+                               "(max_neighbors_per_leaf / max_interactions_per_node).")
+        if cnt["far_counts"].sum() > (1 << 31) - 1 or cnt["near_counts"].sum() > (1 << 31) - 1:
+            raise RuntimeError("Dual-tree traversal pair totals exceed signed int32 capacity; "
+                               "refine the tree or use a lower-memory traversal strategy.")
+        cnt["max_wf"] = int(max(1, cnt["gen_sizes"].max(initial=1)))
+        return cnt
+
+    result = None
+    success = False
+    for Q in qc:
+        queue_retry = False
+        for K in inter:
+            for Kn in nbr:
+                result = walk(topo, geom, queue_cap=Q, far_cap=K, near_cap=Kn, **kw)
+                result["attempt_caps"] = (Q, K, Kn)
+                if result["queue_overflow"]:
+                    if events is not None:
+                        events.append(("queue_overflow", Q, K))
+                    queue_retry = True
+                    break
+                if result["far_overflow"]:
+                    raise RuntimeError(FAR_ERR)
+                if result["near_overflow"]:
+                    if events is not None:
+                        events.append(("neighbor_overflow", Q, K))
+                    continue
+                success = True
+                if events is not None and len(events) > 0:
+                    events.append(("success", Q, K))
+                break
+            if queue_retry or success:
+                break
+        else:
+            continue
+        if queue_retry and not success:
+            continue
+        if success:
+            break
+    else:
+        if result["queue_overflow"]:
+            raise RuntimeError(qerr)
+    if not success and result["far_overflow"]:
+        raise RuntimeError(FAR_ERR)
+    if result["far_overflow"]:
+        raise RuntimeError(FAR_ERR)
+    if result["near_overflow"]:
+        raise RuntimeError(NEAR_ERR)
+    return result
+
+
+def to_interactions(raw, topo):
+    """_result_to_interactions (3561-3637)."""
+    T = topo["parent"].shape[0]
+    counts = raw["far_counts"]
+    total = int(counts.sum())
+    num_levels = int(topo["num_levels"])
+    level_idx = np.asarray(topo["level_offsets"], np.int64)[: num_levels + 1]
+    nbl = np.asarray(topo["nodes_by_level"], np.int64)
+    cbl = counts[nbl]
+    obl = np.concatenate([[0], np.cumsum(cbl)]).astype(np.int64)
+    node_off = np.zeros(T, np.int64)
+    node_off[nbl] = obl[:-1]
+    node_off = np.concatenate([node_off, [total]]).astype(np.int64)
+    if total == 0:
+        return dict(offsets=np.zeros(T + 1, np.int64), sources=np.zeros(0, np.int64),
+                    targets=np.zeros(0, np.int64), counts=counts,
+                    level_offsets=np.zeros(num_levels + 1, np.int64),
+                    target_levels=np.zeros(0, np.int64))
+    # raw entries are node-id ordered rows; regroup rows in level-major order.
+    row_start = np.concatenate([[0], np.cumsum(counts)])
+    src = np.empty(total, np.int64)
+    tgt = np.empty(total, np.int64)
+    for node in range(T):
+        c = counts[node]
+        if c:
+            o = node_off[node]
+            src[o:o + c] = raw["far_entries"][row_start[node]:row_start[node] + c]
+            tgt[o:o + c] = node
+    levels = np.asarray(topo["node_level"], np.int64)[tgt]
+    return dict(offsets=node_off, sources=src, targets=tgt, counts=counts,
+                level_offsets=obl[level_idx], target_levels=levels)
+
+
+def to_neighbors(raw, topo):
+    """_result_to_neighbors (3640-3837) with the default target block size 0."""
+    counts = raw["near_counts"]
+    L = counts.shape[0]
+    leaf_indices, _, po, p2n = leaf_ordering(topo)
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    nbrs = np.asarray(raw["near_entries"], np.int64)
+    T = topo["parent"].shape[0]
+    mx = int(counts.max(initial=0))
+    if mx == 0:
+        nlp = np.zeros((L, 0), np.int64)
+    else:
+        lookup = np.full(T, -1, np.int64)
+        lookup[leaf_indices] = np.arange(L)
+        cols = np.arange(mx)
+        valid = cols[None, :] < counts[:, None]
+        idx = np.where(valid, offsets[:-1, None] + cols[None, :], 0)
+        nlp = np.where(valid, lookup[nbrs[idx]] if nbrs.size else -1, -1)
+    return dict(offsets=offsets, neighbors=nbrs, leaf_indices=leaf_indices, counts=counts,
+                particle_order_leaf_indices=po, particle_order_to_native_leaf=p2n,
+                neighbor_leaf_positions=nlp.astype(np.int64),
+                target_block_leaf_ids=np.zeros(0, np.int64),
+                target_block_source_leaf_ids=np.zeros((0, 0), np.int64),
+                target_block_valid_mask=np.zeros((0, 0), bool),
+                target_block_offsets=np.zeros(L + 1, np.int64), target_block_size=0)
+
+
+def build_interactions_and_neighbors(topo, geom, theta=0.5, max_interactions_per_node=None,
+                                     max_neighbors_per_leaf=2048, max_pair_queue=None,
+                                     process_block=None, traversal_config=None,
+                                     mac_type="bh", dehnen_radius_scale=1.0, events=None):
+    """_interactions_impl.py:6032-6144 with return_result=False."""
+    raw = run_walk_with_retries(
+        topo, geom, theta, mac_type, dehnen_radius_scale,
+        max_interactions_per_node=max_interactions_per_node,
+        max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
+        traversal_config=traversal_config, events=events)
+    return to_interactions(raw, topo), to_neighbors(raw, topo), raw

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP pipeline against the CPU oracle on identical inputs.
+
+Integer outputs (codes, permutation, topology, octree tables, counts,
+offsets, per-row interaction/neighbour sets) must be identical; float32
+geometry must match within 1e-6 relative (and is checked bit-exact).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests import parity
+from yggdrax_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(c):
+    if c is None:
+        return None
+    from yggdrax_amd import DualTreeTraversalConfig
+    return DualTreeTraversalConfig(*c)
+
+
+class _OCfg:
+    def __init__(self, c):
+        self.max_pair_queue, self.process_block, self.max_interactions_per_node, \
+            self.max_neighbors_per_leaf = c
+
+
+CASES = {
+    "c1_uniform512_octree": (lambda: inputs.uniform_cube(512, 0), "octree", 16, 0.6, "dehnen",
+                             (8192, 256, 2048, 2048)),
+    "getting_started_radix": (lambda: inputs.uniform_cube(256, 7), "radix", 16, 0.8, "dehnen",
+                              (8192, 256, 2048, 2048)),
+    "plummer_16k_radix": (lambda: inputs.plummer_sphere(1 << 14, 2), "radix", 32, 0.5, "dehnen",
+                          None),
+    "clustered_32k_octree": (lambda: inputs.clustered(1 << 15, 3), "octree", 64, 0.5, "dehnen",
+                             None),
+    "uniform4k_bh_radix": (lambda: inputs.uniform_cube(4096, 11), "radix", 8, 0.7, "bh", None),
+    "plummer4k_engblom_octree": (lambda: inputs.plummer_sphere(4096, 5, a=0.3), "octree", 8, 0.5,
+                                 "engblom", None),
+}
+
+
+@pytest.fixture(scope="module")
+def ygg():
+    import yggdrax_amd
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return yggdrax_amd
+
+
+def _run_both(ygg, name):
+    gen, kind, leaf, theta, mac, cfg = CASES[name]
+    print(f"[parity] case={name}", flush=True)
+    pos, mass = gen()
+    build = oracle.build_octree if kind == "octree" else oracle.build_tree
+    ref = build(pos, mass, leaf_size=leaf)
+    rgeom = oracle.compute_tree_geometry(ref, ref["positions_sorted"])
+    rint, rnb, _ = oracle.build_interactions_and_neighbors(
+        ref, rgeom, theta=theta, mac_type=mac, traversal_config=None if cfg is None else _OCfg(cfg))
+    gbuild = ygg.build_octree if kind == "octree" else ygg.build_tree
+    tree, pos_sorted, _, _ = gbuild(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
+                                    leaf_size=leaf, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, pos_sorted)
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=theta, mac_type=mac,
+                                                       traversal_config=_cfg(cfg))
+    torch.cuda.synchronize()
+    return (ref, rgeom, rint, rnb), (tree, pos_sorted, geom, inter, nbrs), kind
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_pipeline_parity(ygg, name):
+    (ref, rgeom, rint, rnb), (tree, pos_sorted, geom, inter, nbrs), kind = _run_both(ygg, name)
+    parity.assert_topology_equal(tree, ref, octree=(kind == "octree"))
+    assert np.array_equal(parity.np_(pos_sorted), ref["positions_sorted"])
+    parity.assert_geometry_close(geom, rgeom)
+    parity.assert_interactions_equal(inter, rint)
+    parity.assert_neighbors_equal(nbrs, rnb)
+
+
+def test_getting_started_known_answer(ygg):
+    """examples/getting_started.ipynb:33-52: 31 nodes, 16 leaves, 4 far, 236 near."""
+    pos, mass = inputs.uniform_cube(256, 7)
+    tree = ygg.build_tree(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(), leaf_size=16)
+    ps = torch.from_numpy(pos).cuda()[tree.particle_indices]
+    geom = ygg.compute_tree_geometry(tree, ps)
+    cfg = ygg.DualTreeTraversalConfig(8192, 256, 2048, 2048)
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=0.8, mac_type="dehnen",
+                                                       traversal_config=cfg)
+    assert tree.num_nodes == 31 and tree.num_leaves == 16
+    assert int(inter.counts.sum()) == 4
+    assert int(nbrs.counts.sum()) == 236
+
+
+@pytest.mark.parametrize("n", [1 << 20])
+def test_sort_large_uniform(ygg, n):
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    lo, hi = oracle.infer_bounds(pos)
+    codes = oracle.morton_encode(pos, (lo, hi))
+    perm = np.argsort(codes, kind="stable")
+    tree = ygg.build_tree(torch.from_numpy(pos).cuda(), None, leaf_size=32)
+    assert np.array_equal(parity.np_(tree.particle_indices), perm)
+    assert np.array_equal(parity.np_(tree.morton_codes), codes[perm])
+
+
+def test_identical_points_stable(ygg):
+    """tests/unit/test_tree.py:193-214: identical points keep input order."""
+    pos = np.full((64, 3), 0.25, np.float32)
+    tree = ygg.build_tree(torch.from_numpy(pos).cuda(), None, leaf_size=4)
+    assert np.array_equal(parity.np_(tree.particle_indices), np.arange(64))
+    ref = oracle.build_tree(pos, None, leaf_size=4)
+    parity.assert_topology_equal(tree, ref)
+
+
+@pytest.mark.parametrize("n,leaf", [(1, 8), (2, 1), (17, 4), (33, 32), (1000, 1)])
+def test_small_sizes(ygg, n, leaf):
+    rng = np.random.default_rng(n)
+    pos = rng.normal(size=(n, 3)).astype(np.float32)
+    ref = oracle.build_octree(pos, None, leaf_size=leaf)
+    tree, ps, _, _ = ygg.build_octree(torch.from_numpy(pos).cuda(), None, leaf_size=leaf,
+                                      return_reordered=True)
+    parity.assert_topology_equal(tree, ref, octree=True)
+    rgeom = oracle.compute_tree_geometry(ref, ref["positions_sorted"])
+    geom = ygg.compute_tree_geometry(tree, ps)
+    parity.assert_geometry_close(geom, rgeom)
+    rint, rnb, _ = oracle.build_interactions_and_neighbors(ref, rgeom, theta=0.5,
+                                                           mac_type="dehnen")
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=0.5, mac_type="dehnen")
+    parity.assert_interactions_equal(inter, rint)
+    parity.assert_neighbors_equal(nbrs, rnb)
+
+
+def test_overflow_errors_match(ygg):
+    """Tiny capacities raise the reference's errors (queue ladder and far cap)."""
+    pos, mass = inputs.uniform_cube(512, 0)
+    ref = oracle.build_tree(pos, mass, leaf_size=8)
+    rgeom = oracle.compute_tree_geometry(ref, ref["positions_sorted"])
+    tree, ps, _, _ = ygg.build_tree(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
+                                    leaf_size=8, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    with pytest.raises(RuntimeError, match="Interaction list capacity exceeded"):
+        oracle.build_interactions_and_neighbors(ref, rgeom, theta=0.6, mac_type="dehnen",
+                                                traversal_config=_OCfg((8192, 256, 1, 2048)))
+    with pytest.raises(RuntimeError, match="Interaction list capacity exceeded"):
+        ygg.build_interactions_and_neighbors(
+            tree, geom, theta=0.6, mac_type="dehnen",
+            traversal_config=ygg.DualTreeTraversalConfig(8192, 256, 1, 2048))
+    # tiny neighbour capacity: both grow it through the ladder and succeed
+    rint, rnb, _ = oracle.build_interactions_and_neighbors(
+        ref, rgeom, theta=0.6, mac_type="dehnen", traversal_config=_OCfg((8192, 256, 2048, 2)))
+    inter, nbrs = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.6, mac_type="dehnen",
+        traversal_config=ygg.DualTreeTraversalConfig(8192, 256, 2048, 2))
+    parity.assert_interactions_equal(inter, rint)
+    parity.assert_neighbors_equal(nbrs, rnb)
+    # explicit tiny queue without config: queue error
+    with pytest.raises(RuntimeError, match="Pair queue capacity exceeded"):
+        ygg.build_interactions_and_neighbors(tree, geom, theta=0.6, mac_type="dehnen",
+                                             max_pair_queue=4, max_interactions_per_node=64)

@@ -1,0 +1,166 @@
+// Shared device helpers for the yggdrax MI355X (gfx950) kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/yggdrax_amd.h"
+
+#define YGG_WAVE 64
+
+#define YGG_CHECK(expr)                                   \
+  do {                                                    \
+    hipError_t _e = (expr);                               \
+    if (_e != hipSuccess) return YGG_ERR_HIP;             \
+  } while (0)
+
+#define YGG_LAUNCH_CHECK() YGG_CHECK(hipGetLastError())
+
+namespace ygg {
+
+__device__ __forceinline__ int clz64(uint64_t x) { return x == 0 ? 64 : __clzll((long long)x); }
+
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {
+  x &= 0x1FFFFFull;
+  x = (x | (x << 32)) & 0x1F00000000FFFFull;
+  x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+  x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+  x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+// Common octree depth of two Morton codes (floor((leading common bits - 1)/3)).
+__device__ __forceinline__ int common_depth(uint64_t a, uint64_t b) {
+  uint64_t x = a ^ b;
+  int bits = (x == 0) ? 63 : max(clz64(x) - 1, 0);
+  return bits / 3;
+}
+
+__device__ __forceinline__ uint64_t prefix_code(uint64_t c, int d) {
+  if (d <= 0) return 0;
+  int s = max(0, 63 - 3 * d);
+  return (c >> s) << s;
+}
+
+__device__ __forceinline__ int octant_at_depth(uint64_t c, int d) {
+  if (d <= 0) return 0;
+  int s = max(0, 63 - 3 * d);
+  return (int)((c >> s) & 7ull);
+}
+
+// (8^d - 1)/7 + (code >> (63 - 3d)): sortable key of an octree cell.
+__device__ __forceinline__ uint64_t oct_address(uint64_t c, int d) {
+  uint64_t du = (uint64_t)max(0, d);
+  uint64_t shift = 63ull - 3ull * du;
+  uint64_t pre = c >> shift;
+  uint64_t p8 = 1ull << (3ull * du);
+  return (p8 - 1ull) / 7ull + pre;
+}
+
+// Fixed-round binary search returning `high` after bit_length(n) halvings
+// of [low, high), comparing q <= a[mid] (left) or q < a[mid] (right). On a
+// sorted array this is lower/upper bound; it is also well defined on the
+// unsorted resolved leaf-cell codes the octree ranges search.
+__device__ __forceinline__ int fixed_round_search(const uint64_t* a, int n, uint64_t q, bool left) {
+  int levels = 32 - __clz(n);
+  int low = 0, high = n;
+  for (int it = 0; it < levels; ++it) {
+    int mid = (int)(((unsigned)low + (unsigned)high) >> 1);
+    int m = mid < n ? mid : n - 1;
+    uint64_t v = a[m];
+    bool go_left = left ? (q <= v) : (q < v);
+    if (go_left)
+      high = mid;
+    else
+      low = mid;
+  }
+  return high;
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// Number of set bits of `mask` in lanes below the calling lane.
+__device__ __forceinline__ unsigned mbcnt64(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Wave-wide inclusive prefix sum of a 32-bit value (wave64).
+__device__ __forceinline__ unsigned wave_inclusive_sum(unsigned v) {
+  const int lane = (int)lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T atomic_load_agent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_store_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+inline int grid_for(int64_t n, int threads, int max_blocks = 1 << 20) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (int)b;
+}
+
+inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
+
+// Bump allocator over a caller-provided workspace.
+struct Arena {
+  char* base;
+  size_t cap;
+  size_t off;
+  __host__ Arena(void* p, size_t c) : base((char*)p), cap(c), off(0) {}
+  template <typename T>
+  __host__ T* take(size_t count) {
+    size_t bytes = align_up(count * sizeof(T));
+    if (base == nullptr) {  // sizing pass
+      off += bytes;
+      return nullptr;
+    }
+    if (off + bytes > cap) return nullptr;
+    T* r = (T*)(base + off);
+    off += bytes;
+    return r;
+  }
+};
+
+// ---- primitives implemented in ygg_sort.hip --------------------------------
+// Stable LSD radix sort of (key, u32 value) pairs on key bits [begin, end).
+// vals_in == nullptr means values are the input positions 0..n-1.
+// A precomputed per-pass digit histogram (passes x 256, u32) may be passed
+// through `hist` with hist_ready = true.
+size_t sort_workspace_bytes(int64_t n, int key_bytes, int passes);
+int sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                   uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                   size_t ws_bytes, hipStream_t st, const uint32_t* hist = nullptr);
+int sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                   uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                   size_t ws_bytes, hipStream_t st);
+
+// Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:
+// out[n] = total). `gather`, if non-null, reads counts[gather[i]].
+size_t scan_workspace_bytes(int64_t n);
+int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int64_t* out,
+                              int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
+
+int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+
+}  // namespace ygg

@@ -1,0 +1,384 @@
+// Stable LSD radix sort (8-bit digits) for gfx950 wave64, plus scans.
+//
+// Per digit pass one kernel ("one sweep"): each 256-thread workgroup takes
+// the next 4096-key tile from an atomic tile counter, ranks its keys stably
+// with wave64 ballot "match" masks and per-wave LDS digit counters, publishes
+// its digit counts for decoupled look-back, reorders the tile in LDS by digit
+// and writes every digit run out contiguously. A single histogram kernel
+// counts all passes' digits up front, so a pass reads and writes each
+// (key, value) pair exactly once.
+#include "ygg_common.h"
+
+namespace ygg {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kTile = kSortThreads * kSortItems;  // 4096 keys
+constexpr int kBins = 256;
+constexpr int kWaves = kSortThreads / YGG_WAVE;      // 4
+constexpr int kWaveKeys = kTile / kWaves;            // 1024 keys per wave
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagPre = 2u << 30;
+constexpr uint32_t kCountMask = (1u << 30) - 1u;
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, int64_t n,
+                                                    int begin_bit, int passes,
+                                                    uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[8 * kBins];
+  for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    K k = keys[i];
+    for (int p = 0; p < passes; ++p) {
+      unsigned d = (unsigned)((k >> (begin_bit + 8 * p)) & 0xFF);
+      atomicAdd(&h[p * kBins + d], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// In-place exclusive scan of each pass's 256-bin histogram (one block/pass).
+__global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
+  __shared__ uint32_t wsum[kWaves];
+  uint32_t* h = hist + blockIdx.x * kBins;
+  unsigned v = h[threadIdx.x];
+  unsigned inc = wave_inclusive_sum(v);
+  int w = threadIdx.x / YGG_WAVE;
+  if (lane_id() == 63) wsum[w] = inc;
+  __syncthreads();
+  unsigned base = 0;
+  for (int k = 0; k < w; ++k) base += wsum[k];
+  h[threadIdx.x] = base + inc - v;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
+                                                  const uint32_t* __restrict__ vals_in,
+                                                  K* __restrict__ keys_out,
+                                                  uint32_t* __restrict__ vals_out, int64_t n,
+                                                  int shift, const uint32_t* __restrict__ hist_scan,
+                                                  uint32_t* __restrict__ lookback,
+                                                  uint32_t* __restrict__ tile_counter) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wave_hist[kWaves][kBins];
+  __shared__ uint32_t s_digit_start[kBins];
+  __shared__ uint32_t s_global[kBins];
+  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ K s_keys[kTile];
+  __shared__ uint32_t s_vals[kTile];
+
+  const int tid = threadIdx.x;
+  const int w = tid / YGG_WAVE;
+  const unsigned lane = lane_id();
+  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (int i = tid; i < kWaves * kBins; i += kSortThreads) (&s_wave_hist[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * kTile;
+  const int64_t wbase = base + (int64_t)w * kWaveKeys;
+
+  K key[kSortItems];
+  uint32_t val[kSortItems];
+  uint32_t rank[kSortItems];
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    int64_t idx = wbase + k * YGG_WAVE + lane;
+    bool ok = idx < n;
+    key[k] = ok ? keys_in[idx] : (K)~(K)0;
+    val[k] = ok ? (vals_in ? vals_in[idx] : (uint32_t)idx) : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    unsigned d = (unsigned)((key[k] >> shift) & 0xFF);
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    unsigned below = mbcnt64(peers);
+    unsigned cnt = (unsigned)__popcll(peers);
+    unsigned before = s_wave_hist[w][d];
+    rank[k] = before + below;
+    if (below == 0) s_wave_hist[w][d] = before + cnt;
+  }
+  __syncthreads();
+  // Thread `tid` owns digit `tid`.
+  const unsigned d0 = (unsigned)tid;
+  unsigned c[kWaves];
+  unsigned total = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) {
+    c[q] = s_wave_hist[q][d0];
+    s_wave_hist[q][d0] = total;  // exclusive prefix over waves
+    total += c[q];
+  }
+  uint32_t* lb = lookback + tile * kBins;
+  atomic_store_agent(&lb[d0], (tile == 0 ? kFlagPre : kFlagAgg) | total);
+  unsigned inc = wave_inclusive_sum(total);
+  if (lane == 63) s_wsum[w] = inc;
+  unsigned excl_lb = 0;
+  if (tile > 0) {
+    int64_t pt = tile - 1;
+    while (true) {
+      uint32_t v = atomic_load_agent(&lookback[pt * kBins + d0]);
+      uint32_t f = v & ~kCountMask;
+      if (f == 0) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl_lb += v & kCountMask;
+      if (f == kFlagPre) break;
+      --pt;
+    }
+    atomic_store_agent(&lb[d0], kFlagPre | (excl_lb + total));
+  }
+  __syncthreads();
+  unsigned wb = 0;
+  for (int q = 0; q < w; ++q) wb += s_wsum[q];
+  const unsigned dstart = wb + inc - total;
+  s_digit_start[d0] = dstart;
+  s_global[d0] = hist_scan[d0] + excl_lb - dstart;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    unsigned d = (unsigned)((key[k] >> shift) & 0xFF);
+    unsigned pos = s_digit_start[d] + s_wave_hist[w][d] + rank[k];
+    s_keys[pos] = key[k];
+    s_vals[pos] = val[k];
+  }
+  __syncthreads();
+  const int64_t valid = (n - base) < kTile ? (n - base) : kTile;
+  for (int p = tid; p < valid; p += kSortThreads) {
+    K kk = s_keys[p];
+    unsigned d = (unsigned)((kk >> shift) & 0xFF);
+    int64_t g = (int64_t)s_global[d] + p;
+    keys_out[g] = kk;
+    vals_out[g] = s_vals[p];
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
+  if (n <= 0) return YGG_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, v, n);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+static inline int64_t num_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
+
+size_t sort_workspace_bytes(int64_t n, int key_bytes, int passes) {
+  Arena a(nullptr, 0);
+  int64_t t = num_tiles(n);
+  a.take<char>((size_t)n * key_bytes);          // alternate keys
+  a.take<uint32_t>((size_t)n);                  // alternate values
+  a.take<uint32_t>((size_t)passes * kBins);     // histograms
+  a.take<uint32_t>((size_t)passes + (size_t)passes * t * kBins);  // counters + look-back
+  return a.off;
+}
+
+template <typename K>
+static int sort_pairs_impl(const K* keys_in, const uint32_t* vals_in, K* keys_out,
+                           uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                           size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre) {
+  if (n <= 0) return YGG_OK;
+  if (n > (int64_t)kCountMask) return YGG_ERR_ARG;
+  if (end_bit <= begin_bit || keys_in == keys_out) return YGG_ERR_ARG;
+  const int passes = (end_bit - begin_bit + 7) / 8;
+  if (passes > 8) return YGG_ERR_ARG;
+  const int64_t tiles = num_tiles(n);
+  Arena a(ws, ws_bytes);
+  K* kalt = (K*)a.take<char>((size_t)n * sizeof(K));
+  uint32_t* valt = a.take<uint32_t>((size_t)n);
+  uint32_t* hist = a.take<uint32_t>((size_t)passes * kBins);
+  uint32_t* ctr = a.take<uint32_t>((size_t)passes + (size_t)passes * tiles * kBins);
+  if (!kalt || !valt || !hist || !ctr) return YGG_ERR_WORKSPACE;
+  uint32_t* lookback = ctr + passes;
+  int rc = fill_u32(ctr, 0u, (int64_t)passes + (int64_t)passes * tiles * kBins, st);
+  if (rc) return rc;
+  if (hist_pre) {
+    YGG_CHECK(hipMemcpyAsync(hist, hist_pre, sizeof(uint32_t) * passes * kBins,
+                             hipMemcpyDeviceToDevice, st));
+  } else {
+    rc = fill_u32(hist, 0u, (int64_t)passes * kBins, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_digit_hist<K>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, keys_in,
+                       n, begin_bit, passes, hist);
+    YGG_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(256), 0, st, hist);
+  YGG_LAUNCH_CHECK();
+  const K* src_k = keys_in;
+  const uint32_t* src_v = vals_in;
+  for (int p = 0; p < passes; ++p) {
+    bool to_out = ((passes - 1 - p) % 2) == 0;
+    K* dk = to_out ? keys_out : kalt;
+    uint32_t* dv = to_out ? vals_out : valt;
+    hipLaunchKernelGGL(k_onesweep<K>, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, src_k,
+                       src_v, dk, dv, n, begin_bit + 8 * p, hist + p * kBins,
+                       lookback + (int64_t)p * tiles * kBins, ctr + p);
+    YGG_LAUNCH_CHECK();
+    src_k = dk;
+    src_v = dv;
+  }
+  return YGG_OK;
+}
+
+int sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                   uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                   size_t ws_bytes, hipStream_t st, const uint32_t* hist) {
+  return sort_pairs_impl<uint64_t>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
+                                   ws, ws_bytes, st, hist);
+}
+
+int sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                   uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                   size_t ws_bytes, hipStream_t st) {
+  return sort_pairs_impl<uint32_t>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
+                                   ws, ws_bytes, st, nullptr);
+}
+
+// ---- exclusive scan: int32 counts -> int64 offsets --------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int64_t wave_inclusive_sum64(int64_t v) {
+  const int lane = (int)lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int64_t o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ int64_t block_exclusive_scan64(int64_t v, int64_t* out_excl) {
+  __shared__ int64_t wsum[kScanThreads / YGG_WAVE];
+  int64_t inc = wave_inclusive_sum64(v);
+  int w = threadIdx.x / YGG_WAVE;
+  if (lane_id() == 63) wsum[w] = inc;
+  __syncthreads();
+  int64_t base = 0, total = 0;
+  for (int q = 0; q < kScanThreads / YGG_WAVE; ++q) {
+    if (q < w) base += wsum[q];
+    total += wsum[q];
+  }
+  *out_excl = base + inc - v;
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int32_t* __restrict__ c,
+                                                     const int32_t* __restrict__ gather,
+                                                     int64_t n, int64_t* __restrict__ partial) {
+  int64_t b = (int64_t)blockIdx.x * kScanTile;
+  int64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = b + (int64_t)k * kScanThreads + threadIdx.x;
+    if (i < n) s += c[gather ? gather[i] : i];
+  }
+  int64_t ex;
+  int64_t tot = block_exclusive_scan64(s, &ex);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_partials(int64_t* partial, int64_t nb) {
+  // single block: sequential chunks of 256
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+    int64_t i = b0 + threadIdx.x;
+    int64_t v = i < nb ? partial[i] : 0;
+    int64_t ex;
+    int64_t tot = block_exclusive_scan64(v, &ex);
+    if (i < nb) partial[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[nb] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const int32_t* __restrict__ c,
+                                                    const int32_t* __restrict__ gather,
+                                                    int64_t n, const int64_t* __restrict__ partial,
+                                                    int64_t* __restrict__ out) {
+  int64_t b = (int64_t)blockIdx.x * kScanTile;
+  // blocked: thread t owns items [b + t*items, ...)
+  int64_t vals[kScanItems];
+  int64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = b + (int64_t)threadIdx.x * kScanItems + k;
+    vals[k] = i < n ? (int64_t)c[gather ? gather[i] : i] : 0;
+    s += vals[k];
+  }
+  int64_t ex;
+  block_exclusive_scan64(s, &ex);
+  int64_t run = partial[blockIdx.x] + ex;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = b + (int64_t)threadIdx.x * kScanItems + k;
+    if (i < n) out[i] = run;
+    run += vals[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = partial[gridDim.x];
+}
+
+size_t scan_workspace_bytes(int64_t n) {
+  return align_up(sizeof(int64_t) * (size_t)((n + kScanTile - 1) / kScanTile + 2));
+}
+
+int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int64_t* out,
+                              int64_t n, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (n <= 0) return fill_u32((uint32_t*)out, 0u, 2, st);
+  int64_t nb = (n + kScanTile - 1) / kScanTile;
+  if (ws_bytes < scan_workspace_bytes(n)) return YGG_ERR_WORKSPACE;
+  int64_t* partial = (int64_t*)ws;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, st, counts, gather,
+                     n, partial);
+  YGG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, st, partial, nb);
+  YGG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, st, counts, gather,
+                     n, partial, out);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+}  // namespace ygg
+
+extern "C" {
+
+size_t ygg_sort_workspace_bytes(int64_t n, int key_bytes, int begin_bit, int end_bit) {
+  int passes = (end_bit - begin_bit + 7) / 8;
+  return ygg::sort_workspace_bytes(n, key_bytes, passes);
+}
+
+int ygg_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                       size_t ws_bytes, void* stream) {
+  return ygg::sort_pairs_u64(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit, ws,
+                             ws_bytes, (hipStream_t)stream);
+}
+
+int ygg_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                       size_t ws_bytes, void* stream) {
+  return ygg::sort_pairs_u32(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit, ws,
+                             ws_bytes, (hipStream_t)stream);
+}
+
+}  // extern "C"

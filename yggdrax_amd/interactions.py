@@ -1,0 +1,679 @@
+"""Dual-tree interaction and neighbour lists (yggdrax/interactions.py:338-448,
+yggdrax/_interactions_impl.py).
+
+The walk runs on the GPU (ygg_walk_bfs); this module restates the
+reference's host-side control: capacity ladders and retries
+(_run_dual_tree_walk_raw, 3840-4440), the queue-capacity LRU cache
+(245-293), the count-pass sizing path (3989-4299) and the output shaping
+(_result_to_interactions / _result_to_neighbors, 3561-3837).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import logging
+import os
+from collections import OrderedDict
+from dataclasses import dataclass
+from typing import Callable, List, NamedTuple, Optional
+
+import torch
+
+from . import _lib
+from .dtypes import INDEX_DTYPE
+from .geometry import TreeGeometry
+from .tree import DeviceTopology, device_topology, resolve_tree_topology
+
+logger = logging.getLogger(__name__)
+
+_DEFAULT_MAX_NEIGHBORS = 2048
+DEFAULT_PAIR_QUEUE_MULTIPLIER = 128
+_COUNT_PASS_FILL_INTERACTION_CAP = 4096
+_AUTO_INTERACTION_MAX = 16384
+MAX_GENERATIONS = 1024
+# Opt-in per-generation kernel timing (used by the benchmark's roofline leg).
+WALK_TIMING = {"enabled": False, "log": []}
+MAC_TYPES = {"bh": 0, "dehnen": 1, "engblom": 2}
+NODE_RECORD_BYTES = 32
+
+
+@dataclass(frozen=True)
+class DualTreeTraversalConfig:
+    """Fixed traversal parameters (yggdrax/_interactions_impl.py:235-242)."""
+
+    max_pair_queue: int
+    process_block: int
+    max_interactions_per_node: int
+    max_neighbors_per_leaf: int = _DEFAULT_MAX_NEIGHBORS
+
+
+_GLOBAL_DUAL_TREE_CONFIG: Optional[DualTreeTraversalConfig] = None
+_MAX_DUAL_TREE_QUEUE_CACHE_ENTRIES = 128
+_DUAL_TREE_QUEUE_CACHE: "OrderedDict[tuple, int]" = OrderedDict()
+
+
+def set_default_dual_tree_config(config: Optional[DualTreeTraversalConfig]) -> None:
+    if config is not None:
+        for name in ("max_pair_queue", "process_block", "max_interactions_per_node",
+                     "max_neighbors_per_leaf"):
+            if getattr(config, name) < 1:
+                raise ValueError(f"{name} must be >= 1")
+    global _GLOBAL_DUAL_TREE_CONFIG
+    _GLOBAL_DUAL_TREE_CONFIG = config
+
+
+def _get_cached_queue_capacity(key):
+    v = _DUAL_TREE_QUEUE_CACHE.get(key)
+    if v is None:
+        return None
+    _DUAL_TREE_QUEUE_CACHE.move_to_end(key)
+    return int(v)
+
+
+def _store_cached_queue_capacity(key, cap: int) -> None:
+    _DUAL_TREE_QUEUE_CACHE[key] = int(cap)
+    _DUAL_TREE_QUEUE_CACHE.move_to_end(key)
+    while len(_DUAL_TREE_QUEUE_CACHE) > _MAX_DUAL_TREE_QUEUE_CACHE_ENTRIES:
+        _DUAL_TREE_QUEUE_CACHE.popitem(last=False)
+
+
+class DualTreeRetryEvent(NamedTuple):
+    attempt: int
+    queue_capacity: int
+    interaction_capacity: int
+    status: str
+    far_pair_count: int
+    near_pair_count: int
+
+
+class NodeInteractionList(NamedTuple):
+    offsets: torch.Tensor
+    sources: torch.Tensor
+    targets: torch.Tensor
+    counts: torch.Tensor
+    level_offsets: torch.Tensor
+    target_levels: torch.Tensor
+
+
+class NodeNeighborList(NamedTuple):
+    offsets: torch.Tensor
+    neighbors: torch.Tensor
+    leaf_indices: torch.Tensor
+    counts: torch.Tensor
+    particle_order_leaf_indices: torch.Tensor
+    particle_order_to_native_leaf: torch.Tensor
+    neighbor_leaf_positions: torch.Tensor
+    target_block_leaf_ids: torch.Tensor
+    target_block_source_leaf_ids: torch.Tensor
+    target_block_valid_mask: torch.Tensor
+    target_block_offsets: torch.Tensor
+    target_block_size: int
+
+
+class DualTreeWalkResult(NamedTuple):
+    interaction_offsets: torch.Tensor
+    interaction_sources: torch.Tensor
+    interaction_targets: torch.Tensor
+    interaction_tags: torch.Tensor
+    interaction_counts: torch.Tensor
+    neighbor_offsets: torch.Tensor
+    neighbor_indices: torch.Tensor
+    neighbor_counts: torch.Tensor
+    leaf_indices: torch.Tensor
+    far_pair_count: int
+    near_pair_count: int
+    queue_overflow: bool
+    far_overflow: bool
+    near_overflow: bool
+    accept_decisions: int
+    near_decisions: int
+    refine_decisions: int
+
+
+# ---- capacity ladders (_interactions_impl.py:296-316, 451-563) -------------
+
+def _queue_candidates_bounded(*, max_capacity: int, process_block: int) -> list[int]:
+    qmax = max(4, int(max_capacity))
+    base = min(max(1024, int(process_block) * 16), qmax)
+    c = [base]
+    while c[-1] < qmax:
+        nxt = min(qmax, c[-1] * 2)
+        if nxt == c[-1]:
+            break
+        c.append(nxt)
+    return c
+
+
+def _auto_pair_queue_candidates(total_nodes: int, num_internal: int) -> List[int]:
+    conservative = DEFAULT_PAIR_QUEUE_MULTIPLIER * max(4, total_nodes)
+    start = min(max(4096, num_internal * 2), conservative)
+    if start >= conservative:
+        return [int(conservative)]
+    c: List[int] = []
+    cap = start
+    while cap < conservative:
+        c.append(int(cap))
+        cap = max(cap * 2, cap + 1024)
+    c.append(int(conservative))
+    return list(dict.fromkeys(c))
+
+
+def _interaction_capacity_candidates(requested, total_nodes):
+    if requested is not None:
+        if requested < 1:
+            raise ValueError("max_interactions_per_node must be >= 1")
+        return [int(requested)], False
+    scaled = min(max(total_nodes // 4, 2048), _AUTO_INTERACTION_MAX)
+    p2 = 1 << (max(1, scaled) - 1).bit_length()
+    return [int(min(_AUTO_INTERACTION_MAX, max(2048, p2)))], False
+
+
+def _neighbor_capacity_candidates(requested: int, *, num_leaves: int) -> list[int]:
+    base = max(1, int(requested))
+    hard = max(1, ((1 << 31) - 1) // max(1, int(num_leaves)))
+    base = min(base, hard)
+    c = [int(base)]
+    while c[-1] < hard:
+        nxt = min(hard, max(c[-1] * 2, c[-1] + 1024))
+        if nxt == c[-1]:
+            break
+        c.append(int(nxt))
+    return list(dict.fromkeys(c))
+
+
+QUEUE_ERR_USER = "Pair queue capacity exceeded; increase max_pair_queue and rebuild."
+QUEUE_ERR_AUTO = ("Pair queue capacity exceeded; pass a larger max_pair_queue value to "
+                  "override auto-scaling.")
+FAR_ERR = ("Interaction list capacity exceeded; increase max_interactions_per_node and "
+           "rebuild.")
+NEAR_ERR = "Neighbor list capacity exceeded; increase max_neighbors_per_leaf and rebuild."
+
+
+# ---- one GPU walk attempt ---------------------------------------------------
+
+@dataclass
+class WalkOutcome:
+    """Device results of one walk attempt plus the host-visible status."""
+
+    far_counts: torch.Tensor        # int32 [T] (raw, may exceed capacity on overflow)
+    near_counts: torch.Tensor       # int32 [L]
+    far_entries: Optional[torch.Tensor]
+    near_entries: Optional[torch.Tensor]
+    far_layout: tuple               # (fixed_cap, row_start or None)
+    near_layout: tuple
+    gen_sizes: list                 # wavefront size per generation (clipped to queue cap)
+    queue_overflow: bool
+    far_overflow: bool
+    near_overflow: bool
+    decisions: tuple                # accept, near, refine
+    queue_capacity: int
+
+    @property
+    def max_wavefront(self) -> int:
+        return max([1] + list(self.gen_sizes))
+
+
+class _WalkContext:
+    """Per (tree, geometry, MAC) state shared by all attempts of one call."""
+
+    def __init__(self, dt: DeviceTopology, geometry: TreeGeometry, theta: float, mac_type: str,
+                 dehnen_radius_scale: float):
+        self.lib = _lib.load()
+        self.dt = dt
+        self.dev = geometry.center.device
+        self.stream = _lib.stream_ptr(self.dev)
+        if mac_type not in MAC_TYPES:
+            raise ValueError(f"Unknown mac_type: {mac_type}")
+        self.mac = MAC_TYPES[mac_type]
+        th = ctypes.c_float(float(theta)).value
+        th_sq = ctypes.c_float(ctypes.c_float(th * th).value).value
+        self.theta_sq = th_sq
+        self.theta_root = float(torch.sqrt(torch.tensor(th_sq, dtype=torch.float32)))
+        self.nodes = torch.empty(dt.num_nodes * NODE_RECORD_BYTES, dtype=torch.uint8,
+                                 device=self.dev)
+        g = geometry
+        left = dt.left if dt.left.numel() else None
+        right = dt.right if dt.right.numel() else None
+        _lib.check(self.lib.ygg_walk_prepare_nodes(
+            g.center.contiguous().data_ptr(), g.radius.contiguous().data_ptr(),
+            g.max_extent.contiguous().data_ptr(), _lib.ptr(left), _lib.ptr(right),
+            dt.leaf_position.data_ptr(), dt.num_nodes, self.mac,
+            ctypes.c_float(float(dehnen_radius_scale)).value, self.nodes.data_ptr(),
+            self.stream), "walk_prepare_nodes")
+        self.host_status = torch.zeros(MAX_GENERATIONS + 8, dtype=torch.int32, pin_memory=True)
+        self.gen_sizes = torch.empty(MAX_GENERATIONS + 1, dtype=torch.int32, device=self.dev)
+        self.flags = torch.empty(4, dtype=torch.int32, device=self.dev)
+        self.decisions = torch.empty(3, dtype=torch.int64, device=self.dev)
+        self.timing = WALK_TIMING["enabled"]
+        self.kernel_log = WALK_TIMING["log"]
+
+    def run(self, *, mode: int, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
+            far_row_start=None, far_row_cap=None, near_row_start=None, near_row_cap=None,
+            far_capacity_total: int = 0, near_capacity_total: int = 0,
+            collect_far: bool = True, collect_near: bool = True) -> WalkOutcome:
+        dt = self.dt
+        dev = self.dev
+        Q = max(4, int(queue_cap))
+        wf = torch.empty((2, Q), dtype=torch.int64, device=dev)
+        far_counts = torch.empty(dt.num_nodes, dtype=torch.int32, device=dev)
+        near_counts = torch.empty(dt.num_leaves, dtype=torch.int32, device=dev)
+        far_entries = near_entries = None
+        if mode == 1:
+            far_entries = torch.empty(max(1, dt.num_nodes * far_cap), dtype=torch.int32, device=dev)
+            near_entries = torch.empty(max(1, dt.num_leaves * near_cap), dtype=torch.int32,
+                                       device=dev)
+        elif mode == 2:
+            far_entries = torch.empty(max(1, far_capacity_total), dtype=torch.int32, device=dev)
+            near_entries = torch.empty(max(1, near_capacity_total), dtype=torch.int32, device=dev)
+        p = _lib.WalkParams()
+        p.num_nodes, p.num_internal, p.num_leaves = dt.num_nodes, dt.num_internal, dt.num_leaves
+        p.root = 0
+        p.mac_type, p.mode = self.mac, mode
+        p.collect_far, p.collect_near = int(collect_far), int(collect_near)
+        p.theta_sq, p.theta_root = self.theta_sq, self.theta_root
+        p.far_cap, p.near_cap = int(far_cap), int(near_cap)
+        p.queue_cap = Q
+        p.max_generations = MAX_GENERATIONS
+        p.nodes = self.nodes.data_ptr()
+        p.wf0, p.wf1 = wf[0].data_ptr(), wf[1].data_ptr()
+        p.gen_sizes, p.flags = self.gen_sizes.data_ptr(), self.flags.data_ptr()
+        p.decisions = self.decisions.data_ptr()
+        p.far_counts, p.near_counts = far_counts.data_ptr(), near_counts.data_ptr()
+        p.far_row_start, p.far_row_cap = _lib.ptr(far_row_start), _lib.ptr(far_row_cap)
+        p.near_row_start, p.near_row_cap = _lib.ptr(near_row_start), _lib.ptr(near_row_cap)
+        p.far_entries, p.near_entries = _lib.ptr(far_entries), _lib.ptr(near_entries)
+        ngen = ctypes.c_int32(0)
+        gen_ms = (ctypes.c_float * MAX_GENERATIONS)() if self.timing else None
+        _lib.check(self.lib.ygg_walk_bfs(ctypes.byref(p), self.host_status.data_ptr(),
+                                         ctypes.byref(ngen), gen_ms, self.stream), "walk")
+        if gen_ms is not None:
+            self.kernel_log.append(dict(mode=mode, gen_ms=list(gen_ms)[: ngen.value],
+                                        decisions=tuple(int(v) for v in self.decisions.cpu()),
+                                        gen_sizes=[min(int(self.host_status[k]), Q)
+                                                   for k in range(ngen.value + 1)]))
+        hs = self.host_status
+        gens = [min(int(hs[k]) & 0xFFFFFFFF, Q) for k in range(ngen.value + 1)]
+        G = MAX_GENERATIONS
+        fl = [int(hs[G + 1 + k]) for k in range(3)]
+        dec = tuple(int(v) for v in self.decisions.cpu())
+        layout_far = (far_cap if mode == 1 else 0, far_row_start)
+        layout_near = (near_cap if mode == 1 else 0, near_row_start)
+        return WalkOutcome(far_counts, near_counts, far_entries, near_entries, layout_far,
+                           layout_near, gens, bool(fl[0]), bool(fl[1]), bool(fl[2]), dec, Q)
+
+
+def _scan(counts: torch.Tensor, gather: Optional[torch.Tensor] = None) -> torch.Tensor:
+    lib = _lib.load()
+    n = counts.numel() if gather is None else gather.numel()
+    out = torch.empty(n + 1, dtype=torch.int64, device=counts.device)
+    ws = _lib.workspace(lib.ygg_scan_workspace_bytes(n), counts.device)
+    _lib.check(lib.ygg_exclusive_scan_i32(counts.data_ptr(), _lib.ptr(gather), out.data_ptr(), n,
+                                          ws.data_ptr(), ws.numel(),
+                                          _lib.stream_ptr(counts.device)), "scan")
+    return out
+
+
+def _capped(counts: torch.Tensor, fixed_cap: int, caps: Optional[torch.Tensor]) -> torch.Tensor:
+    c = counts.clone()
+    if fixed_cap > 0 or caps is not None:
+        _lib.check(_lib.load().ygg_cap_counts(c.data_ptr(), c.numel(), int(fixed_cap),
+                                              _lib.ptr(caps), _lib.stream_ptr(c.device)), "cap")
+    return c
+
+
+class _RawWalk(NamedTuple):
+    outcome: WalkOutcome
+    far_counts: torch.Tensor   # int32, capped
+    near_counts: torch.Tensor  # int32, capped
+    far_caps: Optional[torch.Tensor]
+    near_caps: Optional[torch.Tensor]
+
+
+def _emit(retry_logger, status, attempt, q, k, res: _RawWalk):
+    if retry_logger is None:
+        return
+    ev = DualTreeRetryEvent(attempt=int(attempt), queue_capacity=int(q),
+                            interaction_capacity=int(k), status=status,
+                            far_pair_count=int(res.far_counts.sum()),
+                            near_pair_count=int(res.near_counts.sum()))
+    try:
+        retry_logger(ev)
+    except Exception:  # pragma: no cover - defensive, as the reference
+        logger.exception("retry_logger raised", exc_info=True)
+
+
+def _run_walk_raw(tree, geometry: TreeGeometry, theta: float, mac_type: str, *,
+                  max_interactions_per_node, max_neighbors_per_leaf, max_pair_queue,
+                  traversal_config, dehnen_radius_scale, process_block, retry_logger,
+                  collect_far=True, collect_near=True) -> Optional[_RawWalk]:
+    """_run_dual_tree_walk_raw (3840-4440) over the GPU walk."""
+    dt = device_topology(tree)
+    if float(dehnen_radius_scale) <= 0.0:
+        raise ValueError("dehnen_radius_scale must be > 0")
+    T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
+    if NI == 0:
+        return None
+    config = traversal_config if traversal_config is not None else _GLOBAL_DUAL_TREE_CONFIG
+    ctx = _WalkContext(dt, geometry, theta, mac_type, dehnen_radius_scale)
+    if config is not None:
+        qmax = max(4, int(config.max_pair_queue))
+        inter = [int(config.max_interactions_per_node)]
+        nlimit = int(config.max_neighbors_per_leaf)
+        nbr = _neighbor_capacity_candidates(nlimit, num_leaves=L)
+        pov = int(config.process_block)
+        user_q = True
+        cache_key = (T, NI, inter[0], nlimit, qmax, pov, bool(collect_far), bool(collect_near),
+                     str(mac_type))
+        cached = _get_cached_queue_capacity(cache_key)
+        qc = [cached] if cached is not None else _queue_candidates_bounded(
+            max_capacity=qmax, process_block=pov)
+        for cap in _auto_pair_queue_candidates(T, NI):
+            if cap > qc[-1]:
+                qc.append(cap)
+        qc = list(dict.fromkeys(qc))
+    else:
+        cache_key = None
+        user_q = max_pair_queue is not None
+        qc = [max(4, int(max_pair_queue))] if user_q else _auto_pair_queue_candidates(T, NI)
+        inter, _ = _interaction_capacity_candidates(max_interactions_per_node, T)
+        nbr = _neighbor_capacity_candidates(max_neighbors_per_leaf, num_leaves=L)
+    use_count = (max_interactions_per_node is None and max_pair_queue is None
+                 and traversal_config is None)
+    qerr = QUEUE_ERR_USER if user_q else QUEUE_ERR_AUTO
+
+    if use_count:
+        return _count_then_fill(ctx, dt, retry_logger, collect_far, collect_near)
+
+    result: Optional[_RawWalk] = None
+    success = False
+    attempt = 0
+    for Q in qc:
+        queue_retry = False
+        for K in inter:
+            for Kn in nbr:
+                attempt += 1
+                o = ctx.run(mode=1, queue_cap=Q, far_cap=K, near_cap=Kn,
+                            collect_far=collect_far, collect_near=collect_near)
+                result = _RawWalk(o, _capped(o.far_counts, K, None),
+                                  _capped(o.near_counts, Kn, None), None, None)
+                if o.queue_overflow:
+                    _emit(retry_logger, "queue_overflow", attempt, Q, K, result)
+                    queue_retry = True
+                    break
+                if o.far_overflow:
+                    raise RuntimeError(FAR_ERR)
+                if collect_near and o.near_overflow:
+                    _emit(retry_logger, "neighbor_overflow", attempt, Q, K, result)
+                    continue
+                success = True
+                if attempt > 1:
+                    _emit(retry_logger, "success", attempt, Q, K, result)
+                break
+            if queue_retry or success:
+                break
+        else:
+            continue
+        if queue_retry and not success:
+            continue
+        if success:
+            if cache_key is not None:
+                _store_cached_queue_capacity(cache_key, Q)
+            break
+    else:
+        if result is not None and result.outcome.queue_overflow:
+            raise RuntimeError(qerr)
+    assert result is not None
+    if collect_far and result.outcome.far_overflow:
+        raise RuntimeError(FAR_ERR)
+    if collect_near and result.outcome.near_overflow:
+        raise RuntimeError(NEAR_ERR)
+    return result
+
+
+def _count_then_fill(ctx: _WalkContext, dt: DeviceTopology, retry_logger, collect_far,
+                     collect_near) -> _RawWalk:
+    """Count pass, then an exact-length fill (3989-4299)."""
+    T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
+    qcount = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
+    cnt = ctx.run(mode=0, queue_cap=qcount, collect_far=collect_far, collect_near=collect_near)
+    if cnt.queue_overflow:
+        raise RuntimeError("Count-pass traversal wavefront overflowed its conservative buffer.")
+    return fill_from_counts(ctx, dt, cnt.far_counts, cnt.near_counts, cnt.max_wavefront,
+                            retry_logger=retry_logger, collect_far=collect_far,
+                            collect_near=collect_near)
+
+
+def fill_from_counts(ctx: _WalkContext, dt: DeviceTopology, far_caps: torch.Tensor,
+                     near_caps: torch.Tensor, max_wavefront: int, *, retry_logger=None,
+                     collect_far=True, collect_near=True, queue_candidates=None) -> _RawWalk:
+    """Exact-length fill at level-major far offsets and native near offsets,
+    sized by per-row capacities (exact counts or predicted capacities)."""
+    T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
+    far_lm = _scan(far_caps, dt.nodes_by_level)
+    near_off = _scan(near_caps)
+    totals = torch.stack([far_lm[-1], near_off[-1]]).cpu()
+    far_total, near_total = int(totals[0]), int(totals[1])
+    max_near = int(near_caps.max()) if L else 0
+    if max_near > max(1, ((1 << 31) - 1) // max(1, L)):
+        raise RuntimeError(
+            "Suggested per-leaf neighbor capacity is too large to safely allocate on the host "
+            "(would overflow int32). Enable local refinement or pass explicit capacities "
+            "(max_neighbors_per_leaf / max_interactions_per_node).")
+    if far_total > (1 << 31) - 1 or near_total > (1 << 31) - 1:
+        raise RuntimeError("Dual-tree traversal pair totals exceed signed int32 capacity; "
+                           "refine the tree or use a lower-memory traversal strategy.")
+    far_start = torch.empty(T, dtype=torch.int64, device=ctx.dev)
+    _lib.check(ctx.lib.ygg_rows_from_offsets(dt.nodes_by_level.data_ptr(), far_lm.data_ptr(), T,
+                                             far_start.data_ptr(), ctx.stream), "rows")
+    if queue_candidates is None:
+        q_suggest = max(4, max_wavefront + max_wavefront // 2)
+        auto = _auto_pair_queue_candidates(T, NI)
+        queue_candidates = [c for c in auto if c >= q_suggest]
+        if not queue_candidates or queue_candidates[0] != q_suggest:
+            queue_candidates = [q_suggest, *queue_candidates]
+        queue_candidates = list(dict.fromkeys(queue_candidates))
+    result = None
+    for attempt, Q in enumerate(queue_candidates, start=1):
+        o = ctx.run(mode=2, queue_cap=Q, far_row_start=far_start, far_row_cap=far_caps,
+                    near_row_start=near_off, near_row_cap=near_caps,
+                    far_capacity_total=far_total, near_capacity_total=near_total,
+                    collect_far=collect_far, collect_near=collect_near)
+        result = _RawWalk(o, _capped(o.far_counts, 0, far_caps), _capped(o.near_counts, 0, near_caps),
+                          far_caps, near_caps)
+        if o.queue_overflow:
+            _emit(retry_logger, "queue_overflow", attempt, Q, int(far_caps.max()), result)
+            continue
+        if attempt > 1:
+            _emit(retry_logger, "success", attempt, Q, int(far_caps.max()), result)
+        break
+    assert result is not None
+    if result.outcome.queue_overflow:
+        raise RuntimeError(QUEUE_ERR_USER)
+    if collect_far and result.outcome.far_overflow:
+        raise RuntimeError(FAR_ERR)
+    if collect_near and result.outcome.near_overflow:
+        raise RuntimeError(NEAR_ERR)
+    return result
+
+
+def _empty_outputs(dt: DeviceTopology, topo):
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    T, L = dt.num_nodes, dt.num_leaves
+    nl = int(topo.num_levels)
+    inter = NodeInteractionList(
+        offsets=torch.zeros(T + 1, dtype=idx, device=dev), sources=torch.zeros(0, dtype=idx, device=dev),
+        targets=torch.zeros(0, dtype=idx, device=dev), counts=torch.zeros(T, dtype=idx, device=dev),
+        level_offsets=torch.zeros(nl + 1, dtype=idx, device=dev),
+        target_levels=torch.zeros(0, dtype=idx, device=dev))
+    return inter
+
+
+def _to_interactions(raw: _RawWalk, dt: DeviceTopology, topo) -> NodeInteractionList:
+    """_result_to_interactions (3561-3637)."""
+    lib = _lib.load()
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    T = dt.num_nodes
+    counts = raw.far_counts
+    lm = _scan(counts, dt.nodes_by_level)
+    total = int(lm[-1])
+    nl = int(topo.num_levels)
+    counts_out = counts.to(idx)
+    if total == 0:
+        e = _empty_outputs(dt, topo)
+        return e._replace(counts=counts_out)
+    sources = torch.empty(total, dtype=idx, device=dev)
+    targets = torch.empty(total, dtype=idx, device=dev)
+    levels = torch.empty(total, dtype=idx, device=dev)
+    offsets = torch.empty(T + 1, dtype=idx, device=dev)
+    fixed_cap, row_start = raw.outcome.far_layout
+    _lib.check(lib.ygg_far_output(T, dt.nodes_by_level.data_ptr(), lm.data_ptr(), counts.data_ptr(),
+                                  int(fixed_cap), _lib.ptr(row_start),
+                                  raw.outcome.far_entries.data_ptr(), dt.node_level.data_ptr(),
+                                  int(idx == torch.int64), sources.data_ptr(), targets.data_ptr(),
+                                  levels.data_ptr(), offsets.data_ptr(), _lib.stream_ptr(dev)),
+               "far_output")
+    level_idx = dt.level_offsets[: nl + 1].long()
+    level_offsets = lm[level_idx].to(idx)
+    return NodeInteractionList(offsets, sources, targets, counts_out, level_offsets, levels)
+
+
+def _leaf_tables(dt: DeviceTopology):
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    L, NI = dt.num_leaves, dt.num_internal
+    po = torch.arange(NI, NI + L, dtype=idx, device=dev)
+    p2n = torch.empty(L, dtype=idx, device=dev)
+    native = dt.leaf_indices.long() - NI
+    p2n[native] = torch.arange(L, dtype=idx, device=dev)
+    return dt.leaf_indices.to(idx), po, p2n
+
+
+def _target_block_size() -> int:
+    try:
+        return max(0, int(os.environ.get("YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE", "0")))
+    except Exception:
+        return 0
+
+
+def _to_neighbors(raw: Optional[_RawWalk], dt: DeviceTopology) -> NodeNeighborList:
+    """_result_to_neighbors (3640-3837)."""
+    lib = _lib.load()
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    L = dt.num_leaves
+    leaf_indices, po, p2n = _leaf_tables(dt)
+    if raw is None:
+        counts = torch.zeros(L, dtype=torch.int32, device=dev)
+    else:
+        counts = raw.near_counts
+    off = _scan(counts)
+    stats = torch.stack([off[-1], counts.max().to(torch.int64)]).cpu() if L else torch.zeros(2)
+    total, maxc = int(stats[0]), int(stats[1])
+    neighbors = torch.empty(total, dtype=idx, device=dev)
+    nlp = torch.empty((L, maxc), dtype=idx, device=dev)
+    if total > 0:
+        fixed_cap, row_start = raw.outcome.near_layout
+        _lib.check(lib.ygg_near_output(L, off.data_ptr(), counts.data_ptr(), int(fixed_cap),
+                                       _lib.ptr(row_start), raw.outcome.near_entries.data_ptr(),
+                                       dt.leaf_position.data_ptr(), maxc, int(idx == torch.int64),
+                                       neighbors.data_ptr(), nlp.data_ptr(), _lib.stream_ptr(dev)),
+                   "near_output")
+    k = _target_block_size()
+    if k > 0:
+        raise NotImplementedError("YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE > 0 is not supported")
+    return NodeNeighborList(
+        offsets=off.to(idx), neighbors=neighbors, leaf_indices=leaf_indices,
+        counts=counts.to(idx), particle_order_leaf_indices=po,
+        particle_order_to_native_leaf=p2n, neighbor_leaf_positions=nlp,
+        target_block_leaf_ids=torch.zeros(0, dtype=idx, device=dev),
+        target_block_source_leaf_ids=torch.zeros((0, 0), dtype=idx, device=dev),
+        target_block_valid_mask=torch.zeros((0, 0), dtype=torch.bool, device=dev),
+        target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev), target_block_size=0)
+
+
+def _to_result(raw: Optional[_RawWalk], dt: DeviceTopology, inter: NodeInteractionList,
+               nbrs: NodeNeighborList) -> DualTreeWalkResult:
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    T = dt.num_nodes
+    counts = inter.counts
+    node_off = torch.zeros(T + 1, dtype=idx, device=dev)
+    node_off[1:] = torch.cumsum(counts, 0)
+    o = raw.outcome if raw is not None else None
+    return DualTreeWalkResult(
+        interaction_offsets=node_off, interaction_sources=inter.sources,
+        interaction_targets=inter.targets,
+        interaction_tags=torch.full((inter.sources.numel(),), -1, dtype=idx, device=dev),
+        interaction_counts=counts, neighbor_offsets=nbrs.offsets,
+        neighbor_indices=nbrs.neighbors, neighbor_counts=nbrs.counts,
+        leaf_indices=nbrs.leaf_indices, far_pair_count=int(inter.sources.numel()),
+        near_pair_count=int(nbrs.neighbors.numel()),
+        queue_overflow=bool(o.queue_overflow) if o else False,
+        far_overflow=bool(o.far_overflow) if o else False,
+        near_overflow=bool(o.near_overflow) if o else False,
+        accept_decisions=o.decisions[0] if o else 0, near_decisions=o.decisions[1] if o else 0,
+        refine_decisions=o.decisions[2] if o else 0)
+
+
+def build_interactions_and_neighbors(
+    tree, geometry: TreeGeometry, theta: float = 0.5,
+    max_interactions_per_node: Optional[int] = None,
+    max_neighbors_per_leaf: int = _DEFAULT_MAX_NEIGHBORS,
+    max_pair_queue: Optional[int] = None, process_block: Optional[int] = None,
+    traversal_config: Optional[DualTreeTraversalConfig] = None,
+    retry_logger: Optional[Callable[[DualTreeRetryEvent], object]] = None,
+    mac_type: str = "bh", dehnen_radius_scale: float = 1.0, pair_policy=None,
+    policy_state: object = None, *, return_result: bool = False,
+    return_compact_far_pairs: bool = False, return_interactions: bool = True,
+    return_grouped: bool = False):
+    """Far-field interaction lists and near-field neighbour lists from one
+    dual-tree walk (yggdrax/interactions.py:338-448)."""
+    if pair_policy is not None:
+        from .policy import build_with_pair_policy
+        return build_with_pair_policy(
+            tree, geometry, theta, mac_type, dehnen_radius_scale, pair_policy, policy_state,
+            max_interactions_per_node=max_interactions_per_node,
+            max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
+            traversal_config=traversal_config, return_result=return_result)
+    if return_grouped:
+        raise NotImplementedError("grouped interaction buffers are not part of this build")
+    if return_compact_far_pairs:
+        raise NotImplementedError("compact far pairs are not part of this build")
+    dt = device_topology(tree)
+    topo = resolve_tree_topology(tree)
+    raw = _run_walk_raw(tree, geometry, theta, mac_type,
+                        max_interactions_per_node=max_interactions_per_node,
+                        max_neighbors_per_leaf=max_neighbors_per_leaf,
+                        max_pair_queue=max_pair_queue, traversal_config=traversal_config,
+                        dehnen_radius_scale=dehnen_radius_scale, process_block=process_block,
+                        retry_logger=retry_logger)
+    inter = _to_interactions(raw, dt, topo) if raw is not None else _empty_outputs(dt, topo)
+    nbrs = _to_neighbors(raw, dt)
+    out_inter = inter if return_interactions else None
+    if return_result:
+        return out_inter, nbrs, _to_result(raw, dt, inter, nbrs)
+    return out_inter, nbrs
+
+
+def interactions_for_node(data: NodeInteractionList, node: int) -> torch.Tensor:
+    start = int(data.offsets[node])
+    return data.sources[start:start + int(data.counts[node])]
+
+
+def neighbors_for_leaf(data: NodeNeighborList, leaf_node: int) -> torch.Tensor:
+    m = (data.leaf_indices == int(leaf_node)).nonzero()
+    if m.numel() == 0:
+        raise ValueError("leaf_node not present in neighbor list")
+    pos = int(m[0, 0])
+    start = int(data.offsets[pos])
+    return data.neighbors[start:start + int(data.counts[pos])]
+
+
+__all__ = [
+    "DEFAULT_PAIR_QUEUE_MULTIPLIER", "DualTreeRetryEvent", "DualTreeTraversalConfig",
+    "DualTreeWalkResult", "NodeInteractionList", "NodeNeighborList",
+    "build_interactions_and_neighbors", "interactions_for_node", "neighbors_for_leaf",
+    "set_default_dual_tree_config",
+]

@@ -1,0 +1,551 @@
+"""Tree construction: the drop-in surface of yggdrax/tree.py and morton.py.
+
+``build_tree`` (radix/LBVH, yggdrax/tree.py:1413-1478) and ``build_octree``
+(radix topology plus explicit octree metadata, yggdrax/tree.py:1482-1549)
+run on the GPU: bounds, Morton encode fused with the sort's digit
+histograms, a stable radix sort, the Karras build with node levels and the
+octree metadata kernels. Topology fields carry the reference's names and
+dtypes (indices widened to INDEX_DTYPE, Morton codes as uint64).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Callable, NamedTuple, Optional
+
+import torch
+
+from . import _lib
+from .dtypes import INDEX_DTYPE
+
+MAX_TREE_LEVELS = 64
+
+
+class RadixTreeTopology(NamedTuple):
+    """Fields of the reference's RadixTree (yggdrax/_tree_impl.py:24-71)."""
+
+    parent: torch.Tensor
+    left_child: torch.Tensor
+    right_child: torch.Tensor
+    left_is_leaf: torch.Tensor
+    right_is_leaf: torch.Tensor
+    particle_indices: torch.Tensor
+    morton_codes: torch.Tensor
+    node_ranges: torch.Tensor
+    num_particles: int
+    num_internal_nodes: int
+    node_level: torch.Tensor
+    level_offsets: torch.Tensor
+    nodes_by_level: torch.Tensor
+    num_levels: int
+    bounds_min: torch.Tensor
+    bounds_max: torch.Tensor
+    leaf_codes: torch.Tensor
+    leaf_depths: torch.Tensor
+    use_morton_geometry: torch.Tensor
+    leaf_size: Optional[int]
+
+
+class OctreeTopology(NamedTuple):
+    """Radix fields plus explicit octree metadata (yggdrax/octree.py:73-110)."""
+
+    parent: torch.Tensor
+    left_child: torch.Tensor
+    right_child: torch.Tensor
+    left_is_leaf: torch.Tensor
+    right_is_leaf: torch.Tensor
+    particle_indices: torch.Tensor
+    morton_codes: torch.Tensor
+    node_ranges: torch.Tensor
+    num_particles: int
+    num_internal_nodes: int
+    node_level: torch.Tensor
+    level_offsets: torch.Tensor
+    nodes_by_level: torch.Tensor
+    num_levels: int
+    bounds_min: torch.Tensor
+    bounds_max: torch.Tensor
+    leaf_codes: torch.Tensor
+    leaf_depths: torch.Tensor
+    use_morton_geometry: torch.Tensor
+    leaf_size: Optional[int]
+    oct_parent: torch.Tensor
+    oct_children: torch.Tensor
+    oct_child_counts: torch.Tensor
+    oct_child_mask: torch.Tensor
+    oct_valid_mask: torch.Tensor
+    oct_node_codes: torch.Tensor
+    oct_node_depths: torch.Tensor
+    oct_node_ranges: torch.Tensor
+    oct_nodes_by_level: torch.Tensor
+    oct_level_offsets: torch.Tensor
+    oct_num_levels: int
+    oct_leaf_mask: torch.Tensor
+    oct_leaf_nodes: torch.Tensor
+    radix_node_to_oct: torch.Tensor
+    radix_leaf_to_oct: torch.Tensor
+
+
+@dataclass
+class DeviceTopology:
+    """int32 device arrays the kernels consume (kept beside the public fields)."""
+
+    num_particles: int
+    num_nodes: int
+    num_internal: int
+    num_leaves: int
+    leaf_size: int
+    parent: torch.Tensor
+    left: torch.Tensor
+    right: torch.Tensor
+    node_ranges: torch.Tensor
+    node_level: torch.Tensor
+    nodes_by_level: torch.Tensor
+    level_offsets: torch.Tensor
+    leaf_indices: torch.Tensor
+    leaf_position: torch.Tensor
+    num_levels: int
+    max_level: int
+    malformed: bool
+    is_octree: bool
+
+
+@dataclass(frozen=True)
+class TreeBuildConfig:
+    """yggdrax/tree.py:28-43."""
+
+    leaf_size: int = 8
+    return_reordered: bool = False
+    workspace: object = None
+    return_workspace: bool = False
+
+
+@dataclass(frozen=True)
+class RadixTreeWorkspace:
+    """Accepted for signature compatibility; device buffers are pooled by torch."""
+
+    parent: object = None
+
+
+@dataclass(frozen=True)
+class Tree:
+    """Public base container (yggdrax/tree.py:184-330)."""
+
+    @property
+    def num_nodes(self) -> int:
+        return int(self.topology.parent.shape[0])
+
+    @property
+    def num_particles(self) -> int:
+        return int(self.topology.num_particles)
+
+    @property
+    def num_leaves(self) -> int:
+        return int(self.topology.parent.shape[0]) - int(self.topology.num_internal_nodes)
+
+    def __getattr__(self, name):
+        if name in ("topology", "_device"):
+            raise AttributeError(name)
+        return getattr(self.topology, name)
+
+
+@dataclass(frozen=True)
+class RadixTree(Tree):
+    topology: RadixTreeTopology
+    build_mode: str = "adaptive"
+    positions_sorted: Optional[torch.Tensor] = None
+    masses_sorted: Optional[torch.Tensor] = None
+    inverse_permutation: Optional[torch.Tensor] = None
+    workspace: object = None
+    _device: Optional[DeviceTopology] = None
+
+    @property
+    def tree_type(self) -> str:
+        return "radix"
+
+    @classmethod
+    def from_particles(cls, positions, masses, *, build_mode="adaptive", bounds=None,
+                       return_reordered=True, leaf_size=8, **_ignored):
+        if build_mode != "adaptive":
+            raise ValueError(f"Unsupported build_mode '{build_mode}'. Supported: ('adaptive',)")
+        out = build_tree(positions, masses, bounds, return_reordered=return_reordered,
+                         leaf_size=leaf_size)
+        return out[0] if isinstance(out, tuple) else out
+
+
+@dataclass(frozen=True)
+class OctreeTree(RadixTree):
+    @property
+    def tree_type(self) -> str:
+        return "octree"
+
+    @property
+    def oct_num_nodes(self) -> int:
+        return int(self.topology.oct_valid_mask.sum())
+
+    @property
+    def oct_num_leaf_nodes(self) -> int:
+        return int(self.topology.oct_leaf_mask.sum())
+
+    @classmethod
+    def from_particles(cls, positions, masses, *, build_mode="adaptive", bounds=None,
+                       return_reordered=True, leaf_size=8, **_ignored):
+        if build_mode != "adaptive":
+            raise ValueError(f"Unsupported build_mode '{build_mode}'. Supported: ('adaptive',)")
+        out = build_octree(positions, masses, bounds, return_reordered=return_reordered,
+                           leaf_size=leaf_size)
+        return out[0] if isinstance(out, tuple) else out
+
+
+@dataclass(frozen=True)
+class TreeBuildRequest:
+    """yggdrax/tree.py:110-130 (adaptive subset)."""
+
+    positions: object
+    masses: object
+    build_mode: str
+    bounds: object
+    return_reordered: bool
+    leaf_size: int
+
+
+TreeBuilder = Callable[[TreeBuildRequest], Tree]
+
+
+def _as_positions(positions) -> torch.Tensor:
+    t = torch.as_tensor(positions)
+    if not t.is_cuda:
+        if not torch.cuda.is_available():
+            raise RuntimeError("yggdrax_amd runs on the GPU only: no HIP device is available")
+        t = t.to("cuda")
+    if t.dtype != torch.float32:
+        raise TypeError("positions must be float32")
+    if t.ndim != 2 or t.shape[1] != 3:
+        raise ValueError("positions must have shape (N, 3)")
+    return t.contiguous()
+
+
+def infer_bounds(positions) -> tuple[torch.Tensor, torch.Tensor]:
+    """yggdrax/bounds.py:11-33 on the GPU."""
+    lib = _lib.load()
+    pos = _as_positions(positions)
+    b6 = torch.empty(6, dtype=torch.float32, device=pos.device)
+    ws = _lib.workspace(lib.ygg_bounds_workspace_bytes(), pos.device)
+    _lib.check(lib.ygg_infer_bounds(pos.data_ptr(), pos.shape[0], b6.data_ptr(), ws.data_ptr(),
+                                    ws.numel(), _lib.stream_ptr(pos.device)), "infer_bounds")
+    return b6[:3], b6[3:]
+
+
+def _bounds6(pos: torch.Tensor, bounds) -> torch.Tensor:
+    if bounds is None:
+        lo, hi = infer_bounds(pos)
+        return torch.cat([lo, hi])
+    lo = torch.as_tensor(bounds[0], dtype=torch.float32, device=pos.device).reshape(3)
+    hi = torch.as_tensor(bounds[1], dtype=torch.float32, device=pos.device).reshape(3)
+    return torch.cat([lo, hi]).contiguous()
+
+
+def morton_encode(positions, bounds) -> torch.Tensor:
+    """yggdrax/morton.py:43-80: uint64 codes (21 bits per axis)."""
+    lib = _lib.load()
+    pos = _as_positions(positions)
+    b6 = _bounds6(pos, bounds)
+    codes = torch.empty(pos.shape[0], dtype=torch.int64, device=pos.device)
+    _lib.check(lib.ygg_morton_encode(pos.data_ptr(), pos.shape[0], b6.data_ptr(),
+                                     codes.data_ptr(), _lib.stream_ptr(pos.device)),
+               "morton_encode")
+    return codes.view(torch.uint64)
+
+
+class _Scratch:
+    """Pinned host words for the few scalar reads a build needs."""
+
+    def __init__(self):
+        self.buf = torch.zeros(16, dtype=torch.int32, pin_memory=True)
+
+
+def _alloc_radix(L: int, device):
+    T = 2 * L - 1
+    NI = L - 1
+    i32 = dict(dtype=torch.int32, device=device)
+    return dict(
+        parent=torch.empty(T, **i32),
+        left_child=torch.empty(max(NI, 0), **i32),
+        right_child=torch.empty(max(NI, 0), **i32),
+        left_is_leaf=torch.empty(max(NI, 0), dtype=torch.uint8, device=device),
+        right_is_leaf=torch.empty(max(NI, 0), dtype=torch.uint8, device=device),
+        node_ranges=torch.empty((T, 2), **i32),
+        node_level=torch.empty(T, **i32),
+        level_offsets=torch.empty(65, **i32),
+        nodes_by_level=torch.empty(T, **i32),
+        leaf_codes=torch.empty(L, dtype=torch.int64, device=device),
+        scalars=torch.zeros(8, **i32),
+    )
+
+
+def _ptrs(struct_cls, arrays: dict):
+    s = struct_cls()
+    for name, _ in struct_cls._fields_:
+        t = arrays.get(name)
+        setattr(s, name, None if t is None or t.numel() == 0 else t.data_ptr())
+    return s
+
+
+def _alloc_octree(L: int, device):
+    M = max(1, 2 * L - 1)
+    T = 2 * L - 1
+    i32 = dict(dtype=torch.int32, device=device)
+    u8 = dict(dtype=torch.uint8, device=device)
+    return dict(
+        oct_parent=torch.empty(M, **i32), oct_children=torch.empty((M, 8), **i32),
+        oct_child_counts=torch.empty(M, **i32), oct_child_mask=torch.empty((M, 8), **u8),
+        oct_valid_mask=torch.empty(M, **u8),
+        oct_node_codes=torch.empty(M, dtype=torch.int64, device=device),
+        oct_node_depths=torch.empty(M, **i32), oct_node_ranges=torch.empty((M, 2), **i32),
+        oct_nodes_by_level=torch.empty(M, **i32), oct_level_offsets=torch.empty(23, **i32),
+        oct_leaf_mask=torch.empty(M, **u8), oct_leaf_nodes=torch.empty(M, **i32),
+        radix_node_to_oct=torch.empty(T, **i32), leaf_indices=torch.empty(L, **i32),
+        leaf_position=torch.empty(T, **i32), scalars=torch.zeros(4, **i32),
+    )
+
+
+def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
+           octree: bool):
+    lib = _lib.load()
+    pos = _as_positions(positions)
+    dev = pos.device
+    n = int(pos.shape[0])
+    if n < 1:
+        raise ValueError("Need at least one particle")
+    if leaf_size < 1:
+        raise ValueError("leaf_size must be >= 1")
+    if n >= (1 << 30):
+        raise ValueError("at most 2^30 - 1 particles per tree")
+    st = _lib.stream_ptr(dev)
+    b6 = _bounds6(pos, bounds)
+    sorted_codes = torch.empty(n, dtype=torch.int64, device=dev)
+    perm = torch.empty(n, dtype=torch.int32, device=dev)
+    ws = _lib.workspace(lib.ygg_morton_sort_workspace_bytes(n), dev)
+    _lib.check(lib.ygg_morton_sort(pos.data_ptr(), n, b6.data_ptr(), sorted_codes.data_ptr(),
+                                   perm.data_ptr(), ws.data_ptr(), ws.numel(), st),
+               "morton_sort")
+    L = (n + leaf_size - 1) // leaf_size
+    T = 2 * L - 1
+    NI = L - 1
+    rad = _alloc_radix(L, dev)
+    rptr = _ptrs(_lib.RadixTopologyPtrs, rad)
+    rws = _lib.workspace(lib.ygg_radix_build_workspace_bytes(n, leaf_size), dev)
+    _lib.check(lib.ygg_radix_build(sorted_codes.data_ptr(), n, leaf_size, ctypes.byref(rptr),
+                                   rws.data_ptr(), rws.numel(), st), "radix_build")
+    if octree:
+        octa = _alloc_octree(L, dev)
+        optr = _ptrs(_lib.OctreeMetaPtrs, octa)
+        ows = _lib.workspace(lib.ygg_octree_meta_workspace_bytes(L), dev)
+        _lib.check(lib.ygg_octree_meta(sorted_codes.data_ptr(), rad["node_ranges"].data_ptr(), L,
+                                       ctypes.byref(optr), ows.data_ptr(), ows.numel(), st),
+                   "octree_meta")
+        leaf_indices, leaf_position = octa["leaf_indices"], octa["leaf_position"]
+    else:
+        leaf_indices = torch.empty(L, dtype=torch.int32, device=dev)
+        leaf_position = torch.empty(T, dtype=torch.int32, device=dev)
+        _lib.check(lib.ygg_radix_leaf_order(L, leaf_indices.data_ptr(), leaf_position.data_ptr(),
+                                            st), "leaf_order")
+    pos_sorted = mass_sorted = inv = None
+    if return_reordered:
+        pos_sorted = torch.empty_like(pos)
+        inv = torch.empty(n, dtype=torch.int32, device=dev)
+        m = None
+        if masses is not None:
+            m = torch.as_tensor(masses, dtype=torch.float32, device=dev).contiguous()
+            mass_sorted = torch.empty_like(m)
+        _lib.check(lib.ygg_reorder(pos.data_ptr(), _lib.ptr(m), perm.data_ptr(), n,
+                                   pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(),
+                                   st), "reorder")
+    # One host read: level count, depth flags and octree level count.
+    scal = rad["scalars"].cpu()
+    if int(scal[2]) != 0:  # levels beyond the fast walk: exact pointer doubling
+        _lib.check(lib.ygg_levels_pointer_doubling(ctypes.byref(rptr), T, rws.data_ptr(),
+                                                   rws.numel(), st), "levels")
+        scal = rad["scalars"].cpu()
+    num_levels = int(scal[0])
+    max_level = int(scal[1])
+    malformed = bool(int(scal[3])) or bool(int(scal[2]))
+    if malformed and max_level >= 65536:
+        raise RuntimeError("tree depth exceeds 65535 levels (cyclic parent links from "
+                           "duplicate Morton codes)")
+    idx = INDEX_DTYPE
+    fields = dict(
+        parent=rad["parent"].to(idx),
+        left_child=rad["left_child"].to(idx),
+        right_child=rad["right_child"].to(idx),
+        left_is_leaf=rad["left_is_leaf"].view(torch.bool),
+        right_is_leaf=rad["right_is_leaf"].view(torch.bool),
+        particle_indices=perm.to(idx),
+        morton_codes=sorted_codes.view(torch.uint64),
+        node_ranges=rad["node_ranges"].to(idx),
+        num_particles=n,
+        num_internal_nodes=NI,
+        node_level=rad["node_level"].to(idx),
+        level_offsets=rad["level_offsets"].to(idx),
+        nodes_by_level=rad["nodes_by_level"].to(idx),
+        num_levels=num_levels,
+        bounds_min=b6[:3].clone(),
+        bounds_max=b6[3:].clone(),
+        leaf_codes=rad["leaf_codes"].view(torch.uint64),
+        leaf_depths=torch.full((L,), -1, dtype=idx, device=dev),
+        use_morton_geometry=torch.zeros((), dtype=torch.bool, device=dev),
+        leaf_size=int(leaf_size),
+    )
+    if octree:
+        oscal = octa["scalars"].cpu()
+        fields.update(
+            oct_parent=octa["oct_parent"].to(idx),
+            oct_children=octa["oct_children"].to(idx),
+            oct_child_counts=octa["oct_child_counts"].to(idx),
+            oct_child_mask=octa["oct_child_mask"].view(torch.bool),
+            oct_valid_mask=octa["oct_valid_mask"].view(torch.bool),
+            oct_node_codes=octa["oct_node_codes"].view(torch.uint64),
+            oct_node_depths=octa["oct_node_depths"].to(idx),
+            oct_node_ranges=octa["oct_node_ranges"].to(idx),
+            oct_nodes_by_level=octa["oct_nodes_by_level"].to(idx),
+            oct_level_offsets=octa["oct_level_offsets"].to(idx),
+            oct_num_levels=int(oscal[0]),
+            oct_leaf_mask=octa["oct_leaf_mask"].view(torch.bool),
+            oct_leaf_nodes=octa["oct_leaf_nodes"].to(idx),
+            radix_node_to_oct=octa["radix_node_to_oct"].to(idx),
+            radix_leaf_to_oct=octa["radix_node_to_oct"][NI:].to(idx),
+        )
+        topo = OctreeTopology(**fields)
+    else:
+        topo = RadixTreeTopology(**fields)
+    dt = DeviceTopology(
+        num_particles=n, num_nodes=T, num_internal=NI, num_leaves=L, leaf_size=int(leaf_size),
+        parent=rad["parent"], left=rad["left_child"], right=rad["right_child"],
+        node_ranges=rad["node_ranges"], node_level=rad["node_level"],
+        nodes_by_level=rad["nodes_by_level"], level_offsets=rad["level_offsets"],
+        leaf_indices=leaf_indices, leaf_position=leaf_position, num_levels=num_levels,
+        max_level=max_level, malformed=malformed, is_octree=octree)
+    cls = OctreeTree if octree else RadixTree
+    tree = cls(topology=topo, build_mode="adaptive", positions_sorted=pos_sorted,
+               masses_sorted=mass_sorted,
+               inverse_permutation=None if inv is None else inv.to(idx), _device=dt)
+    if return_reordered:
+        return tree, pos_sorted, mass_sorted, tree.inverse_permutation
+    return tree
+
+
+def build_tree(positions, masses=None, bounds=None, *, return_reordered=False, leaf_size=8,
+               workspace=None, return_workspace=False, config: Optional[TreeBuildConfig] = None):
+    """Adaptive radix (LBVH) tree (yggdrax/tree.py:1413-1478)."""
+    if config is not None:
+        return_reordered = config.return_reordered
+        leaf_size = config.leaf_size
+        return_workspace = config.return_workspace
+    out = _build(positions, masses, bounds, leaf_size=leaf_size,
+                 return_reordered=return_reordered, octree=False)
+    if return_workspace:
+        return (*out, RadixTreeWorkspace()) if isinstance(out, tuple) else (out, RadixTreeWorkspace())
+    return out
+
+
+def build_octree(positions, masses=None, bounds=None, *, return_reordered=False, leaf_size=8,
+                 workspace=None, return_workspace=False, config: Optional[TreeBuildConfig] = None):
+    """Radix topology augmented with explicit octree metadata
+    (yggdrax/tree.py:1482-1549)."""
+    if config is not None:
+        return_reordered = config.return_reordered
+        leaf_size = config.leaf_size
+        return_workspace = config.return_workspace
+    out = _build(positions, masses, bounds, leaf_size=leaf_size,
+                 return_reordered=return_reordered, octree=True)
+    if return_workspace:
+        return (*out, RadixTreeWorkspace()) if isinstance(out, tuple) else (out, RadixTreeWorkspace())
+    return out
+
+
+def _radix_builder(req: TreeBuildRequest) -> Tree:
+    return RadixTree.from_particles(req.positions, req.masses, bounds=req.bounds,
+                                    return_reordered=req.return_reordered,
+                                    leaf_size=req.leaf_size)
+
+
+def _octree_builder(req: TreeBuildRequest) -> Tree:
+    return OctreeTree.from_particles(req.positions, req.masses, bounds=req.bounds,
+                                     return_reordered=req.return_reordered,
+                                     leaf_size=req.leaf_size)
+
+
+_TREE_BUILDERS: dict[str, TreeBuilder] = {"radix": _radix_builder, "octree": _octree_builder}
+
+
+def register_tree_builder(tree_type: str, builder: TreeBuilder, *, overwrite: bool = False) -> None:
+    """yggdrax/tree.py:1308-1321."""
+    name = tree_type.strip()
+    if not name:
+        raise ValueError("tree_type must be a non-empty string")
+    if name in _TREE_BUILDERS and not overwrite:
+        raise ValueError(f"tree_type '{name}' is already registered; pass overwrite=True to replace it")
+    _TREE_BUILDERS[name] = builder
+
+
+def available_tree_types() -> tuple[str, ...]:
+    return tuple(sorted(_TREE_BUILDERS))
+
+
+def tree_from_particles(positions, masses, *, tree_type="radix", build_mode="adaptive",
+                        bounds=None, return_reordered=True, leaf_size=8) -> Tree:
+    """Tree.from_particles dispatch through the builder registry (tree.py:245-330)."""
+    builder = _TREE_BUILDERS.get(tree_type)
+    if builder is None:
+        supported = ", ".join(f"'{k}'" for k in sorted(_TREE_BUILDERS))
+        raise ValueError(f"Unsupported tree_type '{tree_type}'. Supported: ({supported})")
+    return builder(TreeBuildRequest(positions, masses, build_mode, bounds, return_reordered,
+                                    leaf_size))
+
+
+def resolve_tree_topology(tree):
+    topo = getattr(tree, "topology", None)
+    return tree if topo is None else topo
+
+
+def device_topology(tree) -> DeviceTopology:
+    """int32 device view of a tree or a bare topology."""
+    dt = getattr(tree, "_device", None)
+    if dt is not None:
+        return dt
+    topo = resolve_tree_topology(tree)
+    lib = _lib.load()
+    dev = topo.parent.device
+    T = int(topo.parent.shape[0])
+    NI = int(topo.num_internal_nodes)
+    L = T - NI
+    i32 = lambda t: torch.as_tensor(t, device=dev).to(torch.int32).contiguous()  # noqa: E731
+    leaf_indices = torch.empty(L, dtype=torch.int32, device=dev)
+    leaf_position = torch.empty(T, dtype=torch.int32, device=dev)
+    octree = hasattr(topo, "radix_leaf_to_oct")
+    if octree:
+        order = torch.argsort(i32(topo.radix_leaf_to_oct), stable=True).to(torch.int32)
+        leaf_indices = order + NI
+        leaf_position = torch.full((T,), -1, dtype=torch.int32, device=dev)
+        leaf_position[leaf_indices.long()] = torch.arange(L, dtype=torch.int32, device=dev)
+    else:
+        _lib.check(lib.ygg_radix_leaf_order(L, leaf_indices.data_ptr(), leaf_position.data_ptr(),
+                                            _lib.stream_ptr(dev)), "leaf_order")
+    lvl = i32(topo.node_level)
+    return DeviceTopology(
+        num_particles=int(topo.num_particles), num_nodes=T, num_internal=NI, num_leaves=L,
+        leaf_size=int(topo.leaf_size or 0), parent=i32(topo.parent), left=i32(topo.left_child),
+        right=i32(topo.right_child), node_ranges=i32(topo.node_ranges), node_level=lvl,
+        nodes_by_level=i32(topo.nodes_by_level), level_offsets=i32(topo.level_offsets),
+        leaf_indices=leaf_indices, leaf_position=leaf_position,
+        num_levels=int(topo.num_levels), max_level=int(lvl.max()), malformed=True,
+        is_octree=octree)
+
+
+__all__ = [
+    "MAX_TREE_LEVELS", "OctreeTopology", "OctreeTree", "RadixTree", "RadixTreeTopology",
+    "RadixTreeWorkspace", "Tree", "TreeBuildConfig", "TreeBuildRequest", "available_tree_types",
+    "build_octree", "build_tree", "device_topology", "infer_bounds", "morton_encode",
+    "register_tree_builder", "resolve_tree_topology", "tree_from_particles",
+]
