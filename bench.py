@@ -37,10 +37,18 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# The reference's count-pass path rejects this workload (a background leaf
+# next to a dense cluster has more than (2^31-1)/L near neighbours, which its
+# dense (L x capacity) buffers cannot index), so the benchmark call lifts that
+# guard and skips the dense neighbour-position matrix; the CSR lists are
+# complete and exact.
+WALK_KW = dict(mac_type="dehnen", neighbor_positions=False, strict_int32=False)
+
+
 def run_pipeline(ygg, pos, mass, leaf, theta):
     tree, ps, _, _ = ygg.build_octree(pos, mass, leaf_size=leaf, return_reordered=True)
     geom = ygg.compute_tree_geometry(tree, ps)
-    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=theta, mac_type="dehnen")
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=theta, **WALK_KW)
     return tree, geom, inter, nbrs
 
 
@@ -51,7 +59,7 @@ def stage_times(ygg, torch, pos, mass, leaf, theta):
     ev[1].record()
     geom = ygg.compute_tree_geometry(tree, ps)
     ev[2].record()
-    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=theta, mac_type="dehnen")
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=theta, **WALK_KW)
     ev[3].record()
     torch.cuda.synchronize()
     return dict(build_ms=ev[0].elapsed_time(ev[1]), geometry_ms=ev[1].elapsed_time(ev[2]),
@@ -61,6 +69,8 @@ def stage_times(ygg, torch, pos, mass, leaf, theta):
 
 
 def walk_roofline(ygg, torch, pos, mass, leaf, theta):
+    """Roofline of the walk's generation kernel (emission pass), timed with
+    HIP events recorded on the launch stream around every generation."""
     from yggdrax_amd import interactions as I
     I.WALK_TIMING["enabled"] = True
     I.WALK_TIMING["log"].clear()
@@ -69,27 +79,29 @@ def walk_roofline(ygg, torch, pos, mass, leaf, theta):
         torch.cuda.synchronize()
     finally:
         I.WALK_TIMING["enabled"] = False
-    total_ms = 0.0
-    total_bytes = 0.0
-    launches = 0
-    pairs_total = 0
-    for rec in I.WALK_TIMING["log"]:
-        acc, near, ref = rec["decisions"]
-        pairs = acc + near + ref
-        pushes = sum(rec["gen_sizes"][1:])
-        entries = 2 * (acc + near)
-        total_bytes += pairs * (2 * 32 + 8) + pushes * 8 + entries * 4
-        total_ms += sum(rec["gen_ms"])
-        launches += sum(1 for v in rec["gen_sizes"][:-1] if v > 0)
-        pairs_total += pairs
+    recs = [r for r in I.WALK_TIMING["log"] if r.get("emit")]
+    rec = recs[-1]
+    acc, near, ref = rec["decisions"]
+    pairs = acc + near + ref
+    pushes = sum(rec["gen_sizes"][1:])
+    entries = rec["far_total"] + rec["near_total"]
+    # per pair: pair read 8 B + two 32 B node records; 8 B per pushed child
+    # pair; 8 B (row key + value) per emitted entry
+    total_bytes = pairs * (8 + 2 * 32) + pushes * 8 + entries * 8
+    gen_ms = [m for m, g in zip(rec["gen_ms"], rec["gen_sizes"]) if g > 0]
+    total_ms = sum(gen_ms)
+    launches = len(gen_ms)
     avg_ms = total_ms / max(1, launches)
     avg_bytes = total_bytes / max(1, launches)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    count_ms = sum(sum(r["gen_ms"]) for r in I.WALK_TIMING["log"] if not r.get("emit"))
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_walk_gen", launches=launches, avg_launch_ms=round(avg_ms, 5),
-                bytes_per_launch=round(avg_bytes), pairs_per_s=round(pairs_total / (total_ms * 1e-3)),
-                bytes_per_pair=round(total_bytes / max(1, pairs_total), 2))
+                kernel="k_walk_flat<true> (emission pass)", launches=launches,
+                avg_launch_ms=round(avg_ms, 5), bytes_per_launch=round(avg_bytes),
+                pairs=pairs, pairs_per_s=round(pairs / (total_ms * 1e-3)),
+                bytes_per_pair=round(total_bytes / max(1, pairs), 2),
+                emit_walk_ms=round(total_ms, 4), count_walk_ms=round(count_ms, 4))
 
 
 def cpu_baseline(sample_n, leaf, theta):
@@ -100,7 +112,9 @@ def cpu_baseline(sample_n, leaf, theta):
     topo = oracle.build_octree(pos, mass, leaf_size=leaf)
     geom = oracle.compute_tree_geometry(topo, topo["positions_sorted"])
     from oracle import oracle as O
-    O.run_walk_with_retries(topo, geom, theta, "dehnen")
+    # the oracle's count walk (no int32 guard needed: it only counts)
+    cap = max(topo["parent"].shape[0] * 128, 1 << 16)
+    O.walk(topo, geom, theta, "dehnen", queue_cap=cap, far_cap=1 << 62, near_cap=1 << 62)
     dt = time.perf_counter() - t0
     return dict(value=round(sample_n / dt / 1e6, 4), unit="Mpts/s", cores=1, kind="port",
                 sample=f"oracle build_octree+geometry+count walk on {sample_n} points of the "

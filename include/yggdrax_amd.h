@@ -137,13 +137,17 @@ int ygg_octree_meta(const uint64_t* sorted_codes, const int32_t* node_ranges, in
 /* compute_tree_geometry (yggdrax/geometry.py:18-75,
  * yggdrax/_geometry_impl.py:176-338): per-node AABB centre, half extent
  * (clamped to >= 1e-6), bounding-sphere radius and max half extent from the
- * Morton-sorted positions. level_sync = 0 uses the bottom-up merge (valid for
- * well-formed trees, YggRadixTopology.scalars[3] == 0); level_sync = 1 runs
- * the reference's level-by-level pass using node_level and max_level. */
+ * Morton-sorted positions. leaf_size is the fixed-count leaf partition
+ * (leaf j = particles [j*leaf_size, ...)); leaf_cap bounds the particles read
+ * per leaf (max_leaf_size). level_sync = 0 takes each internal box as a
+ * segment-tree range reduction over its leaves (valid for well-formed trees,
+ * YggRadixTopology.scalars[3] == 0); level_sync = 1 runs the reference's
+ * level-by-level pass using node_level and max_level. */
 size_t ygg_geometry_workspace_bytes(int64_t num_nodes);
 int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int32_t* left,
                  const int32_t* right, const int32_t* parent, const int32_t* node_level,
-                 int64_t num_nodes, int64_t leaf_cap, int level_sync, int32_t max_level,
+                 int64_t num_nodes, int64_t leaf_size, int64_t leaf_cap, int level_sync,
+                 int32_t max_level,
                  float* center, float* half_extent, float* radius, float* max_extent, void* ws,
                  size_t ws_bytes, void* stream);
 
@@ -164,11 +168,12 @@ int ygg_radix_leaf_order(int64_t num_leaves, int32_t* leaf_indices, int32_t* lea
 
 /* Node records for the walk: 32 bytes per node (centre, MAC extent of
  * _build_mac_extents, yggdrax/_interactions_impl.py:993-1061, children,
- * native leaf row). */
+ * native leaf row, level-major rank from nodes_by_level). */
 int ygg_walk_prepare_nodes(const float* center, const float* radius, const float* max_extent,
                            const int32_t* left, const int32_t* right,
-                           const int32_t* leaf_position, int64_t num_nodes, int mac_type,
-                           float dehnen_radius_scale, void* nodes_out, void* stream);
+                           const int32_t* leaf_position, const int32_t* nodes_by_level,
+                           int64_t num_nodes, int mac_type, float dehnen_radius_scale,
+                           void* nodes_out, void* stream);
 
 typedef struct YggWalkParams {
   int32_t num_nodes, num_internal, num_leaves, root;
@@ -236,6 +241,62 @@ int ygg_near_output(int64_t num_leaves, const int64_t* offsets, const int32_t* c
                     int32_t fixed_cap, const int64_t* row_start, const int32_t* entries,
                     const int32_t* leaf_position, int64_t max_count, int out_i64,
                     void* neighbors, void* neighbor_leaf_positions, void* stream);
+
+/* Flat-emission walk: the same pair semantics as ygg_walk_bfs over the full
+ * walk, with workgroup-aggregated appends to three streams: the next
+ * wavefront, far entries (key = level-major rank of the target row, value =
+ * source node) and near entries (key = native leaf row, value = neighbour
+ * leaf node). emit = 0 only counts (entry totals and the exact wavefront
+ * size of every generation; the accept / near / refine decision counts
+ * follow as far/2, near/2 and the remaining evaluated pairs). A stream that would exceed its capacity sets its
+ * flag (queue, far, near) and the walk stops after that generation.
+ * host_status and gen_ms (optional per-generation kernel times from HIP
+ * events on `stream`) are as for ygg_walk_bfs. */
+typedef struct YggFlatWalkParams {
+  int32_t num_nodes, num_internal, num_leaves, root;
+  int32_t mac_type, emit, max_generations, grid_blocks;
+  float theta_sq, theta_root;
+  int32_t gen_hint, pad0;     /* expected generation count (first launch batch), 0 = 24 */
+  int64_t queue_cap;          /* pairs per wavefront buffer */
+  int64_t far_cap, near_cap;  /* entries per flat stream */
+  const void* nodes;
+  uint64_t* wf0;
+  uint64_t* wf1;
+  uint32_t* gen_sizes;        /* [max_generations + 1] */
+  uint32_t* flags;            /* [4] */
+  uint64_t* counters;         /* [2]: far total, near total */
+  uint32_t* far_keys;
+  uint32_t* far_vals;
+  uint32_t* near_keys;
+  uint32_t* near_vals;
+} YggFlatWalkParams;
+
+int ygg_walk_flat(const YggFlatWalkParams* params, uint32_t* host_status,
+                  int32_t* num_generations, float* gen_ms, void* stream);
+
+/* offsets[r] = lower_bound(sorted_keys, r) for r in [0, rows]. */
+int ygg_row_offsets(const uint32_t* sorted_keys, int64_t n, int64_t rows, int64_t* offsets,
+                    void* stream);
+
+/* Far lists from key-sorted flat entries (level-major CSR of
+ * _result_to_interactions, yggdrax/_interactions_impl.py:3561-3637):
+ * sources/targets/target_levels[n], counts[T] and node_offsets[T + 1] in the
+ * output index type, plus int32 counts32[T]. */
+int ygg_far_lists_from_sorted(int64_t num_nodes, const uint32_t* keys, const uint32_t* vals,
+                              int64_t n, const int64_t* lm_offsets, const int32_t* nodes_by_level,
+                              const int32_t* node_level, int out_i64, void* sources,
+                              void* targets, void* target_levels, void* counts,
+                              void* node_offsets, int32_t* counts32, void* stream);
+
+/* Near lists from row-sorted flat entries (_result_to_neighbors,
+ * yggdrax/_interactions_impl.py:3640-3837): neighbors[n], the dense
+ * neighbor_leaf_positions[L][max_count] (may be NULL), counts[L] and
+ * offsets[L + 1] in the output index type. */
+int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
+                               const uint32_t* vals, const int32_t* leaf_position,
+                               int64_t max_count, int out_i64, void* neighbors,
+                               void* neighbor_leaf_positions, void* counts, void* offsets,
+                               void* stream);
 
 #ifdef __cplusplus
 }

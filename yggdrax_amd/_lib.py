@@ -54,6 +54,18 @@ class WalkParams(ctypes.Structure):
     ]
 
 
+class FlatWalkParams(ctypes.Structure):
+    _fields_ = [
+        ("num_nodes", _I32), ("num_internal", _I32), ("num_leaves", _I32), ("root", _I32),
+        ("mac_type", _I32), ("emit", _I32), ("max_generations", _I32), ("grid_blocks", _I32),
+        ("theta_sq", _F), ("theta_root", _F), ("gen_hint", _I32), ("pad0", _I32),
+        ("queue_cap", _I64), ("far_cap", _I64),
+        ("near_cap", _I64), ("nodes", _P), ("wf0", _P), ("wf1", _P), ("gen_sizes", _P),
+        ("flags", _P), ("counters", _P), ("far_keys", _P), ("far_vals", _P),
+        ("near_keys", _P), ("near_vals", _P),
+    ]
+
+
 _SIGS = {
     "ygg_sort_workspace_bytes": (_SZ, [_I64, _INT, _INT, _INT]),
     "ygg_sort_pairs_u64": (_INT, [_P, _P, _P, _P, _I64, _INT, _INT, _P, _SZ, _P]),
@@ -70,10 +82,15 @@ _SIGS = {
     "ygg_octree_meta_workspace_bytes": (_SZ, [_I64]),
     "ygg_octree_meta": (_INT, [_P, _P, _I64, ctypes.POINTER(OctreeMetaPtrs), _P, _SZ, _P]),
     "ygg_geometry_workspace_bytes": (_SZ, [_I64]),
-    "ygg_geometry": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _INT, _I32, _P, _P, _P, _P,
-                            _P, _SZ, _P]),
+    "ygg_geometry": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I32, _P, _P, _P,
+                            _P, _P, _SZ, _P]),
     "ygg_radix_leaf_order": (_INT, [_I64, _P, _P, _P]),
-    "ygg_walk_prepare_nodes": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _INT, _F, _P, _P]),
+    "ygg_walk_prepare_nodes": (_INT, [_P, _P, _P, _P, _P, _P, _P, _I64, _INT, _F, _P, _P]),
+    "ygg_walk_flat": (_INT, [ctypes.POINTER(FlatWalkParams), _P, ctypes.POINTER(_I32), _P, _P]),
+    "ygg_row_offsets": (_INT, [_P, _I64, _I64, _P, _P]),
+    "ygg_far_lists_from_sorted": (_INT, [_I64, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P,
+                                         _P, _P, _P]),
+    "ygg_near_lists_from_sorted": (_INT, [_I64, _P, _P, _P, _I64, _INT, _P, _P, _P, _P, _P]),
     "ygg_walk_bfs": (_INT, [ctypes.POINTER(WalkParams), _P, ctypes.POINTER(_I32), _P, _P]),
     "ygg_scan_workspace_bytes": (_SZ, [_I64]),
     "ygg_exclusive_scan_i32": (_INT, [_P, _P, _P, _I64, _P, _SZ, _P]),

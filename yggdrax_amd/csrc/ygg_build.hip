@@ -100,12 +100,19 @@ __global__ __launch_bounds__(256) void k_encode_hist(const float* __restrict__ p
   __syncthreads();
   float lo[3] = {b6[0], b6[1], b6[2]};
   float w[3] = {b6[3] - b6[0], b6[4] - b6[1], b6[5] - b6[2]};
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t c = encode_point(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], lo, w);
-    codes[i] = c;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // uniform trip count per wave: every lane takes part in the ballots
+  for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+    const int64_t i = b0 + threadIdx.x;
+    const bool valid = i < n;
+    uint64_t c = 0;
+    if (valid) {
+      c = encode_point(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], lo, w);
+      codes[i] = c;
+    }
 #pragma unroll
-    for (int p = 0; p < 8; ++p) atomicAdd(&h[p * 256 + (unsigned)((c >> (8 * p)) & 0xFF)], 1u);
+    for (int p = 0; p < 8; ++p)
+      wave_hist_add(&h[p * 256], (unsigned)((c >> (8 * p)) & 0xFF), valid);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x)

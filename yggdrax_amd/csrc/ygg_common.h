@@ -86,6 +86,20 @@ __device__ __forceinline__ unsigned mbcnt64(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
+// Histogram update aggregated over the wave: lanes holding the same 8-bit
+// digit are found with eight ballots and the lowest of them adds the group
+// size, so clustered digits cost one LDS atomic per distinct digit.
+__device__ __forceinline__ void wave_hist_add(uint32_t* h, unsigned d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    bool bit = (d >> b) & 1u;
+    uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  if (valid && mbcnt64(peers) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+}
+
 // Wave-wide inclusive prefix sum of a 32-bit value (wave64).
 __device__ __forceinline__ unsigned wave_inclusive_sum(unsigned v) {
   const int lane = (int)lane_id();

@@ -1,11 +1,12 @@
 // Per-node geometry (compute_tree_geometry, _geometry_impl.py:176-338).
 //
 // Leaf boxes: one wave64 per leaf reduces the leaf's particles (contiguous in
-// Morton order) with cross-lane min/max. Internal boxes: Karras-style
-// bottom-up merge; the second child to arrive at a parent (device-scope
-// acq_rel counter) merges both children, so every node is finalised exactly
-// once without level-by-level launches. min/max are exact, so this gives the
-// reference's level-synchronous result bit for bit on well-formed trees;
+// Morton order) with cross-lane min/max and writes them into the bottom row
+// of a power-of-two segment tree. Two multi-level LDS reductions build the
+// tree; every internal node then takes the min/max over its leaf range
+// [first, last] in O(log L) segment-tree nodes. min/max are exact, so this
+// gives the reference's level-synchronous result bit for bit on well-formed
+// trees (every internal node's particles are exactly its leaf range);
 // malformed trees (duplicate leaf codes) take the level-synchronous path.
 #include "ygg_common.h"
 
@@ -30,20 +31,45 @@ __device__ __forceinline__ void finalize_node(int64_t k, const float* mn, const 
   maxext[k] = fmaxf(fmaxf(h[0], h[1]), h[2]);
 }
 
-__global__ __launch_bounds__(256) void k_geom_bottom_up(
-    const float* __restrict__ pos, const int32_t* __restrict__ ranges,
-    const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-    const int32_t* __restrict__ parent, int T, int NI, int leaf_cap, float* __restrict__ bmin,
-    float* __restrict__ bmax, uint32_t* __restrict__ visit, float* __restrict__ center,
+struct __attribute__((aligned(16))) Box8 {
+  float mn[4];
+  float mx[4];
+};
+
+__device__ __forceinline__ Box8 empty_box() {
+  Box8 b;
+  for (int d = 0; d < 4; ++d) {
+    b.mn[d] = INFINITY;
+    b.mx[d] = -INFINITY;
+  }
+  return b;
+}
+
+__device__ __forceinline__ void merge_into(Box8& a, const Box8& b) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    a.mn[d] = fminf(a.mn[d], b.mn[d]);
+    a.mx[d] = fmaxf(a.mx[d], b.mx[d]);
+  }
+}
+
+// One wave per segment-tree leaf slot j < P: real leaves reduce their
+// particles, padding slots get the empty box.
+__global__ __launch_bounds__(256) void k_seg_leaves(
+    const float* __restrict__ pos, const int32_t* __restrict__ ranges, int T, int NI,
+    int leaf_cap, int P, Box8* __restrict__ seg, float* __restrict__ center,
     float* __restrict__ half, float* __restrict__ radius, float* __restrict__ maxext) {
   const int L = T - NI;
-  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE;
+  int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE;
   const unsigned lane = lane_id();
-  if (wave >= L) return;
-  const int64_t node = NI + wave;
+  if (j >= P) return;
+  if (j >= L) {
+    if (lane == 0) seg[P + j] = empty_box();
+    return;
+  }
+  const int64_t node = NI + j;
   const int a = ranges[2 * node];
-  int cnt = ranges[2 * node + 1] - a + 1;
-  cnt = min(cnt, leaf_cap);
+  int cnt = min(ranges[2 * node + 1] - a + 1, leaf_cap);
   float lo[3] = {INFINITY, INFINITY, INFINITY};
   float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int m = lane; m < cnt; m += YGG_WAVE) {
@@ -64,31 +90,68 @@ __global__ __launch_bounds__(256) void k_geom_bottom_up(
     }
   }
   if (lane != 0) return;
+  Box8 b;
   for (int d = 0; d < 3; ++d) {
-    bmin[3 * node + d] = lo[d];
-    bmax[3 * node + d] = hi[d];
+    b.mn[d] = lo[d];
+    b.mx[d] = hi[d];
   }
+  b.mn[3] = 0.f;
+  b.mx[3] = 0.f;
+  seg[P + j] = b;
   finalize_node(node, lo, hi, center, half, radius, maxext);
-  int64_t cur = node;
-  for (int guard = 0; guard < 4096; ++guard) {  // depth bound; malformed trees go level-sync
-    int32_t p = parent[cur];
-    if (p < 0) break;
-    uint32_t old = __hip_atomic_fetch_add(&visit[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) break;
-    int32_t l = left[p], r = right[p];
-    float nlo[3], nhi[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      nlo[d] = fminf(bmin[3 * (int64_t)l + d], bmin[3 * (int64_t)r + d]);
-      nhi[d] = fmaxf(bmax[3 * (int64_t)l + d], bmax[3 * (int64_t)r + d]);
-    }
-    for (int d = 0; d < 3; ++d) {
-      bmin[3 * (int64_t)p + d] = nlo[d];
-      bmax[3 * (int64_t)p + d] = nhi[d];
-    }
-    finalize_node(p, nlo, nhi, center, half, radius, maxext);
-    cur = p;
+}
+
+// Reduce up to 9 segment-tree levels: children [S, 2S) -> parents down to
+// level S / 512 (or the root). Block b owns children [S + 512 b, S + 512 b + 512).
+__global__ __launch_bounds__(256) void k_seg_up(Box8* __restrict__ seg, int S) {
+  __shared__ Box8 sb[2][256];
+  const int tid = threadIdx.x;
+  const int64_t c0 = (int64_t)S + (int64_t)blockIdx.x * 512 + 2 * tid;
+  int n = S / 2 - blockIdx.x * 256;  // parents this block owns at the first level
+  if (n > 256) n = 256;
+  Box8 b = empty_box();
+  if (tid < n) {
+    b = seg[c0];
+    merge_into(b, seg[c0 + 1]);
+    seg[S / 2 + blockIdx.x * 256 + tid] = b;
   }
+  sb[0][tid] = b;
+  __syncthreads();
+  int level = S / 4, width = n / 2, cur = 0;
+  int64_t base = (int64_t)blockIdx.x * 128;
+  while (width >= 1 && level >= 1) {
+    if (tid < width) {  // ping-pong buffers: reads of one level never race its writes
+      Box8 x = sb[cur][2 * tid];
+      merge_into(x, sb[cur][2 * tid + 1]);
+      seg[level + base + tid] = x;
+      sb[1 - cur][tid] = x;
+    }
+    __syncthreads();
+    cur = 1 - cur;
+    level >>= 1;
+    width >>= 1;
+    base >>= 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_seg_query(const Box8* __restrict__ seg, int P,
+                                                   const int32_t* __restrict__ ranges, int NI,
+                                                   int leaf, float* __restrict__ center,
+                                                   float* __restrict__ half,
+                                                   float* __restrict__ radius,
+                                                   float* __restrict__ maxext) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NI) return;
+  int l = ranges[2 * i] / leaf + P;
+  int r = ranges[2 * i + 1] / leaf + P + 1;
+  Box8 acc = empty_box();
+  while (l < r) {
+    if (l & 1) merge_into(acc, seg[l++]);
+    if (r & 1) merge_into(acc, seg[--r]);
+    l >>= 1;
+    r >>= 1;
+  }
+  finalize_node(i, acc.mn, acc.mx, center, half, radius, maxext);
 }
 
 // Level-synchronous upward pass for malformed trees (_geometry_impl.py:302-329):
@@ -156,19 +219,26 @@ using namespace ygg;
 
 extern "C" {
 
+static int64_t seg_width(int64_t num_nodes) {
+  int64_t L = num_nodes - (num_nodes - 1) / 2, P = 1;
+  while (P < L) P <<= 1;
+  return P;
+}
+
 size_t ygg_geometry_workspace_bytes(int64_t num_nodes) {
   Arena a(nullptr, 0);
   a.take<float>((size_t)num_nodes * 3);
   a.take<float>((size_t)num_nodes * 3);
   a.take<float>((size_t)num_nodes * 3);
   a.take<float>((size_t)num_nodes * 3);
-  a.take<uint32_t>((size_t)num_nodes);
+  a.take<Box8>((size_t)(2 * seg_width(num_nodes)));
   return a.off;
 }
 
 int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int32_t* left,
                  const int32_t* right, const int32_t* parent, const int32_t* node_level,
-                 int64_t num_nodes, int64_t leaf_cap, int level_sync, int32_t max_level,
+                 int64_t num_nodes, int64_t leaf_size, int64_t leaf_cap, int level_sync,
+                 int32_t max_level,
                  float* center, float* half_extent, float* radius, float* max_extent, void* ws,
                  size_t ws_bytes, void* stream) {
   if (num_nodes < 1 || leaf_cap < 1) return YGG_ERR_ARG;
@@ -181,16 +251,23 @@ int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int3
   float* bmax = a.take<float>((size_t)T * 3);
   float* tmin = a.take<float>((size_t)T * 3);
   float* tmax = a.take<float>((size_t)T * 3);
-  uint32_t* visit = a.take<uint32_t>((size_t)T);
-  if (!bmin || !bmax || !tmin || !tmax || !visit) return YGG_ERR_WORKSPACE;
+  Box8* seg = a.take<Box8>((size_t)(2 * seg_width(T)));
+  if (!bmin || !bmax || !tmin || !tmax || !seg) return YGG_ERR_WORKSPACE;
+  if (leaf_size < 1) level_sync = 1;  // leaf ranges are not a fixed-count partition
   const int lc = (int)(leaf_cap > 0x7FFFFFFF ? 0x7FFFFFFF : leaf_cap);
   if (!level_sync) {
-    int rc = fill_u32(visit, 0u, T, st);
-    if (rc) return rc;
-    int64_t threads = (int64_t)L * YGG_WAVE;
-    hipLaunchKernelGGL(k_geom_bottom_up, dim3(grid_for(threads, 256)), dim3(256), 0, st,
-                       pos_sorted, node_ranges, left, right, parent, T, NI, lc, bmin, bmax, visit,
-                       center, half_extent, radius, max_extent);
+    const int P = (int)seg_width(T);
+    int64_t threads = (int64_t)P * YGG_WAVE;
+    hipLaunchKernelGGL(k_seg_leaves, dim3(grid_for(threads, 256)), dim3(256), 0, st, pos_sorted,
+                       node_ranges, T, NI, lc, P, seg, center, half_extent, radius, max_extent);
+    for (int S = P; S > 1; S /= 512) {
+      int blocks = (S / 2 + 255) / 256;
+      hipLaunchKernelGGL(k_seg_up, dim3(blocks), dim3(256), 0, st, seg, S);
+    }
+    if (NI > 0)
+      hipLaunchKernelGGL(k_seg_query, dim3(grid_for(NI, 256)), dim3(256), 0, st, seg, P,
+                         node_ranges, NI, (int)leaf_size, center, half_extent, radius,
+                         max_extent);
     YGG_LAUNCH_CHECK();
     return YGG_OK;
   }

@@ -28,13 +28,13 @@ __global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, 
   __shared__ uint32_t h[8 * kBins];
   for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    K k = keys[i];
-    for (int p = 0; p < passes; ++p) {
-      unsigned d = (unsigned)((k >> (begin_bit + 8 * p)) & 0xFF);
-      atomicAdd(&h[p * kBins + d], 1u);
-    }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+    const int64_t i = b0 + threadIdx.x;
+    const bool valid = i < n;
+    K k = valid ? keys[i] : (K)0;
+    for (int p = 0; p < passes; ++p)
+      wave_hist_add(&h[p * kBins], (unsigned)((k >> (begin_bit + 8 * p)) & 0xFF), valid);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x)

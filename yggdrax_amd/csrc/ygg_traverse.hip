@@ -45,6 +45,11 @@ __global__ void k_prep_nodes(const float* __restrict__ center, const float* __re
   out[k] = r;
 }
 
+__global__ void k_set_rank(const int32_t* __restrict__ nbl, int T, NodeRec* __restrict__ out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < T) out[nbl[r]].pad = r;
+}
+
 __global__ void k_walk_init(YggWalkParams p) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -269,13 +274,18 @@ extern "C" {
 
 int ygg_walk_prepare_nodes(const float* center, const float* radius, const float* max_extent,
                            const int32_t* left, const int32_t* right,
-                           const int32_t* leaf_position, int64_t num_nodes, int mac_type,
-                           float dehnen_radius_scale, void* nodes_out, void* stream) {
+                           const int32_t* leaf_position, const int32_t* nodes_by_level,
+                           int64_t num_nodes, int mac_type, float dehnen_radius_scale,
+                           void* nodes_out, void* stream) {
   if (num_nodes < 1 || !nodes_out) return YGG_ERR_ARG;
   int T = (int)num_nodes, NI = (T - 1) / 2;
-  hipLaunchKernelGGL(k_prep_nodes, dim3(grid_for(T, 256)), dim3(256), 0, (hipStream_t)stream,
-                     center, radius, max_extent, left, right, leaf_position, T, NI, mac_type,
-                     dehnen_radius_scale, (NodeRec*)nodes_out);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_prep_nodes, dim3(grid_for(T, 256)), dim3(256), 0, st, center, radius,
+                     max_extent, left, right, leaf_position, T, NI, mac_type, dehnen_radius_scale,
+                     (NodeRec*)nodes_out);
+  if (nodes_by_level)
+    hipLaunchKernelGGL(k_set_rank, dim3(grid_for(T, 256)), dim3(256), 0, st, nodes_by_level, T,
+                       (NodeRec*)nodes_out);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

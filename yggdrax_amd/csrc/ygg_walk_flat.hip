@@ -1,0 +1,463 @@
+// Dual-tree walk with flat, block-aggregated emission.
+//
+// Same pair semantics as the per-row-slot walk (MAC, accept / near / refine,
+// split rule and child order of _dual_tree_walk_impl,
+// _interactions_impl.py:1379-1454), but instead of one device-scope atomic
+// per list entry, each 256-thread workgroup evaluates a chunk of 1024 pairs,
+// prefix-sums its outputs in LDS and reserves space in three append-only
+// streams (next wavefront, far entries, near entries) with one atomic each.
+// A far entry is (level-major rank of its target row, source node); a near
+// entry is (native leaf row, neighbour leaf node). Rows are formed afterwards
+// by a stable radix sort on the row key, so no per-row counters are needed.
+// Wavefront sizes per generation are exact, which lets the host reproduce
+// the reference's pair-queue capacity decisions.
+#include "ygg_common.h"
+#include "ygg_walk.h"
+
+namespace ygg {
+
+constexpr int kFlatThreads = 256;
+constexpr int kFlatPer = 4;
+constexpr int kFlatChunk = kFlatThreads * kFlatPer;
+
+struct FlatArgs {
+  int32_t num_internal;
+  int32_t mac_type;
+  float theta_sq, theta_root;
+  int64_t queue_cap;
+  int64_t far_cap, near_cap;
+  const NodeRec* nodes;
+  uint64_t* wf0;
+  uint64_t* wf1;
+  uint32_t* gen_sizes;
+  uint32_t* flags;                 // [0] queue, [1] far stream, [2] near stream
+  unsigned long long* counters;    // [0] far total, [1] near total
+  uint32_t* far_keys;
+  uint32_t* far_vals;
+  uint32_t* near_keys;
+  uint32_t* near_vals;
+};
+
+__device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v) {
+  const int lane = (int)lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned long long o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
+  if (atomic_load_agent(&a.flags[0]) | atomic_load_agent(&a.flags[1]) |
+      atomic_load_agent(&a.flags[2]))
+    return;
+  const int64_t raw = a.gen_sizes[g];
+  const int64_t size = raw < a.queue_cap ? raw : a.queue_cap;
+  if ((int64_t)blockIdx.x * kFlatChunk >= size) return;
+  const uint64_t* __restrict__ wf_in = (g & 1) ? a.wf1 : a.wf0;
+  uint64_t* __restrict__ wf_out = (g & 1) ? a.wf0 : a.wf1;
+  const NodeRec* __restrict__ nodes = a.nodes;
+  const int NI = a.num_internal;
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / YGG_WAVE;
+  __shared__ unsigned long long s_wave[kFlatThreads / YGG_WAVE];
+  __shared__ unsigned long long s_base[3];
+
+  for (int64_t chunk = (int64_t)blockIdx.x * kFlatChunk; chunk < size;
+       chunk += (int64_t)gridDim.x * kFlatChunk) {
+    // mode: 0 none, 1 accept, 2 near, 3 both-same, 4 both-cross, 5 split t, 6 split s
+    int mode[kFlatPer];
+    uint32_t pt[kFlatPer], ps[kFlatPer], x0[kFlatPer], x1[kFlatPer], x2[kFlatPer],
+        x3[kFlatPer];
+    unsigned nc = 0, nf = 0, nn = 0;
+#pragma unroll
+    for (int k = 0; k < kFlatPer; ++k) {
+      const int64_t idx = chunk + k * kFlatThreads + threadIdx.x;
+      mode[k] = 0;
+      if (idx >= size) continue;
+      const uint64_t pr = wf_in[idx];
+      const int32_t t = (int32_t)(pr >> 32), s = (int32_t)(uint32_t)pr;
+      const NodeRec nt = nodes[t], ns = nodes[s];
+      float dx = nt.cx - ns.cx, dy = nt.cy - ns.cy, dz = nt.cz - ns.cz;
+      float d2 = dx * dx;
+      d2 = d2 + dy * dy;
+      d2 = d2 + dz * dz;
+      const float et = nt.ext, es = ns.ext;
+      const bool diff = t != s;
+      bool ok;
+      if (a.mac_type == YGG_MAC_ENGBLOM) {
+        float rb = fmaxf(et, es), rs = fminf(et, es);
+        float lhs = rb + a.theta_root * rs;
+        ok = diff && d2 > 0.0f && (lhs * lhs <= a.theta_sq * d2);
+      } else {
+        float sum = et + es;
+        ok = diff && d2 > 0.0f && (sum * sum <= a.theta_sq * d2);
+      }
+      const bool t_int = t < NI, s_int = s < NI;
+      pt[k] = (uint32_t)t;
+      ps[k] = (uint32_t)s;
+      if (ok) {
+        mode[k] = 1;
+        x0[k] = (uint32_t)nt.pad;  // level-major rank of t
+        x1[k] = (uint32_t)ns.pad;
+        nf += 2;
+      } else if (!t_int && !s_int && diff) {
+        mode[k] = 2;
+        x0[k] = (uint32_t)nt.leafpos;
+        x1[k] = (uint32_t)ns.leafpos;
+        nn += 2;
+      } else {
+        const bool same = !diff;
+        const bool split_t = t_int && (same || !s_int || et >= es);
+        const bool split_s = s_int && (same || !t_int || es > et);
+        x0[k] = (uint32_t)nt.left;
+        x1[k] = (uint32_t)nt.right;
+        x2[k] = (uint32_t)ns.left;
+        x3[k] = (uint32_t)ns.right;
+        if (split_t && split_s) {
+          mode[k] = same ? 3 : 4;
+          nc += same ? 3 : 4;
+        } else if (split_t) {
+          mode[k] = 5;
+          nc += 2;
+        } else if (split_s) {
+          mode[k] = 6;
+          nc += 2;
+        }
+      }
+    }
+    // Block exclusive scan of (children, far, near) packed in 21-bit fields.
+    const unsigned long long packed =
+        (unsigned long long)nc | ((unsigned long long)nf << 21) | ((unsigned long long)nn << 42);
+    const unsigned long long incl = wave_incl_u64(packed);
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    unsigned long long wbase = 0, btot = 0;
+#pragma unroll
+    for (int q = 0; q < kFlatThreads / YGG_WAVE; ++q) {
+      if (q < w) wbase += s_wave[q];
+      btot += s_wave[q];
+    }
+    const unsigned long long mask = (1ull << 21) - 1;
+    if (threadIdx.x == 0) {
+      const unsigned long long tc = btot & mask, tf = (btot >> 21) & mask,
+                               tn = (btot >> 42) & mask;
+      s_base[0] = tc ? atomicAdd(&a.gen_sizes[g + 1], (uint32_t)tc) : 0;
+      s_base[1] = tf ? atomicAdd(&a.counters[0], tf) : 0;
+      s_base[2] = tn ? atomicAdd(&a.counters[1], tn) : 0;
+    }
+    __syncthreads();
+    const unsigned long long ex = wbase + incl - packed;
+    int64_t cslot = (int64_t)s_base[0] + (int64_t)(ex & mask);
+    int64_t fslot = (int64_t)s_base[1] + (int64_t)((ex >> 21) & mask);
+    int64_t nslot = (int64_t)s_base[2] + (int64_t)((ex >> 42) & mask);
+    bool qover = false, fover = false, nover = false;
+#pragma unroll
+    for (int k = 0; k < kFlatPer; ++k) {
+      const int m = mode[k];
+      if (m == 0) continue;
+      if (m == 1) {
+        if (EMIT) {
+          if (fslot + 1 < a.far_cap) {
+            a.far_keys[fslot] = x0[k];
+            a.far_vals[fslot] = ps[k];
+            a.far_keys[fslot + 1] = x1[k];
+            a.far_vals[fslot + 1] = pt[k];
+          } else {
+            fover = true;
+          }
+        }
+        fslot += 2;
+      } else if (m == 2) {
+        if (EMIT) {
+          if (nslot + 1 < a.near_cap) {
+            a.near_keys[nslot] = x0[k];
+            a.near_vals[nslot] = ps[k];
+            a.near_keys[nslot + 1] = x1[k];
+            a.near_vals[nslot + 1] = pt[k];
+          } else {
+            nover = true;
+          }
+        }
+        nslot += 2;
+      } else {
+        uint32_t ca[4], cb[4];
+        int cnt;
+        if (m == 3) {
+          ca[0] = x0[k]; cb[0] = x0[k];
+          ca[1] = x0[k]; cb[1] = x1[k];
+          ca[2] = x1[k]; cb[2] = x1[k];
+          cnt = 3;
+        } else if (m == 4) {
+          ca[0] = x0[k]; cb[0] = x2[k];
+          ca[1] = x0[k]; cb[1] = x3[k];
+          ca[2] = x1[k]; cb[2] = x2[k];
+          ca[3] = x1[k]; cb[3] = x3[k];
+          cnt = 4;
+        } else if (m == 5) {
+          ca[0] = x0[k]; cb[0] = ps[k];
+          ca[1] = x1[k]; cb[1] = ps[k];
+          cnt = 2;
+        } else {
+          ca[0] = pt[k]; cb[0] = x2[k];
+          ca[1] = pt[k]; cb[1] = x3[k];
+          cnt = 2;
+        }
+        for (int j = 0; j < cnt; ++j, ++cslot) {
+          uint32_t lo = ca[j] < cb[j] ? ca[j] : cb[j], hi = ca[j] < cb[j] ? cb[j] : ca[j];
+          if (cslot < a.queue_cap)
+            wf_out[cslot] = ((uint64_t)lo << 32) | hi;
+          else
+            qover = true;
+        }
+      }
+    }
+    if (qover) atomicOr(&a.flags[0], 1u);
+    if (fover) atomicOr(&a.flags[1], 1u);
+    if (nover) atomicOr(&a.flags[2], 1u);
+    __syncthreads();  // s_wave / s_base reuse
+  }
+}
+
+__global__ void k_flat_init(FlatArgs a, int root, int G) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = i; k <= G; k += gridDim.x * blockDim.x) a.gen_sizes[k] = 0;
+  if (i < 4) a.flags[i] = 0;
+  if (i < 5) a.counters[i] = 0;
+  if (i == 0) {
+    a.gen_sizes[0] = 1;
+    a.wf0[0] = ((uint64_t)(uint32_t)root << 32) | (uint32_t)root;
+  }
+}
+
+// offsets[r] = lower_bound(keys, r) for r in [0, rows], keys sorted ascending.
+__global__ void k_row_offsets(const uint32_t* __restrict__ keys, int64_t n, int rows,
+                              int64_t* __restrict__ off) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > rows) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < (uint32_t)r)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  off[r] = lo;
+}
+
+template <typename OutT>
+__global__ void k_far_from_sorted(const uint32_t* __restrict__ keys,
+                                  const uint32_t* __restrict__ vals, int64_t n,
+                                  const int32_t* __restrict__ nbl,
+                                  const int32_t* __restrict__ node_level,
+                                  OutT* __restrict__ sources, OutT* __restrict__ targets,
+                                  OutT* __restrict__ levels) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t node = nbl[keys[i]];
+    sources[i] = (OutT)vals[i];
+    targets[i] = (OutT)node;
+    levels[i] = (OutT)node_level[node];
+  }
+}
+
+template <typename OutT>
+__global__ void k_far_rows(const int64_t* __restrict__ off, int T,
+                           const int32_t* __restrict__ nbl, OutT* __restrict__ counts,
+                           OutT* __restrict__ node_offsets, int32_t* __restrict__ counts32) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= T) return;
+  int32_t node = nbl[r];
+  int64_t c = off[r + 1] - off[r];
+  counts[node] = (OutT)c;
+  counts32[node] = (int32_t)c;
+  node_offsets[node] = (OutT)off[r];
+  if (r == 0) node_offsets[T] = (OutT)off[T];
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(256) void k_near_from_sorted(
+    int L, const int64_t* __restrict__ off, const uint32_t* __restrict__ vals,
+    OutT* __restrict__ neighbors, OutT* __restrict__ counts, OutT* __restrict__ offsets_out) {
+  int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE;
+  const unsigned lane = lane_id();
+  if (r >= L) return;
+  const int64_t o = off[r];
+  const int64_t c = off[r + 1] - o;
+  if (lane == 0) {
+    counts[r] = (OutT)c;
+    offsets_out[r] = (OutT)o;
+    if (r == L - 1) offsets_out[L] = (OutT)off[L];
+  }
+  for (int64_t e = lane; e < c; e += YGG_WAVE) neighbors[o + e] = (OutT)vals[o + e];
+}
+
+// Dense [L][maxc] neighbour leaf rows, -1 padded: one thread per element,
+// contiguous over the whole matrix.
+template <typename OutT>
+__global__ __launch_bounds__(256) void k_near_dense(int L, const int64_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ vals,
+                                                    const int32_t* __restrict__ leaf_position,
+                                                    int64_t maxc, OutT* __restrict__ nlp) {
+  const int64_t total = (int64_t)L * maxc;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / maxc, e = i - r * maxc;
+    const int64_t o = off[r];
+    OutT v = (OutT)-1;
+    if (o + e < off[r + 1]) v = (OutT)leaf_position[vals[o + e]];
+    __builtin_nontemporal_store(v, &nlp[i]);
+  }
+}
+
+}  // namespace ygg
+
+using namespace ygg;
+
+extern "C" {
+
+static hipEvent_t g_flat_events[2 * 1025];
+static int g_flat_events_ready = 0;
+
+int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* num_generations,
+                  float* gen_ms, void* stream) {
+  if (!p || !host_status || !num_generations) return YGG_ERR_ARG;
+  if (gen_ms && p->max_generations > 1024) return YGG_ERR_ARG;
+  if (gen_ms && !g_flat_events_ready) {
+    for (int k = 0; k < 2 * 1025; ++k) YGG_CHECK(hipEventCreate(&g_flat_events[k]));
+    g_flat_events_ready = 1;
+  }
+  if (p->num_internal < 1 || p->queue_cap < 4 || p->max_generations < 2) return YGG_ERR_ARG;
+  if (p->emit && (!p->far_keys || !p->far_vals || !p->near_keys || !p->near_vals))
+    return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  FlatArgs a;
+  a.num_internal = p->num_internal;
+  a.mac_type = p->mac_type;
+  a.theta_sq = p->theta_sq;
+  a.theta_root = p->theta_root;
+  a.queue_cap = p->queue_cap;
+  a.far_cap = p->far_cap;
+  a.near_cap = p->near_cap;
+  a.nodes = (const NodeRec*)p->nodes;
+  a.wf0 = p->wf0;
+  a.wf1 = p->wf1;
+  a.gen_sizes = p->gen_sizes;
+  a.flags = p->flags;
+  a.counters = (unsigned long long*)p->counters;
+  a.far_keys = p->far_keys;
+  a.far_vals = p->far_vals;
+  a.near_keys = p->near_keys;
+  a.near_vals = p->near_vals;
+  const int G = p->max_generations;
+  hipLaunchKernelGGL(k_flat_init, dim3(grid_for(G + 1, 256)), dim3(256), 0, st, a, p->root, G);
+  YGG_LAUNCH_CHECK();
+  const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
+  int g = 0, batch = p->gen_hint > 0 ? p->gen_hint : 24;
+  while (true) {
+    int end = g + batch < G ? g + batch : G;
+    for (int k = g; k < end; ++k) {
+      if (gen_ms) YGG_CHECK(hipEventRecord(g_flat_events[2 * k], st));
+      if (p->emit)
+        hipLaunchKernelGGL(k_walk_flat<true>, dim3(blocks), dim3(256), 0, st, a, k);
+      else
+        hipLaunchKernelGGL(k_walk_flat<false>, dim3(blocks), dim3(256), 0, st, a, k);
+      if (gen_ms) YGG_CHECK(hipEventRecord(g_flat_events[2 * k + 1], st));
+    }
+    YGG_LAUNCH_CHECK();
+    YGG_CHECK(hipMemcpyAsync(host_status, p->gen_sizes, sizeof(uint32_t) * (G + 1),
+                             hipMemcpyDeviceToHost, st));
+    YGG_CHECK(hipMemcpyAsync(host_status + G + 1, p->flags, sizeof(uint32_t) * 4,
+                             hipMemcpyDeviceToHost, st));
+    YGG_CHECK(hipStreamSynchronize(st));
+    const uint32_t* fl = host_status + G + 1;
+    int done = -1;
+    for (int k = 0; k <= end; ++k)
+      if (host_status[k] == 0) {
+        done = k;
+        break;
+      }
+    if ((fl[0] | fl[1] | fl[2]) || done >= 0) {
+      *num_generations = done >= 0 ? done : end;
+      if (gen_ms) {
+        for (int k = 0; k < G; ++k) gen_ms[k] = 0.f;
+        for (int k = 0; k < end; ++k)
+          YGG_CHECK(hipEventElapsedTime(&gen_ms[k], g_flat_events[2 * k],
+                                        g_flat_events[2 * k + 1]));
+      }
+      return YGG_OK;
+    }
+    if (end >= G) return YGG_ERR_UNSUPPORTED;
+    g = end;
+    batch = 8;
+  }
+}
+
+int ygg_row_offsets(const uint32_t* sorted_keys, int64_t n, int64_t rows, int64_t* offsets,
+                    void* stream) {
+  if (rows < 0 || n < 0) return YGG_ERR_ARG;
+  hipLaunchKernelGGL(k_row_offsets, dim3(grid_for(rows + 1, 256)), dim3(256), 0,
+                     (hipStream_t)stream, sorted_keys, n, (int)rows, offsets);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_far_lists_from_sorted(int64_t num_nodes, const uint32_t* keys, const uint32_t* vals,
+                              int64_t n, const int64_t* lm_offsets, const int32_t* nodes_by_level,
+                              const int32_t* node_level, int out_i64, void* sources,
+                              void* targets, void* target_levels, void* counts,
+                              void* node_offsets, int32_t* counts32, void* stream) {
+  if (num_nodes < 1 || n < 0) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  int T = (int)num_nodes;
+  dim3 ge(grid_for(n > 0 ? n : 1, 256, 16384)), gr(grid_for(T, 256));
+  if (out_i64) {
+    if (n > 0)
+      hipLaunchKernelGGL(k_far_from_sorted<int64_t>, ge, dim3(256), 0, st, keys, vals, n,
+                         nodes_by_level, node_level, (int64_t*)sources, (int64_t*)targets,
+                         (int64_t*)target_levels);
+    hipLaunchKernelGGL(k_far_rows<int64_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
+                       (int64_t*)counts, (int64_t*)node_offsets, counts32);
+  } else {
+    if (n > 0)
+      hipLaunchKernelGGL(k_far_from_sorted<int32_t>, ge, dim3(256), 0, st, keys, vals, n,
+                         nodes_by_level, node_level, (int32_t*)sources, (int32_t*)targets,
+                         (int32_t*)target_levels);
+    hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
+                       (int32_t*)counts, (int32_t*)node_offsets, counts32);
+  }
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
+                               const uint32_t* vals, const int32_t* leaf_position,
+                               int64_t max_count, int out_i64, void* neighbors,
+                               void* neighbor_leaf_positions, void* counts, void* offsets,
+                               void* stream) {
+  if (num_leaves < 1) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  int L = (int)num_leaves;
+  dim3 grid(grid_for((int64_t)L * YGG_WAVE, 256));
+  dim3 gd(grid_for(L * max_count > 0 ? (int64_t)L * max_count : 1, 256, 32768));
+  if (out_i64) {
+    hipLaunchKernelGGL(k_near_from_sorted<int64_t>, grid, dim3(256), 0, st, L, row_offsets, vals,
+                       (int64_t*)neighbors, (int64_t*)counts, (int64_t*)offsets);
+    if (neighbor_leaf_positions && max_count > 0)
+      hipLaunchKernelGGL(k_near_dense<int64_t>, gd, dim3(256), 0, st, L, row_offsets, vals,
+                         leaf_position, max_count, (int64_t*)neighbor_leaf_positions);
+  } else {
+    hipLaunchKernelGGL(k_near_from_sorted<int32_t>, grid, dim3(256), 0, st, L, row_offsets, vals,
+                       (int32_t*)neighbors, (int32_t*)counts, (int32_t*)offsets);
+    if (neighbor_leaf_positions && max_count > 0)
+      hipLaunchKernelGGL(k_near_dense<int32_t>, gd, dim3(256), 0, st, L, row_offsets, vals,
+                         leaf_position, max_count, (int32_t*)neighbor_leaf_positions);
+  }
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+}  // extern "C"

@@ -46,7 +46,8 @@ def compute_tree_geometry(tree, positions_sorted, *, max_leaf_size=None,
     right = dt.right if dt.right.numel() else None
     _lib.check(lib.ygg_geometry(
         pos.data_ptr(), dt.node_ranges.data_ptr(), _lib.ptr(left), _lib.ptr(right),
-        dt.parent.data_ptr(), dt.node_level.data_ptr(), T, int(cap), int(dt.malformed),
+        dt.parent.data_ptr(), dt.node_level.data_ptr(), T, int(dt.leaf_size), int(cap),
+        int(dt.malformed),
         int(dt.max_level), center.data_ptr(), half.data_ptr(), radius.data_ptr(),
         maxext.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr(dev)), "geometry")
     return TreeGeometry(center, half, radius, maxext)

@@ -34,6 +34,10 @@ MAX_GENERATIONS = 1024
 # Opt-in per-generation kernel timing (used by the benchmark's roofline leg).
 WALK_TIMING = {"enabled": False, "log": []}
 MAC_TYPES = {"bh": 0, "dehnen": 1, "engblom": 2}
+FLAT_GRID_BLOCKS = 2048
+# Forces the attempt-by-attempt walk with per-row slots (the reference's
+# fixed-capacity method) instead of the flat walk plus ladder emulation.
+EXACT_ATTEMPTS = os.environ.get("YGGDRAX_EXACT_ATTEMPTS", "0") == "1"
 NODE_RECORD_BYTES = 32
 
 
@@ -237,7 +241,7 @@ class _WalkContext:
         _lib.check(self.lib.ygg_walk_prepare_nodes(
             g.center.contiguous().data_ptr(), g.radius.contiguous().data_ptr(),
             g.max_extent.contiguous().data_ptr(), _lib.ptr(left), _lib.ptr(right),
-            dt.leaf_position.data_ptr(), dt.num_nodes, self.mac,
+            dt.leaf_position.data_ptr(), dt.nodes_by_level.data_ptr(), dt.num_nodes, self.mac,
             ctypes.c_float(float(dehnen_radius_scale)).value, self.nodes.data_ptr(),
             self.stream), "walk_prepare_nodes")
         self.host_status = torch.zeros(MAX_GENERATIONS + 8, dtype=torch.int32, pin_memory=True)
@@ -245,7 +249,59 @@ class _WalkContext:
         self.flags = torch.empty(4, dtype=torch.int32, device=self.dev)
         self.decisions = torch.empty(3, dtype=torch.int64, device=self.dev)
         self.timing = WALK_TIMING["enabled"]
+        self.gen_hint = int(1.5 * max(1, dt.num_levels)) + 4
         self.kernel_log = WALK_TIMING["log"]
+
+    def run_flat(self, *, emit: bool, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
+                 gen_hint: int = 0):
+        """One flat-emission walk over the whole tree (ygg_walk_flat)."""
+        dt = self.dt
+        dev = self.dev
+        Q = max(4, int(queue_cap))
+        wf = torch.empty((2, Q), dtype=torch.int64, device=dev)
+        counters = torch.empty(2, dtype=torch.int64, device=dev)
+        fk = fv = nk = nv = None
+        if emit:
+            fk = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
+            fv = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
+            nk = torch.empty(max(1, near_cap), dtype=torch.int32, device=dev)
+            nv = torch.empty(max(1, near_cap), dtype=torch.int32, device=dev)
+        p = _lib.FlatWalkParams()
+        p.num_nodes, p.num_internal, p.num_leaves, p.root = (dt.num_nodes, dt.num_internal,
+                                                             dt.num_leaves, 0)
+        p.mac_type, p.emit, p.max_generations = self.mac, int(emit), MAX_GENERATIONS
+        p.grid_blocks = FLAT_GRID_BLOCKS
+        p.gen_hint = int(gen_hint) if gen_hint else int(self.gen_hint)
+        p.theta_sq, p.theta_root = self.theta_sq, self.theta_root
+        p.queue_cap, p.far_cap, p.near_cap = Q, int(far_cap), int(near_cap)
+        p.nodes = self.nodes.data_ptr()
+        p.wf0, p.wf1 = wf[0].data_ptr(), wf[1].data_ptr()
+        p.gen_sizes, p.flags = self.gen_sizes.data_ptr(), self.flags.data_ptr()
+        p.counters = counters.data_ptr()
+        p.far_keys, p.far_vals = _lib.ptr(fk), _lib.ptr(fv)
+        p.near_keys, p.near_vals = _lib.ptr(nk), _lib.ptr(nv)
+        ngen = ctypes.c_int32(0)
+        gen_ms = (ctypes.c_float * MAX_GENERATIONS)() if self.timing else None
+        _lib.check(self.lib.ygg_walk_flat(ctypes.byref(p), self.host_status.data_ptr(),
+                                          ctypes.byref(ngen), gen_ms, self.stream), "walk_flat")
+        hs = self.host_status
+        G = MAX_GENERATIONS
+        raw_gens = [int(hs[k]) & 0xFFFFFFFF for k in range(min(ngen.value + 2, G + 1))]
+        c = [int(v) for v in counters.cpu()]
+        # every evaluated pair is exactly one of accept (2 far entries),
+        # near (2 near entries) or refine
+        pairs = sum(min(v, Q) for v in raw_gens)
+        dec = (c[0] // 2, c[1] // 2, pairs - c[0] // 2 - c[1] // 2)
+        self.gen_hint = ngen.value + 1
+        if gen_ms is not None:
+            self.kernel_log.append(dict(kernel="k_walk_flat", emit=bool(emit),
+                                        gen_ms=list(gen_ms)[: ngen.value], decisions=dec,
+                                        gen_sizes=raw_gens, far_total=c[0], near_total=c[1]))
+        return dict(far_total=c[0], near_total=c[1], decisions=dec,
+                    gen_sizes=raw_gens, peak=max(raw_gens + [1]),
+                    queue_overflow=bool(int(hs[G + 1])), far_overflow=bool(int(hs[G + 2])),
+                    near_overflow=bool(int(hs[G + 3])), far_keys=fk, far_vals=fv,
+                    near_keys=nk, near_vals=nv, queue_capacity=Q)
 
     def run(self, *, mode: int, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
             far_row_start=None, far_row_cap=None, near_row_start=None, near_row_cap=None,
@@ -342,26 +398,26 @@ def _emit(retry_logger, status, attempt, q, k, res: _RawWalk):
         logger.exception("retry_logger raised", exc_info=True)
 
 
-def _run_walk_raw(tree, geometry: TreeGeometry, theta: float, mac_type: str, *,
-                  max_interactions_per_node, max_neighbors_per_leaf, max_pair_queue,
-                  traversal_config, dehnen_radius_scale, process_block, retry_logger,
-                  collect_far=True, collect_near=True) -> Optional[_RawWalk]:
-    """_run_dual_tree_walk_raw (3840-4440) over the GPU walk."""
-    dt = device_topology(tree)
-    if float(dehnen_radius_scale) <= 0.0:
-        raise ValueError("dehnen_radius_scale must be > 0")
+class _Ladder(NamedTuple):
+    queue: list
+    inter: list
+    nbr: list
+    user_q: bool
+    cache_key: Optional[tuple]
+    use_count: bool
+
+
+def _ladder(dt: DeviceTopology, mac_type, *, max_interactions_per_node, max_neighbors_per_leaf,
+            max_pair_queue, traversal_config, collect_far=True, collect_near=True) -> _Ladder:
+    """Capacity candidates of _run_dual_tree_walk_raw (3917-4003)."""
     T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
-    if NI == 0:
-        return None
     config = traversal_config if traversal_config is not None else _GLOBAL_DUAL_TREE_CONFIG
-    ctx = _WalkContext(dt, geometry, theta, mac_type, dehnen_radius_scale)
     if config is not None:
         qmax = max(4, int(config.max_pair_queue))
         inter = [int(config.max_interactions_per_node)]
         nlimit = int(config.max_neighbors_per_leaf)
         nbr = _neighbor_capacity_candidates(nlimit, num_leaves=L)
         pov = int(config.process_block)
-        user_q = True
         cache_key = (T, NI, inter[0], nlimit, qmax, pov, bool(collect_far), bool(collect_near),
                      str(mac_type))
         cached = _get_cached_queue_capacity(cache_key)
@@ -370,18 +426,37 @@ def _run_walk_raw(tree, geometry: TreeGeometry, theta: float, mac_type: str, *,
         for cap in _auto_pair_queue_candidates(T, NI):
             if cap > qc[-1]:
                 qc.append(cap)
-        qc = list(dict.fromkeys(qc))
-    else:
-        cache_key = None
-        user_q = max_pair_queue is not None
-        qc = [max(4, int(max_pair_queue))] if user_q else _auto_pair_queue_candidates(T, NI)
-        inter, _ = _interaction_capacity_candidates(max_interactions_per_node, T)
-        nbr = _neighbor_capacity_candidates(max_neighbors_per_leaf, num_leaves=L)
+        return _Ladder(list(dict.fromkeys(qc)), inter, nbr, True, cache_key, False)
+    user_q = max_pair_queue is not None
+    qc = [max(4, int(max_pair_queue))] if user_q else _auto_pair_queue_candidates(T, NI)
+    inter, _ = _interaction_capacity_candidates(max_interactions_per_node, T)
+    nbr = _neighbor_capacity_candidates(max_neighbors_per_leaf, num_leaves=L)
     use_count = (max_interactions_per_node is None and max_pair_queue is None
                  and traversal_config is None)
-    qerr = QUEUE_ERR_USER if user_q else QUEUE_ERR_AUTO
+    return _Ladder(qc, inter, nbr, user_q, None, use_count)
 
-    if use_count:
+
+def _run_walk_raw(tree, geometry: TreeGeometry, theta: float, mac_type: str, *,
+                  max_interactions_per_node, max_neighbors_per_leaf, max_pair_queue,
+                  traversal_config, dehnen_radius_scale, process_block, retry_logger,
+                  collect_far=True, collect_near=True, ctx=None) -> Optional[_RawWalk]:
+    """_run_dual_tree_walk_raw (3840-4440), attempt by attempt, over the GPU walk
+    with per-row slots (the reference's fixed-capacity method)."""
+    dt = device_topology(tree)
+    if float(dehnen_radius_scale) <= 0.0:
+        raise ValueError("dehnen_radius_scale must be > 0")
+    T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
+    if NI == 0:
+        return None
+    if ctx is None:
+        ctx = _WalkContext(dt, geometry, theta, mac_type, dehnen_radius_scale)
+    lad = _ladder(dt, mac_type, max_interactions_per_node=max_interactions_per_node,
+                  max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
+                  traversal_config=traversal_config, collect_far=collect_far,
+                  collect_near=collect_near)
+    qc, inter, nbr, cache_key = lad.queue, lad.inter, lad.nbr, lad.cache_key
+    qerr = QUEUE_ERR_USER if lad.user_q else QUEUE_ERR_AUTO
+    if lad.use_count:
         return _count_then_fill(ctx, dt, retry_logger, collect_far, collect_near)
 
     result: Optional[_RawWalk] = None
@@ -428,6 +503,166 @@ def _run_walk_raw(tree, geometry: TreeGeometry, theta: float, mac_type: str, *,
     if collect_near and result.outcome.near_overflow:
         raise RuntimeError(NEAR_ERR)
     return result
+
+
+# ---- flat-emission path -----------------------------------------------------
+
+class _FlatResult(NamedTuple):
+    far_keys: torch.Tensor     # sorted level-major row keys (uint32 as int32)
+    far_vals: torch.Tensor
+    near_vals: torch.Tensor    # sorted by native leaf row
+    far_off: torch.Tensor      # int64 [T + 1] level-major row offsets
+    near_off: torch.Tensor     # int64 [L + 1]
+    stats: dict
+
+
+def _sort_entries(keys, vals, n, rows):
+    """Stable radix sort of (row key, value) pairs by row key."""
+    lib = _lib.load()
+    dev = keys.device
+    bits = max(1, int(rows).bit_length())
+    end_bit = ((bits + 7) // 8) * 8
+    ko = torch.empty(max(1, n), dtype=torch.int32, device=dev)
+    vo = torch.empty(max(1, n), dtype=torch.int32, device=dev)
+    if n > 0:
+        ws = _lib.workspace(lib.ygg_sort_workspace_bytes(n, 4, 0, end_bit), dev)
+        _lib.check(lib.ygg_sort_pairs_u32(keys.data_ptr(), vals.data_ptr(), ko.data_ptr(),
+                                          vo.data_ptr(), n, 0, end_bit, ws.data_ptr(), ws.numel(),
+                                          _lib.stream_ptr(dev)), "sort entries")
+    off = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    _lib.check(lib.ygg_row_offsets(ko.data_ptr(), n, rows, off.data_ptr(), _lib.stream_ptr(dev)),
+               "row offsets")
+    return ko, vo, off
+
+
+def _emulate_ladder(lad: _Ladder, peak: int, max_far: int, max_near: int):
+    """Outcome of the reference's retry loop from full-walk statistics.
+
+    Returns the queue capacity of the successful attempt, raises the
+    reference's error, or returns None when the outcome depends on the
+    generation at which two overflows happen (then the attempt-by-attempt
+    walk decides)."""
+    success_q = None
+    for Q in lad.queue:
+        q_over = peak > Q
+        for K in lad.inter:
+            f_over = max_far > K
+            if q_over and f_over:
+                return None
+            if f_over:
+                raise RuntimeError(FAR_ERR)
+            for Kn in lad.nbr:
+                n_over = max_near > Kn
+                if q_over or n_over:
+                    continue
+                success_q = Q
+                break
+            if success_q is not None:
+                break
+        if success_q is not None:
+            break
+    if success_q is None:
+        if peak > lad.queue[-1]:
+            raise RuntimeError(QUEUE_ERR_USER if lad.user_q else QUEUE_ERR_AUTO)
+        raise RuntimeError(NEAR_ERR)
+    if lad.cache_key is not None:
+        _store_cached_queue_capacity(lad.cache_key, success_q)
+    return success_q
+
+
+def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
+                   sizes: Optional[dict] = None, strict_int32: bool = True
+                   ) -> Optional[_FlatResult]:
+    """Full walk with flat emission. `sizes` (queue, far, near capacities)
+    skips the counting walk; on any stream overflow the counting walk runs
+    and the emission walk is repeated at exact sizes."""
+    T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
+    qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
+    em = None
+    if sizes is not None:
+        em = ctx.run_flat(emit=True, queue_cap=sizes["queue"], far_cap=sizes["far"],
+                          near_cap=sizes["near"])
+        if em["queue_overflow"] or em["far_overflow"] or em["near_overflow"]:
+            em = None
+    if em is None:
+        tot = ctx.run_flat(emit=False, queue_cap=qbig)
+        if tot["queue_overflow"]:
+            if lad.use_count:
+                raise RuntimeError("Count-pass traversal wavefront overflowed its conservative "
+                                   "buffer.")
+            return None
+        if (strict_int32 and lad.use_count
+                and max(tot["far_total"], tot["near_total"]) > (1 << 31) - 1):
+            raise RuntimeError("Dual-tree traversal pair totals exceed signed int32 capacity; "
+                               "refine the tree or use a lower-memory traversal strategy.")
+        em = ctx.run_flat(emit=True, queue_cap=tot["peak"], far_cap=tot["far_total"],
+                          near_cap=tot["near_total"])
+        assert not (em["queue_overflow"] or em["far_overflow"] or em["near_overflow"])
+    nf, nn = em["far_total"], em["near_total"]
+    fk, fv, far_off = _sort_entries(em["far_keys"], em["far_vals"], nf, T)
+    _, nv, near_off = _sort_entries(em["near_keys"], em["near_vals"], nn, L)
+    mx = torch.stack([torch.diff(far_off).max(), torch.diff(near_off).max()]).cpu()
+    max_far, max_near = int(mx[0]), int(mx[1])
+    stats = dict(peak=em["peak"], gen_sizes=em["gen_sizes"], decisions=em["decisions"],
+                 max_far=max_far, max_near=max_near, far_total=nf, near_total=nn)
+    if lad.use_count:
+        if strict_int32 and max_near > max(1, ((1 << 31) - 1) // max(1, L)):
+            raise RuntimeError(
+                "Suggested per-leaf neighbor capacity is too large to safely allocate on the "
+                "host (would overflow int32). Enable local refinement or pass explicit "
+                "capacities (max_neighbors_per_leaf / max_interactions_per_node).")
+    else:
+        q = _emulate_ladder(lad, em["peak"], max_far, max_near)
+        if q is None:
+            return None
+        stats["queue_capacity"] = q
+    return _FlatResult(fk, fv, nv, far_off, near_off, stats)
+
+
+def _flat_outputs(fr: _FlatResult, dt: DeviceTopology, topo, neighbor_positions: bool = True):
+    lib = _lib.load()
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    T, L = dt.num_nodes, dt.num_leaves
+    i64 = int(idx == torch.int64)
+    st = _lib.stream_ptr(dev)
+    nf = fr.stats["far_total"]
+    nn = fr.stats["near_total"]
+    sources = torch.empty(nf, dtype=idx, device=dev)
+    targets = torch.empty(nf, dtype=idx, device=dev)
+    levels = torch.empty(nf, dtype=idx, device=dev)
+    counts = torch.empty(T, dtype=idx, device=dev)
+    counts32 = torch.empty(T, dtype=torch.int32, device=dev)
+    node_off = torch.empty(T + 1, dtype=idx, device=dev)
+    _lib.check(lib.ygg_far_lists_from_sorted(
+        T, fr.far_keys.data_ptr(), fr.far_vals.data_ptr(), nf, fr.far_off.data_ptr(),
+        dt.nodes_by_level.data_ptr(), dt.node_level.data_ptr(), i64, sources.data_ptr(),
+        targets.data_ptr(), levels.data_ptr(), counts.data_ptr(), node_off.data_ptr(),
+        counts32.data_ptr(), st), "far lists")
+    nl = int(topo.num_levels)
+    level_offsets = fr.far_off[dt.level_offsets[: nl + 1].long()].to(idx)
+    if nf == 0:
+        node_off = torch.zeros(T + 1, dtype=idx, device=dev)
+        level_offsets = torch.zeros(nl + 1, dtype=idx, device=dev)
+    inter = NodeInteractionList(node_off, sources, targets, counts, level_offsets, levels)
+    maxc = fr.stats["max_near"] if neighbor_positions else 0
+    neighbors = torch.empty(nn, dtype=idx, device=dev)
+    nlp = torch.empty((L, maxc), dtype=idx, device=dev)
+    ncounts = torch.empty(L, dtype=idx, device=dev)
+    noff = torch.empty(L + 1, dtype=idx, device=dev)
+    _lib.check(lib.ygg_near_lists_from_sorted(
+        L, fr.near_off.data_ptr(), fr.near_vals.data_ptr(), dt.leaf_position.data_ptr(), maxc,
+        i64, neighbors.data_ptr(), nlp.data_ptr() if maxc > 0 else None, ncounts.data_ptr(),
+        noff.data_ptr(), st), "near lists")
+    leaf_indices, po, p2n = _leaf_tables(dt)
+    nbrs = NodeNeighborList(
+        offsets=noff, neighbors=neighbors, leaf_indices=leaf_indices, counts=ncounts,
+        particle_order_leaf_indices=po, particle_order_to_native_leaf=p2n,
+        neighbor_leaf_positions=nlp, target_block_leaf_ids=torch.zeros(0, dtype=idx, device=dev),
+        target_block_source_leaf_ids=torch.zeros((0, 0), dtype=idx, device=dev),
+        target_block_valid_mask=torch.zeros((0, 0), dtype=torch.bool, device=dev),
+        target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev), target_block_size=0)
+    return inter, nbrs
 
 
 def _count_then_fill(ctx: _WalkContext, dt: DeviceTopology, retry_logger, collect_far,
@@ -601,7 +836,12 @@ def _to_result(raw: Optional[_RawWalk], dt: DeviceTopology, inter: NodeInteracti
     counts = inter.counts
     node_off = torch.zeros(T + 1, dtype=idx, device=dev)
     node_off[1:] = torch.cumsum(counts, 0)
-    o = raw.outcome if raw is not None else None
+    if isinstance(raw, _FlatResult):
+        dec = raw.stats["decisions"]
+        o = WalkOutcome(None, None, None, None, (0, None), (0, None), raw.stats["gen_sizes"],
+                        False, False, False, dec, raw.stats.get("queue_capacity", 0))
+    else:
+        o = raw.outcome if raw is not None else None
     return DualTreeWalkResult(
         interaction_offsets=node_off, interaction_sources=inter.sources,
         interaction_targets=inter.targets,
@@ -627,9 +867,21 @@ def build_interactions_and_neighbors(
     mac_type: str = "bh", dehnen_radius_scale: float = 1.0, pair_policy=None,
     policy_state: object = None, *, return_result: bool = False,
     return_compact_far_pairs: bool = False, return_interactions: bool = True,
-    return_grouped: bool = False):
+    return_grouped: bool = False, capacity_hint: Optional[dict] = None,
+    neighbor_positions: bool = True, strict_int32: bool = True):
     """Far-field interaction lists and near-field neighbour lists from one
-    dual-tree walk (yggdrax/interactions.py:338-448)."""
+    dual-tree walk (yggdrax/interactions.py:338-448).
+
+    ``capacity_hint`` ({"queue", "far", "near"} stream capacities, e.g. from
+    the learned capacity model) lets the walk size its buffers in one pass;
+    an overflow falls back to counting first, so outputs never depend on it.
+
+    ``neighbor_positions=False`` leaves NodeNeighborList.neighbor_leaf_positions
+    as an [L, 0] array instead of the dense -1 padded [L, max_count] matrix,
+    and ``strict_int32=False`` drops the reference's host-allocation guards
+    (per-leaf neighbour count <= (2^31 - 1) / L, totals <= 2^31 - 1) of the
+    count-pass path, which exist for its dense (L x capacity) buffers; the
+    CSR lists need neither. Both default to the reference's behaviour."""
     if pair_policy is not None:
         from .policy import build_with_pair_policy
         return build_with_pair_policy(
@@ -643,14 +895,31 @@ def build_interactions_and_neighbors(
         raise NotImplementedError("compact far pairs are not part of this build")
     dt = device_topology(tree)
     topo = resolve_tree_topology(tree)
-    raw = _run_walk_raw(tree, geometry, theta, mac_type,
-                        max_interactions_per_node=max_interactions_per_node,
-                        max_neighbors_per_leaf=max_neighbors_per_leaf,
-                        max_pair_queue=max_pair_queue, traversal_config=traversal_config,
-                        dehnen_radius_scale=dehnen_radius_scale, process_block=process_block,
-                        retry_logger=retry_logger)
-    inter = _to_interactions(raw, dt, topo) if raw is not None else _empty_outputs(dt, topo)
-    nbrs = _to_neighbors(raw, dt)
+    if _target_block_size() > 0:
+        raise NotImplementedError("YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE > 0 is not supported")
+    if float(dehnen_radius_scale) <= 0.0:
+        raise ValueError("dehnen_radius_scale must be > 0")
+    kw = dict(max_interactions_per_node=max_interactions_per_node,
+              max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
+              traversal_config=traversal_config)
+    if dt.num_internal == 0:
+        inter, nbrs, raw = _empty_outputs(dt, topo), _to_neighbors(None, dt), None
+    else:
+        ctx = _WalkContext(dt, geometry, theta, mac_type, dehnen_radius_scale)
+        fr = None
+        if retry_logger is None and not EXACT_ATTEMPTS:
+            fr = _run_walk_flat(ctx, dt, _ladder(dt, mac_type, **kw), sizes=capacity_hint,
+                                strict_int32=strict_int32)
+        if fr is not None:
+            inter, nbrs = _flat_outputs(fr, dt, topo, neighbor_positions)
+            raw = fr
+        else:
+            raw = _run_walk_raw(tree, geometry, theta, mac_type,
+                                dehnen_radius_scale=dehnen_radius_scale,
+                                process_block=process_block, retry_logger=retry_logger, ctx=ctx,
+                                **kw)
+            inter = _to_interactions(raw, dt, topo)
+            nbrs = _to_neighbors(raw, dt)
     out_inter = inter if return_interactions else None
     if return_result:
         return out_inter, nbrs, _to_result(raw, dt, inter, nbrs)
