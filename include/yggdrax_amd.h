@@ -269,6 +269,8 @@ typedef struct YggFlatWalkParams {
   uint32_t* far_vals;
   uint32_t* near_keys;
   uint32_t* near_vals;
+  int32_t hist_gen, pad1;     /* node_hist[t] counts generation hist_gen's pairs by target */
+  int32_t* node_hist;         /* [num_nodes] or NULL */
 } YggFlatWalkParams;
 
 int ygg_walk_flat(const YggFlatWalkParams* params, uint32_t* host_status,
@@ -297,6 +299,55 @@ int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
                                int64_t max_count, int out_i64, void* neighbors,
                                void* neighbor_leaf_positions, void* counts, void* offsets,
                                void* stream);
+
+/* ---- learned capacity model -------------------------------------------- */
+
+/* 8 -> 64 -> 64 -> 64 -> 3 ReLU MLP, bfloat16 weights, float32 accumulation.
+ * No counterpart in the reference: it sizes the walk's streams in one pass
+ * (the reference counts first or retries on overflow). Parameters are laid
+ * out W1[64][8] b1 W2[64][64] b2 W3[64][64] b3 W4[3][64] b4. */
+int ygg_mlp_num_params(void);
+int ygg_mlp_forward(const uint16_t* params_bf16, const float* features, int64_t n,
+                    float* outputs, void* stream);
+int64_t ygg_mlp_train_workspace_floats(int64_t batch);
+/* One Adam step (lr > 0) or gradient only (lr == 0) of the masked MSE on a
+ * batch; grads receives dLoss/dparams (float32, parameter layout). */
+int ygg_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, float* adam_v,
+                       int64_t step, float lr, const float* x, const float* target,
+                       const float* mask, int64_t batch, float* grads, float* ws,
+                       float* loss_out, void* stream);
+int ygg_mlp_params_to_bf16(const float* params, uint16_t* params_bf16, void* stream);
+/* Per-node features: level, particle count, max half extent, radius, parent
+ * far count (may be NULL), sibling particle count, density, leaf flag. */
+int ygg_mlp_features(int64_t num_nodes, const int32_t* node_level, const int32_t* node_ranges,
+                     const int32_t* parent, const int32_t* left, const int32_t* right,
+                     const float* half_extent, const float* radius, const float* max_extent,
+                     const float* parent_far, float* features, void* stream);
+/* totals[3] (double) = per-output sums of 2^y - 1 (near: leaves only). */
+int ygg_mlp_totals(const float* outputs, int64_t num_nodes, double* totals, void* stream);
+int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float* far_counts, void* stream);
+
+/* ---- custom pair policies ------------------------------------------------- */
+
+/* Geometry of one generation's pairs for a host-evaluated pair policy
+ * (PairPolicy keyword arrays of _resolve_pair_actions,
+ * yggdrax/_interactions_impl.py:731-810). */
+int ygg_policy_geometry(const void* nodes, const uint64_t* pairs, int64_t n, int64_t num_internal,
+                        int mac_type, float theta_sq, float theta_root, int32_t* targets,
+                        int32_t* sources, float* center_target, float* center_source,
+                        float* dist_sq, float* extent_target, float* extent_source,
+                        uint8_t* mac_ok, uint8_t* target_leaf, uint8_t* source_leaf,
+                        void* stream);
+/* Applies both directions' actions and tags: accept only if both accept,
+ * near only if both say near, else refine; appends tagged far entries, near
+ * entries and child pairs (pairs_out at *out_size, reserved atomically). */
+int ygg_policy_apply(const void* nodes, const uint64_t* pairs, int64_t n, int64_t num_internal,
+                     const int32_t* actions_fwd, const int32_t* actions_rev,
+                     const int32_t* tags_fwd, const int32_t* tags_rev, uint64_t* pairs_out,
+                     int64_t queue_cap, uint32_t* out_size, uint32_t* far_keys,
+                     uint32_t* far_vals, int32_t* far_tags, int64_t far_cap,
+                     uint32_t* near_keys, uint32_t* near_vals, int64_t near_cap,
+                     uint64_t* counters, uint32_t* flags, void* stream);
 
 #ifdef __cplusplus
 }

@@ -495,3 +495,82 @@ def build_interactions_and_neighbors(topo, geom, theta=0.5, max_interactions_per
         max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
         traversal_config=traversal_config, events=events)
     return to_interactions(raw, topo), to_neighbors(raw, topo), raw
+
+
+def walk_with_policy(topo, geom, theta, mac_type, dehnen_radius_scale, policy, policy_state):
+    """Generation-synchronous walk with a custom pair policy evaluated in both
+    directions (_resolve_pair_actions, 731-810, inside 1131-1785). Returns
+    node-order rows of (source, tag) far entries and near rows."""
+    T = topo["parent"].shape[0]
+    NI = (T - 1) // 2
+    L = T - NI
+    leaf_indices, leaf_pos, _, _ = leaf_ordering(topo)
+    ext = mac_extents(geom, mac_type, dehnen_radius_scale)
+    c = np.asarray(geom["center"], np.float32)
+    th = np.float32(theta)
+    th2 = np.float32(th * th)
+    left = np.asarray(topo["left_child"], np.int64)
+    right = np.asarray(topo["right_child"], np.int64)
+    far = [[] for _ in range(T)]
+    near = [[] for _ in range(L)]
+    wt = np.array([0], np.int64)
+    ws = np.array([0], np.int64)
+    gens = [1]
+    while wt.size:
+        d = (c[wt] - c[ws]).astype(np.float32)
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]).astype(np.float32)
+        d2 = (d2 + d[:, 2] * d[:, 2]).astype(np.float32)
+        et, es = ext[wt], ext[ws]
+        diff = wt != ws
+        if mac_type == "engblom":
+            rb, rs = np.maximum(et, es), np.minimum(et, es)
+            lhs = (rb + np.float32(np.sqrt(th2)) * rs).astype(np.float32)
+            ok = diff & (d2 > 0) & (lhs * lhs <= th2 * d2)
+        else:
+            sm = (et + es).astype(np.float32)
+            ok = diff & (d2 > 0) & (sm * sm <= th2 * d2)
+        tl, sl = wt >= NI, ws >= NI
+        valid = np.ones(wt.size, bool)
+        common = dict(valid_pairs=valid, mac_ok=ok, different_nodes=diff, same_node=~diff,
+                      dist_sq=d2)
+        fa, ft = policy(policy_state, target_leaf=tl, source_leaf=sl, target_nodes=wt,
+                        source_nodes=ws, center_target=c[wt], center_source=c[ws],
+                        extent_target=et, extent_source=es, **common)
+        ra, rt = policy(policy_state, target_leaf=sl, source_leaf=tl, target_nodes=ws,
+                        source_nodes=wt, center_target=c[ws], center_source=c[wt],
+                        extent_target=es, extent_source=et, **common)
+        fa, ra, ft, rt = (np.asarray(x, np.int64) for x in (fa, ra, ft, rt))
+        acc = (fa == 0) & (ra == 0)
+        nr = (fa == 1) & (ra == 1) & ~acc
+        nt_, ns_ = [], []
+        for p in range(wt.size):
+            t, s = int(wt[p]), int(ws[p])
+            if acc[p]:
+                far[t].append((s, int(ft[p])))
+                far[s].append((t, int(rt[p])))
+            elif nr[p]:
+                near[leaf_pos[t]].append(s)
+                near[leaf_pos[s]].append(t)
+            else:
+                same = t == s
+                ti, si = t < NI, s < NI
+                st = ti and (same or not si or et[p] >= es[p])
+                ss = si and (same or not ti or es[p] > et[p])
+                if st and ss and same:
+                    kids = [(left[t], left[t]), (left[t], right[t]), (right[t], right[t])]
+                elif st and ss:
+                    kids = [(left[t], left[s]), (left[t], right[s]), (right[t], left[s]),
+                            (right[t], right[s])]
+                elif st:
+                    kids = [(left[t], s), (right[t], s)]
+                elif ss:
+                    kids = [(t, left[s]), (t, right[s])]
+                else:
+                    kids = []
+                for a, b in kids:
+                    nt_.append(min(a, b))
+                    ns_.append(max(a, b))
+        wt = np.array(nt_, np.int64)
+        ws = np.array(ns_, np.int64)
+        gens.append(int(wt.size))
+    return far, near, gens

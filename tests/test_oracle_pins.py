@@ -461,3 +461,43 @@ def test_capacity_ladder_errors():
     with pytest.raises(RuntimeError, match="Pair queue capacity exceeded"):
         oracle.build_interactions_and_neighbors(t, g, theta=0.6, mac_type="dehnen",
                                                 max_pair_queue=4, max_interactions_per_node=64)
+
+
+# ---- capacity-model reference: backprop pinned by finite differences --------
+
+def test_mlp_reference_gradient_matches_finite_differences():
+    from oracle import mlp_ref
+    rng = np.random.default_rng(0)
+    p = rng.normal(0, 0.3, 9091)
+    x = rng.normal(0, 1, (16, 8))
+    t = rng.normal(0, 1, (16, 3))
+    m = (rng.uniform(size=(16, 3)) > 0.3).astype(np.float64)
+    loss, g = mlp_ref.loss_and_grad(p, x, t, m, rounding=False)
+    idx = rng.choice(p.size, 60, replace=False)
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        lp, _ = mlp_ref.loss_and_grad(p + e, x, t, m, rounding=False)
+        lm, _ = mlp_ref.loss_and_grad(p - e, x, t, m, rounding=False)
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - g[i]) <= 1e-5 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_oracle_policy_walk_with_mac_policy_equals_walk():
+    from oracle import oracle as O
+    pos, _ = inputs.plummer_sphere(1500, 9, a=0.4)
+    t = oracle.build_octree(pos, None, leaf_size=8)
+    g = oracle.compute_tree_geometry(t, t["positions_sorted"])
+
+    def mac_policy(state, *, valid_pairs, mac_ok, different_nodes, target_leaf, source_leaf,
+                   same_node, target_nodes, source_nodes, center_target, center_source, dist_sq,
+                   extent_target, extent_source):
+        near = valid_pairs & ~mac_ok & target_leaf & source_leaf & different_nodes
+        return np.where(mac_ok, 0, np.where(near, 1, 2)), np.where(mac_ok, 1, -1)
+    far, near, _ = O.walk_with_policy(t, g, 0.55, "dehnen", 1.0, mac_policy, None)
+    inter, nbrs, _ = oracle.build_interactions_and_neighbors(t, g, theta=0.55, mac_type="dehnen")
+    for k in range(len(far)):
+        row = inter["sources"][inter["offsets"][k]:inter["offsets"][k] + inter["counts"][k]]
+        assert sorted(s for s, _ in far[k]) == sorted(row.tolist())
+    for r in range(len(near)):
+        assert sorted(near[r]) == sorted(nbrs["neighbors"][nbrs["offsets"][r]:nbrs["offsets"][r + 1]].tolist())

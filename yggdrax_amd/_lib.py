@@ -62,7 +62,8 @@ class FlatWalkParams(ctypes.Structure):
         ("queue_cap", _I64), ("far_cap", _I64),
         ("near_cap", _I64), ("nodes", _P), ("wf0", _P), ("wf1", _P), ("gen_sizes", _P),
         ("flags", _P), ("counters", _P), ("far_keys", _P), ("far_vals", _P),
-        ("near_keys", _P), ("near_vals", _P),
+        ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("pad1", _I32),
+        ("node_hist", _P),
     ]
 
 
@@ -98,6 +99,17 @@ _SIGS = {
     "ygg_cap_counts": (_INT, [_P, _I64, _I32, _P, _P]),
     "ygg_far_output": (_INT, [_I64, _P, _P, _P, _I32, _P, _P, _P, _INT, _P, _P, _P, _P, _P]),
     "ygg_near_output": (_INT, [_I64, _P, _P, _I32, _P, _P, _P, _I64, _INT, _P, _P, _P]),
+    "ygg_policy_geometry": (_INT, [_P, _P, _I64, _I64, _INT, _F, _F] + [_P] * 11 + [_P]),
+    "ygg_policy_apply": (_INT, [_P, _P, _I64, _I64] + [_P] * 5 + [_I64] + [_P] * 4 + [_I64] +
+                         [_P] * 2 + [_I64] + [_P] * 3),
+    "ygg_mlp_num_params": (_INT, []),
+    "ygg_mlp_forward": (_INT, [_P, _P, _I64, _P, _P]),
+    "ygg_mlp_train_workspace_floats": (_I64, [_I64]),
+    "ygg_mlp_train_step": (_INT, [_P, _P, _P, _P, _I64, _F, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "ygg_mlp_params_to_bf16": (_INT, [_P, _P, _P]),
+    "ygg_mlp_features": (_INT, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ygg_mlp_totals": (_INT, [_P, _I64, _P, _P]),
+    "ygg_mlp_far_counts": (_INT, [_P, _I64, _P, _P]),
 }
 
 _lib = None

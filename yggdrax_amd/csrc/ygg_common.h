@@ -111,6 +111,12 @@ __device__ __forceinline__ unsigned wave_inclusive_sum(unsigned v) {
   return v;
 }
 
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __device__ __forceinline__ unsigned wave_sum(unsigned v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -1,0 +1,345 @@
+// Capacity model: an 8 -> 64 -> 64 -> 64 -> 3 ReLU MLP over per-node
+// features, with bfloat16 weights and float32 accumulation, trained and run
+// by these kernels. It predicts, per node, log2(1 + far-list count),
+// log2(1 + near-list count) (leaves) and log2(1 + pairs in the peak
+// wavefront with this node as target); summed, these size the walk's
+// streams in one pass. A misprediction only costs a fallback to counting:
+// the walk detects every stream overflow exactly.
+//
+// Parameter layout (P = 9091 floats): W1[64][8] b1[64] W2[64][64] b2[64]
+// W3[64][64] b3[64] W4[3][64] b4[3]; W[o][i] multiplies input i into output o.
+#include "ygg_common.h"
+
+// The model's arithmetic is not matched against the reference, so products
+// and sums may fuse here.
+#pragma clang fp contract(fast)
+
+namespace ygg {
+
+constexpr int kIn = 8, kH = 64, kOut = 3;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kH * kIn, kOffW2 = kOffB1 + kH,
+              kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH * kH,
+              kOffW4 = kOffB3 + kH, kOffB4 = kOffW4 + kOut * kH, kParams = kOffB4 + kOut;
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+
+// Round-to-nearest-even float32 -> bfloat16 (NaN kept quiet).
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40);
+  uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+
+// ---- inference: one thread per sample, weights staged once per block in LDS.
+__global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ wbf,
+                                                   const float* __restrict__ x, int64_t n,
+                                                   float* __restrict__ y) {
+  __shared__ float w[kParams];
+  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = bf16_to_f32(wbf[i]);
+  __syncthreads();
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    float in[kIn];
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) in[i] = x[s * kIn + i];
+    float h1[kH], h2[kH];
+#pragma unroll
+    for (int o = 0; o < kH; ++o) {
+      float acc = w[kOffB1 + o];
+#pragma unroll
+      for (int i = 0; i < kIn; ++i) acc += w[kOffW1 + o * kIn + i] * in[i];
+      h1[o] = fmaxf(acc, 0.0f);
+    }
+#pragma unroll
+    for (int o = 0; o < kH; ++o) {
+      float acc = w[kOffB2 + o];
+#pragma unroll
+      for (int i = 0; i < kH; ++i) acc += w[kOffW2 + o * kH + i] * h1[i];
+      h2[o] = fmaxf(acc, 0.0f);
+    }
+#pragma unroll
+    for (int o = 0; o < kH; ++o) {
+      float acc = w[kOffB3 + o];
+#pragma unroll
+      for (int i = 0; i < kH; ++i) acc += w[kOffW3 + o * kH + i] * h2[i];
+      h1[o] = fmaxf(acc, 0.0f);
+    }
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) {
+      float acc = w[kOffB4 + o];
+#pragma unroll
+      for (int i = 0; i < kH; ++i) acc += w[kOffW4 + o * kH + i] * h1[i];
+      y[s * kOut + o] = acc;
+    }
+  }
+}
+
+// ---- training: layer kernels over a batch (row-major activations).
+// y[b][o] = act(sum_i W[o][i] x[b][i] + bias[o])
+__global__ void k_lin_fwd(const uint16_t* __restrict__ W, const uint16_t* __restrict__ bias,
+                          const float* __restrict__ x, int B, int nin, int nout, int relu,
+                          float* __restrict__ y) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * nout) return;
+  int b = (int)(e / nout), o = (int)(e % nout);
+  float acc = bf16_to_f32(bias[o]);
+  for (int i = 0; i < nin; ++i) acc += bf16_to_f32(W[o * nin + i]) * x[(int64_t)b * nin + i];
+  y[e] = relu ? fmaxf(acc, 0.0f) : acc;
+}
+
+// dx[b][i] = sum_o W[o][i] dy[b][o], times relu'(x) when the input came from a ReLU.
+__global__ void k_lin_bwd_data(const uint16_t* __restrict__ W, const float* __restrict__ dy,
+                               const float* __restrict__ xin, int B, int nin, int nout,
+                               int relu_in, float* __restrict__ dx) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)B * nin) return;
+  int b = (int)(e / nin), i = (int)(e % nin);
+  float acc = 0.0f;
+  for (int o = 0; o < nout; ++o) acc += bf16_to_f32(W[o * nin + i]) * dy[(int64_t)b * nout + o];
+  if (relu_in && !(xin[e] > 0.0f)) acc = 0.0f;
+  dx[e] = acc;
+}
+
+// dW[o][i] = sum_b dy[b][o] x[b][i]; db[o] = sum_b dy[b][o] (one thread per weight).
+__global__ void k_lin_bwd_weight(const float* __restrict__ dy, const float* __restrict__ x,
+                                 int B, int nin, int nout, float* __restrict__ dW,
+                                 float* __restrict__ db) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nout * nin) {
+    int o = e / nin, i = e % nin;
+    float acc = 0.0f;
+    for (int b = 0; b < B; ++b) acc += dy[(int64_t)b * nout + o] * x[(int64_t)b * nin + i];
+    dW[e] = acc;
+  } else if (e < nout * nin + nout) {
+    int o = e - nout * nin;
+    float acc = 0.0f;
+    for (int b = 0; b < B; ++b) acc += dy[(int64_t)b * nout + o];
+    db[o] = acc;
+  }
+}
+
+// Masked mean squared error: dy = 2 (y - t) mask / B, loss accumulated.
+__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t,
+                      const float* __restrict__ mask, int B, float* __restrict__ dy,
+                      float* __restrict__ loss) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (e < B * kOut) {
+    float d = (y[e] - t[e]) * mask[e];
+    dy[e] = 2.0f * d / (float)B;
+    l = d * d / (float)B;
+  }
+  l = wave_sum_f(l);
+  if (lane_id() == 0 && l != 0.0f) atomicAdd(loss, l);
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, uint16_t* __restrict__ pbf, int n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.0f - b1) * gi;
+  float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  float np = p[i] - upd;
+  p[i] = np;
+  pbf[i] = f32_to_bf16(np);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ q, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = f32_to_bf16(p[i]);
+}
+
+__global__ void k_zero_words(uint32_t* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// ---- per-node features ------------------------------------------------------
+__global__ void k_mlp_features(int T, int NI, const int32_t* __restrict__ node_level,
+                               const int32_t* __restrict__ ranges,
+                               const int32_t* __restrict__ parent,
+                               const int32_t* __restrict__ left,
+                               const int32_t* __restrict__ right,
+                               const float* __restrict__ half, const float* __restrict__ radius,
+                               const float* __restrict__ maxext,
+                               const float* __restrict__ parent_far, float* __restrict__ x) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= T) return;
+  const float root_ext = maxext[0], root_rad = radius[0];
+  const float root_vol = 8.0f * half[0] * half[1] * half[2];
+  float cnt = (float)(ranges[2 * k + 1] - ranges[2 * k] + 1);
+  int p = parent[k];
+  float sib = 0.0f;
+  if (p >= 0) {
+    int sb = left[p] == k ? right[p] : left[p];
+    sib = (float)(ranges[2 * sb + 1] - ranges[2 * sb] + 1);
+  }
+  float vol = 8.0f * half[3 * k] * half[3 * k + 1] * half[3 * k + 2];
+  float pf = (parent_far && p >= 0) ? parent_far[p] : 0.0f;
+  float* o = x + (int64_t)k * kIn;
+  o[0] = (float)node_level[k] / 64.0f;
+  o[1] = log2f(cnt) / 24.0f;
+  o[2] = log2f(maxext[k] / root_ext) / 32.0f + 1.0f;
+  o[3] = log2f(radius[k] / root_rad) / 32.0f + 1.0f;
+  o[4] = log2f(1.0f + fmaxf(pf, 0.0f)) / 16.0f;
+  o[5] = log2f(1.0f + sib) / 24.0f;
+  float rel = (cnt / fmaxf(vol, 1e-30f)) / ((float)(ranges[1] + 1) / fmaxf(root_vol, 1e-30f));
+  o[6] = fminf(fmaxf(log2f(fmaxf(rel, 1e-30f)) / 64.0f, -1.0f), 1.0f);
+  o[7] = k >= NI ? 1.0f : 0.0f;
+}
+
+// Sum of 2^y - 1 per output over [0, n) (capacity totals), y row-major [n][3].
+__global__ void k_mlp_totals(const float* __restrict__ y, int64_t n, int64_t NI,
+                             double* __restrict__ tot) {
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    a0 += (double)fmaxf(exp2f(y[3 * s]) - 1.0f, 0.0f);
+    if (s >= NI) a1 += (double)fmaxf(exp2f(y[3 * s + 1]) - 1.0f, 0.0f);
+    a2 += (double)fmaxf(exp2f(y[3 * s + 2]) - 1.0f, 0.0f);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, 64);
+    a1 += __shfl_xor(a1, off, 64);
+    a2 += __shfl_xor(a2, off, 64);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(&tot[0], a0);
+    atomicAdd(&tot[1], a1);
+    atomicAdd(&tot[2], a2);
+  }
+}
+
+// parent_far[k] = 2^y[k][0] - 1 (the first pass's far-count prediction).
+__global__ void k_mlp_far_of(const float* __restrict__ y, int T, float* __restrict__ pf) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < T) pf[k] = fmaxf(exp2f(y[3 * k]) - 1.0f, 0.0f);
+}
+
+}  // namespace ygg
+
+using namespace ygg;
+
+extern "C" {
+
+int ygg_mlp_num_params(void) { return kParams; }
+
+int ygg_mlp_forward(const uint16_t* params_bf16, const float* features, int64_t n,
+                    float* outputs, void* stream) {
+  if (n < 0 || !params_bf16) return YGG_ERR_ARG;
+  if (n == 0) return YGG_OK;
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
+                     (hipStream_t)stream, params_bf16, features, n, outputs);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+// One training step on a batch: forward, masked MSE, backward, Adam.
+// ws: float workspace of ygg_mlp_train_workspace_floats(B) entries.
+int64_t ygg_mlp_train_workspace_floats(int64_t B) {
+  return (int64_t)B * (3 * kH + kOut) * 2 + kParams + 16;
+}
+
+int ygg_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, float* adam_v,
+                       int64_t step, float lr, const float* x, const float* target,
+                       const float* mask, int64_t B64, float* grads, float* ws,
+                       float* loss_out, void* stream) {
+  if (B64 < 1 || !params || !grads || !ws) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int B = (int)B64;
+  float* a1 = ws;
+  float* a2 = a1 + (int64_t)B * kH;
+  float* a3 = a2 + (int64_t)B * kH;
+  float* yo = a3 + (int64_t)B * kH;
+  float* d1 = yo + (int64_t)B * kOut;
+  float* d2 = d1 + (int64_t)B * kH;
+  float* d3 = d2 + (int64_t)B * kH;
+  float* dyo = d3 + (int64_t)B * kH;
+  const uint16_t* W = params_bf16;
+  auto g = [](int64_t n) { return dim3(grid_for(n, 256)); };
+  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW1, W + kOffB1, x,
+                     B, kIn, kH, 1, a1);
+  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW2, W + kOffB2, a1,
+                     B, kH, kH, 1, a2);
+  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW3, W + kOffB3, a2,
+                     B, kH, kH, 1, a3);
+  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kOut), dim3(256), 0, st, W + kOffW4, W + kOffB4,
+                     a3, B, kH, kOut, 0, yo);
+  if (loss_out) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)loss_out, 1);
+  hipLaunchKernelGGL(k_mse, g((int64_t)B * kOut), dim3(256), 0, st, yo, target, mask, B, dyo,
+                     loss_out);
+  // layer 4
+  hipLaunchKernelGGL(k_lin_bwd_weight, g(kOut * kH + kOut), dim3(256), 0, st, dyo, a3, B, kH,
+                     kOut, grads + kOffW4, grads + kOffB4);
+  hipLaunchKernelGGL(k_lin_bwd_data, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW4, dyo, a3,
+                     B, kH, kOut, 1, d3);
+  // layer 3
+  hipLaunchKernelGGL(k_lin_bwd_weight, g(kH * kH + kH), dim3(256), 0, st, d3, a2, B, kH, kH,
+                     grads + kOffW3, grads + kOffB3);
+  hipLaunchKernelGGL(k_lin_bwd_data, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW3, d3, a2, B,
+                     kH, kH, 1, d2);
+  // layer 2
+  hipLaunchKernelGGL(k_lin_bwd_weight, g(kH * kH + kH), dim3(256), 0, st, d2, a1, B, kH, kH,
+                     grads + kOffW2, grads + kOffB2);
+  hipLaunchKernelGGL(k_lin_bwd_data, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW2, d2, a1, B,
+                     kH, kH, 1, d1);
+  // layer 1
+  hipLaunchKernelGGL(k_lin_bwd_weight, g(kH * kIn + kH), dim3(256), 0, st, d1, x, B, kIn, kH,
+                     grads + kOffW1, grads + kOffB1);
+  YGG_LAUNCH_CHECK();
+  if (lr > 0.0f) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, g(kParams), dim3(256), 0, st, params, grads, adam_m, adam_v,
+                       params_bf16, kParams, lr, b1, b2, 1e-8f, c1, c2);
+    YGG_LAUNCH_CHECK();
+  }
+  (void)yo;
+  return YGG_OK;
+}
+
+int ygg_mlp_params_to_bf16(const float* params, uint16_t* params_bf16, void* stream) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(kParams, 256)), dim3(256), 0, (hipStream_t)stream,
+                     params, params_bf16, kParams);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_mlp_features(int64_t num_nodes, const int32_t* node_level, const int32_t* node_ranges,
+                     const int32_t* parent, const int32_t* left, const int32_t* right,
+                     const float* half_extent, const float* radius, const float* max_extent,
+                     const float* parent_far, float* features, void* stream) {
+  if (num_nodes < 1) return YGG_ERR_ARG;
+  int T = (int)num_nodes, NI = (T - 1) / 2;
+  hipLaunchKernelGGL(k_mlp_features, dim3(grid_for(T, 256)), dim3(256), 0, (hipStream_t)stream, T,
+                     NI, node_level, node_ranges, parent, left, right, half_extent, radius,
+                     max_extent, parent_far, features);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_mlp_totals(const float* outputs, int64_t num_nodes, double* totals, void* stream) {
+  if (num_nodes < 1) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)totals, 6);
+  hipLaunchKernelGGL(k_mlp_totals, dim3(grid_for(num_nodes, 256, 1024)), dim3(256), 0, st,
+                     outputs, num_nodes, (num_nodes - 1) / 2, totals);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float* far_counts, void* stream) {
+  hipLaunchKernelGGL(k_mlp_far_of, dim3(grid_for(num_nodes, 256)), dim3(256), 0,
+                     (hipStream_t)stream, outputs, (int)num_nodes, far_counts);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+}  // extern "C"

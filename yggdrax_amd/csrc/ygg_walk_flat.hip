@@ -36,6 +36,8 @@ struct FlatArgs {
   uint32_t* far_vals;
   uint32_t* near_keys;
   uint32_t* near_vals;
+  int32_t hist_gen;     // generation whose pairs are histogrammed by target node
+  int32_t* node_hist;   // [T] or null
 };
 
 __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v) {
@@ -96,6 +98,7 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         ok = diff && d2 > 0.0f && (sum * sum <= a.theta_sq * d2);
       }
       const bool t_int = t < NI, s_int = s < NI;
+      if (a.node_hist && g == a.hist_gen) atomicAdd(&a.node_hist[t], 1);
       pt[k] = (uint32_t)t;
       ps[k] = (uint32_t)s;
       if (ok) {
@@ -352,6 +355,8 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
   a.far_vals = p->far_vals;
   a.near_keys = p->near_keys;
   a.near_vals = p->near_vals;
+  a.hist_gen = p->hist_gen;
+  a.node_hist = p->node_hist;
   const int G = p->max_generations;
   hipLaunchKernelGGL(k_flat_init, dim3(grid_for(G + 1, 256)), dim3(256), 0, st, a, p->root, G);
   YGG_LAUNCH_CHECK();

@@ -253,7 +253,7 @@ class _WalkContext:
         self.kernel_log = WALK_TIMING["log"]
 
     def run_flat(self, *, emit: bool, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
-                 gen_hint: int = 0):
+                 gen_hint: int = 0, hist_gen: int = -1, node_hist=None):
         """One flat-emission walk over the whole tree (ygg_walk_flat)."""
         dt = self.dt
         dev = self.dev
@@ -280,6 +280,7 @@ class _WalkContext:
         p.counters = counters.data_ptr()
         p.far_keys, p.far_vals = _lib.ptr(fk), _lib.ptr(fv)
         p.near_keys, p.near_vals = _lib.ptr(nk), _lib.ptr(nv)
+        p.hist_gen, p.node_hist = int(hist_gen), _lib.ptr(node_hist)
         ngen = ctypes.c_int32(0)
         gen_ms = (ctypes.c_float * MAX_GENERATIONS)() if self.timing else None
         _lib.check(self.lib.ygg_walk_flat(ctypes.byref(p), self.host_status.data_ptr(),
@@ -883,7 +884,7 @@ def build_interactions_and_neighbors(
     count-pass path, which exist for its dense (L x capacity) buffers; the
     CSR lists need neither. Both default to the reference's behaviour."""
     if pair_policy is not None:
-        from .policy import build_with_pair_policy
+        from .policy import build_with_pair_policy  # noqa: E402
         return build_with_pair_policy(
             tree, geometry, theta, mac_type, dehnen_radius_scale, pair_policy, policy_state,
             max_interactions_per_node=max_interactions_per_node,

@@ -1,0 +1,148 @@
+"""Custom pair policies (PairPolicy, yggdrax/_interactions_impl.py:324, 731-810).
+
+A policy is a Python callable ``policy(policy_state, **kw) -> (actions, tags)``
+over arrays of one walk generation (torch tensors on the device here, with
+the reference's keyword names). It is evaluated in both directions; a pair
+is accepted only when both directions accept and is near only when both
+say near, otherwise it is refined. Accepted pairs carry the directed tags
+(forward tag on the target's row, reverse tag on the source's row), exposed
+in DualTreeWalkResult.interaction_tags.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .dtypes import INDEX_DTYPE
+from .interactions import (MAX_GENERATIONS, _FlatResult, _WalkContext, _emulate_ladder,
+                           _flat_outputs, _ladder, _to_result, DEFAULT_PAIR_QUEUE_MULTIPLIER)
+from .tree import device_topology, resolve_tree_topology
+
+ACTION_ACCEPT, ACTION_NEAR, ACTION_REFINE = 0, 1, 2
+
+def _lib_policy():
+    return _lib.load()
+
+
+def _policy_walk(ctx: _WalkContext, dt, pair_policy, policy_state, qcap, fcap, ncap):
+    lib = _lib_policy()
+    dev, st = ctx.dev, ctx.stream
+    idx = INDEX_DTYPE
+    wf = [torch.empty(qcap, dtype=torch.int64, device=dev) for _ in range(2)]
+    wf[0][0] = 0  # root pair (0, 0)
+    far_k = torch.empty(fcap, dtype=torch.int32, device=dev)
+    far_v = torch.empty(fcap, dtype=torch.int32, device=dev)
+    far_t = torch.empty(fcap, dtype=torch.int32, device=dev)
+    near_k = torch.empty(ncap, dtype=torch.int32, device=dev)
+    near_v = torch.empty(ncap, dtype=torch.int32, device=dev)
+    counters = torch.zeros(2, dtype=torch.int64, device=dev)
+    flags = torch.zeros(4, dtype=torch.int32, device=dev)
+    out_size = torch.zeros(1, dtype=torch.int32, device=dev)
+    size, gens, cur = 1, [1], 0
+    while size > 0:
+        if len(gens) > MAX_GENERATIONS:
+            raise RuntimeError("pair-policy walk exceeded the generation limit")
+        n = size
+        f32 = dict(dtype=torch.float32, device=dev)
+        t = torch.empty(n, dtype=torch.int32, device=dev)
+        s = torch.empty(n, dtype=torch.int32, device=dev)
+        ct = torch.empty((n, 3), **f32)
+        cs = torch.empty((n, 3), **f32)
+        d2 = torch.empty(n, **f32)
+        et = torch.empty(n, **f32)
+        es = torch.empty(n, **f32)
+        mok = torch.empty(n, dtype=torch.uint8, device=dev)
+        tl = torch.empty(n, dtype=torch.uint8, device=dev)
+        sl = torch.empty(n, dtype=torch.uint8, device=dev)
+        _lib.check(lib.ygg_policy_geometry(
+            ctx.nodes.data_ptr(), wf[cur].data_ptr(), n, dt.num_internal, ctx.mac,
+            ctx.theta_sq, ctx.theta_root, t.data_ptr(), s.data_ptr(), ct.data_ptr(),
+            cs.data_ptr(), d2.data_ptr(), et.data_ptr(), es.data_ptr(), mok.data_ptr(),
+            tl.data_ptr(), sl.data_ptr(), st), "policy geometry")
+        valid = torch.ones(n, dtype=torch.bool, device=dev)
+        ti, si = t.to(idx), s.to(idx)
+        same = ti == si
+        common = dict(valid_pairs=valid, mac_ok=mok.bool(), different_nodes=~same,
+                      same_node=same, dist_sq=d2)
+        fa, ftag = pair_policy(policy_state, target_leaf=tl.bool(), source_leaf=sl.bool(),
+                               target_nodes=ti, source_nodes=si, center_target=ct,
+                               center_source=cs, extent_target=et, extent_source=es, **common)
+        ra, rtag = pair_policy(policy_state, target_leaf=sl.bool(), source_leaf=tl.bool(),
+                               target_nodes=si, source_nodes=ti, center_target=cs,
+                               center_source=ct, extent_target=es, extent_source=et, **common)
+        conv = lambda a: torch.as_tensor(a, device=dev).to(torch.int32).reshape(n).contiguous()  # noqa
+        fa, ra, ftag, rtag = conv(fa), conv(ra), conv(ftag), conv(rtag)
+        out_size.zero_()
+        _lib.check(lib.ygg_policy_apply(
+            ctx.nodes.data_ptr(), wf[cur].data_ptr(), n, dt.num_internal, fa.data_ptr(),
+            ra.data_ptr(), ftag.data_ptr(), rtag.data_ptr(), wf[1 - cur].data_ptr(), qcap,
+            out_size.data_ptr(), far_k.data_ptr(), far_v.data_ptr(), far_t.data_ptr(), fcap,
+            near_k.data_ptr(), near_v.data_ptr(), ncap, counters.data_ptr(), flags.data_ptr(),
+            st), "policy apply")
+        size = int(out_size)
+        gens.append(size)
+        cur = 1 - cur
+        if int(flags.max()) != 0:
+            return None
+    c = [int(v) for v in counters.cpu()]
+    return dict(far_total=c[0], near_total=c[1], gen_sizes=gens, peak=max(gens),
+                far_keys=far_k, far_vals=far_v, far_tags=far_t, near_keys=near_k,
+                near_vals=near_v, decisions=(c[0] // 2, c[1] // 2, sum(gens) - (c[0] + c[1]) // 2))
+
+
+def build_with_pair_policy(tree, geometry, theta, mac_type, dehnen_radius_scale, pair_policy,
+                           policy_state, *, max_interactions_per_node=None,
+                           max_neighbors_per_leaf=2048, max_pair_queue=None,
+                           traversal_config=None, return_result=False):
+    """build_interactions_and_neighbors with a custom pair policy."""
+    from .interactions import _empty_outputs, _sort_entries, _to_neighbors
+
+    dt = device_topology(tree)
+    topo = resolve_tree_topology(tree)
+    if float(dehnen_radius_scale) <= 0.0:
+        raise ValueError("dehnen_radius_scale must be > 0")
+    if dt.num_internal == 0:
+        inter, nbrs = _empty_outputs(dt, topo), _to_neighbors(None, dt)
+        return (inter, nbrs, _to_result(None, dt, inter, nbrs)) if return_result else (inter, nbrs)
+    ctx = _WalkContext(dt, geometry, theta, mac_type, dehnen_radius_scale)
+    lad = _ladder(dt, mac_type, max_interactions_per_node=max_interactions_per_node,
+                  max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
+                  traversal_config=traversal_config)
+    T, L = dt.num_nodes, dt.num_leaves
+    qcap = T * DEFAULT_PAIR_QUEUE_MULTIPLIER
+    fcap = ncap = 16 * T + 1024
+    res = None
+    for _ in range(8):
+        res = _policy_walk(ctx, dt, pair_policy, policy_state, qcap, fcap, ncap)
+        if res is not None:
+            break
+        fcap, ncap = fcap * 4, ncap * 4
+    if res is None:
+        raise RuntimeError("pair-policy walk did not fit its buffers")
+    nf, nn = res["far_total"], res["near_total"]
+    fk, perm, far_off = _sort_entries(res["far_keys"], torch.arange(
+        max(1, nf), dtype=torch.int32, device=ctx.dev)[:max(1, nf)], nf, T)
+    pl = perm[:nf].long()
+    fv = res["far_vals"][pl]
+    tags = res["far_tags"][pl]
+    _, nv, near_off = _sort_entries(res["near_keys"], res["near_vals"], nn, L)
+    mx = torch.stack([torch.diff(far_off).max(), torch.diff(near_off).max()]).cpu()
+    stats = dict(peak=res["peak"], gen_sizes=res["gen_sizes"], decisions=res["decisions"],
+                 max_far=int(mx[0]), max_near=int(mx[1]), far_total=nf, near_total=nn)
+    if not lad.use_count:
+        q = _emulate_ladder(lad, res["peak"], stats["max_far"], stats["max_near"])
+        if q is None:
+            raise RuntimeError("overflow order of this capacity configuration needs the "
+                               "per-generation policy walk with capacities")
+        stats["queue_capacity"] = q
+    fr = _FlatResult(fk, fv, nv, far_off, near_off, stats)
+    inter, nbrs = _flat_outputs(fr, dt, topo)
+    if return_result:
+        r = _to_result(fr, dt, inter, nbrs)
+        r = r._replace(interaction_tags=tags.to(INDEX_DTYPE))
+        return inter, nbrs, r
+    return inter, nbrs
+
+
+__all__ = ["ACTION_ACCEPT", "ACTION_NEAR", "ACTION_REFINE", "build_with_pair_policy"]
