@@ -104,6 +104,67 @@ def walk_roofline(ygg, torch, pos, mass, leaf, theta):
                 emit_walk_ms=round(total_ms, 4), count_walk_ms=round(count_ms, 4))
 
 
+def _same_lists(torch, a, b, num_rows):
+    """Both CSR lists hold the same per-row sets (row order of entries is
+    unspecified): compare counts/offsets exactly and the (row, value) pairs
+    after a sort."""
+    if not (torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])):
+        return False
+    # entry e belongs to the row whose [offset, offset + count) holds it
+    order = torch.argsort(a[0][:num_rows], stable=True)
+    rows = torch.repeat_interleave(order, a[1][order].long())
+    ka = torch.sort(rows * (1 << 32) + a[2].long())[0]
+    kb = torch.sort(rows * (1 << 32) + b[2].long())[0]
+    return bool(torch.equal(ka, kb))
+
+
+def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
+    """Walk time of the plain path (counting walk, then the emission walk at
+    exact sizes) against the learned path (capacity MLP inference, then one
+    emission walk at the predicted sizes), on the same tree, with a check
+    that both produce the same lists."""
+    from yggdrax_amd import learned
+    model = learned.CapacityModel.load()
+    tree, ps, _, _ = ygg.build_octree(pos, mass, leaf_size=leaf, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+
+    def plain():
+        return ygg.build_interactions_and_neighbors(tree, geom, theta=theta, **WALK_KW)
+
+    def with_model():
+        hint = model.capacity_hint(tree, geom)
+        return ygg.build_interactions_and_neighbors(tree, geom, theta=theta, capacity_hint=hint,
+                                                    **WALK_KW), hint
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3, out
+
+    from yggdrax_amd import interactions as I
+    t_plain, (pi, pn) = timed(plain)
+    runs0 = I.FLAT_STATS["count_walks"]
+    t_learn, ((li, ln), hint) = timed(with_model)
+    fallbacks = I.FLAT_STATS["count_walks"] - runs0
+    T = int(tree.num_nodes)
+    L = int(pn.counts.numel())
+    same = (_same_lists(torch, (pi.offsets, pi.counts, pi.sources), (li.offsets, li.counts,
+                                                                      li.sources), T)
+            and torch.equal(pi.targets, li.targets)
+            and _same_lists(torch, (pn.offsets, pn.counts, pn.neighbors),
+                            (ln.offsets, ln.counts, ln.neighbors), L))
+    nf, nn = int(pi.sources.numel()), int(pn.neighbors.numel())
+    return dict(plain_walk_ms=round(t_plain, 3), learned_walk_ms=round(t_learn, 3),
+                walk_speedup=round(t_plain / t_learn, 3), identical_lists=same,
+                fallback_walks=fallbacks, hint=hint, true_far=nf, true_near=nn,
+                far_overalloc=round(hint["far"] / max(1, nf), 3),
+                near_overalloc=round(hint["near"] / max(1, nn), 3))
+
+
 def cpu_baseline(sample_n, leaf, theta):
     import oracle
     from yggdrax_amd import inputs
@@ -131,6 +192,7 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--cpu-sample-log2n", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -167,6 +229,10 @@ def main():
     _log(f"stages {stages}")
     roof = walk_roofline(ygg, torch, pos, mass, args.leaf, args.theta)
     _log(f"roofline {roof}")
+    lrn = None
+    if not args.no_learned:
+        lrn = learned_leg(ygg, torch, pos, mass, args.leaf, args.theta, args.steps)
+        _log(f"learned {lrn}")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(1 << args.cpu_sample_log2n, args.leaf, args.theta)
@@ -181,7 +247,7 @@ def main():
                                f"uniform background, seed 3), build_octree leaf {args.leaf}, "
                                f"dehnen theta {args.theta}, count pass + exact fill, int64 outputs",
                    "parallelism": "single GPU"},
-        "stages": stages, "roofline": roof, "cpu_baseline": cpu,
+        "stages": stages, "learned": lrn, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

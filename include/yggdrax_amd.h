@@ -30,17 +30,21 @@ extern "C" {
 /* ---- primitives ---------------------------------------------------------- */
 
 /* Stable LSD radix sort of (key, u32 value) pairs over key bits
- * [begin_bit, end_bit), 8 bits per pass. vals_in may be NULL (values become
- * 0..n-1). keys_in must differ from keys_out. Replaces the XLA sorts behind
+ * [begin_bit, end_bit), digit_bits (8..11) bits per pass; digit_bits 0 takes
+ * ygg_sort_default_digit_bits. The workspace query must use the same
+ * digit_bits as the sort. vals_in may be NULL (values become 0..n-1).
+ * keys_in must differ from keys_out. Replaces the XLA sorts behind
  * jnp.argsort(stable=True) (yggdrax/_tree_impl.py:401, yggdrax/octree.py:259,
  * yggdrax/_interactions_impl.py:1082). */
-size_t ygg_sort_workspace_bytes(int64_t n, int key_bytes, int begin_bit, int end_bit);
+size_t ygg_sort_workspace_bytes(int64_t n, int key_bytes, int begin_bit, int end_bit,
+                                int digit_bits);
+int ygg_sort_default_digit_bits(int key_bytes, int begin_bit, int end_bit);
 int ygg_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
-                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                       size_t ws_bytes, void* stream);
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, int digit_bits,
+                       void* ws, size_t ws_bytes, void* stream);
 int ygg_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                       size_t ws_bytes, void* stream);
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, int digit_bits,
+                       void* ws, size_t ws_bytes, void* stream);
 
 /* ---- bounds and Morton codes -------------------------------------------- */
 
@@ -310,7 +314,8 @@ int ygg_mlp_num_params(void);
 int ygg_mlp_forward(const uint16_t* params_bf16, const float* features, int64_t n,
                     float* outputs, void* stream);
 int64_t ygg_mlp_train_workspace_floats(int64_t batch);
-/* One Adam step (lr > 0) or gradient only (lr == 0) of the masked MSE on a
+/* One Adam step (lr > 0) or gradient only (lr == 0) of the masked Poisson
+ * negative log-likelihood (rate 2^y, target = count / per-output scale) on a
  * batch; grads receives dLoss/dparams (float32, parameter layout). */
 int ygg_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, float* adam_v,
                        int64_t step, float lr, const float* x, const float* target,
@@ -323,9 +328,12 @@ int ygg_mlp_features(int64_t num_nodes, const int32_t* node_level, const int32_t
                      const int32_t* parent, const int32_t* left, const int32_t* right,
                      const float* half_extent, const float* radius, const float* max_extent,
                      const float* parent_far, float* features, void* stream);
-/* totals[3] (double) = per-output sums of 2^y - 1 (near: leaves only). */
-int ygg_mlp_totals(const float* outputs, int64_t num_nodes, double* totals, void* stream);
-int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float* far_counts, void* stream);
+/* totals[3] (double) = per-output sums of scale_j * 2^y_j (near: leaves only). */
+int ygg_mlp_totals(const float* outputs, int64_t num_nodes, float scale_far, float scale_near,
+                   float scale_queue, double* totals, void* stream);
+/* far_counts[k] = scale_far * 2^y[k][0]: predicted far count of every node. */
+int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float scale_far,
+                       float* far_counts, void* stream);
 
 /* ---- custom pair policies ------------------------------------------------- */
 

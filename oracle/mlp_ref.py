@@ -2,7 +2,8 @@
 
 Same layout as the HIP kernels: W1[64][8] b1 W2[64][64] b2 W3[64][64] b3
 W4[3][64] b4, bfloat16-rounded weights, float32 activations. Used to check
-the HIP forward pass and the HIP gradients of the masked MSE loss.
+the HIP forward pass and the HIP gradients of the masked Poisson negative
+log-likelihood (rate 2^y, y clamped at 30).
 """
 
 from __future__ import annotations
@@ -41,16 +42,23 @@ def forward(p, x, rounding=True):
     return a3 @ W4.T + b4, (x, a1, a2, a3)
 
 
+LN2 = float(np.log(2.0))
+MAX_LOG_RATE = 30.0
+
+
 def loss_and_grad(p, x, t, m, rounding=True):
-    """Masked MSE (mean over the batch) and its gradient in the parameter
-    layout, computed in float64 (bfloat16-rounded weights unless
-    rounding=False)."""
+    """Masked Poisson NLL (mean over the batch) of rates 2^y against counts t,
+    and its gradient in the parameter layout, computed in float64
+    (bfloat16-rounded weights unless rounding=False)."""
     W1, b1, W2, b2, W3, b3, W4, b4 = _weights(p, rounding)
     y, (x0, a1, a2, a3) = forward(p, x, rounding)
     B = x.shape[0]
-    d = (y - t) * m
-    loss = float((d * d).sum() / B)
-    dy = 2 * d / B
+    t = np.asarray(t, np.float64)
+    m = np.asarray(m, np.float64)
+    yc = np.minimum(y, MAX_LOG_RATE)
+    lam = np.exp2(yc)
+    loss = float((m * (lam - t * yc * LN2)).sum() / B)
+    dy = np.where(y > MAX_LOG_RATE, 0.0, m * (lam - t) * LN2 / B)
     gW4 = dy.T @ a3
     gb4 = dy.sum(0)
     d3 = (dy @ W4) * (a3 > 0)

@@ -49,6 +49,24 @@ def test_ctypes_binding_covers_declarations(lib_path):
     assert set(_lib.declared_symbols()) <= _declared()
 
 
+def _declared_arity():
+    """Parameter count of every ygg_* function declared in include/*.h."""
+    out = {}
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for name, args in re.findall(r"\b(ygg_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+            args = args.strip()
+            out[name] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_ctypes_argument_counts_match_header():
+    from yggdrax_amd import _lib
+    arity = _declared_arity()
+    bad = {n: (len(a), arity[n]) for n, (_, a) in _lib._SIGS.items() if len(a) != arity.get(n, -1)}
+    assert not bad, f"ctypes argtypes vs header parameter count: {bad}"
+
+
 def test_struct_layouts_match_header(tmp_path):
     """ctypes mirrors of the header's structs have the C compiler's layout."""
     import ctypes

@@ -109,6 +109,48 @@ def test_sort_large_uniform(ygg, n):
     assert np.array_equal(parity.np_(tree.morton_codes), codes[perm])
 
 
+@pytest.mark.parametrize("key_bytes,range_bits,n", [(4, 18, 300007), (4, 20, 1 << 20),
+                                                    (4, 32, 70001), (8, 63, 500009),
+                                                    (8, 21, 4097)])
+@pytest.mark.parametrize("digit_bits", [8, 9, 10, 11])
+def test_radix_sort_every_digit_width(ygg, key_bytes, range_bits, n, digit_bits):
+    """Every digit width gives numpy's stable argsort (heavy duplicates in
+    the narrow-range cases exercise stability)."""
+    from yggdrax_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(range_bits * 7 + digit_bits)
+    hi = 1 << range_bits
+    if key_bytes == 8:
+        keys = rng.integers(0, hi, n, dtype=np.uint64)
+        kt = torch.from_numpy(keys.view(np.int64)).cuda()
+        fn = lib.ygg_sort_pairs_u64
+    else:
+        keys = rng.integers(0, min(hi, 1 << 32), n, dtype=np.uint64).astype(np.uint32)
+        kt = torch.from_numpy(keys.view(np.int32)).cuda()
+        fn = lib.ygg_sort_pairs_u32
+    ko = torch.empty_like(kt)
+    vo = torch.empty(n, dtype=torch.int32, device="cuda")
+    ws = _lib.workspace(lib.ygg_sort_workspace_bytes(n, key_bytes, 0, range_bits, digit_bits),
+                        kt.device)
+    _lib.check(fn(kt.data_ptr(), None, ko.data_ptr(), vo.data_ptr(), n, 0, range_bits,
+                  digit_bits, ws.data_ptr(), ws.numel(), _lib.stream_ptr(kt.device)), "sort")
+    perm = np.argsort(keys, kind="stable")
+    assert np.array_equal(vo.cpu().numpy().astype(np.int64), perm)
+    out = ko.cpu().numpy().view(np.uint64 if key_bytes == 8 else np.uint32)
+    assert np.array_equal(out, keys[perm])
+
+
+def test_autotuned_sort_plan_is_a_valid_width(ygg):
+    from yggdrax_amd import autotune
+    autotune.enable(True)
+    try:
+        bits = autotune.sort_digit_bits(4, 19, 1 << 18)
+    finally:
+        autotune.enable(False)
+    assert bits in autotune.SORT_DIGIT_BITS
+    assert autotune.plans()["sort/k4/r19/n19"] == bits
+
+
 def test_identical_points_stable(ygg):
     """tests/unit/test_tree.py:193-214: identical points keep input order."""
     pos = np.full((64, 3), 0.25, np.float32)

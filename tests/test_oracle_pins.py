@@ -470,7 +470,7 @@ def test_mlp_reference_gradient_matches_finite_differences():
     rng = np.random.default_rng(0)
     p = rng.normal(0, 0.3, 9091)
     x = rng.normal(0, 1, (16, 8))
-    t = rng.normal(0, 1, (16, 3))
+    t = rng.poisson(3.0, (16, 3)).astype(np.float64)
     m = (rng.uniform(size=(16, 3)) > 0.3).astype(np.float64)
     loss, g = mlp_ref.loss_and_grad(p, x, t, m, rounding=False)
     idx = rng.choice(p.size, 60, replace=False)
@@ -481,6 +481,23 @@ def test_mlp_reference_gradient_matches_finite_differences():
         lm, _ = mlp_ref.loss_and_grad(p - e, x, t, m, rounding=False)
         fd = (lp - lm) / 2e-6
         assert abs(fd - g[i]) <= 1e-5 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_mlp_reference_poisson_optimum_is_the_masked_mean():
+    """With every weight zero the output is b4; the Poisson NLL is stationary
+    exactly where 2^b4 equals the masked mean count (the MLE of a rate)."""
+    from oracle import mlp_ref
+    rng = np.random.default_rng(1)
+    t = rng.poisson(7.0, (64, 3)).astype(np.float64)
+    m = (rng.uniform(size=(64, 3)) > 0.4).astype(np.float64)
+    p = np.zeros(9091)
+    mean = (t * m).sum(0) / m.sum(0)
+    p[-3:] = np.log2(mean)
+    _, g = mlp_ref.loss_and_grad(p, rng.normal(size=(64, 8)), t, m, rounding=False)
+    assert np.allclose(g[-3:], 0.0, atol=1e-12)
+    p[-3:] += 0.1
+    _, g = mlp_ref.loss_and_grad(p, rng.normal(size=(64, 8)), t, m, rounding=False)
+    assert np.all(g[-3:] > 0)
 
 
 def test_oracle_policy_walk_with_mac_policy_equals_walk():

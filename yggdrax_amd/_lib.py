@@ -68,9 +68,10 @@ class FlatWalkParams(ctypes.Structure):
 
 
 _SIGS = {
-    "ygg_sort_workspace_bytes": (_SZ, [_I64, _INT, _INT, _INT]),
-    "ygg_sort_pairs_u64": (_INT, [_P, _P, _P, _P, _I64, _INT, _INT, _P, _SZ, _P]),
-    "ygg_sort_pairs_u32": (_INT, [_P, _P, _P, _P, _I64, _INT, _INT, _P, _SZ, _P]),
+    "ygg_sort_workspace_bytes": (_SZ, [_I64, _INT, _INT, _INT, _INT]),
+    "ygg_sort_default_digit_bits": (_INT, [_INT, _INT, _INT]),
+    "ygg_sort_pairs_u64": (_INT, [_P, _P, _P, _P, _I64, _INT, _INT, _INT, _P, _SZ, _P]),
+    "ygg_sort_pairs_u32": (_INT, [_P, _P, _P, _P, _I64, _INT, _INT, _INT, _P, _SZ, _P]),
     "ygg_bounds_workspace_bytes": (_SZ, []),
     "ygg_infer_bounds": (_INT, [_P, _I64, _P, _P, _SZ, _P]),
     "ygg_morton_encode": (_INT, [_P, _I64, _P, _P, _P]),
@@ -99,7 +100,7 @@ _SIGS = {
     "ygg_cap_counts": (_INT, [_P, _I64, _I32, _P, _P]),
     "ygg_far_output": (_INT, [_I64, _P, _P, _P, _I32, _P, _P, _P, _INT, _P, _P, _P, _P, _P]),
     "ygg_near_output": (_INT, [_I64, _P, _P, _I32, _P, _P, _P, _I64, _INT, _P, _P, _P]),
-    "ygg_policy_geometry": (_INT, [_P, _P, _I64, _I64, _INT, _F, _F] + [_P] * 11 + [_P]),
+    "ygg_policy_geometry": (_INT, [_P, _P, _I64, _I64, _INT, _F, _F] + [_P] * 10 + [_P]),
     "ygg_policy_apply": (_INT, [_P, _P, _I64, _I64] + [_P] * 5 + [_I64] + [_P] * 4 + [_I64] +
                          [_P] * 2 + [_I64] + [_P] * 3),
     "ygg_mlp_num_params": (_INT, []),
@@ -108,8 +109,8 @@ _SIGS = {
     "ygg_mlp_train_step": (_INT, [_P, _P, _P, _P, _I64, _F, _P, _P, _P, _I64, _P, _P, _P, _P]),
     "ygg_mlp_params_to_bf16": (_INT, [_P, _P, _P]),
     "ygg_mlp_features": (_INT, [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
-    "ygg_mlp_totals": (_INT, [_P, _I64, _P, _P]),
-    "ygg_mlp_far_counts": (_INT, [_P, _I64, _P, _P]),
+    "ygg_mlp_totals": (_INT, [_P, _I64, _F, _F, _F, _P, _P]),
+    "ygg_mlp_far_counts": (_INT, [_P, _I64, _F, _P, _P]),
 }
 
 _lib = None

@@ -574,7 +574,7 @@ int ygg_morton_encode(const float* pos, int64_t n, const float* bounds6, uint64_
 
 size_t ygg_morton_sort_workspace_bytes(int64_t n) {
   return align_up(sizeof(uint64_t) * (size_t)n) + align_up(sizeof(uint32_t) * 8 * 256) +
-         ygg::sort_workspace_bytes(n, 8, 8);
+         ygg::sort_workspace_bytes(n, 8, 64);
 }
 
 int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t* sorted_codes,
@@ -584,7 +584,7 @@ int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t*
   Arena a(ws, ws_bytes);
   uint64_t* codes = a.take<uint64_t>((size_t)n);
   uint32_t* hist = a.take<uint32_t>(8 * 256);
-  size_t rest = ygg::sort_workspace_bytes(n, 8, 8);
+  size_t rest = ygg::sort_workspace_bytes(n, 8, 64);
   char* sws = a.take<char>(rest);
   if (!codes || !hist || !sws) return YGG_ERR_WORKSPACE;
   int rc = fill_u32(hist, 0u, 8 * 256, st);
@@ -613,7 +613,7 @@ size_t ygg_radix_build_workspace_bytes(int64_t n, int64_t leaf_size) {
   a.take<uint32_t>((size_t)T);  // parent encodings
   a.take<uint32_t>(64);         // level counts
   a.take<uint32_t>((size_t)T);  // sorted level keys
-  a.take<char>(ygg::sort_workspace_bytes(T, 4, 4));
+  a.take<char>(ygg::sort_workspace_bytes(T, 4, 32));
   a.take<int32_t>((size_t)T * 2);  // pointer doubling sc
   a.take<int64_t>((size_t)T * 2);  // pointer doubling dist
   return a.off;
@@ -643,7 +643,7 @@ int ygg_radix_build(const uint64_t* sorted_codes, int64_t n, int64_t leaf_size,
   uint32_t* penc = a.take<uint32_t>((size_t)T);
   uint32_t* lcount = a.take<uint32_t>(64);
   uint32_t* skeys = a.take<uint32_t>((size_t)T);
-  size_t sbytes = ygg::sort_workspace_bytes(T, 4, 4);
+  size_t sbytes = ygg::sort_workspace_bytes(T, 4, 32);
   char* sws = a.take<char>(sbytes);
   if (!penc || !lcount || !skeys || !sws) return YGG_ERR_WORKSPACE;
   if ((rc = fill_u32(penc, 0u, T, st))) return rc;
@@ -673,7 +673,7 @@ int ygg_levels_pointer_doubling(const YggRadixTopology* o, int64_t num_nodes, vo
   uint32_t* penc = a.take<uint32_t>((size_t)T);
   uint32_t* lcount = a.take<uint32_t>(64);
   uint32_t* skeys = a.take<uint32_t>((size_t)T);
-  size_t sbytes = ygg::sort_workspace_bytes(T, 4, 4);
+  size_t sbytes = ygg::sort_workspace_bytes(T, 4, 32);
   char* sws = a.take<char>(sbytes);
   int32_t* sc = a.take<int32_t>((size_t)T * 2);
   int64_t* dist = a.take<int64_t>((size_t)T * 2);
@@ -719,8 +719,8 @@ size_t ygg_octree_meta_workspace_bytes(int64_t L) {
   a.take<uint32_t>(32);             // level counts
   a.take<uint32_t>((size_t)L);      // leaf keys
   a.take<uint32_t>((size_t)L);      // native order
-  size_t s1 = ygg::sort_workspace_bytes(M, 8, 8);
-  size_t s2 = ygg::sort_workspace_bytes(M > T ? M : T, 4, 4);
+  size_t s1 = ygg::sort_workspace_bytes(M, 8, 64);
+  size_t s2 = ygg::sort_workspace_bytes(M > T ? M : T, 4, 32);
   a.take<char>(s1 > s2 ? s1 : s2);
   a.take<char>(ygg::scan_workspace_bytes(M));
   return a.off;
@@ -749,8 +749,8 @@ int ygg_octree_meta(const uint64_t* sc, const int32_t* node_ranges, int64_t L64,
   uint32_t* lcount = a.take<uint32_t>(32);
   uint32_t* lkey = a.take<uint32_t>((size_t)L);
   uint32_t* native = a.take<uint32_t>((size_t)L);
-  size_t s1 = ygg::sort_workspace_bytes(M, 8, 8);
-  size_t s2 = ygg::sort_workspace_bytes(M > T ? M : T, 4, 4);
+  size_t s1 = ygg::sort_workspace_bytes(M, 8, 64);
+  size_t s2 = ygg::sort_workspace_bytes(M > T ? M : T, 4, 32);
   size_t sb = s1 > s2 ? s1 : s2;
   char* sws = a.take<char>(sb);
   size_t scb = ygg::scan_workspace_bytes(M);
@@ -793,8 +793,7 @@ int ygg_octree_meta(const uint64_t* sc, const int32_t* node_ranges, int64_t L64,
   hipLaunchKernelGGL(k_radix_to_oct, dim3(gT), dim3(256), 0, st, sc, node_ranges, rc, rd, NI, T,
                      oaddr, M, o->radix_node_to_oct, lkey);
   YGG_LAUNCH_CHECK();
-  int bits = 32 - __builtin_clz((unsigned)M);  // values in [0, M]
-  int end_bit = ((bits + 7) / 8) * 8;
+  int end_bit = 32 - __builtin_clz((unsigned)M);  // values in [0, M]
   if ((rcode = sort_pairs_u32(lkey, nullptr, stmp, native, L, 0, end_bit, sws, sb, st)))
     return rcode;
   hipLaunchKernelGGL(k_leaf_order, dim3(gT), dim3(256), 0, st, native, NI, L, T, o->leaf_indices,

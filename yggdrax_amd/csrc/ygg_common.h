@@ -89,15 +89,26 @@ __device__ __forceinline__ unsigned mbcnt64(uint64_t mask) {
 // Histogram update aggregated over the wave: lanes holding the same 8-bit
 // digit are found with eight ballots and the lowest of them adds the group
 // size, so clustered digits cost one LDS atomic per distinct digit.
-__device__ __forceinline__ void wave_hist_add(uint32_t* h, unsigned d, bool valid) {
-  uint64_t peers = __ballot(valid);
+// Lanes of `peers` whose BITS-bit digit equals this lane's (BITS ballots).
+template <int BITS>
+__device__ __forceinline__ uint64_t wave_match_bits(unsigned d, uint64_t peers) {
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
+  for (int b = 0; b < BITS; ++b) {
     bool bit = (d >> b) & 1u;
     uint64_t bal = __ballot(bit);
     peers &= bit ? bal : ~bal;
   }
+  return peers;
+}
+
+template <int BITS>
+__device__ __forceinline__ void wave_hist_add_bits(uint32_t* h, unsigned d, bool valid) {
+  uint64_t peers = wave_match_bits<BITS>(d, __ballot(valid));
   if (valid && mbcnt64(peers) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+}
+
+__device__ __forceinline__ void wave_hist_add(uint32_t* h, unsigned d, bool valid) {
+  wave_hist_add_bits<8>(h, d, valid);
 }
 
 // Wave-wide inclusive prefix sum of a 32-bit value (wave64).
@@ -163,17 +174,21 @@ struct Arena {
 };
 
 // ---- primitives implemented in ygg_sort.hip --------------------------------
-// Stable LSD radix sort of (key, u32 value) pairs on key bits [begin, end).
+// Stable LSD radix sort of (key, u32 value) pairs on key bits [begin, end)
+// with digit_bits-wide digits (8..11; 0 = default_digit_bits).
 // vals_in == nullptr means values are the input positions 0..n-1.
-// A precomputed per-pass digit histogram (passes x 256, u32) may be passed
-// through `hist` with hist_ready = true.
-size_t sort_workspace_bytes(int64_t n, int key_bytes, int passes);
+// A precomputed per-pass digit histogram (passes x 2^digit_bits, u32) may be
+// passed through `hist`.
+constexpr int kMaxDigitBits = 11;
+int default_digit_bits(int key_bytes, int range_bits);
+size_t sort_workspace_bytes(int64_t n, int key_bytes, int range_bits, int digit_bits = 0);
 int sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
                    uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                   size_t ws_bytes, hipStream_t st, const uint32_t* hist = nullptr);
+                   size_t ws_bytes, hipStream_t st, const uint32_t* hist = nullptr,
+                   int digit_bits = 0);
 int sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                    uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                   size_t ws_bytes, hipStream_t st);
+                   size_t ws_bytes, hipStream_t st, int digit_bits = 0);
 
 // Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:
 // out[n] = total). `gather`, if non-null, reads counts[gather[i]].

@@ -1,10 +1,12 @@
 // Capacity model: an 8 -> 64 -> 64 -> 64 -> 3 ReLU MLP over per-node
 // features, with bfloat16 weights and float32 accumulation, trained and run
-// by these kernels. It predicts, per node, log2(1 + far-list count),
-// log2(1 + near-list count) (leaves) and log2(1 + pairs in the peak
-// wavefront with this node as target); summed, these size the walk's
-// streams in one pass. A misprediction only costs a fallback to counting:
-// the walk detects every stream overflow exactly.
+// by these kernels. Output j is the log2 of a Poisson rate: the expected
+// far-list count, near-list count (leaves) and number of pairs in the peak
+// wavefront with this node as target, each divided by a per-output scale.
+// Trained on the Poisson negative log-likelihood, the rates are unbiased, so
+// their sums estimate the walk's stream sizes in one pass. A misprediction
+// only costs a fallback to counting: the walk detects every stream overflow
+// exactly.
 //
 // Parameter layout (P = 9091 floats): W1[64][8] b1[64] W2[64][64] b2[64]
 // W3[64][64] b3[64] W4[3][64] b4[3]; W[o][i] multiplies input i into output o.
@@ -122,15 +124,22 @@ __global__ void k_lin_bwd_weight(const float* __restrict__ dy, const float* __re
 }
 
 // Masked mean squared error: dy = 2 (y - t) mask / B, loss accumulated.
-__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t,
-                      const float* __restrict__ mask, int B, float* __restrict__ dy,
-                      float* __restrict__ loss) {
+// Masked Poisson NLL with rate 2^y (y clamped at 30): per element
+// m (2^y - t y ln 2) / B, gradient m (2^y - t) ln 2 / B.
+constexpr float kLn2 = 0.69314718055994531f;
+constexpr float kMaxLogRate = 30.0f;
+
+__global__ void k_poisson(const float* __restrict__ y, const float* __restrict__ t,
+                          const float* __restrict__ mask, int B, float* __restrict__ dy,
+                          float* __restrict__ loss) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.0f;
   if (e < B * kOut) {
-    float d = (y[e] - t[e]) * mask[e];
-    dy[e] = 2.0f * d / (float)B;
-    l = d * d / (float)B;
+    float m = mask[e];
+    float yc = fminf(y[e], kMaxLogRate);
+    float lam = exp2f(yc);
+    dy[e] = y[e] > kMaxLogRate ? 0.0f : m * (lam - t[e]) * kLn2 / (float)B;
+    l = m * (lam - t[e] * yc * kLn2) / (float)B;
   }
   l = wave_sum_f(l);
   if (lane_id() == 0 && l != 0.0f) atomicAdd(loss, l);
@@ -195,15 +204,16 @@ __global__ void k_mlp_features(int T, int NI, const int32_t* __restrict__ node_l
   o[7] = k >= NI ? 1.0f : 0.0f;
 }
 
-// Sum of 2^y - 1 per output over [0, n) (capacity totals), y row-major [n][3].
-__global__ void k_mlp_totals(const float* __restrict__ y, int64_t n, int64_t NI,
-                             double* __restrict__ tot) {
+// Sum of scale_j 2^y_j per output over [0, n) (stream totals), y row-major
+// [n][3]; output 1 (near) counts leaves only.
+__global__ void k_mlp_totals(const float* __restrict__ y, int64_t n, int64_t NI, float s0,
+                             float s1, float s2, double* __restrict__ tot) {
   double a0 = 0, a1 = 0, a2 = 0;
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
-    a0 += (double)fmaxf(exp2f(y[3 * s]) - 1.0f, 0.0f);
-    if (s >= NI) a1 += (double)fmaxf(exp2f(y[3 * s + 1]) - 1.0f, 0.0f);
-    a2 += (double)fmaxf(exp2f(y[3 * s + 2]) - 1.0f, 0.0f);
+    a0 += (double)(s0 * exp2f(fminf(y[3 * s], kMaxLogRate)));
+    if (s >= NI) a1 += (double)(s1 * exp2f(fminf(y[3 * s + 1], kMaxLogRate)));
+    a2 += (double)(s2 * exp2f(fminf(y[3 * s + 2], kMaxLogRate)));
   }
   for (int off = 32; off > 0; off >>= 1) {
     a0 += __shfl_xor(a0, off, 64);
@@ -217,10 +227,10 @@ __global__ void k_mlp_totals(const float* __restrict__ y, int64_t n, int64_t NI,
   }
 }
 
-// parent_far[k] = 2^y[k][0] - 1 (the first pass's far-count prediction).
-__global__ void k_mlp_far_of(const float* __restrict__ y, int T, float* __restrict__ pf) {
+// far[k] = s0 2^y[k][0] (the first pass's far-count prediction).
+__global__ void k_mlp_far_of(const float* __restrict__ y, int T, float s0, float* __restrict__ pf) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < T) pf[k] = fmaxf(exp2f(y[3 * k]) - 1.0f, 0.0f);
+  if (k < T) pf[k] = s0 * exp2f(fminf(y[3 * k], kMaxLogRate));
 }
 
 }  // namespace ygg
@@ -273,8 +283,8 @@ int ygg_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, floa
   hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kOut), dim3(256), 0, st, W + kOffW4, W + kOffB4,
                      a3, B, kH, kOut, 0, yo);
   if (loss_out) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)loss_out, 1);
-  hipLaunchKernelGGL(k_mse, g((int64_t)B * kOut), dim3(256), 0, st, yo, target, mask, B, dyo,
-                     loss_out);
+  hipLaunchKernelGGL(k_poisson, g((int64_t)B * kOut), dim3(256), 0, st, yo, target, mask, B,
+                     dyo, loss_out);
   // layer 4
   hipLaunchKernelGGL(k_lin_bwd_weight, g(kOut * kH + kOut), dim3(256), 0, st, dyo, a3, B, kH,
                      kOut, grads + kOffW4, grads + kOffB4);
@@ -325,19 +335,23 @@ int ygg_mlp_features(int64_t num_nodes, const int32_t* node_level, const int32_t
   return YGG_OK;
 }
 
-int ygg_mlp_totals(const float* outputs, int64_t num_nodes, double* totals, void* stream) {
-  if (num_nodes < 1) return YGG_ERR_ARG;
+int ygg_mlp_totals(const float* outputs, int64_t num_nodes, float scale_far, float scale_near,
+                   float scale_queue, double* totals, void* stream) {
+  if (num_nodes < 1 || !outputs || !totals) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)totals, 6);
   hipLaunchKernelGGL(k_mlp_totals, dim3(grid_for(num_nodes, 256, 1024)), dim3(256), 0, st,
-                     outputs, num_nodes, (num_nodes - 1) / 2, totals);
+                     outputs, num_nodes, (num_nodes - 1) / 2, scale_far, scale_near, scale_queue,
+                     totals);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
 
-int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float* far_counts, void* stream) {
+int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float scale_far,
+                       float* far_counts, void* stream) {
+  if (num_nodes < 1 || !outputs || !far_counts) return YGG_ERR_ARG;
   hipLaunchKernelGGL(k_mlp_far_of, dim3(grid_for(num_nodes, 256)), dim3(256), 0,
-                     (hipStream_t)stream, outputs, (int)num_nodes, far_counts);
+                     (hipStream_t)stream, outputs, (int)num_nodes, scale_far, far_counts);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

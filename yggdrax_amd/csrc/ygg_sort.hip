@@ -1,4 +1,4 @@
-// Stable LSD radix sort (8-bit digits) for gfx950 wave64, plus scans.
+// Stable LSD radix sort for gfx950 wave64, digit width 8..11 bits, plus scans.
 //
 // Per digit pass one kernel ("one sweep"): each 256-thread workgroup takes
 // the next 4096-key tile from an atomic tile counter, ranks its keys stably
@@ -6,7 +6,10 @@
 // its digit counts for decoupled look-back, reorders the tile in LDS by digit
 // and writes every digit run out contiguously. A single histogram kernel
 // counts all passes' digits up front, so a pass reads and writes each
-// (key, value) pair exactly once.
+// (key, value) pair exactly once. Wider digits mean fewer passes over the
+// data but more look-back words per tile (2^bits per pass) and more LDS;
+// the width is a parameter so callers (and the autotuner) can pick the
+// cheapest pass count for a key range.
 #include "ygg_common.h"
 
 namespace ygg {
@@ -14,18 +17,21 @@ namespace ygg {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kTile = kSortThreads * kSortItems;  // 4096 keys
-constexpr int kBins = 256;
 constexpr int kWaves = kSortThreads / YGG_WAVE;      // 4
 constexpr int kWaveKeys = kTile / kWaves;            // 1024 keys per wave
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPre = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 
-template <typename K>
+constexpr int passes_for(int range_bits, int bits) { return (range_bits + bits - 1) / bits; }
+
+template <typename K, int BITS>
 __global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, int64_t n,
                                                     int begin_bit, int passes,
                                                     uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[8 * kBins];
+  constexpr int kBins = 1 << BITS;
+  constexpr int kMaxPasses = passes_for(8 * (int)sizeof(K), BITS);
+  __shared__ uint32_t h[kMaxPasses * kBins];
   for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) h[i] = 0;
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -34,28 +40,44 @@ __global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, 
     const bool valid = i < n;
     K k = valid ? keys[i] : (K)0;
     for (int p = 0; p < passes; ++p)
-      wave_hist_add(&h[p * kBins], (unsigned)((k >> (begin_bit + 8 * p)) & 0xFF), valid);
+      wave_hist_add_bits<BITS>(&h[p * kBins],
+                               (unsigned)((k >> (begin_bit + BITS * p)) & (kBins - 1)), valid);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x)
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
-// In-place exclusive scan of each pass's 256-bin histogram (one block/pass).
+// In-place exclusive scan of each pass's 2^BITS-bin histogram (one block per
+// pass; thread t owns bins [t * D, t * D + D)).
+template <int BITS>
 __global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
+  constexpr int kBins = 1 << BITS;
+  constexpr int D = kBins / kSortThreads;
   __shared__ uint32_t wsum[kWaves];
-  uint32_t* h = hist + blockIdx.x * kBins;
-  unsigned v = h[threadIdx.x];
-  unsigned inc = wave_inclusive_sum(v);
+  uint32_t* h = hist + (int64_t)blockIdx.x * kBins;
+  unsigned v[D];
+  unsigned t = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    v[j] = h[threadIdx.x * D + j];
+    t += v[j];
+  }
+  unsigned inc = wave_inclusive_sum(t);
   int w = threadIdx.x / YGG_WAVE;
   if (lane_id() == 63) wsum[w] = inc;
   __syncthreads();
-  unsigned base = 0;
-  for (int k = 0; k < w; ++k) base += wsum[k];
-  h[threadIdx.x] = base + inc - v;
+  unsigned run = 0;
+  for (int k = 0; k < w; ++k) run += wsum[k];
+  run += inc - t;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    h[threadIdx.x * D + j] = run;
+    run += v[j];
+  }
 }
 
-template <typename K>
+template <typename K, int BITS>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
                                                   K* __restrict__ keys_out,
@@ -63,6 +85,9 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
                                                   int shift, const uint32_t* __restrict__ hist_scan,
                                                   uint32_t* __restrict__ lookback,
                                                   uint32_t* __restrict__ tile_counter) {
+  constexpr int kBins = 1 << BITS;
+  constexpr unsigned kMask = kBins - 1;
+  constexpr int D = kBins / kSortThreads;  // digits owned per thread
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wave_hist[kWaves][kBins];
   __shared__ uint32_t s_digit_start[kBins];
@@ -93,14 +118,8 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   }
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    unsigned d = (unsigned)((key[k] >> shift) & 0xFF);
-    uint64_t peers = ~0ull;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      bool bit = (d >> b) & 1u;
-      uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
-    }
+    unsigned d = (unsigned)((key[k] >> shift) & kMask);
+    uint64_t peers = wave_match_bits<BITS>(d, ~0ull);
     unsigned below = mbcnt64(peers);
     unsigned cnt = (unsigned)__popcll(peers);
     unsigned before = s_wave_hist[w][d];
@@ -108,46 +127,63 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
     if (below == 0) s_wave_hist[w][d] = before + cnt;
   }
   __syncthreads();
-  // Thread `tid` owns digit `tid`.
-  const unsigned d0 = (unsigned)tid;
-  unsigned c[kWaves];
-  unsigned total = 0;
-#pragma unroll
-  for (int q = 0; q < kWaves; ++q) {
-    c[q] = s_wave_hist[q][d0];
-    s_wave_hist[q][d0] = total;  // exclusive prefix over waves
-    total += c[q];
-  }
+  // Thread `tid` owns digits [tid * D, tid * D + D).
+  unsigned tot[D];
+  unsigned tsum = 0;
   uint32_t* lb = lookback + tile * kBins;
-  atomic_store_agent(&lb[d0], (tile == 0 ? kFlagPre : kFlagAgg) | total);
-  unsigned inc = wave_inclusive_sum(total);
-  if (lane == 63) s_wsum[w] = inc;
-  unsigned excl_lb = 0;
-  if (tile > 0) {
-    int64_t pt = tile - 1;
-    while (true) {
-      uint32_t v = atomic_load_agent(&lookback[pt * kBins + d0]);
-      uint32_t f = v & ~kCountMask;
-      if (f == 0) {
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      excl_lb += v & kCountMask;
-      if (f == kFlagPre) break;
-      --pt;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const unsigned d = (unsigned)tid * D + j;
+    unsigned total = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) {
+      unsigned c = s_wave_hist[q][d];
+      s_wave_hist[q][d] = total;  // exclusive prefix over waves
+      total += c;
     }
-    atomic_store_agent(&lb[d0], kFlagPre | (excl_lb + total));
+    tot[j] = total;
+    tsum += total;
+    atomic_store_agent(&lb[d], (tile == 0 ? kFlagPre : kFlagAgg) | total);
+  }
+  unsigned inc = wave_inclusive_sum(tsum);
+  if (lane == 63) s_wsum[w] = inc;
+  unsigned excl_lb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const unsigned d = (unsigned)tid * D + j;
+    unsigned e = 0;
+    if (tile > 0) {
+      int64_t pt = tile - 1;
+      while (true) {
+        uint32_t v = atomic_load_agent(&lookback[pt * kBins + d]);
+        uint32_t f = v & ~kCountMask;
+        if (f == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        e += v & kCountMask;
+        if (f == kFlagPre) break;
+        --pt;
+      }
+      atomic_store_agent(&lb[d], kFlagPre | (e + tot[j]));
+    }
+    excl_lb[j] = e;
   }
   __syncthreads();
-  unsigned wb = 0;
-  for (int q = 0; q < w; ++q) wb += s_wsum[q];
-  const unsigned dstart = wb + inc - total;
-  s_digit_start[d0] = dstart;
-  s_global[d0] = hist_scan[d0] + excl_lb - dstart;
+  unsigned run = 0;
+  for (int q = 0; q < w; ++q) run += s_wsum[q];
+  run += inc - tsum;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const unsigned d = (unsigned)tid * D + j;
+    s_digit_start[d] = run;
+    s_global[d] = hist_scan[d] + excl_lb[j] - run;
+    run += tot[j];
+  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    unsigned d = (unsigned)((key[k] >> shift) & 0xFF);
+    unsigned d = (unsigned)((key[k] >> shift) & kMask);
     unsigned pos = s_digit_start[d] + s_wave_hist[w][d] + rank[k];
     s_keys[pos] = key[k];
     s_vals[pos] = val[k];
@@ -156,7 +192,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   const int64_t valid = (n - base) < kTile ? (n - base) : kTile;
   for (int p = tid; p < valid; p += kSortThreads) {
     K kk = s_keys[p];
-    unsigned d = (unsigned)((kk >> shift) & 0xFF);
+    unsigned d = (unsigned)((kk >> shift) & kMask);
     int64_t g = (int64_t)s_global[d] + p;
     keys_out[g] = kk;
     vals_out[g] = s_vals[p];
@@ -178,25 +214,35 @@ int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
 
 static inline int64_t num_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
 
-size_t sort_workspace_bytes(int64_t n, int key_bytes, int passes) {
+// 8-bit digits: a 4096-key tile then has runs of ~16 keys per digit, so the
+// scatter writes stay coalesced; measured on gfx950 at 1.5e8-3.3e8 u32
+// pairs, 10-bit digits (runs of ~4 keys) move each byte 2.7x slower, which
+// outweighs the pass saved. Wider digits remain available to the autotuner.
+int default_digit_bits(int key_bytes, int range_bits) {
+  (void)key_bytes;
+  (void)range_bits;
+  return 8;
+}
+
+size_t sort_workspace_bytes(int64_t n, int key_bytes, int range_bits, int digit_bits) {
+  if (digit_bits == 0) digit_bits = default_digit_bits(key_bytes, range_bits);
+  const int passes = passes_for(range_bits, digit_bits);
+  const size_t bins = (size_t)1 << digit_bits;
   Arena a(nullptr, 0);
   int64_t t = num_tiles(n);
-  a.take<char>((size_t)n * key_bytes);          // alternate keys
-  a.take<uint32_t>((size_t)n);                  // alternate values
-  a.take<uint32_t>((size_t)passes * kBins);     // histograms
-  a.take<uint32_t>((size_t)passes + (size_t)passes * t * kBins);  // counters + look-back
+  a.take<char>((size_t)n * key_bytes);                              // alternate keys
+  a.take<uint32_t>((size_t)n);                                      // alternate values
+  a.take<uint32_t>((size_t)passes * bins);                          // histograms
+  a.take<uint32_t>((size_t)passes + (size_t)passes * t * bins);     // counters + look-back
   return a.off;
 }
 
-template <typename K>
-static int sort_pairs_impl(const K* keys_in, const uint32_t* vals_in, K* keys_out,
+template <typename K, int BITS>
+static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_out,
                            uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
                            size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre) {
-  if (n <= 0) return YGG_OK;
-  if (n > (int64_t)kCountMask) return YGG_ERR_ARG;
-  if (end_bit <= begin_bit || keys_in == keys_out) return YGG_ERR_ARG;
-  const int passes = (end_bit - begin_bit + 7) / 8;
-  if (passes > 8) return YGG_ERR_ARG;
+  constexpr int kBins = 1 << BITS;
+  const int passes = passes_for(end_bit - begin_bit, BITS);
   const int64_t tiles = num_tiles(n);
   Arena a(ws, ws_bytes);
   K* kalt = (K*)a.take<char>((size_t)n * sizeof(K));
@@ -213,11 +259,11 @@ static int sort_pairs_impl(const K* keys_in, const uint32_t* vals_in, K* keys_ou
   } else {
     rc = fill_u32(hist, 0u, (int64_t)passes * kBins, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_digit_hist<K>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, keys_in,
-                       n, begin_bit, passes, hist);
+    hipLaunchKernelGGL((k_digit_hist<K, BITS>), dim3(grid_for(n, 256, 2048)), dim3(256), 0, st,
+                       keys_in, n, begin_bit, passes, hist);
     YGG_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(256), 0, st, hist);
+  hipLaunchKernelGGL(k_hist_scan<BITS>, dim3(passes), dim3(256), 0, st, hist);
   YGG_LAUNCH_CHECK();
   const K* src_k = keys_in;
   const uint32_t* src_v = vals_in;
@@ -225,8 +271,8 @@ static int sort_pairs_impl(const K* keys_in, const uint32_t* vals_in, K* keys_ou
     bool to_out = ((passes - 1 - p) % 2) == 0;
     K* dk = to_out ? keys_out : kalt;
     uint32_t* dv = to_out ? vals_out : valt;
-    hipLaunchKernelGGL(k_onesweep<K>, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, src_k,
-                       src_v, dk, dv, n, begin_bit + 8 * p, hist + p * kBins,
+    hipLaunchKernelGGL((k_onesweep<K, BITS>), dim3((unsigned)tiles), dim3(kSortThreads), 0, st,
+                       src_k, src_v, dk, dv, n, begin_bit + BITS * p, hist + p * kBins,
                        lookback + (int64_t)p * tiles * kBins, ctr + p);
     YGG_LAUNCH_CHECK();
     src_k = dk;
@@ -235,18 +281,47 @@ static int sort_pairs_impl(const K* keys_in, const uint32_t* vals_in, K* keys_ou
   return YGG_OK;
 }
 
+template <typename K>
+static int sort_pairs_impl(const K* keys_in, const uint32_t* vals_in, K* keys_out,
+                           uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
+                           size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre,
+                           int digit_bits) {
+  if (n <= 0) return YGG_OK;
+  if (n > (int64_t)kCountMask) return YGG_ERR_ARG;
+  if (end_bit <= begin_bit || keys_in == keys_out || begin_bit < 0 ||
+      end_bit > 8 * (int)sizeof(K))
+    return YGG_ERR_ARG;
+  if (digit_bits == 0) digit_bits = default_digit_bits((int)sizeof(K), end_bit - begin_bit);
+  switch (digit_bits) {
+    case 8:
+      return sort_pairs_bits<K, 8>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
+                                   ws, ws_bytes, st, hist_pre);
+    case 9:
+      return sort_pairs_bits<K, 9>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
+                                   ws, ws_bytes, st, hist_pre);
+    case 10:
+      return sort_pairs_bits<K, 10>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
+                                    ws, ws_bytes, st, hist_pre);
+    case 11:
+      return sort_pairs_bits<K, 11>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
+                                    ws, ws_bytes, st, hist_pre);
+    default:
+      return YGG_ERR_ARG;
+  }
+}
+
 int sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
                    uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                   size_t ws_bytes, hipStream_t st, const uint32_t* hist) {
+                   size_t ws_bytes, hipStream_t st, const uint32_t* hist, int digit_bits) {
   return sort_pairs_impl<uint64_t>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
-                                   ws, ws_bytes, st, hist);
+                                   ws, ws_bytes, st, hist, digit_bits);
 }
 
 int sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                    uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                   size_t ws_bytes, hipStream_t st) {
+                   size_t ws_bytes, hipStream_t st, int digit_bits) {
   return sort_pairs_impl<uint32_t>(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit,
-                                   ws, ws_bytes, st, nullptr);
+                                   ws, ws_bytes, st, nullptr, digit_bits);
 }
 
 // ---- exclusive scan: int32 counts -> int64 offsets --------------------------
@@ -362,23 +437,27 @@ int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int6
 
 extern "C" {
 
-size_t ygg_sort_workspace_bytes(int64_t n, int key_bytes, int begin_bit, int end_bit) {
-  int passes = (end_bit - begin_bit + 7) / 8;
-  return ygg::sort_workspace_bytes(n, key_bytes, passes);
+size_t ygg_sort_workspace_bytes(int64_t n, int key_bytes, int begin_bit, int end_bit,
+                                int digit_bits) {
+  return ygg::sort_workspace_bytes(n, key_bytes, end_bit - begin_bit, digit_bits);
+}
+
+int ygg_sort_default_digit_bits(int key_bytes, int begin_bit, int end_bit) {
+  return ygg::default_digit_bits(key_bytes, end_bit - begin_bit);
 }
 
 int ygg_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
-                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                       size_t ws_bytes, void* stream) {
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, int digit_bits,
+                       void* ws, size_t ws_bytes, void* stream) {
   return ygg::sort_pairs_u64(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit, ws,
-                             ws_bytes, (hipStream_t)stream);
+                             ws_bytes, (hipStream_t)stream, nullptr, digit_bits);
 }
 
 int ygg_sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                       size_t ws_bytes, void* stream) {
+                       uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, int digit_bits,
+                       void* ws, size_t ws_bytes, void* stream) {
   return ygg::sort_pairs_u32(keys_in, vals_in, keys_out, vals_out, n, begin_bit, end_bit, ws,
-                             ws_bytes, (hipStream_t)stream);
+                             ws_bytes, (hipStream_t)stream, digit_bits);
 }
 
 }  // extern "C"

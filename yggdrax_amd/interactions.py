@@ -33,6 +33,9 @@ _AUTO_INTERACTION_MAX = 16384
 MAX_GENERATIONS = 1024
 # Opt-in per-generation kernel timing (used by the benchmark's roofline leg).
 WALK_TIMING = {"enabled": False, "log": []}
+# Number of counting walks run by the flat path (a capacity hint that holds
+# runs none; a hint that overflows falls back to one).
+FLAT_STATS = {"count_walks": 0, "hinted_walks": 0, "hint_overflows": 0}
 MAC_TYPES = {"bh": 0, "dehnen": 1, "engblom": 2}
 FLAT_GRID_BLOCKS = 2048
 # Forces the attempt-by-attempt walk with per-row slots (the reference's
@@ -250,10 +253,11 @@ class _WalkContext:
         self.decisions = torch.empty(3, dtype=torch.int64, device=self.dev)
         self.timing = WALK_TIMING["enabled"]
         self.gen_hint = int(1.5 * max(1, dt.num_levels)) + 4
+        self.grid_blocks = FLAT_GRID_BLOCKS
         self.kernel_log = WALK_TIMING["log"]
 
     def run_flat(self, *, emit: bool, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
-                 gen_hint: int = 0, hist_gen: int = -1, node_hist=None):
+                 gen_hint: int = 0, hist_gen: int = -1, node_hist=None, grid_blocks: int = 0):
         """One flat-emission walk over the whole tree (ygg_walk_flat)."""
         dt = self.dt
         dev = self.dev
@@ -270,7 +274,7 @@ class _WalkContext:
         p.num_nodes, p.num_internal, p.num_leaves, p.root = (dt.num_nodes, dt.num_internal,
                                                              dt.num_leaves, 0)
         p.mac_type, p.emit, p.max_generations = self.mac, int(emit), MAX_GENERATIONS
-        p.grid_blocks = FLAT_GRID_BLOCKS
+        p.grid_blocks = int(grid_blocks) if grid_blocks else self.grid_blocks
         p.gen_hint = int(gen_hint) if gen_hint else int(self.gen_hint)
         p.theta_sq, p.theta_root = self.theta_sq, self.theta_root
         p.queue_cap, p.far_cap, p.near_cap = Q, int(far_cap), int(near_cap)
@@ -518,18 +522,20 @@ class _FlatResult(NamedTuple):
 
 
 def _sort_entries(keys, vals, n, rows):
-    """Stable radix sort of (row key, value) pairs by row key."""
+    """Stable radix sort of (row key, value) pairs by row key; the digit
+    width comes from the autotuner's plan for this size."""
+    from . import autotune
     lib = _lib.load()
     dev = keys.device
-    bits = max(1, int(rows).bit_length())
-    end_bit = ((bits + 7) // 8) * 8
+    end_bit = max(1, int(rows).bit_length())
     ko = torch.empty(max(1, n), dtype=torch.int32, device=dev)
     vo = torch.empty(max(1, n), dtype=torch.int32, device=dev)
     if n > 0:
-        ws = _lib.workspace(lib.ygg_sort_workspace_bytes(n, 4, 0, end_bit), dev)
+        bits = autotune.sort_digit_bits(4, end_bit, n)
+        ws = _lib.workspace(lib.ygg_sort_workspace_bytes(n, 4, 0, end_bit, bits), dev)
         _lib.check(lib.ygg_sort_pairs_u32(keys.data_ptr(), vals.data_ptr(), ko.data_ptr(),
-                                          vo.data_ptr(), n, 0, end_bit, ws.data_ptr(), ws.numel(),
-                                          _lib.stream_ptr(dev)), "sort entries")
+                                          vo.data_ptr(), n, 0, end_bit, bits, ws.data_ptr(),
+                                          ws.numel(), _lib.stream_ptr(dev)), "sort entries")
     off = torch.empty(rows + 1, dtype=torch.int64, device=dev)
     _lib.check(lib.ygg_row_offsets(ko.data_ptr(), n, rows, off.data_ptr(), _lib.stream_ptr(dev)),
                "row offsets")
@@ -579,13 +585,19 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     and the emission walk is repeated at exact sizes."""
     T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
     qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
+    from . import autotune
+    ctx.grid_blocks = autotune.walk_grid_blocks(
+        T, lambda b: ctx.run_flat(emit=False, queue_cap=qbig, grid_blocks=b), FLAT_GRID_BLOCKS)
     em = None
     if sizes is not None:
+        FLAT_STATS["hinted_walks"] += 1
         em = ctx.run_flat(emit=True, queue_cap=sizes["queue"], far_cap=sizes["far"],
                           near_cap=sizes["near"])
         if em["queue_overflow"] or em["far_overflow"] or em["near_overflow"]:
+            FLAT_STATS["hint_overflows"] += 1
             em = None
     if em is None:
+        FLAT_STATS["count_walks"] += 1
         tot = ctx.run_flat(emit=False, queue_cap=qbig)
         if tot["queue_overflow"]:
             if lad.use_count:

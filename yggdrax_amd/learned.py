@@ -1,20 +1,23 @@
 """Learned stream capacities for the dual-tree walk.
 
 A small MLP (8 -> 64 -> 64 -> 64 -> 3, ReLU, bfloat16 weights, float32
-accumulation; HIP kernels in csrc/ygg_mlp.hip) predicts for every node
-log2(1 + far-list count), log2(1 + near-list count) (leaves) and
-log2(1 + number of pairs with this node as target in the largest
-wavefront). Per-node features: level, particle count, max half extent,
-sphere radius, parent far count, sibling particle count, density relative
-to the root, leaf flag (log-scaled). The parent far count is known exactly
-when training (the walk produces it); at inference a first pass predicts
-every node's far count with that feature zeroed and a second pass feeds
-the parent's prediction.
+accumulation; HIP kernels in csrc/ygg_mlp.hip) predicts for every node the
+log2 of a Poisson rate: the expected far-list count, near-list count
+(leaves) and number of pairs with this node as target in the largest
+wavefront, each over a per-output scale. Per-node features: level,
+particle count, max half extent, sphere radius, parent far count, sibling
+particle count, density relative to the root, leaf flag (log-scaled). The
+parent far count is known exactly when training (the walk produces it); at
+inference a first pass predicts every node's far count with that feature
+zeroed and a second pass feeds the parent's prediction.
 
-Summed and calibrated, the predictions give the far/near stream sizes and
-the pair-queue size, so the walk runs once instead of counting first. The
-walk detects every overflow exactly and falls back to counting, so lists
-never depend on the model.
+Trained on the Poisson negative log-likelihood, the rates are unbiased
+estimates of the counts, so their sums estimate the far/near stream sizes
+and the pair-queue size; a calibration factor, a margin and a small
+per-node floor (which covers small trees, whose far totals are tiny and
+erratic) turn them into capacities. The walk then runs once instead of
+counting first. The walk detects every overflow exactly and falls back to
+counting, so lists never depend on the model.
 """
 
 from __future__ import annotations
@@ -76,11 +79,18 @@ def features(dt, geom, parent_far: Optional[torch.Tensor] = None) -> torch.Tenso
     return x
 
 
+# Per-node capacity floors added to the calibrated sums: far entries per
+# node, near entries per leaf, queue pairs per node.
+FLOORS = np.array([8.0, 64.0, 8.0], np.float32)
+
+
 @dataclass
 class CapacityModel:
     params: np.ndarray                      # float32 master parameters
+    scale: np.ndarray = field(default_factory=lambda: np.ones(3, np.float32))
     calib: np.ndarray = field(default_factory=lambda: np.ones(3, np.float32))
     margin: np.ndarray = field(default_factory=lambda: np.full(3, 0.1, np.float32))
+    floor: np.ndarray = field(default_factory=lambda: FLOORS.copy())
     meta: dict = field(default_factory=dict)
     _dev: dict = field(default_factory=dict)
 
@@ -88,13 +98,15 @@ class CapacityModel:
     def load(cls, path: str = DEFAULT_WEIGHTS) -> "CapacityModel":
         z = np.load(path, allow_pickle=False)
         meta = {k[5:]: str(z[k]) for k in z.files if k.startswith("meta_")}
-        return cls(z["params"].astype(np.float32), z["calib"].astype(np.float32),
-                   z["margin"].astype(np.float32), meta)
+        return cls(z["params"].astype(np.float32), z["scale"].astype(np.float32),
+                   z["calib"].astype(np.float32), z["margin"].astype(np.float32),
+                   z["floor"].astype(np.float32), meta)
 
     def save(self, path: str = DEFAULT_WEIGHTS) -> None:
         os.makedirs(os.path.dirname(path), exist_ok=True)
         extra = {f"meta_{k}": np.array(str(v)) for k, v in self.meta.items()}
-        np.savez(path, params=self.params, calib=self.calib, margin=self.margin, **extra)
+        np.savez(path, params=self.params, scale=self.scale, calib=self.calib,
+                 margin=self.margin, floor=self.floor, **extra)
 
     def device_weights(self, device) -> torch.Tensor:
         key = str(device)
@@ -116,35 +128,46 @@ class CapacityModel:
         return y
 
     def predict_nodes(self, tree, geom) -> torch.Tensor:
-        """Two-pass per-node prediction (log2(1 + count) per output)."""
+        """Two-pass per-node prediction (log2 rate / scale per output)."""
         lib = _lib.load()
         dt = device_topology(tree)
         dev = geom.center.device
         y0 = self.forward(features(dt, geom, None))
         pf = torch.empty(dt.num_nodes, dtype=torch.float32, device=dev)
-        _lib.check(lib.ygg_mlp_far_counts(y0.data_ptr(), dt.num_nodes, pf.data_ptr(),
-                                          _lib.stream_ptr(dev)), "far of")
+        _lib.check(lib.ygg_mlp_far_counts(y0.data_ptr(), dt.num_nodes, float(self.scale[0]),
+                                          pf.data_ptr(), _lib.stream_ptr(dev)), "far counts")
         return self.forward(features(dt, geom, pf))
 
     def predict_totals(self, tree, geom) -> np.ndarray:
+        """Uncalibrated sums of the predicted rates (far, near, queue)."""
         dt = device_topology(tree)
         y = self.predict_nodes(tree, geom)
         tot = torch.empty(3, dtype=torch.float64, device=y.device)
-        _lib.check(_lib.load().ygg_mlp_totals(y.data_ptr(), dt.num_nodes, tot.data_ptr(),
-                                              _lib.stream_ptr(y.device)), "totals")
+        s = [float(v) for v in self.scale]
+        _lib.check(_lib.load().ygg_mlp_totals(y.data_ptr(), dt.num_nodes, s[0], s[1], s[2],
+                                              tot.data_ptr(), _lib.stream_ptr(y.device)),
+                   "totals")
         return tot.cpu().numpy()
+
+    @staticmethod
+    def _units(tree_or_nodes) -> np.ndarray:
+        T = int(getattr(tree_or_nodes, "num_nodes", tree_or_nodes))
+        return np.array([T, (T + 1) // 2, T], np.float64)
+
+    def capacities(self, tree, totals: np.ndarray) -> np.ndarray:
+        return totals * self.calib * (1.0 + self.margin) + self.floor * self._units(tree)
 
     def capacity_hint(self, tree, geom) -> dict:
         """Far / near stream and queue capacities for a one-pass walk."""
-        t = self.predict_totals(tree, geom) * self.calib * (1.0 + self.margin)
-        return {"far": int(t[0]) + 1024, "near": int(t[1]) + 1024, "queue": int(t[2]) + 1024}
+        c = self.capacities(tree, self.predict_totals(tree, geom))
+        return {"far": int(c[0]) + 1024, "near": int(c[1]) + 1024, "queue": int(c[2]) + 1024}
 
 
 # ---- training data from the walk itself ------------------------------------
 
 def node_targets(tree, geom, theta, mac_type="dehnen"):
-    """Per-node targets (log2(1 + far), log2(1 + near), log2(1 + peak-wavefront
-    pairs as target)) and masks, from the exact walk."""
+    """Per-node target counts (far-list entries, near-list entries, pairs
+    with the node as target in the peak wavefront) and masks, from the walk."""
     dt = device_topology(tree)
     dev = geom.center.device
     inter, nbrs = build_interactions_and_neighbors(tree, geom, theta=theta, mac_type=mac_type,
@@ -163,60 +186,66 @@ def node_targets(tree, geom, theta, mac_type="dehnen"):
         peak = tot["peak"]
     else:
         peak = 1
-    y = torch.stack([torch.log2(1 + far), torch.log2(1 + near),
-                     torch.log2(1 + hist.to(torch.float32))], dim=1).contiguous()
+    y = torch.stack([far, near, hist.to(torch.float32)], dim=1).contiguous()
     mask = torch.ones_like(y)
     mask[:NI, 1] = 0.0
     totals = np.array([float(inter.counts.sum()), float(nbrs.counts.sum()), float(peak)])
     return y, mask, far, totals
 
 
-def collect(systems, device="cuda"):
-    """systems: iterable of (name, builder, pos, mass, leaf, theta). Returns
-    features (with exact parent far counts), targets, masks and per-system
-    true totals."""
+def collect(systems, device="cuda", keep_trees=False):
+    """systems: iterable of (name, builder, make, leaf, theta), make() ->
+    (positions, masses). Returns features (with exact parent far counts),
+    targets, masks and per-system records (true totals; the tree and
+    geometry too when keep_trees)."""
     import yggdrax_amd as ygg
     xs, ys, ms, recs = [], [], [], []
-    for name, builder, pos, mass, leaf, theta in systems:
+    for name, builder, make, leaf, theta in systems:
+        pos, mass = make()
         build = ygg.build_octree if builder == "octree" else ygg.build_tree
         tree, ps, _, _ = build(torch.from_numpy(pos).to(device), torch.from_numpy(mass).to(device),
                                leaf_size=leaf, return_reordered=True)
         geom = ygg.compute_tree_geometry(tree, ps)
         y, m, far, totals = node_targets(tree, geom, theta)
         dt = device_topology(tree)
-        x = features(dt, geom, far)
-        xs.append(x)
+        xs.append(features(dt, geom, far))
         ys.append(y)
         ms.append(m)
-        recs.append(dict(name=name, tree=tree, geom=geom, totals=totals))
+        rec = dict(name=name, totals=totals, num_nodes=int(tree.num_nodes))
+        if keep_trees:
+            rec.update(tree=tree, geom=geom)
+        recs.append(rec)
     return torch.cat(xs), torch.cat(ys), torch.cat(ms), recs
 
 
 def training_systems(seeds=range(1000, 1200)):
-    """200 seeded systems from the three benchmark generators (uniform cube,
-    Plummer sphere, clustered), at sizes that keep collection fast."""
+    """Seeded systems from the three benchmark generators with each
+    generator's tree parameters: uniform cube (octree, leaf 16, theta 0.6,
+    2^9..2^16 points), Plummer sphere (radix, leaf 32, theta 0.5,
+    2^13..2^20) and clustered (octree, leaf 64, theta 0.5, 2^15..2^24);
+    sizes log-uniform. Positions are generated lazily."""
     from . import inputs
     out = []
     for k, seed in enumerate(seeds):
         kind = k % 3
         rng = np.random.default_rng(seed)
         if kind == 0:
-            n = int(rng.choice([512, 2048, 8192]))
-            pos, mass = inputs.uniform_cube(n, seed)
-            out.append((f"uniform{n}_{seed}", "octree", pos, mass, 16, 0.6))
+            n = int(round(2 ** rng.uniform(9, 16)))
+            out.append((f"uniform{n}_{seed}", "octree",
+                        lambda n=n, s=seed: inputs.uniform_cube(n, s), 16, 0.6))
         elif kind == 1:
-            n = int(2 ** rng.integers(13, 18))
-            pos, mass = inputs.plummer_sphere(n, seed)
-            out.append((f"plummer{n}_{seed}", "radix", pos, mass, 32, 0.5))
+            n = int(round(2 ** rng.uniform(13, 20)))
+            out.append((f"plummer{n}_{seed}", "radix",
+                        lambda n=n, s=seed: inputs.plummer_sphere(n, s), 32, 0.5))
         else:
-            n = int(2 ** rng.integers(15, 20))
-            pos, mass = inputs.clustered(n, seed)
-            out.append((f"clustered{n}_{seed}", "octree", pos, mass, 64, 0.5))
+            n = int(round(2 ** rng.uniform(15, 24)))
+            out.append((f"clustered{n}_{seed}", "octree",
+                        lambda n=n, s=seed: inputs.clustered(n, s), 64, 0.5))
     return out
 
 
 class Trainer:
-    """Adam on the masked MSE with the HIP training kernels."""
+    """Adam on the masked Poisson NLL with the HIP training kernels."""
 
     def __init__(self, params: np.ndarray, batch: int, device="cuda"):
         self.lib = _lib.load()
@@ -255,10 +284,19 @@ class Trainer:
         return self.g.clone()
 
 
+def target_scale(y, m) -> np.ndarray:
+    """Per-output masked mean count (the unit each output's rate is in)."""
+    s = (y * m).sum(0) / m.sum(0).clamp(min=1.0)
+    return np.maximum(s.cpu().numpy().astype(np.float32), 1e-3)
+
+
 def train(x, y, m, *, steps=4000, batch=4096, lr=3e-3, seed=0, log=print):
-    """Trains on (x, y, m). Half of every batch has the parent far-count
-    feature zeroed so one network serves both inference passes."""
+    """Trains on features x, target counts y and masks m; returns (params,
+    scale). Half of every batch has the parent far-count feature zeroed so
+    one network serves both inference passes."""
     g = torch.Generator(device="cpu").manual_seed(seed)
+    scale = target_scale(y, m)
+    ys = (y / torch.from_numpy(scale).to(y.device)).contiguous()
     tr = Trainer(init_params(seed), batch, x.device)
     n = x.shape[0]
     t0 = time.time()
@@ -267,25 +305,27 @@ def train(x, y, m, *, steps=4000, batch=4096, lr=3e-3, seed=0, log=print):
         xb = x[idx].clone()
         xb[: batch // 2, 4] = 0.0
         rate = lr * 0.5 * (1 + math.cos(math.pi * it / steps))
-        tr.step(xb.contiguous(), y[idx].contiguous(), m[idx].contiguous(), rate)
+        tr.step(xb.contiguous(), ys[idx].contiguous(), m[idx].contiguous(), rate)
         if log and (it % 500 == 0 or it == steps - 1):
             log(f"[train] step {it} loss {float(tr.loss):.5f} ({time.time() - t0:.1f} s)")
-    return tr.p.cpu().numpy()
+    return tr.p.cpu().numpy(), scale
 
 
 def calibrate(model: CapacityModel, recs, quantile=0.98) -> None:
     """calib = median(true / predicted total) per output; margin so that the
-    `quantile` of calibration systems fit."""
-    ratios = []
+    `quantile` of calibration systems fit once the per-node floors are
+    added."""
+    ratios, need = [], []
     for r in recs:
-        pred = model.predict_totals(r["tree"], r["geom"])
-        ratios.append(np.asarray(r["totals"], np.float64) / np.maximum(pred, 1.0))
-    ratios = np.array(ratios)
-    med = np.median(ratios, axis=0)
-    q = np.quantile(ratios / med, quantile, axis=0)
+        pred = np.maximum(model.predict_totals(r["tree"], r["geom"]), 1.0)
+        true = np.asarray(r["totals"], np.float64)
+        ratios.append(true / pred)
+        need.append((true - model.floor * model._units(r["tree"]), pred))
+    med = np.median(np.array(ratios), axis=0)
+    req = np.array([(nd / (med * pr)) - 1.0 for nd, pr in need])
     model.calib = med.astype(np.float32)
-    model.margin = np.maximum(q - 1.0, 0.02).astype(np.float32)
+    model.margin = np.maximum(np.quantile(req, quantile, axis=0), 0.05).astype(np.float32)
 
 
 __all__ = ["CapacityModel", "Trainer", "bf16_round", "calibrate", "collect", "features",
-           "init_params", "node_targets", "train", "training_systems"]
+           "init_params", "node_targets", "target_scale", "train", "training_systems"]

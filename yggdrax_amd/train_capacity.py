@@ -3,8 +3,9 @@
     python -m yggdrax_amd.train_capacity [--out PATH] [--steps N]
 
 Training data: 200 seeded systems (seeds 1000-1199) from the uniform-cube,
-Plummer and clustered generators; calibration on 30 held-out systems
-(seeds 2000-2029). Every target comes from the project's own walk.
+Plummer and clustered generators at sizes up to 2^16, 2^20 and 2^24
+points; calibration on 30 held-out systems (seeds 2000-2029). Every target
+comes from the project's own walk.
 """
 
 from __future__ import annotations
@@ -32,20 +33,24 @@ def main():
     x, y, m, _ = learned.collect(learned.training_systems(range(1000, 1200)))
     print(f"[capacity] {x.shape[0]} node samples from 200 systems ({time.time() - t0:.1f} s)",
           flush=True)
-    params = learned.train(x, y, m, steps=args.steps, batch=args.batch, lr=args.lr,
-                           log=lambda s: print(s, flush=True))
-    model = learned.CapacityModel(params)
-    _, _, _, held = learned.collect(learned.training_systems(range(2000, 2030)))
+    
+    params, scale = learned.train(x, y, m, steps=args.steps, batch=args.batch, lr=args.lr,
+                                  log=lambda s: print(s, flush=True))
+    model = learned.CapacityModel(params, scale)
+    _, _, _, held = learned.collect(learned.training_systems(range(2000, 2030)), keep_trees=True)
     learned.calibrate(model, held)
-    fit = 0
+    fit, over = 0, []
     for r in held:
         hint = model.capacity_hint(r["tree"], r["geom"])
         t = r["totals"]
         fit += int(hint["far"] >= t[0] and hint["near"] >= t[1] and hint["queue"] >= t[2])
+        over.append([hint["far"] / max(t[0], 1), hint["near"] / max(t[1], 1)])
     model.meta = dict(train_seeds="1000-1199", calib_seeds="2000-2029", steps=args.steps,
                       held_out_fit=f"{fit}/{len(held)}", created=time.strftime("%Y-%m-%d"))
     model.save(args.out)
-    print(json.dumps(dict(calib=model.calib.tolist(), margin=model.margin.tolist(),
+    print(json.dumps(dict(scale=model.scale.tolist(), calib=model.calib.tolist(),
+                          margin=model.margin.tolist(),
+                          median_overalloc=np.median(np.array(over), axis=0).round(3).tolist(),
                           held_out_fit=f"{fit}/{len(held)}", seconds=round(time.time() - t0, 1),
                           out=args.out)), flush=True)
 
