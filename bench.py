@@ -12,8 +12,12 @@ Prints ONE JSON line. The roofline object is for the dominant kernel (the
 walk's generation kernel); its algorithmic bytes use the per-pair model
 2 x 32 B node records + 8 B queue read per evaluated pair, 8 B per pushed
 child pair and 4 B per emitted list entry (an atomic slot plus the entry in
-the fill pass). The CPU baseline is the oracle (single thread) on a 2^20
-sample from the same generator.
+the fill pass). `build_roofline` prices sort + build + geometry at 287
+compulsory bytes per point against HBM peak. The `learned` object compares
+the walk with learned capacities (one emission walk) against the plain
+count-then-fill walk on the same tree and checks the lists are identical.
+The CPU baseline is the oracle (single thread) on a 2^22-point sample from
+the same generator (build, geometry and the counting walk).
 """
 
 from __future__ import annotations
@@ -68,6 +72,22 @@ def stage_times(ygg, torch, pos, mass, leaf, theta):
                 num_nodes=int(tree.num_nodes), num_levels=int(tree.num_levels))
 
 
+def measured_traffic(log2n, leaf, theta):
+    """HBM bytes per emission-walk launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy), for the
+    workload it was measured on; None otherwise."""
+    if (log2n, leaf, theta) != (24, 64, 0.5):
+        return None
+    path = os.path.join(ROOT, "profiles", "pmc_hbm_traffic_clustered_2p24.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch_k_walk_flat_emit:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def walk_roofline(ygg, torch, pos, mass, leaf, theta):
     """Roofline of the walk's generation kernel (emission pass), timed with
     HIP events recorded on the launch stream around every generation."""
@@ -102,6 +122,21 @@ def walk_roofline(ygg, torch, pos, mass, leaf, theta):
                 pairs=pairs, pairs_per_s=round(pairs / (total_ms * 1e-3)),
                 bytes_per_pair=round(total_bytes / max(1, pairs), 2),
                 emit_walk_ms=round(total_ms, 4), count_walk_ms=round(count_ms, 4))
+
+
+# Compulsory bytes per point of sort + build + geometry with 8-bit radix
+# digits: bounds 12, encode 20, sort 8 passes x 24 + 8 = 200, reorder 40,
+# Karras + levels ~0.6 and octree metadata ~1 (per point at leaf 64),
+# geometry 13.
+BUILD_BYTES_PER_POINT = 287.0
+
+
+def build_roofline(n, stages):
+    ms = stages["build_ms"] + stages["geometry_ms"]
+    achieved = BUILD_BYTES_PER_POINT * n / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), bytes_per_point=BUILD_BYTES_PER_POINT,
+                stages="build_octree + compute_tree_geometry", ms=round(ms, 4))
 
 
 def _same_lists(torch, a, b, num_rows):
@@ -190,9 +225,12 @@ def main():
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--cpu-sample-log2n", type=int, default=20)
+    ap.add_argument("--cpu-sample-log2n", type=int, default=22)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (scales the HBM "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -200,6 +238,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        _log("copy probe: 3 copies of 1 GiB")
+        return
     import yggdrax_amd as ygg
     from yggdrax_amd import inputs
 
@@ -228,7 +274,10 @@ def main():
     stages = stage_times(ygg, torch, pos, mass, args.leaf, args.theta)
     _log(f"stages {stages}")
     roof = walk_roofline(ygg, torch, pos, mass, args.leaf, args.theta)
+    roof["traffic"] = measured_traffic(args.log2n, args.leaf, args.theta)
     _log(f"roofline {roof}")
+    build_roof = build_roofline(n, stages)
+    _log(f"build roofline {build_roof}")
     lrn = None
     if not args.no_learned:
         lrn = learned_leg(ygg, torch, pos, mass, args.leaf, args.theta, args.steps)
@@ -247,7 +296,8 @@ def main():
                                f"uniform background, seed 3), build_octree leaf {args.leaf}, "
                                f"dehnen theta {args.theta}, count pass + exact fill, int64 outputs",
                    "parallelism": "single GPU"},
-        "stages": stages, "learned": lrn, "roofline": roof, "cpu_baseline": cpu,
+        "stages": stages, "learned": lrn, "roofline": roof, "build_roofline": build_roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
