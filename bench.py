@@ -200,6 +200,98 @@ def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
                 near_overalloc=round(hint["near"] / max(1, nn), 3))
 
 
+def batched_leg(torch, steps, warmup):
+    """Batched workload: 1024 Plummer systems, N_i log-uniform in [256, 65536]
+    (PCG64 seed 4), build_octree leaf 32, dehnen theta 0.6. Times the batched
+    pipeline (one set of launches and one forest walk for all systems)
+    against the same systems run one call at a time through the single-system
+    path, and checks a sample of systems' lists are identical."""
+    from yggdrax_amd import batch, inputs
+    import yggdrax_amd as ygg
+    pos_np, mass_np, offsets = inputs.plummer_batch(1024, 4)
+    P = torch.from_numpy(pos_np).cuda()
+    M = torch.from_numpy(mass_np).cuda()
+    kw = dict(mac_type="dehnen")
+
+    def run_batched():
+        bt = batch.build_octree_batched(P, M, offsets, leaf_size=32)
+        g = batch.compute_tree_geometry_batched(bt)
+        return bt, batch.build_interactions_and_neighbors_batched(bt, g, 0.6, **kw)
+
+    from yggdrax_amd import learned
+    from yggdrax_amd import interactions as I
+    model = learned.CapacityModel.load()
+
+    def run_learned():
+        bt = batch.build_octree_batched(P, M, offsets, leaf_size=32)
+        g = batch.compute_tree_geometry_batched(bt)
+        hint = model.capacity_hint_batched(bt, g)
+        return bt, batch.build_interactions_and_neighbors_batched(bt, g, 0.6, capacity_hint=hint,
+                                                                  **kw), hint
+
+    def run_one(k):
+        p0, p1 = int(offsets[k]), int(offsets[k + 1])
+        t, ps, _, _ = ygg.build_octree(P[p0:p1], M[p0:p1], leaf_size=32, return_reordered=True)
+        g = ygg.compute_tree_geometry(t, ps)
+        return ygg.build_interactions_and_neighbors(t, g, theta=0.6, neighbor_positions=False,
+                                                    **kw)
+
+    for _ in range(max(1, warmup)):
+        run_batched()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bt, lists = run_batched()
+    torch.cuda.synchronize()
+    t_b = (time.perf_counter() - t0) / steps * 1e3
+    run_learned()
+    torch.cuda.synchronize()
+    c0 = I.FLAT_STATS["count_walks"]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, llists, hint = run_learned()
+    torch.cuda.synchronize()
+    t_l = (time.perf_counter() - t0) / steps * 1e3
+    learned_fallbacks = I.FLAT_STATS["count_walks"] - c0
+    run_one(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = [run_one(k) for k in range(len(offsets) - 1)]
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) * 1e3
+    same = True
+    for k in range(0, len(offsets) - 1, 97):
+        si, sn = outs[k]
+        bi, bn = lists.system(k)
+        li, ln = llists.system(k)
+        same &= bool(torch.equal(bi.counts, li.counts) and torch.equal(bn.counts, ln.counts)
+                     and _same_lists(torch, (bi.offsets, bi.counts, bi.sources),
+                                     (li.offsets, li.counts, li.sources), bi.counts.numel())
+                     and _same_lists(torch, (bn.offsets, bn.counts, bn.neighbors),
+                                     (ln.offsets, ln.counts, ln.neighbors), bn.counts.numel()))
+        same &= bool(torch.equal(si.counts, bi.counts) and torch.equal(si.offsets, bi.offsets)
+                     and torch.equal(sn.counts, bn.counts) and torch.equal(sn.offsets, bn.offsets)
+                     and _same_lists(torch, (si.offsets, si.counts, si.sources),
+                                     (bi.offsets, bi.counts, bi.sources), si.counts.numel())
+                     and _same_lists(torch, (sn.offsets, sn.counts, sn.neighbors),
+                                     (bn.offsets, bn.counts, bn.neighbors), sn.counts.numel()))
+    n = int(offsets[-1])
+    return dict(workload="1024 Plummer systems, N_i log-uniform [256, 65536] (seed 4), "
+                         "build_octree leaf 32, dehnen theta 0.6",
+                points=n, batched_ms=round(t_b, 3), batched_mpts_s=round(n / t_b / 1e3, 3),
+                learned_batched_ms=round(t_l, 3),
+                learned_batched_mpts_s=round(n / t_l / 1e3, 3),
+                learned_fallback_walks=learned_fallbacks,
+                learned_far_overalloc=round(hint["far"] / max(1, lists.stats["far_total"]), 3),
+                learned_near_overalloc=round(hint["near"] / max(1, lists.stats["near_total"]), 3),
+                per_system_loop_ms=round(t_s, 3), speedup_vs_per_system=round(t_s / t_b, 2),
+                learned_speedup_vs_per_system=round(t_s / t_l, 2),
+                learned_speedup_vs_batched_plain=round(t_b / t_l, 3),
+                far_entries=int(lists.stats["far_total"]),
+                near_entries=int(lists.stats["near_total"]), fallback_systems=len(bt.fallback),
+                sampled_systems_identical=same)
+
+
 def cpu_baseline(sample_n, leaf, theta):
     import oracle
     from yggdrax_amd import inputs
@@ -228,6 +320,7 @@ def main():
     ap.add_argument("--cpu-sample-log2n", type=int, default=22)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scales the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -282,6 +375,10 @@ def main():
     if not args.no_learned:
         lrn = learned_leg(ygg, torch, pos, mass, args.leaf, args.theta, args.steps)
         _log(f"learned {lrn}")
+    bat = None
+    if not args.no_batched:
+        bat = batched_leg(torch, args.steps, args.warmup)
+        _log(f"batched {bat}")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(1 << args.cpu_sample_log2n, args.leaf, args.theta)
@@ -296,7 +393,8 @@ def main():
                                f"uniform background, seed 3), build_octree leaf {args.leaf}, "
                                f"dehnen theta {args.theta}, count pass + exact fill, int64 outputs",
                    "parallelism": "single GPU"},
-        "stages": stages, "learned": lrn, "roofline": roof, "build_roofline": build_roof,
+        "stages": stages, "learned": lrn, "batched": bat, "roofline": roof,
+        "build_roofline": build_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

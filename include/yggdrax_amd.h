@@ -275,10 +275,18 @@ typedef struct YggFlatWalkParams {
   uint32_t* near_vals;
   int32_t hist_gen, pad1;     /* node_hist[t] counts generation hist_gen's pairs by target */
   int32_t* node_hist;         /* [num_nodes] or NULL */
+  const int32_t* roots;       /* [num_roots] seed pairs (r, r) of a forest, or NULL: (root, root) */
+  int64_t num_roots;
 } YggFlatWalkParams;
 
 int ygg_walk_flat(const YggFlatWalkParams* params, uint32_t* host_status,
                   int32_t* num_generations, float* gen_ms, void* stream);
+/* The same walk as exactly max_generations generation launches with no host
+ * synchronisation, so a caller can capture it into a HIP graph. Generation
+ * sizes, counters and overflow flags stay in device memory (gen_sizes,
+ * counters, flags); a wavefront still non-empty after the last launch is
+ * left in gen_sizes[max_generations] for the caller to detect. */
+int ygg_walk_flat_fixed(const YggFlatWalkParams* params, void* stream);
 
 /* offsets[r] = lower_bound(sorted_keys, r) for r in [0, rows]. */
 int ygg_row_offsets(const uint32_t* sorted_keys, int64_t n, int64_t rows, int64_t* offsets,
@@ -334,6 +342,97 @@ int ygg_mlp_totals(const float* outputs, int64_t num_nodes, float scale_far, flo
 /* far_counts[k] = scale_far * 2^y[k][0]: predicted far count of every node. */
 int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float scale_far,
                        float* far_counts, void* stream);
+
+/* ---- batched mode: many independent systems per launch ------------------ */
+
+/* A batch of S systems whose points are concatenated: system s owns points
+ * [point_offsets[s], point_offsets[s+1]) and leaves [leaf_offsets[s],
+ * leaf_offsets[s+1]) (leaf_offsets[s+1] - leaf_offsets[s] =
+ * ceil(n_s / leaf_size)); both tables live in device memory. Per-system
+ * outputs are laid out back to back: internal-node arrays at
+ * leaf_offsets[s] - s, node and octree-slot arrays at 2 leaf_offsets[s] - s,
+ * level_offsets at 65 s, oct_level_offsets at 23 s, radix scalars at 8 s,
+ * octree scalars at 4 s. Every stored index is local to its system, so
+ * each slice equals what the single-system entry point returns for that
+ * system alone. No reference counterpart: the reference builds one tree per
+ * call (yggdrax/tree.py:1482-1549); this is the batched mode of the same
+ * computation. */
+typedef struct YggBatch {
+  int32_t num_systems, pad0;
+  int64_t leaf_size;
+  const int32_t* point_offsets;  /* [S + 1], device */
+  const int32_t* leaf_offsets;   /* [S + 1], device */
+  int64_t num_points;            /* point_offsets[S] */
+  int64_t num_leaves;            /* leaf_offsets[S] */
+  int64_t max_leaves_per_system;
+} YggBatch;
+
+/* Per-system infer_bounds (yggdrax/bounds.py:11-33): bounds6[6 s + k]. */
+int ygg_batch_infer_bounds(const YggBatch* batch, const float* pos, float* bounds6, void* stream);
+/* Morton codes with each system's bounds, stably sorted by (system, code):
+ * perm holds global point indices. */
+size_t ygg_batch_morton_sort_workspace_bytes(const YggBatch* batch);
+int ygg_batch_morton_sort(const YggBatch* batch, const float* pos, const float* bounds6,
+                          uint64_t* sorted_codes, uint32_t* perm, void* ws, size_t ws_bytes,
+                          void* stream);
+/* Reordered positions/masses, per-system particle_indices and inverse
+ * permutation (local indices). */
+int ygg_batch_reorder(const YggBatch* batch, const float* pos, const float* mass,
+                      const uint32_t* perm, float* pos_sorted, float* mass_sorted,
+                      int32_t* perm_local, int32_t* inverse_local, void* stream);
+/* ygg_radix_build for every system (scalars [8 S]). Systems whose
+ * scalars[8 s + 2] (deep levels) is set need the single-system path. */
+size_t ygg_batch_radix_build_workspace_bytes(const YggBatch* batch);
+int ygg_batch_radix_build(const YggBatch* batch, const uint64_t* sorted_codes,
+                          const YggRadixTopology* out, void* ws, size_t ws_bytes, void* stream);
+/* ygg_octree_meta for every system (scalars [4 S]). */
+size_t ygg_batch_octree_meta_workspace_bytes(const YggBatch* batch);
+int ygg_batch_octree_meta(const YggBatch* batch, const uint64_t* sorted_codes,
+                          const int32_t* node_ranges, const YggOctreeMeta* out, void* ws,
+                          size_t ws_bytes, void* stream);
+/* ygg_radix_leaf_order for every system. */
+int ygg_batch_radix_leaf_order(const YggBatch* batch, int32_t* leaf_indices,
+                               int32_t* leaf_position, void* stream);
+/* compute_tree_geometry for every (well-formed) system: segment tree over
+ * all leaves of the batch, one range query per internal node. */
+size_t ygg_batch_geometry_workspace_bytes(const YggBatch* batch);
+int ygg_batch_geometry(const YggBatch* batch, const float* pos_sorted, const int32_t* node_ranges,
+                       int64_t leaf_cap, float* center, float* half_extent, float* radius,
+                       float* max_extent, void* ws, size_t ws_bytes, void* stream);
+/* Walk node records of every system with batch-global child ids, leaf rows
+ * (leaf_offsets[s] + row) and level-major ranks (node offset + rank), so one
+ * walk over the forest (YggFlatWalkParams.roots = every system's root)
+ * evaluates every system's pairs. */
+int ygg_batch_walk_prepare_nodes(const YggBatch* batch, const float* center, const float* radius,
+                                 const float* max_extent, const int32_t* left,
+                                 const int32_t* right, const int32_t* leaf_position,
+                                 const int32_t* nodes_by_level, int mac_type,
+                                 float dehnen_radius_scale, void* nodes_out, int32_t* roots,
+                                 void* stream);
+/* Forest lists -> per-system lists. Far: sources/targets (batch-global node
+ * ids, in place) become local; offsets_out[node_off(s) + s + k] (T_s + 1 per
+ * system) = the system's level-major CSR offsets; far_level_offsets_out
+ * [65 s + l] from the system's radix level_offsets. Near: neighbors (global
+ * leaf node ids, in place) become local; near_offsets_out[leaf_offsets[s] +
+ * s + j] (L_s + 1 per system). row_offsets are the sorted streams' offsets
+ * over batch-global rows (level-major ranks / leaf rows). */
+int ygg_batch_far_localize(const YggBatch* batch, const int64_t* row_offsets,
+                           const int32_t* nodes_by_level_global, const int32_t* level_offsets,
+                           int64_t num_entries, int out_i64, void* sources, void* targets,
+                           void* node_offsets_out, void* far_level_offsets_out, void* stream);
+int ygg_batch_near_localize(const YggBatch* batch, const int64_t* row_offsets,
+                            int64_t num_entries, int out_i64, void* neighbors,
+                            void* near_offsets_out, void* stream);
+
+/* Capacity-model features of every node of every system of a batch (each
+ * with its own tree's root), and batch totals of the predicted rates. */
+int ygg_batch_mlp_features(const YggBatch* batch, const int32_t* node_level,
+                           const int32_t* node_ranges, const int32_t* parent, const int32_t* left,
+                           const int32_t* right, const float* half_extent, const float* radius,
+                           const float* max_extent, const float* parent_far, float* features,
+                           void* stream);
+int ygg_batch_mlp_totals(const YggBatch* batch, const float* outputs, float scale_far,
+                         float scale_near, float scale_queue, double* totals, void* stream);
 
 /* ---- custom pair policies ------------------------------------------------- */
 

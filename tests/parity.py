@@ -67,7 +67,7 @@ def assert_interactions_equal(inter, ref):
         assert np.array_equal(a, b), f"far row {node} differs"
 
 
-def assert_neighbors_equal(nbrs, ref):
+def assert_neighbors_equal(nbrs, ref, dense=True):
     counts = np_(nbrs.counts)
     assert np.array_equal(counts, ref["counts"]), "near counts differ"
     assert np.array_equal(np_(nbrs.offsets), ref["offsets"]), "near offsets differ"
@@ -79,6 +79,8 @@ def assert_neighbors_equal(nbrs, ref):
     off = ref["offsets"][:-1]
     for r, (a, b) in enumerate(zip(_rows(off, counts, got), _rows(off, counts, ref["neighbors"]))):
         assert np.array_equal(a, b), f"near row {r} differs"
+    if not dense:
+        return
     nlp = np_(nbrs.neighbor_leaf_positions)
     assert nlp.shape == ref["neighbor_leaf_positions"].shape
     assert np.array_equal(np.sort(nlp, axis=1), np.sort(ref["neighbor_leaf_positions"], axis=1))

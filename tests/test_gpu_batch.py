@@ -1,0 +1,94 @@
+"""Batched mode on the GPU: every system of a batch equals the oracle's
+single-system result (topology, octree metadata, geometry, far and near
+lists), including one-leaf systems and a malformed system (identical points)
+that takes the per-system fallback."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests import parity
+from yggdrax_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(sizes, seed):
+    rng = np.random.default_rng(seed)
+    pos, mass = [], []
+    for k, n in enumerate(sizes):
+        p, m = inputs.plummer_sphere(int(n), seed * 1000 + k, a=float(rng.uniform(0.05, 0.5)))
+        pos.append(p)
+        mass.append(m)
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    return pos, mass, offsets
+
+
+@pytest.mark.parametrize("octree", [True, False])
+def test_batch_matches_oracle_per_system(octree):
+    from yggdrax_amd import batch
+    sizes = [1, 2, 33, 100, 513, 2000, 4097, 1, 64, 700]
+    pos, mass, offsets = _systems(sizes, 3)
+    pos[6] = np.full_like(pos[6], 0.25)  # identical points: malformed tree, fallback path
+    P = torch.from_numpy(np.concatenate(pos)).cuda()
+    M = torch.from_numpy(np.concatenate(mass)).cuda()
+    leaf, theta = 16, 0.6
+    build = batch.build_octree_batched if octree else batch.build_tree_batched
+    bt = build(P, M, offsets, leaf_size=leaf)
+    assert 6 in bt.fallback
+    geom = batch.compute_tree_geometry_batched(bt)
+    lists = batch.build_interactions_and_neighbors_batched(bt, geom, theta, mac_type="dehnen")
+    ob = oracle.build_octree if octree else oracle.build_tree
+    for k in range(len(sizes)):
+        print("system", k, "n", sizes[k], flush=True)
+        ref = ob(pos[k], mass[k], leaf_size=leaf)
+        t = bt.system(k)
+        parity.assert_topology_equal(t, ref, octree=octree)
+        p0, p1 = offsets[k], offsets[k + 1]
+        assert np.array_equal(parity.np_(bt.positions_sorted[p0:p1]), ref["positions_sorted"])
+        rg = oracle.compute_tree_geometry(ref, ref["positions_sorted"])
+        parity.assert_geometry_close(geom.system(k), rg)
+        ri, rn, _ = oracle.build_interactions_and_neighbors(ref, rg, theta=theta,
+                                                            mac_type="dehnen")
+        inter, nbrs = lists.system(k)
+        parity.assert_interactions_equal(inter, ri)
+        parity.assert_neighbors_equal(nbrs, rn, dense=False)
+
+
+def test_batch_matches_single_system_path_c4_like():
+    """64 Plummer systems with log-uniform sizes: each system's lists equal the
+    single-system GPU path's (itself pinned to the oracle)."""
+    import yggdrax_amd as ygg
+    from yggdrax_amd import batch
+    rng = np.random.default_rng(4)
+    sizes = np.round(np.exp(rng.uniform(np.log(256), np.log(16384), 64))).astype(np.int64)
+    pos, mass, offsets = _systems(sizes, 4)
+    P = torch.from_numpy(np.concatenate(pos)).cuda()
+    M = torch.from_numpy(np.concatenate(mass)).cuda()
+    bt = batch.build_octree_batched(P, M, offsets, leaf_size=32)
+    geom = batch.compute_tree_geometry_batched(bt)
+    lists = batch.build_interactions_and_neighbors_batched(bt, geom, 0.6, mac_type="dehnen")
+    for k in range(0, 64, 7):
+        t, ps, _, _ = ygg.build_octree(torch.from_numpy(pos[k]).cuda(),
+                                       torch.from_numpy(mass[k]).cuda(), leaf_size=32,
+                                       return_reordered=True)
+        g = ygg.compute_tree_geometry(t, ps)
+        si, sn = ygg.build_interactions_and_neighbors(t, g, theta=0.6, mac_type="dehnen",
+                                                      neighbor_positions=False)
+        bi, bn = lists.system(k)
+        for a, b in ((si.counts, bi.counts), (si.offsets, bi.offsets), (si.targets, bi.targets),
+                     (si.level_offsets, bi.level_offsets), (si.target_levels, bi.target_levels),
+                     (sn.counts, bn.counts), (sn.offsets, bn.offsets),
+                     (sn.leaf_indices, bn.leaf_indices)):
+            assert torch.equal(a, b)
+        for (o, c, x1, x2) in ((si.offsets, si.counts, si.sources, bi.sources),
+                               (sn.offsets, sn.counts, sn.neighbors, bn.neighbors)):
+            off, cnt = parity.np_(o), parity.np_(c)
+            order = np.argsort(off[: len(cnt)], kind="stable")
+            rows = np.repeat(order, cnt[order]).astype(np.int64)
+            k1 = np.sort(rows << 32 | parity.np_(x1).astype(np.int64))
+            k2 = np.sort(rows << 32 | parity.np_(x2).astype(np.int64))
+            assert np.array_equal(k1, k2)

@@ -63,11 +63,39 @@ class FlatWalkParams(ctypes.Structure):
         ("near_cap", _I64), ("nodes", _P), ("wf0", _P), ("wf1", _P), ("gen_sizes", _P),
         ("flags", _P), ("counters", _P), ("far_keys", _P), ("far_vals", _P),
         ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("pad1", _I32),
-        ("node_hist", _P),
+        ("node_hist", _P), ("roots", _P), ("num_roots", _I64),
     ]
 
 
+class Batch(ctypes.Structure):
+    _fields_ = [
+        ("num_systems", _I32), ("pad0", _I32), ("leaf_size", _I64), ("point_offsets", _P),
+        ("leaf_offsets", _P), ("num_points", _I64), ("num_leaves", _I64),
+        ("max_leaves_per_system", _I64),
+    ]
+
+
+_BP = ctypes.POINTER(Batch)
+
 _SIGS = {
+    "ygg_walk_flat_fixed": (_INT, [ctypes.POINTER(FlatWalkParams), _P]),
+    "ygg_batch_infer_bounds": (_INT, [_BP, _P, _P, _P]),
+    "ygg_batch_morton_sort_workspace_bytes": (_SZ, [_BP]),
+    "ygg_batch_morton_sort": (_INT, [_BP, _P, _P, _P, _P, _P, _SZ, _P]),
+    "ygg_batch_reorder": (_INT, [_BP, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ygg_batch_radix_build_workspace_bytes": (_SZ, [_BP]),
+    "ygg_batch_radix_build": (_INT, [_BP, _P, ctypes.POINTER(RadixTopologyPtrs), _P, _SZ, _P]),
+    "ygg_batch_octree_meta_workspace_bytes": (_SZ, [_BP]),
+    "ygg_batch_octree_meta": (_INT, [_BP, _P, _P, ctypes.POINTER(OctreeMetaPtrs), _P, _SZ, _P]),
+    "ygg_batch_radix_leaf_order": (_INT, [_BP, _P, _P, _P]),
+    "ygg_batch_geometry_workspace_bytes": (_SZ, [_BP]),
+    "ygg_batch_geometry": (_INT, [_BP, _P, _P, _I64, _P, _P, _P, _P, _P, _SZ, _P]),
+    "ygg_batch_walk_prepare_nodes": (_INT, [_BP, _P, _P, _P, _P, _P, _P, _P, _INT, _F, _P, _P,
+                                            _P]),
+    "ygg_batch_far_localize": (_INT, [_BP, _P, _P, _P, _I64, _INT, _P, _P, _P, _P, _P]),
+    "ygg_batch_near_localize": (_INT, [_BP, _P, _I64, _INT, _P, _P, _P]),
+    "ygg_batch_mlp_features": (_INT, [_BP] + [_P] * 11),
+    "ygg_batch_mlp_totals": (_INT, [_BP, _P, _F, _F, _F, _P, _P]),
     "ygg_sort_workspace_bytes": (_SZ, [_I64, _INT, _INT, _INT, _INT]),
     "ygg_sort_default_digit_bits": (_INT, [_INT, _INT, _INT]),
     "ygg_sort_pairs_u64": (_INT, [_P, _P, _P, _P, _I64, _INT, _INT, _INT, _P, _SZ, _P]),

@@ -4,17 +4,9 @@
 // separately rounded IEEE operations (the library is built without FMA
 // contraction) in the order the reference writes them.
 #include "ygg_common.h"
+#include "ygg_build_dev.h"
 
 namespace ygg {
-
-// Order-preserving float <-> uint32 mapping for atomic min/max.
-__device__ __forceinline__ uint32_t f2ord(float f) {
-  uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord2f(uint32_t u) {
-  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
-}
 
 // ---- bounds (bounds.py:29-33) ----------------------------------------------
 // Flat grid-stride over 3n floats with a stride that is a multiple of 3, so
@@ -54,32 +46,10 @@ __global__ void k_bounds_init(uint32_t* acc) {
 
 __global__ void k_bounds_final(const uint32_t* __restrict__ acc, float* __restrict__ b6) {
   int d = threadIdx.x;
-  if (d < 3) {
-    float mn = ord2f(acc[d]);
-    float mx = ord2f(acc[3 + d]);
-    float span = mx - mn;
-    float pad = span * 0.05f;
-    if (pad < 1e-6f) pad = 1e-6f;
-    b6[d] = mn - pad;
-    b6[3 + d] = mx + pad;
-  }
+  if (d < 3) padded_bounds(ord2f(acc[d]), ord2f(acc[3 + d]), &b6[d], &b6[3 + d]);
 }
 
 // ---- Morton encode (morton.py:68-77) ----------------------------------------
-__device__ __forceinline__ uint64_t encode_point(float x, float y, float z, const float* lo,
-                                                 const float* w) {
-  float p[3] = {x, y, z};
-  uint64_t c[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    float v = __fdiv_rn(p[d] - lo[d], w[d]);
-    v = v < 0.0f ? 0.0f : v;
-    v = v > 1.0f ? 1.0f : v;
-    c[d] = (uint64_t)rintf(v * 2097151.0f);
-  }
-  return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
-}
-
 __global__ __launch_bounds__(256) void k_encode(const float* __restrict__ pos, int64_t n,
                                                 const float* __restrict__ b6,
                                                 uint64_t* __restrict__ codes) {
@@ -152,18 +122,6 @@ __global__ __launch_bounds__(256) void k_leaves(const uint64_t* __restrict__ sc,
   ranges[2 * node + 1] = (int32_t)(e - 1);
 }
 
-__device__ __forceinline__ int delta_node(const uint64_t* lc, int L, int64_t i, int64_t j) {
-  if (j < 0 || j >= L) return -1;
-  uint64_t ci = lc[i], cj = lc[j];
-  if (ci == cj) return 64 + clz64((uint64_t)i ^ (uint64_t)j);
-  return clz64(ci ^ cj);
-}
-
-__device__ __forceinline__ int lcp_codes(uint64_t a, uint64_t b) {
-  uint64_t x = a ^ b;
-  return x == 0 ? 64 : clz64(x);
-}
-
 __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ lc, int L,
                                                 int64_t n, int64_t leaf,
                                                 int32_t* __restrict__ left,
@@ -173,47 +131,8 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ lc,
                                                 int32_t* __restrict__ ranges,
                                                 uint32_t* __restrict__ penc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int NI = L - 1;
-  if (i >= NI) return;
-  int dp = delta_node(lc, L, i, (int64_t)i + 1);
-  int dm = delta_node(lc, L, i, (int64_t)i - 1);
-  int64_t d = (dp > dm) ? 1 : ((dp < dm) ? -1 : 0);
-  if (i == 0) d = 1;
-  if (d == 0) d = 1;
-  int dmin = delta_node(lc, L, i, (int64_t)i - d);
-  int64_t span = 0;
-  for (int k = 30; k >= 0; --k) {
-    int64_t step = (int64_t)1 << k;
-    int64_t cand = (int64_t)i + (span + step) * d;
-    if (cand >= 0 && cand < L && delta_node(lc, L, i, cand) > dmin) span += step;
-  }
-  int64_t e = (int64_t)i + span * d;
-  int64_t first = min((int64_t)i, e), last = max((int64_t)i, e);
-  ranges[2 * i] = (int32_t)(first * leaf);
-  ranges[2 * i + 1] = (int32_t)(min((last + 1) * leaf, n) - 1);
-  uint64_t fc = lc[first];
-  int common = lcp_codes(fc, lc[last]);
-  int64_t split = first;
-  for (int k = 30; k >= 0; --k) {
-    int64_t mid = split + ((int64_t)1 << k);
-    int64_t safe_mid = mid < last ? mid : last;
-    if (mid < last && lcp_codes(fc, lc[safe_mid]) > common) split = mid;
-  }
-  bool lf = split == first, rf = split + 1 == last;
-  int32_t l = lf ? (int32_t)(NI + first) : (int32_t)split;
-  int32_t r = rf ? (int32_t)(NI + last) : (int32_t)(split + 1);
-  left[i] = l;
-  right[i] = r;
-  lflag[i] = lf;
-  rflag[i] = rf;
-  // parent.at[left].set(i) then parent.at[right].set(i) with sequential
-  // last-writer semantics: right writes beat left writes, larger i wins.
-  atomicMax(&penc[l], (uint32_t)(i + 1));
-  atomicMax(&penc[r], 0x80000000u | (uint32_t)(i + 1));
-}
-
-__device__ __forceinline__ int32_t dec_parent(uint32_t e) {
-  return e == 0 ? -1 : (int32_t)(e & 0x7FFFFFFFu) - 1;
+  if (i >= L - 1) return;
+  karras_node(lc, L, n, leaf, i, left, right, lflag, rflag, ranges, penc);
 }
 
 // scalars: [0] num_levels, [1] max_level, [2] deep, [3] malformed, [4] roots, [5] root
@@ -328,19 +247,6 @@ __global__ void k_oct_resolve(const uint64_t* __restrict__ sc, const int32_t* __
   int d = common_depth(f, l);
   rd[j] = d;
   rc[j] = prefix_code(f, d);
-}
-
-__device__ __forceinline__ void candidate(const uint64_t* rc, const int32_t* rd, int L, int id,
-                                          uint64_t* c, int* d) {
-  if (id < L) {
-    *c = rc[id];
-    *d = rd[id];
-  } else {
-    int j = id - L;
-    int dd = common_depth(rc[j], rc[j + 1]);
-    *d = dd;
-    *c = prefix_code(rc[j], dd);
-  }
 }
 
 __global__ void k_oct_candidates(const uint64_t* __restrict__ rc, const int32_t* __restrict__ rd,

@@ -173,6 +173,50 @@ struct Arena {
   }
 };
 
+// ---- batches of independent trees (segments) --------------------------------
+// System s owns points [pt[s], pt[s+1]) and leaves [leaf[s], leaf[s+1]) of the
+// concatenated arrays; its L_s - 1 internal nodes start at leaf[s] - s and its
+// 2 L_s - 1 nodes at 2 leaf[s] - s. Every index stored in a tree's own arrays
+// is local to that tree, as the single-tree kernels produce it.
+struct Seg {
+  int S;
+  int64_t leaf_size;
+  const int32_t* pt;    // [S + 1]
+  const int32_t* leaf;  // [S + 1]
+  __device__ __forceinline__ int64_t pt_off(int s) const { return pt[s]; }
+  __device__ __forceinline__ int64_t n(int s) const { return pt[s + 1] - pt[s]; }
+  __device__ __forceinline__ int leaf_off(int s) const { return leaf[s]; }
+  __device__ __forceinline__ int L(int s) const { return leaf[s + 1] - leaf[s]; }
+  __device__ __forceinline__ int ni_off(int s) const { return leaf[s] - s; }
+  __device__ __forceinline__ int node_off(int s) const { return 2 * leaf[s] - s; }
+  // System of global index g in a layout with per-system offsets off(s)
+  // (non-decreasing; empty systems are skipped): largest s with off(s) <= g.
+  template <class F>
+  __device__ __forceinline__ int find(int64_t g, F off) const {
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if ((int64_t)off(mid) <= g)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    return lo;
+  }
+  __device__ __forceinline__ int of_point(int64_t g) const {
+    return find(g, [this](int s) { return pt_off(s); });
+  }
+  __device__ __forceinline__ int of_leaf(int64_t g) const {
+    return find(g, [this](int s) { return leaf_off(s); });
+  }
+  __device__ __forceinline__ int of_internal(int64_t g) const {
+    return find(g, [this](int s) { return ni_off(s); });
+  }
+  __device__ __forceinline__ int of_node(int64_t g) const {
+    return find(g, [this](int s) { return node_off(s); });
+  }
+};
+
 // ---- primitives implemented in ygg_sort.hip --------------------------------
 // Stable LSD radix sort of (key, u32 value) pairs on key bits [begin, end)
 // with digit_bits-wide digits (8..11; 0 = default_digit_bits).
@@ -197,5 +241,7 @@ int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int6
                               int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
 
 int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+// Device-to-device copy of n 32-bit words by a kernel (graph-capturable).
+int copy_u32(uint32_t* dst, const uint32_t* src, int64_t n, hipStream_t st);
 
 }  // namespace ygg

@@ -154,6 +154,81 @@ __global__ __launch_bounds__(256) void k_seg_query(const Box8* __restrict__ seg,
   finalize_node(i, acc.mn, acc.mx, center, half, radius, maxext);
 }
 
+// Batched: one wave per segment-tree leaf slot j < P over the batch's
+// concatenated leaves (system of j by the leaf offsets).
+__global__ __launch_bounds__(256) void k_b_seg_leaves(Seg sg, const float* __restrict__ pos,
+                                                      const int32_t* __restrict__ ranges,
+                                                      int64_t Ltot, int leaf_cap, int64_t P,
+                                                      Box8* __restrict__ seg,
+                                                      float* __restrict__ center,
+                                                      float* __restrict__ half,
+                                                      float* __restrict__ radius,
+                                                      float* __restrict__ maxext) {
+  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE;
+  const unsigned lane = lane_id();
+  if (j >= P) return;
+  if (j >= Ltot) {
+    if (lane == 0) seg[P + j] = empty_box();
+    return;
+  }
+  const int s = sg.of_leaf(j);
+  const int64_t node = sg.node_off(s) + (sg.L(s) - 1) + (j - sg.leaf_off(s));
+  const int64_t a = sg.pt_off(s) + ranges[2 * node];
+  const int cnt = min(ranges[2 * node + 1] - ranges[2 * node] + 1, leaf_cap);
+  float lo[3] = {INFINITY, INFINITY, INFINITY};
+  float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int m = lane; m < cnt; m += YGG_WAVE) {
+    const float* p = pos + 3 * (a + m);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float v = p[d];
+      lo[d] = fminf(lo[d], v);
+      hi[d] = fmaxf(hi[d], v);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fminf(lo[d], __shfl_xor(lo[d], off, 64));
+      hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off, 64));
+    }
+  }
+  if (lane != 0) return;
+  Box8 b;
+  for (int d = 0; d < 3; ++d) {
+    b.mn[d] = lo[d];
+    b.mx[d] = hi[d];
+  }
+  b.mn[3] = 0.f;
+  b.mx[3] = 0.f;
+  seg[P + j] = b;
+  finalize_node(node, lo, hi, center, half, radius, maxext);
+}
+
+__global__ __launch_bounds__(256) void k_b_seg_query(Seg sg, const Box8* __restrict__ seg,
+                                                     int64_t P, const int32_t* __restrict__ ranges,
+                                                     int64_t NItot, float* __restrict__ center,
+                                                     float* __restrict__ half,
+                                                     float* __restrict__ radius,
+                                                     float* __restrict__ maxext) {
+  const int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gi >= NItot) return;
+  const int s = sg.of_internal(gi);
+  const int64_t node = sg.node_off(s) + (gi - sg.ni_off(s));
+  const int64_t l0 = sg.leaf_off(s) + P;
+  int64_t l = l0 + ranges[2 * node] / sg.leaf_size;
+  int64_t r = l0 + ranges[2 * node + 1] / sg.leaf_size + 1;
+  Box8 acc = empty_box();
+  while (l < r) {
+    if (l & 1) merge_into(acc, seg[l++]);
+    if (r & 1) merge_into(acc, seg[--r]);
+    l >>= 1;
+    r >>= 1;
+  }
+  finalize_node(node, acc.mn, acc.mx, center, half, radius, maxext);
+}
+
 // Level-synchronous upward pass for malformed trees (_geometry_impl.py:302-329):
 // nodes at `level` read their children's current boxes (phase 0 into tmp),
 // then write them (phase 1), exactly as the reference's masked update.
@@ -282,6 +357,51 @@ int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int3
   }
   hipLaunchKernelGGL(k_geom_finalize, dim3(grid_for(T, 256)), dim3(256), 0, st, T, bmin, bmax,
                      center, half_extent, radius, max_extent);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+static int64_t pow2_at_least(int64_t v) {
+  int64_t P = 1;
+  while (P < v) P <<= 1;
+  return P;
+}
+
+size_t ygg_batch_geometry_workspace_bytes(const YggBatch* b) {
+  if (!b || b->num_leaves < 1) return 0;
+  Arena a(nullptr, 0);
+  a.take<Box8>((size_t)(2 * pow2_at_least(b->num_leaves)));
+  return a.off;
+}
+
+int ygg_batch_geometry(const YggBatch* b, const float* pos_sorted, const int32_t* node_ranges,
+                       int64_t leaf_cap, float* center, float* half_extent, float* radius,
+                       float* max_extent, void* ws, size_t ws_bytes, void* stream) {
+  if (!b || b->num_systems < 1 || b->num_leaves < 1 || leaf_cap < 1 || b->leaf_size < 1)
+    return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  Seg sg;
+  sg.S = b->num_systems;
+  sg.leaf_size = b->leaf_size;
+  sg.pt = b->point_offsets;
+  sg.leaf = b->leaf_offsets;
+  const int64_t Ltot = b->num_leaves, NItot = Ltot - b->num_systems;
+  const int64_t P = pow2_at_least(Ltot);
+  if (P > (1ll << 30)) return YGG_ERR_UNSUPPORTED;
+  Arena a(ws, ws_bytes);
+  Box8* seg = a.take<Box8>((size_t)(2 * P));
+  if (!seg) return YGG_ERR_WORKSPACE;
+  const int lc = (int)(leaf_cap > 0x7FFFFFFF ? 0x7FFFFFFF : leaf_cap);
+  hipLaunchKernelGGL(k_b_seg_leaves, dim3(grid_for(P * YGG_WAVE, 256)), dim3(256), 0, st, sg,
+                     pos_sorted, node_ranges, Ltot, lc, P, seg, center, half_extent, radius,
+                     max_extent);
+  for (int64_t S = P; S > 1; S /= 512) {
+    int blocks = (int)((S / 2 + 255) / 256);
+    hipLaunchKernelGGL(k_seg_up, dim3(blocks), dim3(256), 0, st, seg, (int)S);
+  }
+  if (NItot > 0)
+    hipLaunchKernelGGL(k_b_seg_query, dim3(grid_for(NItot, 256)), dim3(256), 0, st, sg, seg, P,
+                       node_ranges, NItot, center, half_extent, radius, max_extent);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

@@ -171,16 +171,18 @@ __global__ void k_zero_words(uint32_t* p, int n) {
 }
 
 // ---- per-node features ------------------------------------------------------
-__global__ void k_mlp_features(int T, int NI, const int32_t* __restrict__ node_level,
-                               const int32_t* __restrict__ ranges,
-                               const int32_t* __restrict__ parent,
-                               const int32_t* __restrict__ left,
-                               const int32_t* __restrict__ right,
-                               const float* __restrict__ half, const float* __restrict__ radius,
-                               const float* __restrict__ maxext,
-                               const float* __restrict__ parent_far, float* __restrict__ x) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= T) return;
+// Features of node k of one tree whose arrays start at the given bases
+// (root = node 0 of the tree; parent/children/sibling ids local).
+__device__ __forceinline__ void node_features(int k, int NI, const int32_t* __restrict__ node_level,
+                                              const int32_t* __restrict__ ranges,
+                                              const int32_t* __restrict__ parent,
+                                              const int32_t* __restrict__ left,
+                                              const int32_t* __restrict__ right,
+                                              const float* __restrict__ half,
+                                              const float* __restrict__ radius,
+                                              const float* __restrict__ maxext,
+                                              const float* __restrict__ parent_far,
+                                              float* __restrict__ o) {
   const float root_ext = maxext[0], root_rad = radius[0];
   const float root_vol = 8.0f * half[0] * half[1] * half[2];
   float cnt = (float)(ranges[2 * k + 1] - ranges[2 * k] + 1);
@@ -192,7 +194,6 @@ __global__ void k_mlp_features(int T, int NI, const int32_t* __restrict__ node_l
   }
   float vol = 8.0f * half[3 * k] * half[3 * k + 1] * half[3 * k + 2];
   float pf = (parent_far && p >= 0) ? parent_far[p] : 0.0f;
-  float* o = x + (int64_t)k * kIn;
   o[0] = (float)node_level[k] / 64.0f;
   o[1] = log2f(cnt) / 24.0f;
   o[2] = log2f(maxext[k] / root_ext) / 32.0f + 1.0f;
@@ -202,6 +203,62 @@ __global__ void k_mlp_features(int T, int NI, const int32_t* __restrict__ node_l
   float rel = (cnt / fmaxf(vol, 1e-30f)) / ((float)(ranges[1] + 1) / fmaxf(root_vol, 1e-30f));
   o[6] = fminf(fmaxf(log2f(fmaxf(rel, 1e-30f)) / 64.0f, -1.0f), 1.0f);
   o[7] = k >= NI ? 1.0f : 0.0f;
+}
+
+__global__ void k_mlp_features(int T, int NI, const int32_t* __restrict__ node_level,
+                               const int32_t* __restrict__ ranges,
+                               const int32_t* __restrict__ parent,
+                               const int32_t* __restrict__ left,
+                               const int32_t* __restrict__ right,
+                               const float* __restrict__ half, const float* __restrict__ radius,
+                               const float* __restrict__ maxext,
+                               const float* __restrict__ parent_far, float* __restrict__ x) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= T) return;
+  node_features(k, NI, node_level, ranges, parent, left, right, half, radius, maxext, parent_far,
+                x + (int64_t)k * kIn);
+}
+
+// Batched: every node of every system, with its own tree's root.
+__global__ void k_b_mlp_features(Seg sg, int64_t Ttot, const int32_t* __restrict__ node_level,
+                                 const int32_t* __restrict__ ranges,
+                                 const int32_t* __restrict__ parent,
+                                 const int32_t* __restrict__ left,
+                                 const int32_t* __restrict__ right,
+                                 const float* __restrict__ half, const float* __restrict__ radius,
+                                 const float* __restrict__ maxext,
+                                 const float* __restrict__ parent_far, float* __restrict__ x) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= Ttot) return;
+  const int s = sg.of_node(g);
+  const int64_t n0 = sg.node_off(s), ni0 = sg.ni_off(s);
+  node_features((int)(g - n0), sg.L(s) - 1, node_level + n0, ranges + 2 * n0, parent + n0,
+                left + ni0, right + ni0, half + 3 * n0, radius + n0, maxext + n0,
+                parent_far ? parent_far + n0 : nullptr, x + g * kIn);
+}
+
+// Batched totals: output 1 (near) counts each system's leaves.
+__global__ void k_b_mlp_totals(Seg sg, const float* __restrict__ y, int64_t n, float s0, float s1,
+                               float s2, double* __restrict__ tot) {
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int s = sg.of_node(g);
+    a0 += (double)(s0 * exp2f(fminf(y[3 * g], kMaxLogRate)));
+    if (g - sg.node_off(s) >= sg.L(s) - 1)
+      a1 += (double)(s1 * exp2f(fminf(y[3 * g + 1], kMaxLogRate)));
+    a2 += (double)(s2 * exp2f(fminf(y[3 * g + 2], kMaxLogRate)));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, 64);
+    a1 += __shfl_xor(a1, off, 64);
+    a2 += __shfl_xor(a2, off, 64);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(&tot[0], a0);
+    atomicAdd(&tot[1], a1);
+    atomicAdd(&tot[2], a2);
+  }
 }
 
 // Sum of scale_j 2^y_j per output over [0, n) (stream totals), y row-major
@@ -352,6 +409,41 @@ int ygg_mlp_far_counts(const float* outputs, int64_t num_nodes, float scale_far,
   if (num_nodes < 1 || !outputs || !far_counts) return YGG_ERR_ARG;
   hipLaunchKernelGGL(k_mlp_far_of, dim3(grid_for(num_nodes, 256)), dim3(256), 0,
                      (hipStream_t)stream, outputs, (int)num_nodes, scale_far, far_counts);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+static inline Seg mlp_seg(const YggBatch* b) {
+  Seg sg;
+  sg.S = b->num_systems;
+  sg.leaf_size = b->leaf_size;
+  sg.pt = b->point_offsets;
+  sg.leaf = b->leaf_offsets;
+  return sg;
+}
+
+int ygg_batch_mlp_features(const YggBatch* b, const int32_t* node_level,
+                           const int32_t* node_ranges, const int32_t* parent, const int32_t* left,
+                           const int32_t* right, const float* half_extent, const float* radius,
+                           const float* max_extent, const float* parent_far, float* features,
+                           void* stream) {
+  if (!b || b->num_systems < 1 || !features) return YGG_ERR_ARG;
+  const int64_t T = 2 * b->num_leaves - b->num_systems;
+  hipLaunchKernelGGL(k_b_mlp_features, dim3(grid_for(T, 256)), dim3(256), 0, (hipStream_t)stream,
+                     mlp_seg(b), T, node_level, node_ranges, parent, left, right, half_extent,
+                     radius, max_extent, parent_far, features);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_batch_mlp_totals(const YggBatch* b, const float* outputs, float scale_far,
+                         float scale_near, float scale_queue, double* totals, void* stream) {
+  if (!b || b->num_systems < 1 || !outputs || !totals) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t T = 2 * b->num_leaves - b->num_systems;
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)totals, 6);
+  hipLaunchKernelGGL(k_b_mlp_totals, dim3(grid_for(T, 256, 1024)), dim3(256), 0, st, mlp_seg(b),
+                     outputs, T, scale_far, scale_near, scale_queue, totals);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

@@ -205,6 +205,19 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
     p[i] = v;
 }
 
+__global__ void k_copy_u32(uint32_t* __restrict__ d, const uint32_t* __restrict__ s, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+int copy_u32(uint32_t* dst, const uint32_t* src, int64_t n, hipStream_t st) {
+  if (n <= 0) return YGG_OK;
+  hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, dst, src, n);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
 int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
   if (n <= 0) return YGG_OK;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, p, v, n);
@@ -254,8 +267,8 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
   int rc = fill_u32(ctr, 0u, (int64_t)passes + (int64_t)passes * tiles * kBins, st);
   if (rc) return rc;
   if (hist_pre) {
-    YGG_CHECK(hipMemcpyAsync(hist, hist_pre, sizeof(uint32_t) * passes * kBins,
-                             hipMemcpyDeviceToDevice, st));
+    rc = copy_u32(hist, hist_pre, (int64_t)passes * kBins, st);
+    if (rc) return rc;
   } else {
     rc = fill_u32(hist, 0u, (int64_t)passes * kBins, st);
     if (rc) return rc;

@@ -50,6 +50,51 @@ __global__ void k_set_rank(const int32_t* __restrict__ nbl, int T, NodeRec* __re
   if (r < T) out[nbl[r]].pad = r;
 }
 
+// Batched: records of every system's nodes with batch-global child ids,
+// leaf rows and level-major ranks; roots[s] = the system's root node.
+__global__ void k_b_prep_nodes(Seg sg, const float* __restrict__ center,
+                               const float* __restrict__ radius,
+                               const float* __restrict__ maxext,
+                               const int32_t* __restrict__ left,
+                               const int32_t* __restrict__ right,
+                               const int32_t* __restrict__ leaf_position, int64_t Ttot,
+                               int mac_type, float scale, NodeRec* __restrict__ out,
+                               int32_t* __restrict__ roots) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= Ttot) return;
+  const int s = sg.of_node(g);
+  const int64_t n0 = sg.node_off(s);
+  const int NI = sg.L(s) - 1;
+  const int k = (int)(g - n0);
+  NodeRec r;
+  r.cx = center[3 * g];
+  r.cy = center[3 * g + 1];
+  r.cz = center[3 * g + 2];
+  float e;
+  if (mac_type == YGG_MAC_DEHNEN)
+    e = scale * radius[g];
+  else if (mac_type == YGG_MAC_ENGBLOM)
+    e = radius[g];
+  else
+    e = maxext[g];
+  r.ext = e;
+  const int64_t ni = sg.ni_off(s) + k;
+  r.left = k < NI ? (int32_t)(n0 + left[ni]) : -1;
+  r.right = k < NI ? (int32_t)(n0 + right[ni]) : -1;
+  r.leafpos = k < NI ? -1 : sg.leaf_off(s) + leaf_position[g];
+  r.pad = 0;
+  out[g] = r;
+  if (k == 0) roots[s] = (int32_t)n0;
+}
+
+__global__ void k_b_set_rank(Seg sg, const int32_t* __restrict__ nbl, int64_t Ttot,
+                             NodeRec* __restrict__ out) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= Ttot) return;
+  const int64_t n0 = sg.node_off(sg.of_node(g));
+  out[n0 + nbl[g]].pad = (int32_t)g;
+}
+
 __global__ void k_walk_init(YggWalkParams p) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -286,6 +331,29 @@ int ygg_walk_prepare_nodes(const float* center, const float* radius, const float
   if (nodes_by_level)
     hipLaunchKernelGGL(k_set_rank, dim3(grid_for(T, 256)), dim3(256), 0, st, nodes_by_level, T,
                        (NodeRec*)nodes_out);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_batch_walk_prepare_nodes(const YggBatch* b, const float* center, const float* radius,
+                                 const float* max_extent, const int32_t* left,
+                                 const int32_t* right, const int32_t* leaf_position,
+                                 const int32_t* nodes_by_level, int mac_type,
+                                 float dehnen_radius_scale, void* nodes_out, int32_t* roots,
+                                 void* stream) {
+  if (!b || b->num_systems < 1 || !nodes_out || !roots || !nodes_by_level) return YGG_ERR_ARG;
+  Seg sg;
+  sg.S = b->num_systems;
+  sg.leaf_size = b->leaf_size;
+  sg.pt = b->point_offsets;
+  sg.leaf = b->leaf_offsets;
+  const int64_t T = 2 * b->num_leaves - b->num_systems;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_b_prep_nodes, dim3(grid_for(T, 256)), dim3(256), 0, st, sg, center, radius,
+                     max_extent, left, right, leaf_position, T, mac_type, dehnen_radius_scale,
+                     (NodeRec*)nodes_out, roots);
+  hipLaunchKernelGGL(k_b_set_rank, dim3(grid_for(T, 256)), dim3(256), 0, st, sg, nodes_by_level,
+                     T, (NodeRec*)nodes_out);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

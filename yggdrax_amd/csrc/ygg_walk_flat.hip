@@ -38,6 +38,8 @@ struct FlatArgs {
   uint32_t* near_vals;
   int32_t hist_gen;     // generation whose pairs are histogrammed by target node
   int32_t* node_hist;   // [T] or null
+  const int32_t* roots; // forest seeds (r, r), or null for the single root
+  int64_t num_roots;
 };
 
 __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v) {
@@ -61,7 +63,6 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
   const uint64_t* __restrict__ wf_in = (g & 1) ? a.wf1 : a.wf0;
   uint64_t* __restrict__ wf_out = (g & 1) ? a.wf0 : a.wf1;
   const NodeRec* __restrict__ nodes = a.nodes;
-  const int NI = a.num_internal;
   const unsigned lane = lane_id();
   const int w = threadIdx.x / YGG_WAVE;
   __shared__ unsigned long long s_wave[kFlatThreads / YGG_WAVE];
@@ -97,7 +98,9 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         float sum = et + es;
         ok = diff && d2 > 0.0f && (sum * sum <= a.theta_sq * d2);
       }
-      const bool t_int = t < NI, s_int = s < NI;
+      // internal nodes carry children (node ids of a forest are not
+      // split at num_internal)
+      const bool t_int = nt.left >= 0, s_int = ns.left >= 0;
       if (a.node_hist && g == a.hist_gen) atomicAdd(&a.node_hist[t], 1);
       pt[k] = (uint32_t)t;
       ps[k] = (uint32_t)s;
@@ -229,7 +232,13 @@ __global__ void k_flat_init(FlatArgs a, int root, int G) {
   for (int k = i; k <= G; k += gridDim.x * blockDim.x) a.gen_sizes[k] = 0;
   if (i < 4) a.flags[i] = 0;
   if (i < 5) a.counters[i] = 0;
-  if (i == 0) {
+  if (a.roots) {
+    for (int64_t k = i; k < a.num_roots; k += (int64_t)gridDim.x * blockDim.x) {
+      const uint32_t r = (uint32_t)a.roots[k];
+      a.wf0[k] = ((uint64_t)r << 32) | r;
+    }
+    if (i == 0) a.gen_sizes[0] = (uint32_t)a.num_roots;
+  } else if (i == 0) {
     a.gen_sizes[0] = 1;
     a.wf0[0] = ((uint64_t)(uint32_t)root << 32) | (uint32_t)root;
   }
@@ -316,6 +325,70 @@ __global__ __launch_bounds__(256) void k_near_dense(int L, const int64_t* __rest
   }
 }
 
+// ---- batched (forest) lists -> per-system lists ---------------------------------
+template <typename OutT>
+__global__ void k_b_far_localize_entries(Seg sg, int64_t n, OutT* __restrict__ sources,
+                                         OutT* __restrict__ targets) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = (int64_t)targets[e];
+    const int64_t n0 = sg.node_off(sg.of_node(t));
+    targets[e] = (OutT)(t - n0);
+    sources[e] = (OutT)((int64_t)sources[e] - n0);
+  }
+}
+
+// offsets_out[node + s] for every node (ranks r in [0, Ttot)), and the end
+// offset of every system at node_off(s + 1) + s.
+template <typename OutT>
+__global__ void k_b_far_localize_rows(Seg sg, const int64_t* __restrict__ off,
+                                      const int32_t* __restrict__ nbl_global, int64_t Ttot,
+                                      OutT* __restrict__ offsets_out) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r < Ttot) {
+    const int64_t node = nbl_global[r];
+    const int s = sg.of_node(node);
+    offsets_out[node + s] = (OutT)(off[r] - off[sg.node_off(s)]);
+  }
+  if (r < sg.S) {
+    const int s = (int)r;
+    offsets_out[sg.node_off(s + 1) + s] = (OutT)(off[sg.node_off(s + 1)] - off[sg.node_off(s)]);
+  }
+}
+
+template <typename OutT>
+__global__ void k_b_far_level_offsets(Seg sg, const int64_t* __restrict__ off,
+                                      const int32_t* __restrict__ level_offsets,
+                                      OutT* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= 65 * (int64_t)sg.S) return;
+  const int s = (int)(i / 65);
+  const int64_t n0 = sg.node_off(s);
+  out[i] = (OutT)(off[n0 + level_offsets[i]] - off[n0]);
+}
+
+template <typename OutT>
+__global__ void k_b_near_localize(Seg sg, const int64_t* __restrict__ off, int64_t n,
+                                  OutT* __restrict__ neighbors, OutT* __restrict__ offsets_out) {
+  const int64_t Ltot = sg.leaf[sg.S];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = (int64_t)neighbors[e];
+    neighbors[e] = (OutT)(v - sg.node_off(sg.of_node(v)));
+  }
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < Ltot + sg.S;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    if (j < Ltot) {
+      const int s = sg.of_leaf(j);
+      offsets_out[j + s] = (OutT)(off[j] - off[sg.leaf_off(s)]);
+    } else {
+      const int s = (int)(j - Ltot);
+      offsets_out[sg.leaf_off(s + 1) + s] =
+          (OutT)(off[sg.leaf_off(s + 1)] - off[sg.leaf_off(s)]);
+    }
+  }
+}
+
 }  // namespace ygg
 
 using namespace ygg;
@@ -357,8 +430,13 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
   a.near_vals = p->near_vals;
   a.hist_gen = p->hist_gen;
   a.node_hist = p->node_hist;
+  a.roots = p->roots;
+  a.num_roots = p->roots ? p->num_roots : 1;
+  if (p->roots && (p->num_roots < 1 || p->num_roots > p->queue_cap)) return YGG_ERR_ARG;
   const int G = p->max_generations;
-  hipLaunchKernelGGL(k_flat_init, dim3(grid_for(G + 1, 256)), dim3(256), 0, st, a, p->root, G);
+  const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
+  hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
+                     G);
   YGG_LAUNCH_CHECK();
   const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
   int g = 0, batch = p->gen_hint > 0 ? p->gen_hint : 24;
@@ -399,6 +477,54 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
     g = end;
     batch = 8;
   }
+}
+
+// The same walk with a fixed number of generation launches and no host
+// synchronisation (for stream capture into a HIP graph): launches past the
+// last non-empty generation exit at once; counters, generation sizes and
+// overflow flags stay on the device for the caller to read afterwards.
+int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
+  if (!p) return YGG_ERR_ARG;
+  if (p->num_internal < 1 || p->queue_cap < 4 || p->max_generations < 2) return YGG_ERR_ARG;
+  if (p->emit && (!p->far_keys || !p->far_vals || !p->near_keys || !p->near_vals))
+    return YGG_ERR_ARG;
+  if (p->roots && (p->num_roots < 1 || p->num_roots > p->queue_cap)) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  FlatArgs a;
+  a.num_internal = p->num_internal;
+  a.mac_type = p->mac_type;
+  a.theta_sq = p->theta_sq;
+  a.theta_root = p->theta_root;
+  a.queue_cap = p->queue_cap;
+  a.far_cap = p->far_cap;
+  a.near_cap = p->near_cap;
+  a.nodes = (const NodeRec*)p->nodes;
+  a.wf0 = p->wf0;
+  a.wf1 = p->wf1;
+  a.gen_sizes = p->gen_sizes;
+  a.flags = p->flags;
+  a.counters = (unsigned long long*)p->counters;
+  a.far_keys = p->far_keys;
+  a.far_vals = p->far_vals;
+  a.near_keys = p->near_keys;
+  a.near_vals = p->near_vals;
+  a.hist_gen = p->hist_gen;
+  a.node_hist = p->node_hist;
+  a.roots = p->roots;
+  a.num_roots = p->roots ? p->num_roots : 1;
+  const int G = p->max_generations;
+  const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
+  hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
+                     G);
+  const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
+  for (int k = 0; k < G; ++k) {
+    if (p->emit)
+      hipLaunchKernelGGL(k_walk_flat<true>, dim3(blocks), dim3(256), 0, st, a, k);
+    else
+      hipLaunchKernelGGL(k_walk_flat<false>, dim3(blocks), dim3(256), 0, st, a, k);
+  }
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
 }
 
 int ygg_row_offsets(const uint32_t* sorted_keys, int64_t n, int64_t rows, int64_t* offsets,
@@ -461,6 +587,64 @@ int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
       hipLaunchKernelGGL(k_near_dense<int32_t>, gd, dim3(256), 0, st, L, row_offsets, vals,
                          leaf_position, max_count, (int32_t*)neighbor_leaf_positions);
   }
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+static inline Seg seg_of(const YggBatch* b) {
+  Seg sg;
+  sg.S = b->num_systems;
+  sg.leaf_size = b->leaf_size;
+  sg.pt = b->point_offsets;
+  sg.leaf = b->leaf_offsets;
+  return sg;
+}
+
+int ygg_batch_far_localize(const YggBatch* b, const int64_t* row_offsets,
+                           const int32_t* nodes_by_level_global, const int32_t* level_offsets,
+                           int64_t n, int out_i64, void* sources, void* targets,
+                           void* node_offsets_out, void* far_level_offsets_out, void* stream) {
+  if (!b || b->num_systems < 1 || n < 0) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const Seg sg = seg_of(b);
+  const int64_t T = 2 * b->num_leaves - b->num_systems;
+  const int64_t rows = T > b->num_systems ? T : b->num_systems;
+  dim3 ge(grid_for(n > 0 ? n : 1, 256, 16384)), gr(grid_for(rows, 256)),
+      gl(grid_for(65 * (int64_t)b->num_systems, 256));
+  if (out_i64) {
+    if (n > 0)
+      hipLaunchKernelGGL(k_b_far_localize_entries<int64_t>, ge, dim3(256), 0, st, sg, n,
+                         (int64_t*)sources, (int64_t*)targets);
+    hipLaunchKernelGGL(k_b_far_localize_rows<int64_t>, gr, dim3(256), 0, st, sg, row_offsets,
+                       nodes_by_level_global, T, (int64_t*)node_offsets_out);
+    hipLaunchKernelGGL(k_b_far_level_offsets<int64_t>, gl, dim3(256), 0, st, sg, row_offsets,
+                       level_offsets, (int64_t*)far_level_offsets_out);
+  } else {
+    if (n > 0)
+      hipLaunchKernelGGL(k_b_far_localize_entries<int32_t>, ge, dim3(256), 0, st, sg, n,
+                         (int32_t*)sources, (int32_t*)targets);
+    hipLaunchKernelGGL(k_b_far_localize_rows<int32_t>, gr, dim3(256), 0, st, sg, row_offsets,
+                       nodes_by_level_global, T, (int32_t*)node_offsets_out);
+    hipLaunchKernelGGL(k_b_far_level_offsets<int32_t>, gl, dim3(256), 0, st, sg, row_offsets,
+                       level_offsets, (int32_t*)far_level_offsets_out);
+  }
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_batch_near_localize(const YggBatch* b, const int64_t* row_offsets, int64_t n,
+                            int out_i64, void* neighbors, void* near_offsets_out, void* stream) {
+  if (!b || b->num_systems < 1 || n < 0) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const Seg sg = seg_of(b);
+  const int64_t work = (n > b->num_leaves + b->num_systems) ? n : b->num_leaves + b->num_systems;
+  dim3 g(grid_for(work, 256, 16384));
+  if (out_i64)
+    hipLaunchKernelGGL(k_b_near_localize<int64_t>, g, dim3(256), 0, st, sg, row_offsets, n,
+                       (int64_t*)neighbors, (int64_t*)near_offsets_out);
+  else
+    hipLaunchKernelGGL(k_b_near_localize<int32_t>, g, dim3(256), 0, st, sg, row_offsets, n,
+                       (int32_t*)neighbors, (int32_t*)near_offsets_out);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

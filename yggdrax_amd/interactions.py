@@ -285,6 +285,9 @@ class _WalkContext:
         p.far_keys, p.far_vals = _lib.ptr(fk), _lib.ptr(fv)
         p.near_keys, p.near_vals = _lib.ptr(nk), _lib.ptr(nv)
         p.hist_gen, p.node_hist = int(hist_gen), _lib.ptr(node_hist)
+        roots = getattr(self, "roots", None)
+        if roots is not None:
+            p.roots, p.num_roots = roots.data_ptr(), int(roots.numel())
         ngen = ctypes.c_int32(0)
         gen_ms = (ctypes.c_float * MAX_GENERATIONS)() if self.timing else None
         _lib.check(self.lib.ygg_walk_flat(ctypes.byref(p), self.host_status.data_ptr(),

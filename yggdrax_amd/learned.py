@@ -162,6 +162,46 @@ class CapacityModel:
         c = self.capacities(tree, self.predict_totals(tree, geom))
         return {"far": int(c[0]) + 1024, "near": int(c[1]) + 1024, "queue": int(c[2]) + 1024}
 
+    def predict_totals_batched(self, trees, geom) -> np.ndarray:
+        """Uncalibrated predicted totals over every system of a batch
+        (two-pass inference with each system's own root)."""
+        lib = _lib.load()
+        lay = trees.layout
+        dev = geom.center.device
+        st = _lib.stream_ptr(dev)
+        T = lay.num_nodes
+        r = trees.rad
+
+        def feats(pf):
+            x = torch.empty((T, N_IN), dtype=torch.float32, device=dev)
+            _lib.check(lib.ygg_batch_mlp_features(
+                lay.ptr(), r["node_level"].data_ptr(), r["node_ranges"].data_ptr(),
+                r["parent"].data_ptr(), r["left_child"].data_ptr(), r["right_child"].data_ptr(),
+                geom.half_extent.data_ptr(), geom.radius.data_ptr(), geom.max_extent.data_ptr(),
+                _lib.ptr(pf), x.data_ptr(), st), "batch features")
+            return x
+
+        y0 = self.forward(feats(None))
+        pf = torch.empty(T, dtype=torch.float32, device=dev)
+        _lib.check(lib.ygg_mlp_far_counts(y0.data_ptr(), T, float(self.scale[0]), pf.data_ptr(),
+                                          st), "far counts")
+        y = self.forward(feats(pf))
+        tot = torch.empty(3, dtype=torch.float64, device=dev)
+        sc = [float(v) for v in self.scale]
+        _lib.check(lib.ygg_batch_mlp_totals(lay.ptr(), y.data_ptr(), sc[0], sc[1], sc[2],
+                                            tot.data_ptr(), st), "batch totals")
+        return tot.cpu().numpy()
+
+    def capacity_hint_batched(self, trees, geom) -> dict:
+        """Far / near stream and queue capacities for one forest walk over a
+        batch (sums over its systems; floors per node of the whole batch)."""
+        lay = trees.layout
+        units = np.array([lay.num_nodes, lay.num_leaves, lay.num_nodes], np.float64)
+        t = self.predict_totals_batched(trees, geom)
+        c = t * self.calib * (1.0 + self.margin) + self.floor * units
+        return {"far": int(c[0]) + 1024, "near": int(c[1]) + 1024,
+                "queue": int(c[2]) + 1024 + lay.num_systems}
+
 
 # ---- training data from the walk itself ------------------------------------
 

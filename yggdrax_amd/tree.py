@@ -374,6 +374,23 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
     if malformed and max_level >= 65536:
         raise RuntimeError("tree depth exceeds 65535 levels (cyclic parent links from "
                            "duplicate Morton codes)")
+    oct_num_levels = int(octa["scalars"].cpu()[0]) if octree else 0
+    return _assemble_tree(rad, octa if octree else None, perm, sorted_codes, b6, n, leaf_size,
+                          num_levels, max_level, malformed, oct_num_levels, leaf_indices,
+                          leaf_position, pos_sorted, mass_sorted, inv, return_reordered)
+
+
+def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, max_level,
+                   malformed, oct_num_levels, leaf_indices, leaf_position, pos_sorted,
+                   mass_sorted, inv, return_reordered):
+    """Public topology (reference field names and dtypes) and the kernels'
+    int32 view from the build's device arrays; also used for one system's
+    slices of a batched build."""
+    octree = octa is not None
+    dev = rad["parent"].device
+    L = int(rad["leaf_codes"].shape[0])
+    T = 2 * L - 1
+    NI = L - 1
     idx = INDEX_DTYPE
     fields = dict(
         parent=rad["parent"].to(idx),
@@ -398,7 +415,6 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
         leaf_size=int(leaf_size),
     )
     if octree:
-        oscal = octa["scalars"].cpu()
         fields.update(
             oct_parent=octa["oct_parent"].to(idx),
             oct_children=octa["oct_children"].to(idx),
@@ -410,7 +426,7 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
             oct_node_ranges=octa["oct_node_ranges"].to(idx),
             oct_nodes_by_level=octa["oct_nodes_by_level"].to(idx),
             oct_level_offsets=octa["oct_level_offsets"].to(idx),
-            oct_num_levels=int(oscal[0]),
+            oct_num_levels=int(oct_num_levels),
             oct_leaf_mask=octa["oct_leaf_mask"].view(torch.bool),
             oct_leaf_nodes=octa["oct_leaf_nodes"].to(idx),
             radix_node_to_oct=octa["radix_node_to_oct"].to(idx),
