@@ -253,6 +253,16 @@ def batched_leg(torch, steps, warmup):
     torch.cuda.synchronize()
     t_l = (time.perf_counter() - t0) / steps * 1e3
     learned_fallbacks = I.FLAT_STATS["count_walks"] - c0
+    # the same learned batched pipeline captured once as a HIP graph, replayed
+    graph = batch.BatchedGraph(P, M, offsets, leaf_size=32, theta=0.6, model=model, **kw)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, _, glists = graph.replay()
+    torch.cuda.synchronize()
+    t_g = (time.perf_counter() - t0) / steps * 1e3
+    graph_fell_back = graph.last_replay_fell_back
     run_one(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -264,6 +274,12 @@ def batched_leg(torch, steps, warmup):
         si, sn = outs[k]
         bi, bn = lists.system(k)
         li, ln = llists.system(k)
+        gi, gn = glists.system(k)
+        same &= bool(torch.equal(bi.counts, gi.counts) and torch.equal(bn.counts, gn.counts)
+                     and _same_lists(torch, (bi.offsets, bi.counts, bi.sources),
+                                     (gi.offsets, gi.counts, gi.sources), bi.counts.numel())
+                     and _same_lists(torch, (bn.offsets, bn.counts, bn.neighbors),
+                                     (gn.offsets, gn.counts, gn.neighbors), bn.counts.numel()))
         same &= bool(torch.equal(bi.counts, li.counts) and torch.equal(bn.counts, ln.counts)
                      and _same_lists(torch, (bi.offsets, bi.counts, bi.sources),
                                      (li.offsets, li.counts, li.sources), bi.counts.numel())
@@ -282,6 +298,9 @@ def batched_leg(torch, steps, warmup):
                 learned_batched_ms=round(t_l, 3),
                 learned_batched_mpts_s=round(n / t_l / 1e3, 3),
                 learned_fallback_walks=learned_fallbacks,
+                graph_replay_ms=round(t_g, 3), graph_replay_mpts_s=round(n / t_g / 1e3, 3),
+                graph_replay_fell_back=graph_fell_back,
+                graph_speedup_vs_per_system=round(t_s / t_g, 2),
                 learned_far_overalloc=round(hint["far"] / max(1, lists.stats["far_total"]), 3),
                 learned_near_overalloc=round(hint["near"] / max(1, lists.stats["near_total"]), 3),
                 per_system_loop_ms=round(t_s, 3), speedup_vs_per_system=round(t_s / t_b, 2),

@@ -92,3 +92,47 @@ def test_batch_matches_single_system_path_c4_like():
             k1 = np.sort(rows << 32 | parity.np_(x1).astype(np.int64))
             k2 = np.sort(rows << 32 | parity.np_(x2).astype(np.int64))
             assert np.array_equal(k1, k2)
+
+
+def _assert_batches_equal(a, b, systems):
+    for k in systems:
+        ai, an = a.system(k)
+        bi, bn = b.system(k)
+        for x, y in ((ai.counts, bi.counts), (ai.offsets, bi.offsets), (ai.targets, bi.targets),
+                     (ai.level_offsets, bi.level_offsets), (an.counts, bn.counts),
+                     (an.offsets, bn.offsets)):
+            assert torch.equal(x, y), k
+        for (o, c, x1, x2) in ((ai.offsets, ai.counts, ai.sources, bi.sources),
+                               (an.offsets, an.counts, an.neighbors, bn.neighbors)):
+            off, cnt = parity.np_(o), parity.np_(c)
+            order = np.argsort(off[: len(cnt)], kind="stable")
+            rows = np.repeat(order, cnt[order]).astype(np.int64)
+            k1 = np.sort(rows << 32 | parity.np_(x1).astype(np.int64))
+            k2 = np.sort(rows << 32 | parity.np_(x2).astype(np.int64))
+            assert np.array_equal(k1, k2), k
+
+
+def test_batched_graph_replays_match_eager():
+    """The HIP-graph capture of the batched pipeline, replayed several times
+    on new positions of the same layout (eager work and synchronisation in
+    between), gives every replay the eager pipeline's lists."""
+    from yggdrax_amd import batch
+    rng = np.random.default_rng(6)
+    sizes = np.round(np.exp(rng.uniform(np.log(256), np.log(8192), 48))).astype(np.int64)
+    pos, mass, offsets = _systems(sizes, 6)
+    P = torch.from_numpy(np.concatenate(pos)).cuda()
+    M = torch.from_numpy(np.concatenate(mass)).cuda()
+    g = batch.BatchedGraph(P, M, offsets, leaf_size=32, theta=0.6, mac_type="dehnen",
+                           slack=1.3)
+    for step in range(3):
+        Pn = P + torch.from_numpy(rng.normal(0, 1e-3, tuple(P.shape)).astype(np.float32)).cuda()
+        bt, geom, lists = g.replay(Pn, M)
+        print("replay", step, "fell back", g.last_replay_fell_back, flush=True)
+        assert not g.last_replay_fell_back
+        eb = batch.build_octree_batched(Pn, M, offsets, leaf_size=32)
+        eg = batch.compute_tree_geometry_batched(eb)
+        el = batch.build_interactions_and_neighbors_batched(eb, eg, 0.6, mac_type="dehnen")
+        torch.cuda.synchronize()
+        assert torch.equal(bt.sorted_codes, eb.sorted_codes)
+        assert torch.equal(geom.radius, eg.radius)
+        _assert_batches_equal(lists, el, range(0, len(sizes), 5))

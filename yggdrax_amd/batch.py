@@ -28,10 +28,9 @@ import torch
 from . import _lib
 from .dtypes import INDEX_DTYPE
 from .geometry import TreeGeometry
-from .interactions import (MAC_TYPES, NodeInteractionList, NodeNeighborList, _WalkContext,
-                           _ladder, _leaf_tables, _run_walk_flat, FLAT_GRID_BLOCKS)
-from .tree import (DeviceTopology, _alloc_octree, _alloc_radix, _assemble_tree, _ptrs,
-                   build_octree, build_tree)
+from .interactions import (FLAT_GRID_BLOCKS, MAC_TYPES, NodeInteractionList, NodeNeighborList,
+                           _WalkContext, _ladder, _leaf_tables, _run_walk_flat)
+from .tree import _assemble_tree, _ptrs, build_octree, build_tree
 
 
 @dataclass
@@ -224,7 +223,10 @@ def _copy_system(bt: BatchedTrees, k: int, tree) -> None:
 
 
 def _build_batched(positions, masses, offsets, *, leaf_size: int, octree: bool,
-                   return_reordered: bool) -> BatchedTrees:
+                   return_reordered: bool, layout: Optional[BatchLayout] = None,
+                   sync: bool = True) -> BatchedTrees:
+    """sync=False (stream capture): no host reads; the per-system scalars
+    and the deep/malformed fallback are left to the caller."""
     lib = _lib.load()
     pos = positions
     _lib.require_device(pos)
@@ -232,7 +234,7 @@ def _build_batched(positions, masses, offsets, *, leaf_size: int, octree: bool,
     if pos.ndim != 2 or pos.shape[1] != 3:
         raise ValueError("positions must have shape (N, 3)")
     dev = pos.device
-    lay = make_layout(offsets, leaf_size, dev)
+    lay = layout if layout is not None else make_layout(offsets, leaf_size, dev)
     if lay.num_points != pos.shape[0]:
         raise ValueError("offsets[-1] must equal the number of particles")
     st = _lib.stream_ptr(dev)
@@ -298,6 +300,9 @@ def _build_batched(positions, masses, offsets, *, leaf_size: int, octree: bool,
     _lib.check(lib.ygg_batch_reorder(lay.ptr(), pos.data_ptr(), _lib.ptr(m), perm.data_ptr(),
                                      _lib.ptr(pos_sorted), _lib.ptr(mass_sorted),
                                      perm_local.data_ptr(), _lib.ptr(inv), st), "batch reorder")
+    if not sync:
+        return BatchedTrees(lay, octree, rad, octa, sorted_codes, perm_local, b6, None, None,
+                            pos_sorted, mass_sorted, None if inv is None else inv)
     # one host read: every system's scalars
     scal = rad["scalars"].view(S, 8).cpu().numpy().copy()
     osc = octa["scalars"].view(S, 4).cpu().numpy().copy() if octree else None
@@ -438,6 +443,12 @@ class BatchedLists:
     stats: dict
     far_base: np.ndarray = None    # [S] first far entry of each system
     near_base: np.ndarray = None   # [S] first near entry of each system
+    _base_idx: tuple = None
+
+    def set_bases(self) -> None:
+        """Host copies of every system's first far / near entry (one read)."""
+        base = torch.stack(list(self._base_idx)).cpu().numpy()
+        self.far_base, self.near_base = base[0], base[1]
 
     def system(self, k: int):
         lay = self.layout
@@ -490,17 +501,38 @@ def build_interactions_and_neighbors_batched(trees: BatchedTrees, geom: BatchedG
     lad = _ladder(dt, mac_type, max_interactions_per_node=None, max_neighbors_per_leaf=2048,
                   max_pair_queue=None, traversal_config=None)
     fr = _run_walk_flat(ctx, dt, lad, sizes=capacity_hint, strict_int32=False)
-    dev = geom.center.device
+    out = _lists_from_streams(trees, _LayoutIndex(lay, geom.center.device), fr.far_keys,
+                              fr.far_vals, fr.near_vals, fr.far_off, fr.near_off,
+                              fr.stats["far_total"], fr.stats["near_total"], fr.stats)
+    out.set_bases()
+    return out
+
+
+class _LayoutIndex:
+    """Device index tensors that depend on the layout only (built before any
+    stream capture: they need host-to-device copies)."""
+
+    def __init__(self, lay: BatchLayout, dev):
+        S = lay.num_systems
+        self.node_off = torch.from_numpy(lay.node_off[:-1].astype(np.int64)).to(dev)
+        self.leaf_off = torch.from_numpy(lay.leaf_off[:-1].astype(np.int64)).to(dev)
+        self.sys_of_node = torch.repeat_interleave(
+            torch.arange(S, device=dev), torch.from_numpy(np.diff(lay.node_off)).to(dev))
+
+
+def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_vals, near_vals,
+                        far_off, near_off, nf: int, nn: int, stats: dict) -> BatchedLists:
+    """Per-system far/near lists from the forest walk's key-sorted streams.
+    nf / nn may exceed the valid entry counts (capacity-sized streams whose
+    unused keys sort last): the kernels stop at the row offsets' totals."""
+    lib = _lib.load()
+    lay = trees.layout
+    dev = far_keys.device
     st = _lib.stream_ptr(dev)
     idx = INDEX_DTYPE
     i64 = int(idx == torch.int64)
     T, Lt, S = lay.num_nodes, lay.num_leaves, lay.num_systems
-    nf, nn = fr.stats["far_total"], fr.stats["near_total"]
-    # far: rows are batch-global level-major ranks
-    node_off_d = torch.from_numpy(lay.node_off[:-1].astype(np.int64)).to(dev)
-    sys_of_node = torch.repeat_interleave(torch.arange(S, device=dev),
-                                          torch.from_numpy(np.diff(lay.node_off)).to(dev))
-    nbl_global = (trees.rad["nodes_by_level"].long() + node_off_d[sys_of_node]).to(torch.int32)
+    nbl_global = (trees.rad["nodes_by_level"].long() + li.node_off[li.sys_of_node]).to(torch.int32)
     sources = torch.empty(nf, dtype=idx, device=dev)
     targets = torch.empty(nf, dtype=idx, device=dev)
     levels = torch.empty(nf, dtype=idx, device=dev)
@@ -508,38 +540,200 @@ def build_interactions_and_neighbors_batched(trees: BatchedTrees, geom: BatchedG
     counts32 = torch.empty(T, dtype=torch.int32, device=dev)
     node_off_glob = torch.empty(T + 1, dtype=idx, device=dev)
     _lib.check(lib.ygg_far_lists_from_sorted(
-        T, fr.far_keys.data_ptr(), fr.far_vals.data_ptr(), nf, fr.far_off.data_ptr(),
+        T, far_keys.data_ptr(), far_vals.data_ptr(), nf, far_off.data_ptr(),
         nbl_global.data_ptr(), trees.rad["node_level"].data_ptr(), i64, sources.data_ptr(),
         targets.data_ptr(), levels.data_ptr(), counts.data_ptr(), node_off_glob.data_ptr(),
         counts32.data_ptr(), st), "batch far lists")
     far_offsets = torch.empty(T + S, dtype=idx, device=dev)
     far_lvl = torch.empty((S, 65), dtype=idx, device=dev)
     _lib.check(lib.ygg_batch_far_localize(
-        lay.ptr(), fr.far_off.data_ptr(), nbl_global.data_ptr(),
+        lay.ptr(), far_off.data_ptr(), nbl_global.data_ptr(),
         trees.rad["level_offsets"].data_ptr(), nf, i64, sources.data_ptr(), targets.data_ptr(),
         far_offsets.data_ptr(), far_lvl.data_ptr(), st), "batch far localize")
-    # near: rows are batch-global leaf rows
     leaf_position = trees.octa["leaf_position"] if trees.octree else trees.rad["leaf_position"]
     neighbors = torch.empty(nn, dtype=idx, device=dev)
     ncounts = torch.empty(Lt, dtype=idx, device=dev)
     noff_glob = torch.empty(Lt + 1, dtype=idx, device=dev)
     _lib.check(lib.ygg_near_lists_from_sorted(
-        Lt, fr.near_off.data_ptr(), fr.near_vals.data_ptr(), leaf_position.data_ptr(), 0, i64,
+        Lt, near_off.data_ptr(), near_vals.data_ptr(), leaf_position.data_ptr(), 0, i64,
         neighbors.data_ptr(), None, ncounts.data_ptr(), noff_glob.data_ptr(), st),
         "batch near lists")
     near_offsets = torch.empty(Lt + S, dtype=idx, device=dev)
-    _lib.check(lib.ygg_batch_near_localize(lay.ptr(), fr.near_off.data_ptr(), nn, i64,
+    _lib.check(lib.ygg_batch_near_localize(lay.ptr(), near_off.data_ptr(), nn, i64,
                                            neighbors.data_ptr(), near_offsets.data_ptr(), st),
                "batch near localize")
     out = BatchedLists(lay, trees, sources, targets, levels, counts, far_offsets, far_lvl,
-                       neighbors, ncounts, near_offsets, fr.stats)
-    fo = fr.far_off[torch.from_numpy(lay.node_off[:-1]).to(dev)]
-    no = fr.near_off[torch.from_numpy(lay.leaf_off[:-1]).to(dev)]
-    base = torch.stack([fo, no]).cpu().numpy()
-    out.far_base, out.near_base = base[0], base[1]
+                       neighbors, ncounts, near_offsets, stats)
+    out._base_idx = (far_off[li.node_off], near_off[li.leaf_off])
     return out
 
 
-__all__ = ["BatchLayout", "BatchedGeometry", "BatchedLists", "BatchedTrees",
+def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_type: str,
+                dehnen_radius_scale: float, caps: dict, generations: int, grid_blocks: int):
+    """Forest walk with fixed stream capacities and generation count and no
+    host synchronisation (stream-capturable), followed by the row-key sorts
+    over the full capacities (unused stream slots hold key 0xFFFFFFFF and
+    sort last)."""
+    from .interactions import NODE_RECORD_BYTES, _sort_entries
+    lib = _lib.load()
+    lay = trees.layout
+    dev = trees.rad["parent"].device
+    st = _lib.stream_ptr(dev)
+    T, Lt, S = lay.num_nodes, lay.num_leaves, lay.num_systems
+    mac = MAC_TYPES[mac_type]
+    th = ctypes.c_float(float(theta)).value
+    theta_sq = ctypes.c_float(ctypes.c_float(th * th).value).value
+    theta_root = float(np.sqrt(np.float32(theta_sq)))
+    nodes = torch.empty(T * NODE_RECORD_BYTES, dtype=torch.uint8, device=dev)
+    roots = torch.empty(S, dtype=torch.int32, device=dev)
+    leaf_position = trees.octa["leaf_position"] if trees.octree else trees.rad["leaf_position"]
+    _lib.check(lib.ygg_batch_walk_prepare_nodes(
+        lay.ptr(), geom.center.data_ptr(), geom.radius.data_ptr(), geom.max_extent.data_ptr(),
+        trees.rad["left_child"].data_ptr(), trees.rad["right_child"].data_ptr(),
+        leaf_position.data_ptr(), trees.rad["nodes_by_level"].data_ptr(), mac,
+        ctypes.c_float(float(dehnen_radius_scale)).value, nodes.data_ptr(), roots.data_ptr(), st),
+        "graph walk nodes")
+    Q, FC, NC, G = int(caps["queue"]), int(caps["far"]), int(caps["near"]), int(generations)
+    wf = torch.empty((2, Q), dtype=torch.int64, device=dev)
+    counters = torch.empty(2, dtype=torch.int64, device=dev)
+    gen_sizes = torch.empty(G + 1, dtype=torch.int32, device=dev)
+    flags = torch.empty(4, dtype=torch.int32, device=dev)
+    fk = torch.full((FC,), -1, dtype=torch.int32, device=dev)
+    fv = torch.empty(FC, dtype=torch.int32, device=dev)
+    nk = torch.full((NC,), -1, dtype=torch.int32, device=dev)
+    nv = torch.empty(NC, dtype=torch.int32, device=dev)
+    p = _lib.FlatWalkParams()
+    p.num_nodes, p.num_internal, p.num_leaves, p.root = T, max(1, lay.num_internal), Lt, 0
+    p.mac_type, p.emit, p.max_generations, p.grid_blocks = mac, 1, G, int(grid_blocks)
+    p.theta_sq, p.theta_root = theta_sq, theta_root
+    p.queue_cap, p.far_cap, p.near_cap = Q, FC, NC
+    p.nodes, p.wf0, p.wf1 = nodes.data_ptr(), wf[0].data_ptr(), wf[1].data_ptr()
+    p.gen_sizes, p.flags, p.counters = gen_sizes.data_ptr(), flags.data_ptr(), counters.data_ptr()
+    p.far_keys, p.far_vals = fk.data_ptr(), fv.data_ptr()
+    p.near_keys, p.near_vals = nk.data_ptr(), nv.data_ptr()
+    p.hist_gen, p.node_hist = -1, None
+    p.roots, p.num_roots = roots.data_ptr(), S
+    _lib.check(lib.ygg_walk_flat_fixed(ctypes.byref(p), st), "graph walk")
+    fk2, fv2, far_off = _sort_entries(fk, fv, FC, T)
+    _, nv2, near_off = _sort_entries(nk, nv, NC, Lt)
+    return dict(far_keys=fk2, far_vals=fv2, near_vals=nv2, far_off=far_off, near_off=near_off,
+                counters=counters, gen_sizes=gen_sizes, flags=flags, nodes=nodes, wf=wf)
+
+
+class BatchedGraph:
+    """The whole batched pipeline -- build_octree, compute_tree_geometry and
+    the forest walk with its lists -- captured once as a HIP graph for a
+    fixed batch layout. `replay(positions, masses)` reruns it on new inputs
+    of the same layout (e.g. every step of a simulation) with one launch and
+    one small status read.
+
+    Stream capacities and the generation count are fixed at capture: the
+    exact totals of a first eager run (sized by the capacity model when one
+    is given), times `slack`. A replay whose walk
+    overflows, does not finish within the captured generations, or meets a
+    system that needs the single-system path returns the eager pipeline's
+    result instead (`last_replay_fell_back`), so results never depend on the
+    capacities."""
+
+    def __init__(self, positions, masses, offsets, *, leaf_size: int = 32, theta: float = 0.6,
+                 mac_type: str = "dehnen", dehnen_radius_scale: float = 1.0, model=None,
+                 slack: float = 1.1, grid_blocks: int = FLAT_GRID_BLOCKS):
+        from . import autotune
+        _lib.require_device(positions)
+        dev = positions.device
+        self.theta, self.mac_type, self.scale = float(theta), mac_type, float(dehnen_radius_scale)
+        self.leaf_size, self.grid_blocks = int(leaf_size), int(grid_blocks)
+        self.offsets = np.asarray(offsets, dtype=np.int64)
+        self.pos = positions.to(torch.float32).contiguous().clone()
+        self.mass = None if masses is None else torch.as_tensor(
+            masses, dtype=torch.float32, device=dev).contiguous().clone()
+        self.lay = make_layout(self.offsets, leaf_size, dev)
+        self.li = _LayoutIndex(self.lay, dev)
+        # eager run (learned capacities when a model is given): exact stream
+        # totals, generation count and a warm-up of every kernel. The graph's
+        # streams are sized from those totals times `slack`: the capacity-
+        # sized row sorts then do only `slack` times the necessary work.
+        lists = self._eager(model)
+        gens = sum(1 for v in lists.stats["gen_sizes"] if v > 0)
+        caps = {"far": lists.stats["far_total"], "near": lists.stats["near_total"],
+                "queue": lists.stats["peak"]}
+        self.caps = {k: int(v * slack) + 1024 for k, v in caps.items()}
+        self.generations = int(gens * slack) + 4
+        was = autotune.enabled()
+        autotune.enable(False)
+        try:
+            torch.cuda.synchronize(dev)
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self.out = self._pipeline()
+        finally:
+            autotune.enable(was)
+        self.last_replay_fell_back = False
+
+    def _eager(self, model):
+        bt = _build_batched(self.pos, self.mass, self.offsets, leaf_size=self.leaf_size,
+                            octree=True, return_reordered=True, layout=self.lay)
+        geom = compute_tree_geometry_batched(bt)
+        self.hint = model.capacity_hint_batched(bt, geom) if model is not None else None
+        from .interactions import FLAT_STATS
+        c0 = FLAT_STATS["count_walks"]
+        lists = build_interactions_and_neighbors_batched(
+            bt, geom, self.theta, mac_type=self.mac_type, dehnen_radius_scale=self.scale,
+            capacity_hint=self.hint)
+        lists.stats["hinted"] = self.hint is not None and FLAT_STATS["count_walks"] == c0
+        return lists
+
+    def _pipeline(self) -> dict:
+        bt = _build_batched(self.pos, self.mass, None, leaf_size=self.leaf_size, octree=True,
+                            return_reordered=True, layout=self.lay, sync=False)
+        geom = compute_tree_geometry_batched(bt)
+        w = _graph_walk(bt, geom, self.theta, self.mac_type, self.scale, self.caps,
+                        self.generations, self.grid_blocks)
+        lists = _lists_from_streams(bt, self.li, w["far_keys"], w["far_vals"], w["near_vals"],
+                                    w["far_off"], w["near_off"], self.caps["far"],
+                                    self.caps["near"], {})
+        lay = self.lay
+        T, Lt = lay.num_nodes, lay.num_leaves
+        status = torch.cat([
+            w["flags"][:3].long(), w["gen_sizes"][self.generations:].long(), w["counters"],
+            w["far_off"][T:T + 1], w["near_off"][Lt:Lt + 1],
+            bt.rad["scalars"].long(), bt.octa["scalars"].long(),
+            lists._base_idx[0], lists._base_idx[1]])
+        return dict(trees=bt, geom=geom, lists=lists, walk=w, status=status)
+
+    def replay(self, positions=None, masses=None):
+        """(trees, geometry, lists) for the current (or given) inputs."""
+        if positions is not None:
+            self.pos.copy_(positions)
+        if masses is not None and self.mass is not None:
+            self.mass.copy_(torch.as_tensor(masses, dtype=torch.float32))
+        self.graph.replay()
+        S = self.lay.num_systems
+        st = self.out["status"].cpu().numpy()
+        flags, unfinished = st[:3], st[3]
+        far_total, near_total = int(st[4]), int(st[5])     # stream totals of the walk
+        rows_far, rows_near = int(st[6]), int(st[7])       # entries with a valid row key
+        scal = st[8:8 + 8 * S].reshape(S, 8)
+        osc = st[8 + 8 * S:8 + 12 * S].reshape(S, 4)
+        fbase, nbase = st[8 + 12 * S:8 + 13 * S], st[8 + 13 * S:8 + 14 * S]
+        bad = bool(np.any((scal[:, 2] != 0) | (scal[:, 3] != 0)))
+        if (flags.any() or unfinished != 0 or bad or far_total != rows_far
+                or near_total != rows_near):
+            self.last_replay_fell_back = True
+            bt = _build_batched(self.pos, self.mass, self.offsets, leaf_size=self.leaf_size,
+                                octree=True, return_reordered=True, layout=self.lay)
+            geom = compute_tree_geometry_batched(bt)
+            return bt, geom, build_interactions_and_neighbors_batched(
+                bt, geom, self.theta, mac_type=self.mac_type, dehnen_radius_scale=self.scale)
+        self.last_replay_fell_back = False
+        bt, geom, lists = self.out["trees"], self.out["geom"], self.out["lists"]
+        bt.scalars, bt.oct_scalars = scal.astype(np.int32), osc.astype(np.int32)
+        bt._views = {}
+        lists.far_base, lists.near_base = fbase, nbase
+        lists.stats = dict(far_total=far_total, near_total=near_total)
+        return bt, geom, lists
+
+
+__all__ = ["BatchLayout", "BatchedGeometry", "BatchedGraph", "BatchedLists", "BatchedTrees",
            "build_interactions_and_neighbors_batched", "build_octree_batched",
            "build_tree_batched", "compute_tree_geometry_batched", "make_layout"]

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_b_encode_hist(Seg sg, const float* __re
     }
 #pragma unroll
     for (int p = 0; p < 8; ++p)
-      wave_hist_add(&h[p * 256], (unsigned)((c >> (8 * p)) & 0xFF), valid);
+      pass_hist_add<8>(&h[p * 256], (unsigned)((c >> (8 * p)) & 0xFF), valid, p, p == 7 ? 7 : 8);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x)

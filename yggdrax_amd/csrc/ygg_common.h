@@ -107,6 +107,29 @@ __device__ __forceinline__ void wave_hist_add_bits(uint32_t* h, unsigned d, bool
   if (valid && mbcnt64(peers) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
 }
 
+// Histogram add for the digit of one radix pass. The lowest pass of Morton
+// codes and row keys is close to uniform, so one LDS atomic per lane is
+// cheapest there; higher passes repeat digits across a wave and are
+// aggregated with ballots, over only the `used` low bits the key range can
+// set (the top pass of a 19-bit key has 3).
+template <int BITS>
+__device__ __forceinline__ void pass_hist_add(uint32_t* h, unsigned d, bool valid, int pass,
+                                              int used) {
+  if (pass == 0) {
+    if (valid) atomicAdd(&h[d], 1u);
+    return;
+  }
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    if (b >= used) break;
+    bool bit = (d >> b) & 1u;
+    uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  if (valid && mbcnt64(peers) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+}
+
 __device__ __forceinline__ void wave_hist_add(uint32_t* h, unsigned d, bool valid) {
   wave_hist_add_bits<8>(h, d, valid);
 }

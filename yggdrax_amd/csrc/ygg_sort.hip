@@ -10,6 +10,8 @@
 // data but more look-back words per tile (2^bits per pass) and more LDS;
 // the width is a parameter so callers (and the autotuner) can pick the
 // cheapest pass count for a key range.
+#include <type_traits>
+
 #include "ygg_common.h"
 
 namespace ygg {
@@ -27,7 +29,7 @@ constexpr int passes_for(int range_bits, int bits) { return (range_bits + bits -
 
 template <typename K, int BITS>
 __global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, int64_t n,
-                                                    int begin_bit, int passes,
+                                                    int begin_bit, int end_bit, int passes,
                                                     uint32_t* __restrict__ hist) {
   constexpr int kBins = 1 << BITS;
   constexpr int kMaxPasses = passes_for(8 * (int)sizeof(K), BITS);
@@ -39,9 +41,11 @@ __global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, 
     const int64_t i = b0 + threadIdx.x;
     const bool valid = i < n;
     K k = valid ? keys[i] : (K)0;
-    for (int p = 0; p < passes; ++p)
-      wave_hist_add_bits<BITS>(&h[p * kBins],
-                               (unsigned)((k >> (begin_bit + BITS * p)) & (kBins - 1)), valid);
+    for (int p = 0; p < passes; ++p) {
+      const int lo = begin_bit + BITS * p;
+      const int used = end_bit - lo < BITS ? end_bit - lo : BITS;
+      pass_hist_add<BITS>(&h[p * kBins], (unsigned)((k >> lo) & (kBins - 1)), valid, p, used);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x)
@@ -93,8 +97,15 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   __shared__ uint32_t s_digit_start[kBins];
   __shared__ uint32_t s_global[kBins];
   __shared__ uint32_t s_wsum[kWaves];
-  __shared__ K s_keys[kTile];
-  __shared__ uint32_t s_vals[kTile];
+  // Keys, then values, are staged through one buffer (two scatter phases),
+  // with each slot's digit kept beside it: less LDS per tile, more tiles
+  // resident per CU.
+  // (64-bit keys only: for 32-bit keys one buffer each is faster.)
+  using DigT = typename std::conditional<(BITS <= 8), uint8_t, uint16_t>::type;
+  constexpr bool kTwoPhase = sizeof(K) == 8;
+  __shared__ K s_buf[kTile];
+  __shared__ DigT s_dig[kTile];
+  __shared__ uint32_t s_vals[kTwoPhase ? 1 : kTile];
 
   const int tid = threadIdx.x;
   const int w = tid / YGG_WAVE;
@@ -185,18 +196,30 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   for (int k = 0; k < kSortItems; ++k) {
     unsigned d = (unsigned)((key[k] >> shift) & kMask);
     unsigned pos = s_digit_start[d] + s_wave_hist[w][d] + rank[k];
-    s_keys[pos] = key[k];
-    s_vals[pos] = val[k];
+    rank[k] = pos;  // tile slot, reused for the values
+    s_buf[pos] = key[k];
+    s_dig[pos] = (DigT)d;
+    if (!kTwoPhase) s_vals[pos] = val[k];
   }
   __syncthreads();
   const int64_t valid = (n - base) < kTile ? (n - base) : kTile;
-  for (int p = tid; p < valid; p += kSortThreads) {
-    K kk = s_keys[p];
-    unsigned d = (unsigned)((kk >> shift) & kMask);
-    int64_t g = (int64_t)s_global[d] + p;
-    keys_out[g] = kk;
-    vals_out[g] = s_vals[p];
+  if (!kTwoPhase) {
+    for (int p = tid; p < valid; p += kSortThreads) {
+      const int64_t g = (int64_t)s_global[s_dig[p]] + p;
+      keys_out[g] = s_buf[p];
+      vals_out[g] = s_vals[p];
+    }
+    return;
   }
+  for (int p = tid; p < valid; p += kSortThreads)
+    keys_out[(int64_t)s_global[s_dig[p]] + p] = s_buf[p];
+  __syncthreads();
+  uint32_t* s_vb = reinterpret_cast<uint32_t*>(s_buf);
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) s_vb[rank[k]] = val[k];
+  __syncthreads();
+  for (int p = tid; p < valid; p += kSortThreads)
+    vals_out[(int64_t)s_global[s_dig[p]] + p] = s_vb[p];
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
@@ -273,7 +296,7 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
     rc = fill_u32(hist, 0u, (int64_t)passes * kBins, st);
     if (rc) return rc;
     hipLaunchKernelGGL((k_digit_hist<K, BITS>), dim3(grid_for(n, 256, 2048)), dim3(256), 0, st,
-                       keys_in, n, begin_bit, passes, hist);
+                       keys_in, n, begin_bit, end_bit, passes, hist);
     YGG_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_hist_scan<BITS>, dim3(passes), dim3(256), 0, st, hist);

@@ -231,7 +231,7 @@ __global__ void k_flat_init(FlatArgs a, int root, int G) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int k = i; k <= G; k += gridDim.x * blockDim.x) a.gen_sizes[k] = 0;
   if (i < 4) a.flags[i] = 0;
-  if (i < 5) a.counters[i] = 0;
+  if (i < 2) a.counters[i] = 0;  // [0] far total, [1] near total
   if (a.roots) {
     for (int64_t k = i; k < a.num_roots; k += (int64_t)gridDim.x * blockDim.x) {
       const uint32_t r = (uint32_t)a.roots[k];
@@ -262,14 +262,16 @@ __global__ void k_row_offsets(const uint32_t* __restrict__ keys, int64_t n, int 
 
 template <typename OutT>
 __global__ void k_far_from_sorted(const uint32_t* __restrict__ keys,
-                                  const uint32_t* __restrict__ vals, int64_t n,
+                                  const uint32_t* __restrict__ vals, int64_t n, int rows,
                                   const int32_t* __restrict__ nbl,
                                   const int32_t* __restrict__ node_level,
                                   OutT* __restrict__ sources, OutT* __restrict__ targets,
                                   OutT* __restrict__ levels) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t node = nbl[keys[i]];
+    const uint32_t key = keys[i];
+    if (key >= (uint32_t)rows) continue;  // unused stream capacity (filler keys sort last)
+    int32_t node = nbl[key];
     sources[i] = (OutT)vals[i];
     targets[i] = (OutT)node;
     levels[i] = (OutT)node_level[node];
@@ -327,9 +329,11 @@ __global__ __launch_bounds__(256) void k_near_dense(int L, const int64_t* __rest
 
 // ---- batched (forest) lists -> per-system lists ---------------------------------
 template <typename OutT>
-__global__ void k_b_far_localize_entries(Seg sg, int64_t n, OutT* __restrict__ sources,
+__global__ void k_b_far_localize_entries(Seg sg, const int64_t* __restrict__ off, int64_t rows,
+                                         int64_t n, OutT* __restrict__ sources,
                                          OutT* __restrict__ targets) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+  const int64_t total = off[rows] < n ? off[rows] : n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t t = (int64_t)targets[e];
     const int64_t n0 = sg.node_off(sg.of_node(t));
@@ -371,7 +375,8 @@ template <typename OutT>
 __global__ void k_b_near_localize(Seg sg, const int64_t* __restrict__ off, int64_t n,
                                   OutT* __restrict__ neighbors, OutT* __restrict__ offsets_out) {
   const int64_t Ltot = sg.leaf[sg.S];
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+  const int64_t total = off[Ltot] < n ? off[Ltot] : n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = (int64_t)neighbors[e];
     neighbors[e] = (OutT)(v - sg.node_off(sg.of_node(v)));
@@ -547,14 +552,14 @@ int ygg_far_lists_from_sorted(int64_t num_nodes, const uint32_t* keys, const uin
   dim3 ge(grid_for(n > 0 ? n : 1, 256, 16384)), gr(grid_for(T, 256));
   if (out_i64) {
     if (n > 0)
-      hipLaunchKernelGGL(k_far_from_sorted<int64_t>, ge, dim3(256), 0, st, keys, vals, n,
+      hipLaunchKernelGGL(k_far_from_sorted<int64_t>, ge, dim3(256), 0, st, keys, vals, n, T,
                          nodes_by_level, node_level, (int64_t*)sources, (int64_t*)targets,
                          (int64_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int64_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int64_t*)counts, (int64_t*)node_offsets, counts32);
   } else {
     if (n > 0)
-      hipLaunchKernelGGL(k_far_from_sorted<int32_t>, ge, dim3(256), 0, st, keys, vals, n,
+      hipLaunchKernelGGL(k_far_from_sorted<int32_t>, ge, dim3(256), 0, st, keys, vals, n, T,
                          nodes_by_level, node_level, (int32_t*)sources, (int32_t*)targets,
                          (int32_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
@@ -613,16 +618,16 @@ int ygg_batch_far_localize(const YggBatch* b, const int64_t* row_offsets,
       gl(grid_for(65 * (int64_t)b->num_systems, 256));
   if (out_i64) {
     if (n > 0)
-      hipLaunchKernelGGL(k_b_far_localize_entries<int64_t>, ge, dim3(256), 0, st, sg, n,
-                         (int64_t*)sources, (int64_t*)targets);
+      hipLaunchKernelGGL(k_b_far_localize_entries<int64_t>, ge, dim3(256), 0, st, sg,
+                         row_offsets, T, n, (int64_t*)sources, (int64_t*)targets);
     hipLaunchKernelGGL(k_b_far_localize_rows<int64_t>, gr, dim3(256), 0, st, sg, row_offsets,
                        nodes_by_level_global, T, (int64_t*)node_offsets_out);
     hipLaunchKernelGGL(k_b_far_level_offsets<int64_t>, gl, dim3(256), 0, st, sg, row_offsets,
                        level_offsets, (int64_t*)far_level_offsets_out);
   } else {
     if (n > 0)
-      hipLaunchKernelGGL(k_b_far_localize_entries<int32_t>, ge, dim3(256), 0, st, sg, n,
-                         (int32_t*)sources, (int32_t*)targets);
+      hipLaunchKernelGGL(k_b_far_localize_entries<int32_t>, ge, dim3(256), 0, st, sg,
+                         row_offsets, T, n, (int32_t*)sources, (int32_t*)targets);
     hipLaunchKernelGGL(k_b_far_localize_rows<int32_t>, gr, dim3(256), 0, st, sg, row_offsets,
                        nodes_by_level_global, T, (int32_t*)node_offsets_out);
     hipLaunchKernelGGL(k_b_far_level_offsets<int32_t>, gl, dim3(256), 0, st, sg, row_offsets,
