@@ -71,6 +71,11 @@ int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t*
  * mass_sorted[i] = mass[perm[i]] (mass may be NULL), inverse[perm[i]] = i. */
 int ygg_reorder(const float* pos, const float* mass, const uint32_t* perm, int64_t n,
                 float* pos_sorted, float* mass_sorted, int32_t* inverse, void* stream);
+/* The same outputs as ygg_reorder, computed by scattering: the inverse
+ * permutation first, then each particle written to its sorted slot (reads
+ * coalesced, writes scattered). inverse is required. */
+int ygg_reorder_scatter(const float* pos, const float* mass, const uint32_t* perm, int64_t n,
+                        float* pos_sorted, float* mass_sorted, int32_t* inverse, void* stream);
 
 /* ---- radix (LBVH) topology ---------------------------------------------- */
 
@@ -260,7 +265,8 @@ typedef struct YggFlatWalkParams {
   int32_t num_nodes, num_internal, num_leaves, root;
   int32_t mac_type, emit, max_generations, grid_blocks;
   float theta_sq, theta_root;
-  int32_t gen_hint, pad0;     /* expected generation count (first launch batch), 0 = 24 */
+  int32_t gen_hint;           /* expected generation count (first launch batch), 0 = 24 */
+  int32_t pairs_per_thread;   /* queue tile: pairs per thread of a chunk (4, 8 or 16) */
   int64_t queue_cap;          /* pairs per wavefront buffer */
   int64_t far_cap, near_cap;  /* entries per flat stream */
   const void* nodes;
@@ -273,7 +279,9 @@ typedef struct YggFlatWalkParams {
   uint32_t* far_vals;
   uint32_t* near_keys;
   uint32_t* near_vals;
-  int32_t hist_gen, pad1;     /* node_hist[t] counts generation hist_gen's pairs by target */
+  int32_t hist_gen;           /* node_hist[t] counts generation hist_gen's pairs by target */
+  int32_t xcd_groups;         /* > 1: split each generation's chunks into that many contiguous
+                                 ranges, one per XCD (workgroups dispatch round-robin) */
   int32_t* node_hist;         /* [num_nodes] or NULL */
   const int32_t* roots;       /* [num_roots] seed pairs (r, r) of a forest, or NULL: (root, root) */
   int64_t num_roots;

@@ -1,0 +1,40 @@
+"""The oracle reproduces the committed C1 fixture (tests/golden/c1_uniform512.npz,
+written by tests/golden/make_golden.py) byte for byte."""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from tests.golden import make_golden
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_uniform512.npz")
+
+
+def test_oracle_matches_c1_fixture():
+    z = np.load(FIXTURE, allow_pickle=False)
+    got = make_golden.compute()
+    assert sorted(got) == sorted(z.files)
+    for k in z.files:
+        a, b = np.asarray(got[k]), z[k]
+        assert a.shape == b.shape and a.dtype == b.dtype, k
+        if a.dtype.kind == "f":
+            assert np.array_equal(a.view(np.uint32 if a.itemsize == 4 else np.uint64),
+                                  b.view(np.uint32 if b.itemsize == 4 else np.uint64)), k
+        else:
+            assert np.array_equal(a, b), k
+
+
+def test_c1_fixture_known_totals():
+    """Sizes that follow from the configuration alone: 512 points at leaf 16
+    give 32 leaves and 63 radix nodes; every near entry has a symmetric
+    partner; far rows come in (target, source) / (source, target) pairs."""
+    z = np.load(FIXTURE, allow_pickle=False)
+    assert z["topo_parent"].shape == (63,)
+    assert z["near_counts"].shape == (32,)
+    assert int(z["near_counts"].sum()) % 2 == 0 and int(z["far_counts"].sum()) % 2 == 0
+    # symmetric far pairs: (t, s) present iff (s, t) present
+    off, cnt, src = z["far_offsets"], z["far_counts"], z["far_sources"]
+    pairs = {(t, int(s)) for t in range(len(cnt)) for s in src[off[t]:off[t] + cnt[t]]}
+    assert all((s, t) in pairs for t, s in pairs)

@@ -205,3 +205,26 @@ def test_overflow_errors_match(ygg):
     with pytest.raises(RuntimeError, match="Pair queue capacity exceeded"):
         ygg.build_interactions_and_neighbors(tree, geom, theta=0.6, mac_type="dehnen",
                                              max_pair_queue=4, max_interactions_per_node=64)
+
+
+def test_gpu_matches_c1_fixture(ygg):
+    """The HIP path reproduces the committed C1 fixture (the reference's
+    CPU-runnable configuration)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "c1_uniform512.npz"), allow_pickle=False)
+    pos, mass = z["positions"], z["masses"]
+    tree, ps, ms, inv = ygg.build_octree(torch.from_numpy(pos).cuda(),
+                                         torch.from_numpy(mass).cuda(), leaf_size=16,
+                                         return_reordered=True)
+    ref = {k[5:]: z[k] for k in z.files if k.startswith("topo_")}
+    parity.assert_topology_equal(tree, ref, octree=True)
+    assert np.array_equal(parity.np_(ps), ref["positions_sorted"])
+    assert np.array_equal(parity.np_(inv), ref["inverse_permutation"])
+    geom = ygg.compute_tree_geometry(tree, ps)
+    parity.assert_geometry_close(geom, {k[5:]: z[k] for k in z.files if k.startswith("geom_")})
+    cfg = ygg.DualTreeTraversalConfig(8192, 256, 2048, 2048)
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=0.6, mac_type="dehnen",
+                                                       traversal_config=cfg)
+    parity.assert_interactions_equal(inter, {k[4:]: z[k] for k in z.files if k.startswith("far_")})
+    parity.assert_neighbors_equal(nbrs, {k[5:]: z[k] for k in z.files if k.startswith("near_")})

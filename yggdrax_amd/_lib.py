@@ -58,11 +58,11 @@ class FlatWalkParams(ctypes.Structure):
     _fields_ = [
         ("num_nodes", _I32), ("num_internal", _I32), ("num_leaves", _I32), ("root", _I32),
         ("mac_type", _I32), ("emit", _I32), ("max_generations", _I32), ("grid_blocks", _I32),
-        ("theta_sq", _F), ("theta_root", _F), ("gen_hint", _I32), ("pad0", _I32),
+        ("theta_sq", _F), ("theta_root", _F), ("gen_hint", _I32), ("pairs_per_thread", _I32),
         ("queue_cap", _I64), ("far_cap", _I64),
         ("near_cap", _I64), ("nodes", _P), ("wf0", _P), ("wf1", _P), ("gen_sizes", _P),
         ("flags", _P), ("counters", _P), ("far_keys", _P), ("far_vals", _P),
-        ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("pad1", _I32),
+        ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("xcd_groups", _I32),
         ("node_hist", _P), ("roots", _P), ("num_roots", _I64),
     ]
 
@@ -106,6 +106,7 @@ _SIGS = {
     "ygg_morton_sort_workspace_bytes": (_SZ, [_I64]),
     "ygg_morton_sort": (_INT, [_P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "ygg_reorder": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _P]),
+    "ygg_reorder_scatter": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _P]),
     "ygg_radix_build_workspace_bytes": (_SZ, [_I64, _I64]),
     "ygg_radix_build": (_INT, [_P, _I64, _I64, ctypes.POINTER(RadixTopologyPtrs), _P, _SZ, _P]),
     "ygg_levels_pointer_doubling": (_INT, [ctypes.POINTER(RadixTopologyPtrs), _I64, _P, _SZ, _P]),

@@ -9,9 +9,10 @@ Plans can be saved to and loaded from a JSON file (``YGGDRAX_AUTOTUNE_CACHE``).
 Tuned parameters:
   * radix digit width (8..11 bits) of the row-key sorts that turn the
     walk's emission streams into CSR lists, and of the Morton-code sort;
-  * the flat walk's grid size (persistent blocks per generation launch,
-    each taking 1024-pair chunks of the wavefront: the analogue of the
-    reference's process_block).
+  * the flat walk's grid size (persistent blocks per generation launch:
+    the analogue of the reference's process_block) and its queue tile
+    (pairs per thread of one chunk, i.e. 1024-4096-pair chunks, each
+    reserving its outputs with one atomic per stream).
 """
 
 from __future__ import annotations
@@ -32,6 +33,7 @@ _CACHE = os.environ.get("YGGDRAX_AUTOTUNE_CACHE", "")
 
 SORT_DIGIT_BITS = (8, 9, 10, 11)
 WALK_GRID_BLOCKS = (1024, 2048, 4096, 8192)
+WALK_TILES = (4, 8, 16)
 
 
 def enable(flag: bool = True) -> None:
@@ -128,6 +130,13 @@ def sort_digit_bits(key_bytes: int, range_bits: int, n: int) -> int:
     return choose(key, SORT_DIGIT_BITS, timer, default)
 
 
+def walk_tile(num_nodes: int, run_with: Callable[[int], None], default: int) -> int:
+    """Queue tile of the flat walk (pairs per thread of a 256-thread chunk);
+    `run_with(per)` runs one counting walk with that tile."""
+    key = f"walk/tile/n{_size_class(num_nodes)}"
+    return choose(key, WALK_TILES, lambda t: _time_ms(lambda: run_with(t), reps=1), default)
+
+
 def walk_grid_blocks(num_nodes: int, run_with: Callable[[int], None], default: int) -> int:
     """Grid size of the flat walk's generation kernel; `run_with(blocks)`
     runs one counting walk with that grid (used only while tuning)."""
@@ -136,4 +145,4 @@ def walk_grid_blocks(num_nodes: int, run_with: Callable[[int], None], default: i
 
 
 __all__ = ["choose", "enable", "enabled", "load", "plans", "save", "sort_digit_bits",
-           "walk_grid_blocks"]
+           "walk_grid_blocks", "walk_tile"]

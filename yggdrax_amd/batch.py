@@ -419,6 +419,8 @@ class _ForestContext(_WalkContext):
         max_levels = int(trees.scalars[:, 0].max())
         self.gen_hint = int(1.5 * max(1, max_levels)) + 4
         self.grid_blocks = FLAT_GRID_BLOCKS
+        from .interactions import WALK_PAIRS_PER_THREAD
+        self.pairs_per_thread = WALK_PAIRS_PER_THREAD
         self.kernel_log = WALK_TIMING["log"]
         self.dt = SimpleNamespace(num_nodes=T, num_internal=max(1, lay.num_internal),
                                   num_leaves=lay.num_leaves, num_levels=max_levels)
@@ -613,6 +615,9 @@ def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_ty
     p.near_keys, p.near_vals = nk.data_ptr(), nv.data_ptr()
     p.hist_gen, p.node_hist = -1, None
     p.roots, p.num_roots = roots.data_ptr(), S
+    from .interactions import WALK_PAIRS_PER_THREAD, WALK_XCD_GROUPS
+    p.xcd_groups = WALK_XCD_GROUPS
+    p.pairs_per_thread = WALK_PAIRS_PER_THREAD
     _lib.check(lib.ygg_walk_flat_fixed(ctypes.byref(p), st), "graph walk")
     fk2, fv2, far_off = _sort_entries(fk, fv, FC, T)
     _, nv2, near_off = _sort_entries(nk, nv, NC, Lt)

@@ -108,6 +108,31 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ pos,
   }
 }
 
+// Reorder by scattering: the inverse permutation first (coalesced reads of
+// perm), then every particle read in input order and written to its sorted
+// slot, so all reads are coalesced and only writes are scattered.
+__global__ __launch_bounds__(256) void k_inverse(const uint32_t* __restrict__ perm, int64_t n,
+                                                 int32_t* __restrict__ inv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    inv[perm[i]] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_sorted(const float* __restrict__ pos,
+                                                        const float* __restrict__ mass,
+                                                        const int32_t* __restrict__ inv,
+                                                        int64_t n, float* __restrict__ pos_sorted,
+                                                        float* __restrict__ mass_sorted) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = inv[j];
+    pos_sorted[3 * i] = pos[3 * j];
+    pos_sorted[3 * i + 1] = pos[3 * j + 1];
+    pos_sorted[3 * i + 2] = pos[3 * j + 2];
+    if (mass && mass_sorted) mass_sorted[i] = mass[j];
+  }
+}
+
 // ---- Karras radix build (_tree_impl.py:1117-1270) ---------------------------
 __global__ __launch_bounds__(256) void k_leaves(const uint64_t* __restrict__ sc, int64_t n,
                                                 int64_t leaf, int L, uint64_t* __restrict__ lc,
@@ -507,6 +532,18 @@ int ygg_reorder(const float* pos, const float* mass, const uint32_t* perm, int64
   if (n == 0) return YGG_OK;
   hipLaunchKernelGGL(k_reorder, dim3(grid_for(n, 256, 8192)), dim3(256), 0, (hipStream_t)stream,
                      pos, mass, perm, n, pos_sorted, mass_sorted, inverse);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_reorder_scatter(const float* pos, const float* mass, const uint32_t* perm, int64_t n,
+                        float* pos_sorted, float* mass_sorted, int32_t* inverse, void* stream) {
+  if (n < 0 || !inverse || !pos_sorted) return YGG_ERR_ARG;
+  if (n == 0) return YGG_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_inverse, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, perm, n, inverse);
+  hipLaunchKernelGGL(k_scatter_sorted, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, pos, mass,
+                     inverse, n, pos_sorted, mass_sorted);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

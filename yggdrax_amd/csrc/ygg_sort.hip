@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   using DigT = typename std::conditional<(BITS <= 8), uint8_t, uint16_t>::type;
   constexpr bool kTwoPhase = sizeof(K) == 8;
   __shared__ K s_buf[kTile];
-  __shared__ DigT s_dig[kTile];
+  __shared__ DigT s_dig[kTwoPhase ? kTile : 1];
   __shared__ uint32_t s_vals[kTwoPhase ? 1 : kTile];
 
   const int tid = threadIdx.x;
@@ -198,15 +198,18 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
     unsigned pos = s_digit_start[d] + s_wave_hist[w][d] + rank[k];
     rank[k] = pos;  // tile slot, reused for the values
     s_buf[pos] = key[k];
-    s_dig[pos] = (DigT)d;
-    if (!kTwoPhase) s_vals[pos] = val[k];
+    if (kTwoPhase)
+      s_dig[pos] = (DigT)d;
+    else
+      s_vals[pos] = val[k];
   }
   __syncthreads();
   const int64_t valid = (n - base) < kTile ? (n - base) : kTile;
   if (!kTwoPhase) {
     for (int p = tid; p < valid; p += kSortThreads) {
-      const int64_t g = (int64_t)s_global[s_dig[p]] + p;
-      keys_out[g] = s_buf[p];
+      const K kk = s_buf[p];
+      const int64_t g = (int64_t)s_global[(unsigned)((kk >> shift) & kMask)] + p;
+      keys_out[g] = kk;
       vals_out[g] = s_vals[p];
     }
     return;

@@ -17,8 +17,10 @@
 namespace ygg {
 
 constexpr int kFlatThreads = 256;
-constexpr int kFlatPer = 4;
-constexpr int kFlatChunk = kFlatThreads * kFlatPer;
+// Pairs per thread of one chunk (the queue tile): a chunk reserves its
+// outputs with one atomic per stream, so larger tiles mean fewer
+// same-address atomics per generation (they serialise across the XCDs);
+// smaller tiles spread small generations over more workgroups.
 
 struct FlatArgs {
   int32_t num_internal;
@@ -40,6 +42,7 @@ struct FlatArgs {
   int32_t* node_hist;   // [T] or null
   const int32_t* roots; // forest seeds (r, r), or null for the single root
   int64_t num_roots;
+  int32_t xcd_groups;   // > 1: chunk ranges per XCD (see k_walk_flat)
 };
 
 __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v) {
@@ -52,14 +55,28 @@ __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v
   return v;
 }
 
-template <bool EMIT>
+template <bool EMIT, int kFlatPer>
 __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
+  constexpr int kFlatChunk = kFlatThreads * kFlatPer;
   if (atomic_load_agent(&a.flags[0]) | atomic_load_agent(&a.flags[1]) |
       atomic_load_agent(&a.flags[2]))
     return;
   const int64_t raw = a.gen_sizes[g];
   const int64_t size = raw < a.queue_cap ? raw : a.queue_cap;
-  if ((int64_t)blockIdx.x * kFlatChunk >= size) return;
+  // Chunks of kFlatChunk pairs. Workgroups are dispatched round-robin over the
+  // XCDs (workgroup b on XCD b % X); with xcd_groups = X the chunks split
+  // into X contiguous ranges, one per XCD, so each XCD's L2 sees one part of
+  // the wavefront instead of every X-th chunk of all of it.
+  const int64_t nchunks = (size + kFlatChunk - 1) / kFlatChunk;
+  int64_t c_first = blockIdx.x, c_end = nchunks, c_step = gridDim.x;
+  if (a.xcd_groups > 1 && gridDim.x % a.xcd_groups == 0) {
+    const int X = a.xcd_groups;
+    const int g = blockIdx.x % X;
+    c_first = nchunks * g / X + blockIdx.x / X;
+    c_end = nchunks * (g + 1) / X;
+    c_step = gridDim.x / X;
+  }
+  if (c_first >= c_end) return;
   const uint64_t* __restrict__ wf_in = (g & 1) ? a.wf1 : a.wf0;
   uint64_t* __restrict__ wf_out = (g & 1) ? a.wf0 : a.wf1;
   const NodeRec* __restrict__ nodes = a.nodes;
@@ -68,8 +85,8 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
   __shared__ unsigned long long s_wave[kFlatThreads / YGG_WAVE];
   __shared__ unsigned long long s_base[3];
 
-  for (int64_t chunk = (int64_t)blockIdx.x * kFlatChunk; chunk < size;
-       chunk += (int64_t)gridDim.x * kFlatChunk) {
+  for (int64_t ci = c_first; ci < c_end; ci += c_step) {
+    const int64_t chunk = ci * kFlatChunk;
     // mode: 0 none, 1 accept, 2 near, 3 both-same, 4 both-cross, 5 split t, 6 split s
     int mode[kFlatPer];
     uint32_t pt[kFlatPer], ps[kFlatPer], x0[kFlatPer], x1[kFlatPer], x2[kFlatPer],
@@ -292,21 +309,28 @@ __global__ void k_far_rows(const int64_t* __restrict__ off, int T,
   if (r == 0) node_offsets[T] = (OutT)off[T];
 }
 
+// Near lists: entries converted by a flat grid-stride loop (rows can hold
+// anything from a few to tens of thousands of entries), rows by one thread
+// each.
 template <typename OutT>
-__global__ __launch_bounds__(256) void k_near_from_sorted(
-    int L, const int64_t* __restrict__ off, const uint32_t* __restrict__ vals,
-    OutT* __restrict__ neighbors, OutT* __restrict__ counts, OutT* __restrict__ offsets_out) {
-  int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE;
-  const unsigned lane = lane_id();
-  if (r >= L) return;
+__global__ __launch_bounds__(256) void k_near_entries(const int64_t* __restrict__ off, int L,
+                                                      const uint32_t* __restrict__ vals,
+                                                      OutT* __restrict__ neighbors) {
+  const int64_t total = off[L];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    neighbors[e] = (OutT)vals[e];
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(256) void k_near_rows(int L, const int64_t* __restrict__ off,
+                                                   OutT* __restrict__ counts,
+                                                   OutT* __restrict__ offsets_out) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r > L) return;
   const int64_t o = off[r];
-  const int64_t c = off[r + 1] - o;
-  if (lane == 0) {
-    counts[r] = (OutT)c;
-    offsets_out[r] = (OutT)o;
-    if (r == L - 1) offsets_out[L] = (OutT)off[L];
-  }
-  for (int64_t e = lane; e < c; e += YGG_WAVE) neighbors[o + e] = (OutT)vals[o + e];
+  offsets_out[r] = (OutT)o;
+  if (r < L) counts[r] = (OutT)(off[r + 1] - o);
 }
 
 // Dense [L][maxc] neighbour leaf rows, -1 padded: one thread per element,
@@ -400,6 +424,25 @@ using namespace ygg;
 
 extern "C" {
 
+static void launch_flat(const FlatArgs& a, int k, bool emit, int per, int blocks,
+                        hipStream_t st) {
+  if (emit) {
+    if (per >= 16)
+      hipLaunchKernelGGL((k_walk_flat<true, 16>), dim3(blocks), dim3(256), 0, st, a, k);
+    else if (per >= 8)
+      hipLaunchKernelGGL((k_walk_flat<true, 8>), dim3(blocks), dim3(256), 0, st, a, k);
+    else
+      hipLaunchKernelGGL((k_walk_flat<true, 4>), dim3(blocks), dim3(256), 0, st, a, k);
+  } else {
+    if (per >= 16)
+      hipLaunchKernelGGL((k_walk_flat<false, 16>), dim3(blocks), dim3(256), 0, st, a, k);
+    else if (per >= 8)
+      hipLaunchKernelGGL((k_walk_flat<false, 8>), dim3(blocks), dim3(256), 0, st, a, k);
+    else
+      hipLaunchKernelGGL((k_walk_flat<false, 4>), dim3(blocks), dim3(256), 0, st, a, k);
+  }
+}
+
 static hipEvent_t g_flat_events[2 * 1025];
 static int g_flat_events_ready = 0;
 
@@ -437,6 +480,7 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
   a.node_hist = p->node_hist;
   a.roots = p->roots;
   a.num_roots = p->roots ? p->num_roots : 1;
+  a.xcd_groups = p->xcd_groups;
   if (p->roots && (p->num_roots < 1 || p->num_roots > p->queue_cap)) return YGG_ERR_ARG;
   const int G = p->max_generations;
   const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
@@ -449,10 +493,7 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
     int end = g + batch < G ? g + batch : G;
     for (int k = g; k < end; ++k) {
       if (gen_ms) YGG_CHECK(hipEventRecord(g_flat_events[2 * k], st));
-      if (p->emit)
-        hipLaunchKernelGGL(k_walk_flat<true>, dim3(blocks), dim3(256), 0, st, a, k);
-      else
-        hipLaunchKernelGGL(k_walk_flat<false>, dim3(blocks), dim3(256), 0, st, a, k);
+      launch_flat(a, k, p->emit != 0, p->pairs_per_thread, blocks, st);
       if (gen_ms) YGG_CHECK(hipEventRecord(g_flat_events[2 * k + 1], st));
     }
     YGG_LAUNCH_CHECK();
@@ -517,17 +558,13 @@ int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
   a.node_hist = p->node_hist;
   a.roots = p->roots;
   a.num_roots = p->roots ? p->num_roots : 1;
+  a.xcd_groups = p->xcd_groups;
   const int G = p->max_generations;
   const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
   hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
                      G);
   const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
-  for (int k = 0; k < G; ++k) {
-    if (p->emit)
-      hipLaunchKernelGGL(k_walk_flat<true>, dim3(blocks), dim3(256), 0, st, a, k);
-    else
-      hipLaunchKernelGGL(k_walk_flat<false>, dim3(blocks), dim3(256), 0, st, a, k);
-  }
+  for (int k = 0; k < G; ++k) launch_flat(a, k, p->emit != 0, p->pairs_per_thread, blocks, st);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
@@ -577,17 +614,22 @@ int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
   if (num_leaves < 1) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   int L = (int)num_leaves;
-  dim3 grid(grid_for((int64_t)L * YGG_WAVE, 256));
+  dim3 grid(grid_for(max_count > 0 ? (int64_t)L * max_count : (int64_t)L * 256, 256, 16384));
+  dim3 gr(grid_for((int64_t)L + 1, 256));
   dim3 gd(grid_for(L * max_count > 0 ? (int64_t)L * max_count : 1, 256, 32768));
   if (out_i64) {
-    hipLaunchKernelGGL(k_near_from_sorted<int64_t>, grid, dim3(256), 0, st, L, row_offsets, vals,
-                       (int64_t*)neighbors, (int64_t*)counts, (int64_t*)offsets);
+    hipLaunchKernelGGL(k_near_entries<int64_t>, grid, dim3(256), 0, st, row_offsets, L, vals,
+                       (int64_t*)neighbors);
+    hipLaunchKernelGGL(k_near_rows<int64_t>, gr, dim3(256), 0, st, L, row_offsets,
+                       (int64_t*)counts, (int64_t*)offsets);
     if (neighbor_leaf_positions && max_count > 0)
       hipLaunchKernelGGL(k_near_dense<int64_t>, gd, dim3(256), 0, st, L, row_offsets, vals,
                          leaf_position, max_count, (int64_t*)neighbor_leaf_positions);
   } else {
-    hipLaunchKernelGGL(k_near_from_sorted<int32_t>, grid, dim3(256), 0, st, L, row_offsets, vals,
-                       (int32_t*)neighbors, (int32_t*)counts, (int32_t*)offsets);
+    hipLaunchKernelGGL(k_near_entries<int32_t>, grid, dim3(256), 0, st, row_offsets, L, vals,
+                       (int32_t*)neighbors);
+    hipLaunchKernelGGL(k_near_rows<int32_t>, gr, dim3(256), 0, st, L, row_offsets,
+                       (int32_t*)counts, (int32_t*)offsets);
     if (neighbor_leaf_positions && max_count > 0)
       hipLaunchKernelGGL(k_near_dense<int32_t>, gd, dim3(256), 0, st, L, row_offsets, vals,
                          leaf_position, max_count, (int32_t*)neighbor_leaf_positions);

@@ -36,6 +36,12 @@ WALK_TIMING = {"enabled": False, "log": []}
 # Number of counting walks run by the flat path (a capacity hint that holds
 # runs none; a hint that overflows falls back to one).
 FLAT_STATS = {"count_walks": 0, "hinted_walks": 0, "hint_overflows": 0}
+# Walk chunks split into this many contiguous ranges, one per XCD (MI355X
+# has 8; 0 or 1 = plain grid-stride order).
+WALK_XCD_GROUPS = int(os.environ.get("YGGDRAX_WALK_XCD_GROUPS", "0"))
+# Queue tile of the flat walk: pairs per thread of one 256-thread chunk
+# (4, 8 or 16); the autotuner may override per size class.
+WALK_PAIRS_PER_THREAD = int(os.environ.get("YGGDRAX_WALK_PAIRS_PER_THREAD", "8"))
 MAC_TYPES = {"bh": 0, "dehnen": 1, "engblom": 2}
 FLAT_GRID_BLOCKS = 2048
 # Forces the attempt-by-attempt walk with per-row slots (the reference's
@@ -254,6 +260,7 @@ class _WalkContext:
         self.timing = WALK_TIMING["enabled"]
         self.gen_hint = int(1.5 * max(1, dt.num_levels)) + 4
         self.grid_blocks = FLAT_GRID_BLOCKS
+        self.pairs_per_thread = WALK_PAIRS_PER_THREAD
         self.kernel_log = WALK_TIMING["log"]
 
     def run_flat(self, *, emit: bool, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
@@ -285,6 +292,8 @@ class _WalkContext:
         p.far_keys, p.far_vals = _lib.ptr(fk), _lib.ptr(fv)
         p.near_keys, p.near_vals = _lib.ptr(nk), _lib.ptr(nv)
         p.hist_gen, p.node_hist = int(hist_gen), _lib.ptr(node_hist)
+        p.xcd_groups = WALK_XCD_GROUPS
+        p.pairs_per_thread = int(self.pairs_per_thread)
         roots = getattr(self, "roots", None)
         if roots is not None:
             p.roots, p.num_roots = roots.data_ptr(), int(roots.numel())
@@ -591,6 +600,14 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     from . import autotune
     ctx.grid_blocks = autotune.walk_grid_blocks(
         T, lambda b: ctx.run_flat(emit=False, queue_cap=qbig, grid_blocks=b), FLAT_GRID_BLOCKS)
+
+    def with_tile(per):
+        old, ctx.pairs_per_thread = ctx.pairs_per_thread, per
+        try:
+            ctx.run_flat(emit=False, queue_cap=qbig)
+        finally:
+            ctx.pairs_per_thread = old
+    ctx.pairs_per_thread = autotune.walk_tile(T, with_tile, ctx.pairs_per_thread)
     em = None
     if sizes is not None:
         FLAT_STATS["hinted_walks"] += 1

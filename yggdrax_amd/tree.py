@@ -11,6 +11,7 @@ dtypes (indices widened to INDEX_DTYPE, Morton codes as uint64).
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 from typing import Callable, NamedTuple, Optional
 
@@ -20,6 +21,8 @@ from . import _lib
 from .dtypes import INDEX_DTYPE
 
 MAX_TREE_LEVELS = 64
+# Particle reorder by scattering (coalesced reads) instead of gathering.
+REORDER_SCATTER = os.environ.get("YGGDRAX_REORDER_SCATTER", "0") not in ("", "0")
 
 
 class RadixTreeTopology(NamedTuple):
@@ -359,9 +362,10 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
         if masses is not None:
             m = torch.as_tensor(masses, dtype=torch.float32, device=dev).contiguous()
             mass_sorted = torch.empty_like(m)
-        _lib.check(lib.ygg_reorder(pos.data_ptr(), _lib.ptr(m), perm.data_ptr(), n,
-                                   pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(),
-                                   st), "reorder")
+        reorder = lib.ygg_reorder_scatter if REORDER_SCATTER else lib.ygg_reorder
+        _lib.check(reorder(pos.data_ptr(), _lib.ptr(m), perm.data_ptr(), n,
+                           pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(), st),
+                   "reorder")
     # One host read: level count, depth flags and octree level count.
     scal = rad["scalars"].cpu()
     if int(scal[2]) != 0:  # levels beyond the fast walk: exact pointer doubling
