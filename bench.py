@@ -16,8 +16,8 @@ the fill pass). `build_roofline` prices sort + build + geometry at 287
 compulsory bytes per point against HBM peak. The `learned` object compares
 the walk with learned capacities (one emission walk) against the plain
 count-then-fill walk on the same tree and checks the lists are identical.
-The CPU baseline is the oracle (single thread) on a 2^22-point sample from
-the same generator (build, geometry and the counting walk).
+The CPU baseline is the oracle (C++, single thread) on the same 2^24-point
+workload (build, geometry and the counting walk).
 """
 
 from __future__ import annotations
@@ -311,10 +311,11 @@ def batched_leg(torch, steps, warmup):
                 sampled_systems_identical=same)
 
 
-def cpu_baseline(sample_n, leaf, theta):
+def cpu_baseline(sample_n, leaf, theta, pos=None, mass=None):
     import oracle
     from yggdrax_amd import inputs
-    pos, mass = inputs.clustered(sample_n, 3)
+    if pos is None or pos.shape[0] != sample_n:
+        pos, mass = inputs.clustered(sample_n, 3)
     t0 = time.perf_counter()
     topo = oracle.build_octree(pos, mass, leaf_size=leaf)
     geom = oracle.compute_tree_geometry(topo, topo["positions_sorted"])
@@ -324,8 +325,8 @@ def cpu_baseline(sample_n, leaf, theta):
     O.walk(topo, geom, theta, "dehnen", queue_cap=cap, far_cap=1 << 62, near_cap=1 << 62)
     dt = time.perf_counter() - t0
     return dict(value=round(sample_n / dt / 1e6, 4), unit="Mpts/s", cores=1, kind="port",
-                sample=f"oracle build_octree+geometry+count walk on {sample_n} points of the "
-                       f"same clustered generator (seed 3), {dt:.1f} s")
+                sample=f"oracle (C++, one thread) build_octree + geometry + counting walk on "
+                       f"{sample_n} points of the same clustered workload (seed 3), {dt:.1f} s")
 
 
 def main():
@@ -336,7 +337,7 @@ def main():
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--cpu-sample-log2n", type=int, default=22)
+    ap.add_argument("--cpu-sample-log2n", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -400,7 +401,7 @@ def main():
         _log(f"batched {bat}")
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(1 << args.cpu_sample_log2n, args.leaf, args.theta)
+        cpu = cpu_baseline(1 << args.cpu_sample_log2n, args.leaf, args.theta, pos_np, mass_np)
         _log(f"cpu baseline {cpu}")
     value = n / (ms * 1e-3) / 1e6
     out = {
