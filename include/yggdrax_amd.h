@@ -66,6 +66,16 @@ size_t ygg_morton_sort_workspace_bytes(int64_t n);
 int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t* sorted_codes,
                     uint32_t* perm, void* ws, size_t ws_bytes, void* stream);
 
+/* ygg_morton_sort plus the reordered particles in one call: the encode
+ * kernel also packs (x, y, z, m) so the gather reads one 16-byte record per
+ * particle. inverse (int64 if inverse_i64 else int32) may be NULL; masses
+ * may be NULL (mass_sorted is then not written). */
+size_t ygg_morton_sort_reorder_workspace_bytes(int64_t n);
+int ygg_morton_sort_reorder(const float* pos, const float* mass, int64_t n, const float* bounds6,
+                            uint64_t* sorted_codes, uint32_t* perm, float* pos_sorted,
+                            float* mass_sorted, void* inverse, int inverse_i64, void* ws,
+                            size_t ws_bytes, void* stream);
+
 /* Gathers of the Morton-sorted particles (return_reordered of
  * yggdrax/_tree_impl.py:1045-1050): pos_sorted[i] = pos[perm[i]],
  * mass_sorted[i] = mass[perm[i]] (mass may be NULL), inverse[perm[i]] = i. */

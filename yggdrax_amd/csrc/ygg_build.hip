@@ -108,6 +108,58 @@ __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ pos,
   }
 }
 
+// Encode + digit histograms (as k_encode_hist) that also packs (x, y, z, m)
+// into one float4 per particle, so the later reorder gathers one 16-byte
+// record (one cache line) per particle instead of two.
+__global__ __launch_bounds__(256) void k_encode_hist_pack(const float* __restrict__ pos,
+                                                          const float* __restrict__ mass,
+                                                          int64_t n, const float* __restrict__ b6,
+                                                          uint64_t* __restrict__ codes,
+                                                          uint32_t* __restrict__ hist,
+                                                          float4* __restrict__ packed) {
+  __shared__ uint32_t h[8 * 256];
+  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  float lo[3] = {b6[0], b6[1], b6[2]};
+  float w[3] = {b6[3] - b6[0], b6[4] - b6[1], b6[5] - b6[2]};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+    const int64_t i = b0 + threadIdx.x;
+    const bool valid = i < n;
+    uint64_t c = 0;
+    if (valid) {
+      const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+      c = encode_point(x, y, z, lo, w);
+      codes[i] = c;
+      packed[i] = make_float4(x, y, z, mass ? mass[i] : 0.0f);
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+      pass_hist_add<8>(&h[p * 256], (unsigned)((c >> (8 * p)) & 0xFF), valid, p, p == 7 ? 7 : 8);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+template <typename InvT>
+__global__ __launch_bounds__(256) void k_reorder_packed(const float4* __restrict__ packed,
+                                                        const uint32_t* __restrict__ perm,
+                                                        int64_t n, float* __restrict__ pos_sorted,
+                                                        float* __restrict__ mass_sorted,
+                                                        InvT* __restrict__ inv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = perm[i];
+    const float4 r = packed[p];
+    pos_sorted[3 * i] = r.x;
+    pos_sorted[3 * i + 1] = r.y;
+    pos_sorted[3 * i + 2] = r.z;
+    if (mass_sorted) mass_sorted[i] = r.w;
+    if (inv) inv[p] = (InvT)i;
+  }
+}
+
 // Reorder by scattering: the inverse permutation first (coalesced reads of
 // perm), then every particle read in input order and written to its sorted
 // slot, so all reads are coalesced and only writes are scattered.
@@ -524,6 +576,42 @@ int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t*
                      bounds6, codes, hist);
   YGG_LAUNCH_CHECK();
   return sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, 64, sws, rest, st, hist);
+}
+
+size_t ygg_morton_sort_reorder_workspace_bytes(int64_t n) {
+  return align_up(sizeof(float4) * (size_t)n) + ygg_morton_sort_workspace_bytes(n);
+}
+
+int ygg_morton_sort_reorder(const float* pos, const float* mass, int64_t n, const float* bounds6,
+                            uint64_t* sorted_codes, uint32_t* perm, float* pos_sorted,
+                            float* mass_sorted, void* inverse, int inverse_i64, void* ws,
+                            size_t ws_bytes, void* stream) {
+  if (n < 1 || !pos || !pos_sorted) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  Arena a(ws, ws_bytes);
+  float4* packed = a.take<float4>((size_t)n);
+  uint64_t* codes = a.take<uint64_t>((size_t)n);
+  uint32_t* hist = a.take<uint32_t>(8 * 256);
+  size_t rest = ygg::sort_workspace_bytes(n, 8, 64);
+  char* sws = a.take<char>(rest);
+  if (!packed || !codes || !hist || !sws) return YGG_ERR_WORKSPACE;
+  int rc = fill_u32(hist, 0u, 8 * 256, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_encode_hist_pack, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, pos,
+                     mass, n, bounds6, codes, hist, packed);
+  YGG_LAUNCH_CHECK();
+  if ((rc = sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, 64, sws, rest, st, hist)))
+    return rc;
+  if (inverse_i64)
+    hipLaunchKernelGGL(k_reorder_packed<int64_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st,
+                       packed, perm, n, pos_sorted, mass ? mass_sorted : nullptr,
+                       (int64_t*)inverse);
+  else
+    hipLaunchKernelGGL(k_reorder_packed<int32_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st,
+                       packed, perm, n, pos_sorted, mass ? mass_sorted : nullptr,
+                       (int32_t*)inverse);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
 }
 
 int ygg_reorder(const float* pos, const float* mass, const uint32_t* perm, int64_t n,

@@ -329,10 +329,24 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
     b6 = _bounds6(pos, bounds)
     sorted_codes = torch.empty(n, dtype=torch.int64, device=dev)
     perm = torch.empty(n, dtype=torch.int32, device=dev)
-    ws = _lib.workspace(lib.ygg_morton_sort_workspace_bytes(n), dev)
-    _lib.check(lib.ygg_morton_sort(pos.data_ptr(), n, b6.data_ptr(), sorted_codes.data_ptr(),
-                                   perm.data_ptr(), ws.data_ptr(), ws.numel(), st),
-               "morton_sort")
+    pos_sorted = mass_sorted = inv = m = None
+    if masses is not None:
+        m = torch.as_tensor(masses, dtype=torch.float32, device=dev).contiguous()
+    if return_reordered and not REORDER_SCATTER:
+        # sort and gather in one call (packed (x, y, z, m) records)
+        pos_sorted = torch.empty_like(pos)
+        mass_sorted = torch.empty_like(m) if m is not None else None
+        inv = torch.empty(n, dtype=INDEX_DTYPE, device=dev)
+        ws = _lib.workspace(lib.ygg_morton_sort_reorder_workspace_bytes(n), dev)
+        _lib.check(lib.ygg_morton_sort_reorder(
+            pos.data_ptr(), _lib.ptr(m), n, b6.data_ptr(), sorted_codes.data_ptr(),
+            perm.data_ptr(), pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(),
+            int(INDEX_DTYPE == torch.int64), ws.data_ptr(), ws.numel(), st), "morton_sort")
+    else:
+        ws = _lib.workspace(lib.ygg_morton_sort_workspace_bytes(n), dev)
+        _lib.check(lib.ygg_morton_sort(pos.data_ptr(), n, b6.data_ptr(), sorted_codes.data_ptr(),
+                                       perm.data_ptr(), ws.data_ptr(), ws.numel(), st),
+                   "morton_sort")
     L = (n + leaf_size - 1) // leaf_size
     T = 2 * L - 1
     NI = L - 1
@@ -354,15 +368,12 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
         leaf_position = torch.empty(T, dtype=torch.int32, device=dev)
         _lib.check(lib.ygg_radix_leaf_order(L, leaf_indices.data_ptr(), leaf_position.data_ptr(),
                                             st), "leaf_order")
-    pos_sorted = mass_sorted = inv = None
-    if return_reordered:
+    if return_reordered and REORDER_SCATTER:
         pos_sorted = torch.empty_like(pos)
         inv = torch.empty(n, dtype=torch.int32, device=dev)
-        m = None
-        if masses is not None:
-            m = torch.as_tensor(masses, dtype=torch.float32, device=dev).contiguous()
+        if m is not None:
             mass_sorted = torch.empty_like(m)
-        reorder = lib.ygg_reorder_scatter if REORDER_SCATTER else lib.ygg_reorder
+        reorder = lib.ygg_reorder_scatter
         _lib.check(reorder(pos.data_ptr(), _lib.ptr(m), perm.data_ptr(), n,
                            pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(), st),
                    "reorder")
