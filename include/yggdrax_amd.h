@@ -175,6 +175,17 @@ int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int3
 int ygg_radix_leaf_order(int64_t num_leaves, int32_t* leaf_indices, int32_t* leaf_position,
                          void* stream);
 
+/* geometry_to_level_major (yggdrax/_geometry_impl.py:342-433): padded
+ * [num_levels][max_nodes] level-major copies of the node geometry; slot j of
+ * level l holds node nodes_by_level[level_offsets[l] + j] while j is below
+ * the level's count, else zeros and node index -1. */
+int ygg_geometry_level_major(const float* center, const float* half_extent, const float* radius,
+                             const float* max_extent, const int32_t* nodes_by_level,
+                             const int32_t* level_offsets, int32_t num_levels, int64_t max_nodes,
+                             float* out_center, float* out_half_extent, float* out_radius,
+                             float* out_max_extent, void* out_node_indices, int out_i64,
+                             void* stream);
+
 /* ---- dual-tree traversal -------------------------------------------------- */
 
 #define YGG_MAC_BH 0

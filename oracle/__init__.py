@@ -9,6 +9,7 @@ from .oracle import (  # noqa: F401
     build_octree,
     build_tree,
     compute_tree_geometry,
+    geometry_to_level_major,
     infer_bounds,
     lib,
     mac_extents,

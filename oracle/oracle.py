@@ -213,6 +213,30 @@ def compute_tree_geometry(topo, positions_sorted, max_leaf_size=None):
     return dict(center=center, half_extent=half, radius=radius, max_extent=maxext)
 
 
+def geometry_to_level_major(topo, geom):
+    """_geometry_impl.py:342-433: padded [num_levels, max_nodes] level-major
+    copies; slot j of level l is nodes_by_level[level_offsets[l] + j] while
+    j < count(l), else zeros and node index -1."""
+    nl = int(topo["num_levels"])
+    off = np.asarray(topo["level_offsets"], np.int64)[: nl + 1]
+    counts = off[1:] - off[:-1]
+    M = int(np.maximum(counts, 1).max())
+    nbl = np.asarray(topo["nodes_by_level"], np.int64)
+    out = dict(centers=np.zeros((nl, M, 3), np.float32),
+               half_extents=np.zeros((nl, M, 3), np.float32),
+               radii=np.zeros((nl, M), np.float32), max_extents=np.zeros((nl, M), np.float32),
+               level_counts=counts, node_indices=np.full((nl, M), -1, np.int64))
+    for level in range(nl):
+        for j in range(int(counts[level])):
+            node = int(nbl[off[level] + j])
+            out["centers"][level, j] = geom["center"][node]
+            out["half_extents"][level, j] = geom["half_extent"][node]
+            out["radii"][level, j] = geom["radius"][node]
+            out["max_extents"][level, j] = geom["max_extent"][node]
+            out["node_indices"][level, j] = node
+    return out
+
+
 def mac_extents(geom, mac_type, dehnen_radius_scale=1.0):
     """_build_mac_extents (993-1061). Zero-extent propagation is the identity
     here because every half extent is clamped to >= 1e-6."""

@@ -228,3 +228,59 @@ def test_gpu_matches_c1_fixture(ygg):
                                                        traversal_config=cfg)
     parity.assert_interactions_equal(inter, {k[4:]: z[k] for k in z.files if k.startswith("far_")})
     parity.assert_neighbors_equal(nbrs, {k[5:]: z[k] for k in z.files if k.startswith("near_")})
+
+
+def test_compact_far_pairs_match_oracle_and_split_builders(ygg):
+    """build_compact_far_pairs_and_leaf_neighbor_lists (tests/unit/
+    test_interactions_adapter.py:147-188): the compact pairs are the walk's
+    far entries in level-major order with tags -1; shared and split builders
+    give the same pair sets and neighbour lists; per-target sets equal the
+    oracle's."""
+    pos, mass = inputs.uniform_cube(2048, 21)
+    tree, ps, _, _ = ygg.build_tree(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
+                                    leaf_size=8, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    cfg = ygg.DualTreeTraversalConfig(8192, 256, 2048, 2048)
+    far, nbrs = ygg.build_compact_far_pairs_and_leaf_neighbor_lists(
+        tree, geom, theta=0.6, mac_type="dehnen", traversal_config=cfg)
+    split = ygg.build_compact_far_pairs(tree, geom, theta=0.6, mac_type="dehnen",
+                                        traversal_config=cfg)
+    snb = ygg.build_leaf_neighbor_lists(tree, geom, theta=0.6, mac_type="dehnen",
+                                        traversal_config=cfg)
+    n = int(far.far_pair_count)
+    assert far.sources.numel() == n and bool((far.tags == -1).all())
+
+    def pairset(f):
+        return sorted(zip(parity.np_(f.targets).tolist(), parity.np_(f.sources).tolist()))
+    assert pairset(far) == pairset(split)
+    assert torch.equal(nbrs.offsets, snb.offsets) and torch.equal(nbrs.counts, snb.counts)
+    ref = oracle.build_tree(pos, mass, leaf_size=8)
+    rg = oracle.compute_tree_geometry(ref, ref["positions_sorted"])
+    ri, rn, _ = oracle.build_interactions_and_neighbors(ref, rg, theta=0.6, mac_type="dehnen",
+                                                        traversal_config=_OCfg((8192, 256, 2048,
+                                                                                2048)))
+    exp = sorted(zip(np.asarray(ri["targets"]).tolist(), np.asarray(ri["sources"]).tolist()))
+    assert pairset(far) == exp
+    # level-major order: targets' levels never decrease
+    lv = parity.np_(tree.topology.node_level)[parity.np_(far.targets)]
+    assert np.all(np.diff(lv) >= 0)
+    padded, _ = ygg.build_compact_far_pairs_and_leaf_neighbor_lists(
+        tree, geom, theta=0.6, mac_type="dehnen", traversal_config=cfg,
+        compact_far_pair_capacity=n + 5)
+    assert padded.sources.numel() == n + 5 and bool((padded.sources[n:] == -1).all())
+    with pytest.raises(RuntimeError, match="compact far-pair cap exceeded"):
+        ygg.build_compact_far_pairs_and_leaf_neighbor_lists(
+            tree, geom, theta=0.6, mac_type="dehnen", traversal_config=cfg,
+            compact_far_pair_capacity=max(0, n - 1))
+
+
+@pytest.mark.parametrize("name", ["clustered_32k_octree", "getting_started_radix"])
+def test_geometry_to_level_major_matches_oracle(ygg, name):
+    (ref, rgeom, _, _), (tree, _, geom, _, _), _ = _run_both(ygg, name)
+    lm = ygg.geometry_to_level_major(tree, geom)
+    rl = oracle.geometry_to_level_major(ref, rgeom)
+    for f in ("centers", "half_extents", "radii", "max_extents", "level_counts",
+              "node_indices"):
+        got, exp = parity.np_(getattr(lm, f)), np.asarray(rl[f])
+        assert got.shape == exp.shape, f
+        assert np.array_equal(got, exp.astype(got.dtype)), f

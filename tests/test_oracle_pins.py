@@ -518,3 +518,24 @@ def test_oracle_policy_walk_with_mac_policy_equals_walk():
         assert sorted(s for s, _ in far[k]) == sorted(row.tolist())
     for r in range(len(near)):
         assert sorted(near[r]) == sorted(nbrs["neighbors"][nbrs["offsets"][r]:nbrs["offsets"][r + 1]].tolist())
+
+
+def test_level_major_geometry_is_a_per_level_listing():
+    """Every node appears exactly once, in the row of its own level, in
+    nodes_by_level order; padding is zeros with node index -1."""
+    pos, mass = inputs.plummer_sphere(3000, 4, a=0.3)
+    t = oracle.build_octree(pos, mass, leaf_size=8)
+    g = oracle.compute_tree_geometry(t, t["positions_sorted"])
+    lm = oracle.geometry_to_level_major(t, g)
+    lvl = np.asarray(t["node_level"])
+    idx = lm["node_indices"]
+    valid = idx >= 0
+    assert np.array_equal(np.sort(idx[valid]), np.arange(lvl.size))
+    for level in range(idx.shape[0]):
+        row = idx[level][idx[level] >= 0]
+        assert np.all(lvl[row] == level)
+        assert np.array_equal(row, np.asarray(t["nodes_by_level"])[lvl[np.asarray(
+            t["nodes_by_level"])] == level])
+        assert np.array_equal(lm["radii"][level][: row.size], g["radius"][row])
+    assert np.all(lm["centers"][~valid] == 0) and np.all(lm["radii"][~valid] == 0)
+    assert np.array_equal(lm["level_counts"], np.bincount(lvl, minlength=idx.shape[0]))

@@ -8,14 +8,19 @@ torch tensors on the HIP device.
 """
 
 from .dtypes import INDEX_DTYPE, as_index
-from .geometry import TreeGeometry, compute_tree_geometry
+from .geometry import (LevelMajorTreeGeometry, TreeGeometry, compute_tree_geometry,
+                       geometry_to_level_major)
 from .interactions import (
+    CompactTaggedFarPairs,
     DualTreeRetryEvent,
     DualTreeTraversalConfig,
     DualTreeWalkResult,
     NodeInteractionList,
     NodeNeighborList,
+    build_compact_far_pairs,
+    build_compact_far_pairs_and_leaf_neighbor_lists,
     build_interactions_and_neighbors,
+    build_leaf_neighbor_lists,
     interactions_for_node,
     neighbors_for_leaf,
     set_default_dual_tree_config,
@@ -39,8 +44,11 @@ from .tree import (
 
 __all__ = [
     "INDEX_DTYPE", "as_index", "TreeGeometry", "compute_tree_geometry", "DualTreeRetryEvent",
+    "LevelMajorTreeGeometry", "geometry_to_level_major",
     "DualTreeTraversalConfig", "DualTreeWalkResult", "NodeInteractionList", "NodeNeighborList",
     "build_interactions_and_neighbors", "interactions_for_node", "neighbors_for_leaf",
+    "CompactTaggedFarPairs", "build_compact_far_pairs",
+    "build_compact_far_pairs_and_leaf_neighbor_lists", "build_leaf_neighbor_lists",
     "set_default_dual_tree_config", "OctreeTopology", "OctreeTree", "RadixTree",
     "RadixTreeTopology", "Tree", "TreeBuildConfig", "available_tree_types", "build_octree",
     "build_tree", "infer_bounds", "morton_encode", "register_tree_builder",

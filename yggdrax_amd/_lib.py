@@ -115,6 +115,7 @@ _SIGS = {
     "ygg_octree_meta_workspace_bytes": (_SZ, [_I64]),
     "ygg_octree_meta": (_INT, [_P, _P, _I64, ctypes.POINTER(OctreeMetaPtrs), _P, _SZ, _P]),
     "ygg_geometry_workspace_bytes": (_SZ, [_I64]),
+    "ygg_geometry_level_major": (_INT, [_P] * 6 + [_I32, _I64] + [_P] * 5 + [_INT, _P]),
     "ygg_geometry": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I32, _P, _P, _P,
                             _P, _P, _SZ, _P]),
     "ygg_radix_leaf_order": (_INT, [_I64, _P, _P, _P]),

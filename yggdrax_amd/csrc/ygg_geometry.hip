@@ -229,6 +229,45 @@ __global__ __launch_bounds__(256) void k_b_seg_query(Seg sg, const Box8* __restr
   finalize_node(node, acc.mn, acc.mx, center, half, radius, maxext);
 }
 
+// geometry_to_level_major (_geometry_impl.py:342-433): element (l, j) of the
+// padded [num_levels][max_nodes] views holds node nodes_by_level[off[l] + j]
+// for j < count(l), else zeros (node index -1).
+template <typename IdxT>
+__global__ void k_level_major(const float* __restrict__ center, const float* __restrict__ half,
+                              const float* __restrict__ radius, const float* __restrict__ maxext,
+                              const int32_t* __restrict__ nbl, const int32_t* __restrict__ off,
+                              int num_levels, int64_t max_nodes, float* __restrict__ oc,
+                              float* __restrict__ oh, float* __restrict__ orad,
+                              float* __restrict__ omax, IdxT* __restrict__ oidx) {
+  const int64_t total = (int64_t)num_levels * max_nodes;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(e / max_nodes);
+    const int64_t j = e - (int64_t)l * max_nodes;
+    const int64_t cnt = (int64_t)off[l + 1] - off[l];
+    if (j < cnt) {
+      const int32_t node = nbl[off[l] + j];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        oc[3 * e + d] = center[3 * (int64_t)node + d];
+        oh[3 * e + d] = half[3 * (int64_t)node + d];
+      }
+      orad[e] = radius[node];
+      omax[e] = maxext[node];
+      oidx[e] = (IdxT)node;
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        oc[3 * e + d] = 0.0f;
+        oh[3 * e + d] = 0.0f;
+      }
+      orad[e] = 0.0f;
+      omax[e] = 0.0f;
+      oidx[e] = (IdxT)-1;
+    }
+  }
+}
+
 // Level-synchronous upward pass for malformed trees (_geometry_impl.py:302-329):
 // nodes at `level` read their children's current boxes (phase 0 into tmp),
 // then write them (phase 1), exactly as the reference's masked update.
@@ -357,6 +396,30 @@ int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int3
   }
   hipLaunchKernelGGL(k_geom_finalize, dim3(grid_for(T, 256)), dim3(256), 0, st, T, bmin, bmax,
                      center, half_extent, radius, max_extent);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_geometry_level_major(const float* center, const float* half_extent, const float* radius,
+                             const float* max_extent, const int32_t* nodes_by_level,
+                             const int32_t* level_offsets, int32_t num_levels, int64_t max_nodes,
+                             float* out_center, float* out_half_extent, float* out_radius,
+                             float* out_max_extent, void* out_node_indices, int out_i64,
+                             void* stream) {
+  if (num_levels < 1 || max_nodes < 1 || !nodes_by_level || !level_offsets) return YGG_ERR_ARG;
+  const int64_t total = (int64_t)num_levels * max_nodes;
+  dim3 g(grid_for(total, 256, 16384));
+  hipStream_t st = (hipStream_t)stream;
+  if (out_i64)
+    hipLaunchKernelGGL(k_level_major<int64_t>, g, dim3(256), 0, st, center, half_extent, radius,
+                       max_extent, nodes_by_level, level_offsets, num_levels, max_nodes,
+                       out_center, out_half_extent, out_radius, out_max_extent,
+                       (int64_t*)out_node_indices);
+  else
+    hipLaunchKernelGGL(k_level_major<int32_t>, g, dim3(256), 0, st, center, half_extent, radius,
+                       max_extent, nodes_by_level, level_offsets, num_levels, max_nodes,
+                       out_center, out_half_extent, out_radius, out_max_extent,
+                       (int32_t*)out_node_indices);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

@@ -53,4 +53,46 @@ def compute_tree_geometry(tree, positions_sorted, *, max_leaf_size=None,
     return TreeGeometry(center, half, radius, maxext)
 
 
-__all__ = ["TreeGeometry", "compute_tree_geometry"]
+class LevelMajorTreeGeometry(NamedTuple):
+    """Padded level-major geometry views (_geometry_impl.py:342-433)."""
+
+    centers: torch.Tensor        # [num_levels, max_nodes, 3]
+    half_extents: torch.Tensor   # [num_levels, max_nodes, 3]
+    radii: torch.Tensor          # [num_levels, max_nodes]
+    max_extents: torch.Tensor    # [num_levels, max_nodes]
+    level_counts: torch.Tensor   # [num_levels]
+    node_indices: torch.Tensor   # [num_levels, max_nodes], -1 padded
+
+
+def geometry_to_level_major(tree, geometry: TreeGeometry) -> LevelMajorTreeGeometry:
+    """Padded, level-major views of the node geometry
+    (yggdrax/geometry.py:79-102, _geometry_impl.py:342-433), gathered by one
+    kernel."""
+    from .dtypes import INDEX_DTYPE
+    lib = _lib.load()
+    dt = device_topology(tree)
+    num_levels = int(dt.num_levels)
+    if num_levels <= 0:
+        raise ValueError("tree must contain at least one populated level")
+    dev = geometry.center.device
+    off = dt.level_offsets[: num_levels + 1].to(torch.int32).contiguous()
+    counts = (off[1:] - off[:-1]).to(INDEX_DTYPE)
+    max_nodes = int(torch.clamp(counts, min=1).max())
+    f32 = dict(dtype=torch.float32, device=dev)
+    centers = torch.empty((num_levels, max_nodes, 3), **f32)
+    halves = torch.empty((num_levels, max_nodes, 3), **f32)
+    radii = torch.empty((num_levels, max_nodes), **f32)
+    maxes = torch.empty((num_levels, max_nodes), **f32)
+    idx = torch.empty((num_levels, max_nodes), dtype=INDEX_DTYPE, device=dev)
+    g = geometry
+    _lib.check(lib.ygg_geometry_level_major(
+        g.center.contiguous().data_ptr(), g.half_extent.contiguous().data_ptr(),
+        g.radius.contiguous().data_ptr(), g.max_extent.contiguous().data_ptr(),
+        dt.nodes_by_level.data_ptr(), off.data_ptr(), num_levels, max_nodes, centers.data_ptr(),
+        halves.data_ptr(), radii.data_ptr(), maxes.data_ptr(), idx.data_ptr(),
+        int(INDEX_DTYPE == torch.int64), _lib.stream_ptr(dev)), "geometry level major")
+    return LevelMajorTreeGeometry(centers, halves, radii, maxes, counts, idx)
+
+
+__all__ = ["LevelMajorTreeGeometry", "TreeGeometry", "compute_tree_geometry",
+           "geometry_to_level_major"]

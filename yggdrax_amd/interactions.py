@@ -123,6 +123,17 @@ class NodeNeighborList(NamedTuple):
     target_block_size: int
 
 
+class CompactTaggedFarPairs(NamedTuple):
+    """Exact-length far pairs with per-pair tags (_interactions_impl.py:634-644):
+    the walk's far entries in level-major order (the order of
+    NodeInteractionList.sources/targets), tags -1 without a pair policy."""
+
+    sources: torch.Tensor
+    targets: torch.Tensor
+    tags: torch.Tensor
+    far_pair_count: torch.Tensor
+
+
 class DualTreeWalkResult(NamedTuple):
     interaction_offsets: torch.Tensor
     interaction_sources: torch.Tensor
@@ -915,6 +926,21 @@ def build_interactions_and_neighbors(
     (per-leaf neighbour count <= (2^31 - 1) / L, totals <= 2^31 - 1) of the
     count-pass path, which exist for its dense (L x capacity) buffers; the
     CSR lists need neither. Both default to the reference's behaviour."""
+    if return_grouped:
+        raise NotImplementedError("grouped interaction buffers are not part of this build")
+    if return_compact_far_pairs:
+        # the compact pairs come from the result's level-major far arrays
+        inter, nbrs, res = build_interactions_and_neighbors(
+            tree, geometry, theta, max_interactions_per_node, max_neighbors_per_leaf,
+            max_pair_queue, process_block, traversal_config, retry_logger, mac_type,
+            dehnen_radius_scale, pair_policy, policy_state, return_result=True,
+            return_interactions=True, capacity_hint=capacity_hint,
+            neighbor_positions=neighbor_positions, strict_int32=strict_int32)
+        compact = _compact_far_pairs(inter, res)
+        out_inter = inter if return_interactions else None
+        if return_result:
+            return out_inter, nbrs, res, compact
+        return out_inter, nbrs, compact
     if pair_policy is not None:
         from .policy import build_with_pair_policy  # noqa: E402
         return build_with_pair_policy(
@@ -922,10 +948,6 @@ def build_interactions_and_neighbors(
             max_interactions_per_node=max_interactions_per_node,
             max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
             traversal_config=traversal_config, return_result=return_result)
-    if return_grouped:
-        raise NotImplementedError("grouped interaction buffers are not part of this build")
-    if return_compact_far_pairs:
-        raise NotImplementedError("compact far pairs are not part of this build")
     dt = device_topology(tree)
     topo = resolve_tree_topology(tree)
     if _target_block_size() > 0:
@@ -957,6 +979,86 @@ def build_interactions_and_neighbors(
     if return_result:
         return out_inter, nbrs, _to_result(raw, dt, inter, nbrs)
     return out_inter, nbrs
+
+
+def _compact_far_pairs(inter: NodeInteractionList, res) -> CompactTaggedFarPairs:
+    n = int(inter.sources.numel())
+    tags = res.interaction_tags[:n] if res is not None else torch.full(
+        (n,), -1, dtype=INDEX_DTYPE, device=inter.sources.device)
+    return CompactTaggedFarPairs(inter.sources, inter.targets, tags.to(INDEX_DTYPE),
+                                 torch.tensor(n, dtype=INDEX_DTYPE,
+                                              device=inter.sources.device))
+
+
+def build_compact_far_pairs_and_leaf_neighbor_lists(
+        tree, geometry: TreeGeometry, theta: float = 0.5,
+        max_neighbors_per_leaf: int = _DEFAULT_MAX_NEIGHBORS,
+        max_interactions_per_node: Optional[int] = None, mac_type: str = "bh", *,
+        pair_policy=None, policy_state: object = None, max_pair_queue: Optional[int] = None,
+        process_block: Optional[int] = None,
+        traversal_config: Optional[DualTreeTraversalConfig] = None,
+        retry_logger: Optional[Callable[[DualTreeRetryEvent], object]] = None,
+        dehnen_radius_scale: float = 1.0, timing_callback=None,
+        compact_far_pair_capacity: Optional[int] = None, capacity_hint: Optional[dict] = None):
+    """Compact far pairs and leaf neighbour lists from one walk
+    (_interactions_impl.py:5819-5915). With ``compact_far_pair_capacity`` the
+    arrays are padded with -1 to that length and a larger count raises the
+    reference's error (_interactions_impl.py:5346-5372)."""
+    _, nbrs, compact = build_interactions_and_neighbors(
+        tree, geometry, theta, max_interactions_per_node, max_neighbors_per_leaf,
+        max_pair_queue, process_block, traversal_config, retry_logger, mac_type,
+        dehnen_radius_scale, pair_policy, policy_state, return_compact_far_pairs=True,
+        return_interactions=False, capacity_hint=capacity_hint)
+    if compact_far_pair_capacity is not None:
+        cap = int(compact_far_pair_capacity)
+        n = int(compact.far_pair_count)
+        if n > cap:
+            raise RuntimeError("strict fused compact far-pair cap exceeded: "
+                               f"cap={cap}. Increase "
+                               "JACCPOT_STATIC_STRICT_FUSED_COMPACT_FAR_PAIR_CAP.")
+
+        def pad(x):
+            out = torch.full((cap,), -1, dtype=x.dtype, device=x.device)
+            out[:n] = x[:n]
+            return out
+        compact = CompactTaggedFarPairs(pad(compact.sources), pad(compact.targets),
+                                        pad(compact.tags), compact.far_pair_count)
+    return compact, nbrs
+
+
+def build_compact_far_pairs(tree, geometry: TreeGeometry, theta: float = 0.5,
+                            max_interactions_per_node: Optional[int] = None,
+                            mac_type: str = "bh", *, pair_policy=None,
+                            policy_state: object = None, max_pair_queue: Optional[int] = None,
+                            process_block: Optional[int] = None,
+                            traversal_config: Optional[DualTreeTraversalConfig] = None,
+                            retry_logger=None, dehnen_radius_scale: float = 1.0,
+                            capacity_hint: Optional[dict] = None) -> CompactTaggedFarPairs:
+    """Exact-length tagged far pairs (_interactions_impl.py:5765-5816)."""
+    far, _ = build_compact_far_pairs_and_leaf_neighbor_lists(
+        tree, geometry, theta, _DEFAULT_MAX_NEIGHBORS, max_interactions_per_node, mac_type,
+        pair_policy=pair_policy, policy_state=policy_state, max_pair_queue=max_pair_queue,
+        process_block=process_block, traversal_config=traversal_config,
+        retry_logger=retry_logger, dehnen_radius_scale=dehnen_radius_scale,
+        capacity_hint=capacity_hint)
+    return far
+
+
+def build_leaf_neighbor_lists(tree, geometry: TreeGeometry, theta: float = 0.5,
+                              max_neighbors_per_leaf: int = _DEFAULT_MAX_NEIGHBORS,
+                              max_interactions_per_node: Optional[int] = None,
+                              mac_type: str = "bh", *, pair_policy=None,
+                              policy_state: object = None, max_pair_queue: Optional[int] = None,
+                              process_block: Optional[int] = None,
+                              traversal_config: Optional[DualTreeTraversalConfig] = None,
+                              retry_logger=None, dehnen_radius_scale: float = 1.0,
+                              capacity_hint: Optional[dict] = None) -> NodeNeighborList:
+    """Near-field adjacency of the leaves (_interactions_impl.py:5917-5968)."""
+    _, nbrs = build_interactions_and_neighbors(
+        tree, geometry, theta, max_interactions_per_node, max_neighbors_per_leaf,
+        max_pair_queue, process_block, traversal_config, retry_logger, mac_type,
+        dehnen_radius_scale, pair_policy, policy_state, capacity_hint=capacity_hint)
+    return nbrs
 
 
 def interactions_for_node(data: NodeInteractionList, node: int) -> torch.Tensor:
