@@ -186,6 +186,50 @@ int ygg_geometry_level_major(const float* center, const float* half_extent, cons
                              float* out_max_extent, void* out_node_indices, int out_i64,
                              void* stream);
 
+/* ---- downstream far-field layouts ----------------------------------------- */
+
+/* _dense_m2l_buffers (yggdrax/dense_interactions.py:35-140): for every
+ * level-major slot (l, j) and entry k < max_interactions, the k-th far source
+ * of the slot's target node (or -1), its validity, and the target-minus-source
+ * centre displacement (zeros when invalid); out_counts[l][j] is the target's
+ * far count (0 for padding slots). offsets/counts/sources are the level-major
+ * CSR far list in the index dtype (int64 when idx_i64, else int32). Outputs:
+ * out_sources and out_mask [num_levels][max_nodes][max_interactions],
+ * out_displacements [...][3], out_counts [num_levels][max_nodes]. */
+int ygg_dense_interactions(const void* offsets, const void* counts, const void* sources,
+                           int idx_i64, const int32_t* nodes_by_level,
+                           const int32_t* level_offsets, int32_t num_levels, int64_t max_nodes,
+                           int64_t max_interactions, const float* center, void* out_sources,
+                           float* out_displacements, uint8_t* out_mask, void* out_counts,
+                           void* stream);
+
+/* build_grouped_interactions_from_pairs (yggdrax/grouped_interactions.py:
+ * 74-207), split in two calls because the class count sizes the outputs.
+ * ygg_grouped_classify computes each pair's class key (target level, source
+ * level, rint((c_t - c_s) / cell(target level)) per axis, in float64, with
+ * out-of-range values as INT32_MIN as NumPy's cast gives them), sorts the
+ * pairs stably by key into `order` [num_pairs], writes class_scan
+ * [num_pairs + 1] (classes started before each sorted position) and returns
+ * the class count in *num_classes (host memory). It blocks on the stream.
+ * num_pairs must be 1 .. 2^30 - 1. */
+size_t ygg_grouped_workspace_bytes(int64_t num_pairs);
+int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_pairs,
+                         int idx_i64, const int32_t* node_level, const float* center,
+                         const float* bounds_min, const float* bounds_max, uint32_t* order,
+                         int64_t* class_scan, int64_t* num_classes, void* ws, size_t ws_bytes,
+                         void* stream);
+/* ygg_grouped_emit writes the class-major outputs from classify's order and
+ * class_scan: class_sources/class_targets [num_pairs] in sorted order,
+ * class_ids [num_pairs] in input order, class_offsets [num_classes + 1],
+ * class_keys [num_classes][5] and class_displacements [num_classes][3]
+ * (index * cell, float64 rounded to float32). */
+int ygg_grouped_emit(const void* sources, const void* targets, int64_t num_pairs, int idx_i64,
+                     const int32_t* node_level, const float* center, const float* bounds_min,
+                     const float* bounds_max, const uint32_t* order, const int64_t* class_scan,
+                     void* class_sources, void* class_targets, void* class_ids,
+                     void* class_offsets, int32_t* class_keys, float* class_displacements,
+                     void* stream);
+
 /* ---- dual-tree traversal -------------------------------------------------- */
 
 #define YGG_MAC_BH 0

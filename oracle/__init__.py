@@ -9,10 +9,13 @@ from .oracle import (  # noqa: F401
     build_octree,
     build_tree,
     compute_tree_geometry,
+    densify_interactions,
     geometry_to_level_major,
+    grouped_interactions_from_pairs,
     infer_bounds,
     lib,
     mac_extents,
     morton_encode,
+    to_int32_like_numpy_cast,
     walk,
 )

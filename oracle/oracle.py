@@ -4,8 +4,11 @@ Wraps ``libygg_oracle.so`` (a C++ restatement of the reference algorithm)
 and restates, in plain Python/numpy, the reference's host-side pieces:
 topology assembly (yggdrax/tree.py:375-423, 1482-1549), MAC extents
 (yggdrax/_interactions_impl.py:993-1061), leaf ordering (1071-1103), the
-capacity/retry driver (3840-4440) and output shaping (3561-3837).
-Outputs are dicts of numpy arrays whose keys are the reference's NamedTuple
+capacity/retry driver (3840-4440) and output shaping (3561-3837), and
+the downstream far-field layouts: level-major geometry
+(yggdrax/_geometry_impl.py:342-433), dense M2L buffers
+(yggdrax/dense_interactions.py:35-187) and displacement classes
+(yggdrax/grouped_interactions.py:74-207). Outputs are dicts of numpy arrays whose keys are the reference's NamedTuple
 field names; indices are int64 as under the reference's default
 INDEX_DTYPE.
 """
@@ -235,6 +238,94 @@ def geometry_to_level_major(topo, geom):
             out["max_extents"][level, j] = geom["max_extent"][node]
             out["node_indices"][level, j] = node
     return out
+
+
+def densify_interactions(topo, geom, inter):
+    """dense_interactions.py:143-187 with _dense_m2l_buffers (35-140): for
+    each level-major slot holding node t, the first counts[t] entries are
+    t's far sources in CSR row order, mask True, displacement
+    center[t] - center[s] in float32; every other entry is -1 / False / 0.
+    The entry axis has max(1, max count) entries."""
+    lm = geometry_to_level_major(topo, geom)
+    nodes = lm["node_indices"]
+    nl, M = nodes.shape
+    counts = np.asarray(inter["counts"], np.int64)
+    offsets = np.asarray(inter["offsets"], np.int64)
+    sources = np.asarray(inter["sources"], np.int64)
+    K = max(1, int(counts.max()) if counts.size else 0)
+    center = np.asarray(geom["center"], np.float32)
+    out = dict(geometry=lm, m2l_sources=np.full((nl, M, K), -1, np.int64),
+               m2l_mask=np.zeros((nl, M, K), bool), m2l_counts=np.zeros((nl, M), np.int64),
+               m2l_displacements=np.zeros((nl, M, K, 3), np.float32))
+    for level in range(nl):
+        for j in range(M):
+            t = int(nodes[level, j])
+            if t < 0:
+                continue
+            c = int(counts[t])
+            row = sources[offsets[t]:offsets[t] + c]
+            out["m2l_sources"][level, j, :c] = row
+            out["m2l_mask"][level, j, :c] = True
+            out["m2l_counts"][level, j] = c
+            out["m2l_displacements"][level, j, :c] = lm["centers"][level, j] - center[row]
+    return out
+
+
+def _cell_size(extent, level):
+    """grouped_interactions.py:34-38: max(extent / max(2^level, 1), 1e-12)
+    per axis, float64."""
+    denom = np.power(2.0, np.asarray(level, np.float64))[:, None]
+    return np.maximum(np.asarray(extent, np.float64)[None, :] / np.maximum(denom, 1.0), 1e-12)
+
+
+def to_int32_like_numpy_cast(r):
+    """NumPy's float64 -> int32 cast as it behaves on x86-64: integral
+    values inside the int32 range convert, anything else (out of range,
+    NaN) becomes INT32_MIN. Stated explicitly because the C cast underneath
+    is undefined for those values."""
+    r = np.asarray(r, np.float64)
+    ok = (r >= -2147483648.0) & (r <= 2147483647.0)
+    return np.where(ok, r, -2147483648.0).astype(np.int32)
+
+
+def grouped_interactions_from_pairs(topo, geom, sources, targets, level_offsets=None):
+    """build_grouped_interactions_from_pairs (grouped_interactions.py:74-207):
+    class key (target level, source level, rint(delta / cell)) per pair with
+    float64 delta = c_t - c_s and the target level's cell, a stable
+    lexicographic sort of the keys, class starts where the key changes,
+    class displacements index * cell rounded to float32."""
+    src = np.asarray(sources, np.int64).reshape(-1)
+    tgt = np.asarray(targets, np.int64).reshape(-1)
+    nbl = np.asarray(topo["nodes_by_level"], np.int64)
+    lo = (np.zeros(1, np.int64) if level_offsets is None
+          else np.asarray(level_offsets, np.int64))
+    n = src.size
+    if n == 0:
+        return dict(class_keys=np.zeros((0, 5), np.int32),
+                    class_displacements=np.zeros((0, 3), np.float32),
+                    class_offsets=np.zeros(1, np.int64), class_sources=np.zeros(0, np.int64),
+                    class_targets=np.zeros(0, np.int64), class_ids=np.zeros(0, np.int64),
+                    level_offsets=lo, level_nodes=nbl)
+    center = np.asarray(geom["center"], np.float64)
+    level = np.asarray(topo["node_level"], np.int64)
+    extent = (np.asarray(topo["bounds_max"], np.float64).reshape(-1, 3)[0]
+              - np.asarray(topo["bounds_min"], np.float64).reshape(-1, 3)[0])
+    tl, sl = level[tgt], level[src]
+    idx = to_int32_like_numpy_cast(np.rint((center[tgt] - center[src]) / _cell_size(extent, tl)))
+    keys = np.column_stack([tl, sl, idx.astype(np.int64)])
+    order = np.lexsort(tuple(keys[:, k] for k in range(4, -1, -1)))  # stable, column 0 major
+    ks = keys[order]
+    start = np.ones(n, bool)
+    start[1:] = np.any(ks[1:] != ks[:-1], axis=1)
+    starts = np.flatnonzero(start)
+    class_ids = np.empty(n, np.int64)
+    class_ids[order] = np.cumsum(start) - 1
+    ck = ks[starts]
+    disp = (ck[:, 2:5].astype(np.float64) * _cell_size(extent, ck[:, 0])).astype(np.float32)
+    return dict(class_keys=ck.astype(np.int32), class_displacements=disp,
+                class_offsets=np.concatenate([starts, [n]]).astype(np.int64),
+                class_sources=src[order], class_targets=tgt[order], class_ids=class_ids,
+                level_offsets=lo, level_nodes=nbl)
 
 
 def mac_extents(geom, mac_type, dehnen_radius_scale=1.0):

@@ -539,3 +539,126 @@ def test_level_major_geometry_is_a_per_level_listing():
         assert np.array_equal(lm["radii"][level][: row.size], g["radius"][row])
     assert np.all(lm["centers"][~valid] == 0) and np.all(lm["radii"][~valid] == 0)
     assert np.array_equal(lm["level_counts"], np.bincount(lvl, minlength=idx.shape[0]))
+
+
+# ---- downstream layouts: dense level-major buffers and displacement classes ---
+
+def _plummer_lists(n=3000, seed=4, leaf=8, theta=0.6):
+    pos, mass = inputs.plummer_sphere(n, seed, a=0.3)
+    t = oracle.build_octree(pos, mass, leaf_size=leaf)
+    g = oracle.compute_tree_geometry(t, t["positions_sorted"])
+    inter, _, _ = oracle.build_interactions_and_neighbors(t, g, theta=theta, mac_type="dehnen")
+    return t, g, inter
+
+
+def test_dense_buffers_round_trip_to_the_sparse_list():
+    """Reading the dense buffers slot by slot in level-major order gives back
+    the sparse CSR list exactly; every displacement is c_t - c_s within one
+    float32 rounding, with the sign of target minus source."""
+    t, g, inter = _plummer_lists()
+    d = oracle.densify_interactions(t, g, inter)
+    nodes = d["geometry"]["node_indices"]
+    src, mask, cnt, disp = (d["m2l_sources"], d["m2l_mask"], d["m2l_counts"],
+                            d["m2l_displacements"])
+    assert src.shape[2] == max(1, int(inter["counts"].max()))
+    assert np.array_equal(mask, src >= 0)
+    assert np.array_equal(cnt, mask.sum(-1))
+    assert np.all(mask[..., 0] | (cnt == 0))  # valid entries form a prefix
+    rows_t, rows_s = [], []
+    for level in range(nodes.shape[0]):
+        for j in range(nodes.shape[1]):
+            node = nodes[level, j]
+            if node < 0:
+                assert cnt[level, j] == 0
+                continue
+            k = int(cnt[level, j])
+            rows_t.append(np.full(k, node))
+            rows_s.append(src[level, j, :k])
+    assert np.array_equal(np.concatenate(rows_t), inter["targets"])
+    assert np.array_equal(np.concatenate(rows_s), inter["sources"])
+    c = np.asarray(g["center"], np.float64)
+    lv, jj, kk = np.nonzero(mask)
+    exact = c[nodes[lv, jj]] - c[src[lv, jj, kk]]
+    err = np.abs(disp[lv, jj, kk].astype(np.float64) - exact)
+    assert np.all(err <= np.abs(exact) * 2.0 ** -24)
+    assert np.all(disp[~mask] == 0)
+
+
+def test_dense_buffers_getting_started_known_answer():
+    """examples/getting_started.ipynb:33-52: 4 far pairs in total."""
+    _, topo, geom, inter, _ = _getting_started()
+    d = oracle.densify_interactions(topo, geom, inter)
+    assert int(d["m2l_mask"].sum()) == 4 == int(d["m2l_counts"].sum())
+
+
+def test_int32_cast_matches_numpy_astype():
+    """The reference casts rint(delta / cell) with NumPy's astype(int32);
+    the oracle's explicit rule must give what NumPy gives on this host, for
+    small (scalar loop) and large (vector loop) arrays."""
+    vals = np.array([0.0, 2.0, -3.0, 2147483647.0, -2147483648.0, 2147483648.0,
+                     -2147483649.0, 1e20, -1e20, np.inf, -np.inf, np.nan, 4.0e9, -7.0])
+    for arr in (vals, np.tile(vals, 1000)):
+        with np.errstate(invalid="ignore"):
+            expect = arr.astype(np.int32)
+        assert np.array_equal(oracle.to_int32_like_numpy_cast(arr), expect)
+
+
+def test_grouped_known_answer_grid():
+    """Hand-computed classes: bounds [0, 8]^3, so the cell is 8 / 2^level
+    (1 at level 3, 4 at level 1, 2^-37 at level 40). Covers the field order
+    (target level first), the sign (target minus source), round-half-even,
+    the out-of-range index, stability inside a class and the class ids."""
+    INT_MIN = -2 ** 31
+    topo = dict(node_level=np.array([3, 3, 3, 3, 1, 40]),
+                nodes_by_level=np.arange(6), bounds_min=np.zeros(3, np.float32),
+                bounds_max=np.full(3, 8.0, np.float32))
+    geom = dict(center=np.array([[0.5, 0.5, 0.5], [2.5, 0.5, 0.5], [3.0, 3.0, 4.0],
+                                 [-1.0, 3.5, 0.0], [2.0, 2.0, 2.0], [1.5, 0.5, 0.5]],
+                                np.float32))
+    tgt = np.array([1, 2, 0, 4, 5, 0, 1, 2])
+    src = np.array([0, 0, 3, 0, 0, 2, 0, 1])
+    r = oracle.grouped_interactions_from_pairs(topo, geom, src, tgt)
+    assert r["class_keys"].tolist() == [
+        [1, 3, 0, 0, 0], [3, 3, -2, -2, -4], [3, 3, 0, 2, 4], [3, 3, 2, -3, 0],
+        [3, 3, 2, 0, 0], [3, 3, 2, 2, 4], [40, 3, INT_MIN, 0, 0]]
+    assert r["class_offsets"].tolist() == [0, 1, 2, 3, 4, 6, 7, 8]
+    assert r["class_ids"].tolist() == [4, 5, 3, 0, 6, 1, 4, 2]
+    assert r["class_targets"].tolist() == [4, 0, 2, 0, 1, 1, 2, 5]
+    assert r["class_sources"].tolist() == [0, 2, 1, 3, 0, 0, 0, 0]
+    assert r["class_displacements"].tolist() == [
+        [0, 0, 0], [-2, -2, -4], [0, 2, 4], [2, -3, 0], [2, 0, 0], [2, 2, 4],
+        [-2.0 ** -6, 0, 0]]
+    assert r["level_offsets"].tolist() == [0]
+    empty = oracle.grouped_interactions_from_pairs(topo, geom, src[:0], tgt[:0])
+    assert empty["class_keys"].shape == (0, 5) and empty["class_offsets"].tolist() == [0]
+
+
+def test_grouped_classes_reconstruct_the_pairs():
+    """On real far lists: classes are strictly increasing keys; each class's
+    members, taken in input order through class_ids, are exactly its CSR
+    slice (stable order); every member's displacement in cells is within
+    1/2 of its key; class displacements are key * cell in float32."""
+    t, g, inter = _plummer_lists()
+    src, tgt = inter["sources"], inter["targets"]
+    r = oracle.grouped_interactions_from_pairs(t, g, src, tgt, inter["level_offsets"])
+    keys = [tuple(k) for k in r["class_keys"].tolist()]
+    assert all(a < b for a, b in zip(keys, keys[1:]))
+    off = r["class_offsets"]
+    assert off[0] == 0 and off[-1] == src.size and np.all(np.diff(off) > 0)
+    ids = r["class_ids"]
+    lvl = np.asarray(t["node_level"])
+    ext = [float(t["bounds_max"][a]) - float(t["bounds_min"][a]) for a in range(3)]
+    for c in range(len(keys)):
+        members = np.flatnonzero(ids == c)
+        assert np.array_equal(r["class_sources"][off[c]:off[c + 1]], src[members])
+        assert np.array_equal(r["class_targets"][off[c]:off[c + 1]], tgt[members])
+        tl, sl = keys[c][0], keys[c][1]
+        assert np.all(lvl[tgt[members]] == tl) and np.all(lvl[src[members]] == sl)
+        for a in range(3):
+            cell = max(ext[a] / 2.0 ** tl, 1e-12)
+            delta = (g["center"][tgt[members], a].astype(np.float64)
+                     - g["center"][src[members], a].astype(np.float64)) / cell
+            assert np.all(np.abs(delta - keys[c][2 + a]) <= 0.5)
+            want = keys[c][2 + a] * cell
+            got = float(r["class_displacements"][c, a])
+            assert abs(got - want) <= abs(want) * 2.0 ** -24

@@ -1,0 +1,406 @@
+// Downstream layouts of the far-field lists, the formats M2L kernels consume:
+//  * dense level-major buffers (yggdrax/dense_interactions.py:35-140): for
+//    slot (level, j) the target node nodes_by_level[level_offsets[level] + j],
+//    its far sources padded to the longest list, the validity mask, the count
+//    and the target-minus-source centre displacements;
+//  * displacement classes (yggdrax/grouped_interactions.py:74-207): far pairs
+//    keyed by (target level, source level, rint(delta / cell) per axis),
+//    ordered lexicographically by key and, inside a class, by input position,
+//    with CSR class offsets and the class of every input pair.
+#include "ygg_common.h"
+
+namespace ygg {
+namespace {
+
+// One thread per (slot, entry): consecutive threads write consecutive
+// entries of one slot, so every output stream is written coalesced and the
+// slot's CSR row is read once per wavefront.
+template <typename IdxT>
+__global__ void k_dense_m2l(const IdxT* __restrict__ offsets, const IdxT* __restrict__ counts,
+                            const IdxT* __restrict__ sources, const int32_t* __restrict__ nbl,
+                            const int32_t* __restrict__ loff, int num_levels, int64_t max_nodes,
+                            int64_t K, const float* __restrict__ center,
+                            IdxT* __restrict__ out_src, float* __restrict__ out_disp,
+                            uint8_t* __restrict__ out_mask, IdxT* __restrict__ out_counts) {
+  const int64_t total = (int64_t)num_levels * max_nodes * K;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t slot = e / K;
+    const int64_t j = e - slot * K;
+    const int l = (int)(slot / max_nodes);
+    const int64_t s = slot - (int64_t)l * max_nodes;
+    const int64_t lc = (int64_t)loff[l + 1] - loff[l];
+    const int64_t node = s < lc ? (int64_t)nbl[loff[l] + s] : -1;
+    const int64_t cnt = node >= 0 ? (int64_t)counts[node] : 0;
+    const bool valid = j < cnt;
+    const int64_t src = valid ? (int64_t)sources[(int64_t)offsets[node] + j] : -1;
+    out_src[e] = (IdxT)src;
+    out_mask[e] = valid ? 1 : 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      out_disp[3 * e + d] = valid ? center[3 * node + d] - center[3 * src + d] : 0.0f;
+    if (j == 0) out_counts[slot] = (IdxT)cnt;
+  }
+}
+
+// numpy's float64 -> int32 cast on x86-64 (cvttsd2si): values outside the
+// int32 range, and NaN, become INT32_MIN. The argument is already integral.
+__device__ __forceinline__ int32_t to_i32_like_numpy(double r) {
+  return (r >= -2147483648.0 && r <= 2147483647.0) ? (int32_t)r : INT32_MIN;
+}
+
+// Per-axis cell size of a level: max(extent / max(2^level, 1), 1e-12)
+// (grouped_interactions.py:34-38); 2^level and the division by it are exact.
+__device__ __forceinline__ double cell_size(double extent, int level) {
+  return fmax(extent / fmax(ldexp(1.0, level), 1.0), 1e-12);
+}
+
+struct GroupGeom {
+  const int32_t* level;  // node_level [T]
+  const float* center;   // [T, 3]
+  const float* bmin;     // [3]
+  const float* bmax;     // [3]
+};
+
+// Key fields of pair i: target level, source level and the rounded
+// displacement in target-level cells (grouped_interactions.py:139-165).
+// Arithmetic is float64 as in the reference's NumPy code: the centre
+// difference, the IEEE division and the round-half-even are all exact
+// restatements, so the key is bit-identical.
+template <typename IdxT>
+__device__ __forceinline__ void group_fields(const IdxT* __restrict__ src,
+                                             const IdxT* __restrict__ tgt, int64_t i,
+                                             const GroupGeom& g, int32_t f[5]) {
+  const int64_t t = (int64_t)tgt[i], s = (int64_t)src[i];
+  const int tl = g.level[t];
+  f[0] = tl;
+  f[1] = g.level[s];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double ext = (double)g.bmax[a] - (double)g.bmin[a];
+    const double delta = (double)g.center[3 * t + a] - (double)g.center[3 * s + a];
+    f[2 + a] = to_i32_like_numpy(rint(delta / cell_size(ext, tl)));
+  }
+}
+
+__global__ void k_minmax_init(int32_t* mm) {
+  const int i = threadIdx.x;
+  if (i < 5) mm[i] = INT32_MAX;
+  else if (i < 10) mm[i] = INT32_MIN;
+}
+
+// Field-wise min and max over all pairs: wavefront reduction, then one
+// atomic per field and wavefront into mm[0..4] (min) and mm[5..9] (max).
+template <typename IdxT>
+__global__ void k_group_minmax(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt,
+                               int64_t n, GroupGeom g, int32_t* __restrict__ mm) {
+  int32_t lo[5], hi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    lo[k] = INT32_MAX;
+    hi[k] = INT32_MIN;
+  }
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t f[5];
+    group_fields(src, tgt, i, g, f);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      lo[k] = min(lo[k], f[k]);
+      hi[k] = max(hi[k], f[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    for (int o = YGG_WAVE / 2; o > 0; o >>= 1) {
+      lo[k] = min(lo[k], __shfl_xor(lo[k], o));
+      hi[k] = max(hi[k], __shfl_xor(hi[k], o));
+    }
+  }
+  if ((threadIdx.x & (YGG_WAVE - 1)) == 0) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      atomicMin(&mm[k], lo[k]);
+      atomicMax(&mm[5 + k], hi[k]);
+    }
+  }
+}
+
+// Field offsets and widths of the packed key; the fields are concatenated
+// most significant first (target level, source level, dx, dy, dz), each
+// minus its minimum, so unsigned order of the packed value is the
+// lexicographic order of the signed fields.
+struct Pack {
+  int32_t min[5];
+  int32_t width[5];
+};
+
+template <typename IdxT>
+__global__ void k_group_pack(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt,
+                             int64_t n, GroupGeom g, Pack pk, uint64_t* __restrict__ lo_out,
+                             uint64_t* __restrict__ hi_out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t f[5];
+    group_fields(src, tgt, i, g, f);
+    uint64_t lo = 0, hi = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int w = pk.width[k];  // 0..32
+      if (w == 0) continue;
+      const uint64_t u = (uint64_t)((int64_t)f[k] - (int64_t)pk.min[k]);
+      hi = (hi << w) | (lo >> (64 - w));
+      lo = (lo << w) | u;
+    }
+    lo_out[i] = lo;
+    if (hi_out) hi_out[i] = hi;
+  }
+}
+
+__global__ void k_gather_keys(const uint64_t* __restrict__ v, const uint32_t* __restrict__ idx,
+                              int64_t n, uint64_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = v[idx[i]];
+}
+
+__global__ void k_iota_u32(uint32_t* __restrict__ out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)i;
+}
+
+// 1 where sorted position i starts a class (its key differs from i - 1's).
+__global__ void k_group_flags(const uint32_t* __restrict__ order, const uint64_t* __restrict__ lo,
+                              const uint64_t* __restrict__ hi, int64_t n,
+                              int32_t* __restrict__ flags) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t f = 1;
+    if (i > 0) {
+      const uint32_t p = order[i], q = order[i - 1];
+      f = (lo[p] != lo[q]) || (hi && hi[p] != hi[q]);
+    }
+    flags[i] = f;
+  }
+}
+
+// class_scan[i] = classes started before sorted position i, so position i
+// belongs to class class_scan[i + 1] - 1 and starts it iff the two differ.
+template <typename IdxT>
+__global__ void k_group_emit(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt,
+                             int64_t n, GroupGeom g, const uint32_t* __restrict__ order,
+                             const int64_t* __restrict__ scan, IdxT* __restrict__ csrc,
+                             IdxT* __restrict__ ctgt, IdxT* __restrict__ cids,
+                             IdxT* __restrict__ coff, int32_t* __restrict__ ckeys,
+                             float* __restrict__ cdisp) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = order[i];
+    const int64_t c = scan[i + 1] - 1;
+    csrc[i] = src[p];
+    ctgt[i] = tgt[p];
+    cids[p] = (IdxT)c;
+    if (scan[i + 1] != scan[i]) {
+      coff[c] = (IdxT)i;
+      int32_t f[5];
+      group_fields(src, tgt, p, g, f);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) ckeys[5 * c + k] = f[k];
+      // class displacement = index * target-level cell, rounded to float32
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double ext = (double)g.bmax[a] - (double)g.bmin[a];
+        cdisp[3 * c + a] = (float)((double)f[2 + a] * cell_size(ext, f[0]));
+      }
+    }
+    if (i == n - 1) coff[scan[n]] = (IdxT)n;
+  }
+}
+
+inline int bit_width_u32(uint32_t v) { return v == 0 ? 0 : 32 - __builtin_clz(v); }
+
+// The radix sort ranks at most 2^30 - 1 elements per call.
+constexpr int64_t kMaxGroupedPairs = (1ll << 30) - 1;
+
+struct GroupWs {
+  int32_t* mm;
+  uint64_t* lo;
+  uint64_t* hi;
+  uint64_t* hi_sorted_in;
+  uint64_t* keys_tmp;
+  uint32_t* order1;
+  int32_t* flags;
+  void* sort_ws;
+  size_t sort_bytes;
+  void* scan_ws;
+  size_t scan_bytes;
+};
+
+static size_t group_ws_layout(int64_t n, void* ws, size_t ws_bytes, GroupWs* out) {
+  Arena a(ws, ws_bytes);
+  GroupWs w;
+  w.mm = a.take<int32_t>(16);
+  w.lo = a.take<uint64_t>((size_t)n);
+  w.hi = a.take<uint64_t>((size_t)n);
+  w.hi_sorted_in = a.take<uint64_t>((size_t)n);
+  w.keys_tmp = a.take<uint64_t>((size_t)n);
+  w.order1 = a.take<uint32_t>((size_t)n);
+  w.flags = a.take<int32_t>((size_t)n);
+  w.sort_bytes = sort_workspace_bytes(n, 8, 64);
+  w.sort_ws = a.take<char>(w.sort_bytes);
+  w.scan_bytes = scan_workspace_bytes(n);
+  w.scan_ws = a.take<char>(w.scan_bytes);
+  if (out) *out = w;
+  return a.off;
+}
+
+template <typename IdxT>
+int grouped_classify(const IdxT* src, const IdxT* tgt, int64_t n, const GroupGeom& g,
+                     uint32_t* order, int64_t* scan, int64_t* num_classes, void* ws,
+                     size_t ws_bytes, hipStream_t st) {
+  GroupWs w;
+  group_ws_layout(n, ws, ws_bytes, &w);
+  if (!w.mm || !w.lo || !w.hi || !w.hi_sorted_in || !w.keys_tmp || !w.order1 || !w.flags ||
+      !w.sort_ws || !w.scan_ws)
+    return YGG_ERR_WORKSPACE;
+  const int grid = grid_for(n, 256, 4096);
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, w.mm);
+  YGG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_group_minmax<IdxT>, dim3(grid), dim3(256), 0, st, src, tgt, n, g, w.mm);
+  YGG_LAUNCH_CHECK();
+  int32_t mm[10];
+  YGG_CHECK(hipMemcpyAsync(mm, w.mm, sizeof(mm), hipMemcpyDeviceToHost, st));
+  YGG_CHECK(hipStreamSynchronize(st));
+  Pack pk;
+  int W = 0;
+  for (int k = 0; k < 5; ++k) {
+    pk.min[k] = mm[k];
+    pk.width[k] = bit_width_u32((uint32_t)((int64_t)mm[5 + k] - (int64_t)mm[k]));
+    W += pk.width[k];
+  }
+  if (W > 128) return YGG_ERR_UNSUPPORTED;
+  const int pgrid = grid_for(n, 256);
+  uint64_t* hi = W > 64 ? w.hi : nullptr;
+  hipLaunchKernelGGL(k_group_pack<IdxT>, dim3(pgrid), dim3(256), 0, st, src, tgt, n, g, pk, w.lo,
+                     hi);
+  YGG_LAUNCH_CHECK();
+  int rc;
+  if (W == 0) {
+    hipLaunchKernelGGL(k_iota_u32, dim3(pgrid), dim3(256), 0, st, order, n);
+    YGG_LAUNCH_CHECK();
+  } else if (W <= 64) {
+    if ((rc = sort_pairs_u64(w.lo, nullptr, w.keys_tmp, order, n, 0, W, w.sort_ws, w.sort_bytes,
+                             st)))
+      return rc;
+  } else {
+    // stable LSD over two words: low word first, then the high word
+    if ((rc = sort_pairs_u64(w.lo, nullptr, w.keys_tmp, w.order1, n, 0, 64, w.sort_ws,
+                             w.sort_bytes, st)))
+      return rc;
+    hipLaunchKernelGGL(k_gather_keys, dim3(pgrid), dim3(256), 0, st, w.hi, w.order1, n,
+                       w.hi_sorted_in);
+    YGG_LAUNCH_CHECK();
+    if ((rc = sort_pairs_u64(w.hi_sorted_in, w.order1, w.keys_tmp, order, n, 0, W - 64,
+                             w.sort_ws, w.sort_bytes, st)))
+      return rc;
+  }
+  hipLaunchKernelGGL(k_group_flags, dim3(pgrid), dim3(256), 0, st, order, w.lo, hi, n, w.flags);
+  YGG_LAUNCH_CHECK();
+  if ((rc = exclusive_scan_i32_to_i64(w.flags, nullptr, scan, n, w.scan_ws, w.scan_bytes, st)))
+    return rc;
+  int64_t total = 0;
+  YGG_CHECK(hipMemcpyAsync(&total, scan + n, sizeof(total), hipMemcpyDeviceToHost, st));
+  YGG_CHECK(hipStreamSynchronize(st));
+  *num_classes = total;
+  return YGG_OK;
+}
+
+}  // namespace
+}  // namespace ygg
+
+using namespace ygg;
+
+extern "C" {
+
+int ygg_dense_interactions(const void* offsets, const void* counts, const void* sources,
+                           int idx_i64, const int32_t* nodes_by_level,
+                           const int32_t* level_offsets, int32_t num_levels, int64_t max_nodes,
+                           int64_t max_interactions, const float* center, void* out_sources,
+                           float* out_displacements, uint8_t* out_mask, void* out_counts,
+                           void* stream) {
+  if (num_levels < 1 || max_nodes < 1 || max_interactions < 1 || !offsets || !counts ||
+      !sources || !nodes_by_level || !level_offsets || !center || !out_sources ||
+      !out_displacements || !out_mask || !out_counts)
+    return YGG_ERR_ARG;
+  if ((int64_t)num_levels * max_nodes > (1ll << 40) / max_interactions) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t total = (int64_t)num_levels * max_nodes * max_interactions;
+  dim3 g(grid_for(total, 256));
+  if (idx_i64)
+    hipLaunchKernelGGL(k_dense_m2l<int64_t>, g, dim3(256), 0, st, (const int64_t*)offsets,
+                       (const int64_t*)counts, (const int64_t*)sources, nodes_by_level,
+                       level_offsets, num_levels, max_nodes, max_interactions, center,
+                       (int64_t*)out_sources, out_displacements, out_mask,
+                       (int64_t*)out_counts);
+  else
+    hipLaunchKernelGGL(k_dense_m2l<int32_t>, g, dim3(256), 0, st, (const int32_t*)offsets,
+                       (const int32_t*)counts, (const int32_t*)sources, nodes_by_level,
+                       level_offsets, num_levels, max_nodes, max_interactions, center,
+                       (int32_t*)out_sources, out_displacements, out_mask,
+                       (int32_t*)out_counts);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+size_t ygg_grouped_workspace_bytes(int64_t num_pairs) {
+  if (num_pairs < 1) return 0;
+  return group_ws_layout(num_pairs, nullptr, 0, nullptr);
+}
+
+int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_pairs,
+                         int idx_i64, const int32_t* node_level, const float* center,
+                         const float* bounds_min, const float* bounds_max, uint32_t* order,
+                         int64_t* class_scan, int64_t* num_classes, void* ws, size_t ws_bytes,
+                         void* stream) {
+  if (num_pairs < 1 || !sources || !targets || !node_level || !center || !bounds_min ||
+      !bounds_max || !order || !class_scan || !num_classes)
+    return YGG_ERR_ARG;
+  if (num_pairs > kMaxGroupedPairs) return YGG_ERR_UNSUPPORTED;
+  GroupGeom g{node_level, center, bounds_min, bounds_max};
+  hipStream_t st = (hipStream_t)stream;
+  if (idx_i64)
+    return grouped_classify((const int64_t*)sources, (const int64_t*)targets, num_pairs, g,
+                            order, class_scan, num_classes, ws, ws_bytes, st);
+  return grouped_classify((const int32_t*)sources, (const int32_t*)targets, num_pairs, g, order,
+                          class_scan, num_classes, ws, ws_bytes, st);
+}
+
+int ygg_grouped_emit(const void* sources, const void* targets, int64_t num_pairs, int idx_i64,
+                     const int32_t* node_level, const float* center, const float* bounds_min,
+                     const float* bounds_max, const uint32_t* order, const int64_t* class_scan,
+                     void* class_sources, void* class_targets, void* class_ids,
+                     void* class_offsets, int32_t* class_keys, float* class_displacements,
+                     void* stream) {
+  if (num_pairs < 1 || !sources || !targets || !node_level || !center || !bounds_min ||
+      !bounds_max || !order || !class_scan || !class_sources || !class_targets || !class_ids ||
+      !class_offsets || !class_keys || !class_displacements)
+    return YGG_ERR_ARG;
+  GroupGeom g{node_level, center, bounds_min, bounds_max};
+  hipStream_t st = (hipStream_t)stream;
+  dim3 grid(grid_for(num_pairs, 256));
+  if (idx_i64)
+    hipLaunchKernelGGL(k_group_emit<int64_t>, grid, dim3(256), 0, st, (const int64_t*)sources,
+                       (const int64_t*)targets, num_pairs, g, order, class_scan,
+                       (int64_t*)class_sources, (int64_t*)class_targets, (int64_t*)class_ids,
+                       (int64_t*)class_offsets, class_keys, class_displacements);
+  else
+    hipLaunchKernelGGL(k_group_emit<int32_t>, grid, dim3(256), 0, st, (const int32_t*)sources,
+                       (const int32_t*)targets, num_pairs, g, order, class_scan,
+                       (int32_t*)class_sources, (int32_t*)class_targets, (int32_t*)class_ids,
+                       (int32_t*)class_offsets, class_keys, class_displacements);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+}  // extern "C"

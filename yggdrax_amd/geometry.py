@@ -75,6 +75,8 @@ def geometry_to_level_major(tree, geometry: TreeGeometry) -> LevelMajorTreeGeome
     if num_levels <= 0:
         raise ValueError("tree must contain at least one populated level")
     dev = geometry.center.device
+    if tuple(geometry.center.shape) != (dt.num_nodes, 3):
+        raise ValueError(f"geometry.center must have shape ({dt.num_nodes}, 3)")
     off = dt.level_offsets[: num_levels + 1].to(torch.int32).contiguous()
     counts = (off[1:] - off[:-1]).to(INDEX_DTYPE)
     max_nodes = int(torch.clamp(counts, min=1).max())

@@ -925,9 +925,27 @@ def build_interactions_and_neighbors(
     and ``strict_int32=False`` drops the reference's host-allocation guards
     (per-leaf neighbour count <= (2^31 - 1) / L, totals <= 2^31 - 1) of the
     count-pass path, which exist for its dense (L x capacity) buffers; the
-    CSR lists need neither. Both default to the reference's behaviour."""
+    CSR lists need neither. Both default to the reference's behaviour.
+
+    ``return_grouped`` appends the displacement classes of the far pairs
+    (grouped_interactions.build_grouped_interactions_from_pairs over the
+    level-major far list, _interactions_impl.py:6109-6124) as the last
+    element of the returned tuple."""
     if return_grouped:
-        raise NotImplementedError("grouped interaction buffers are not part of this build")
+        if not return_interactions:
+            raise ValueError("grouped interaction return requires return_interactions=True")
+        from .grouped_interactions import build_grouped_interactions_from_pairs
+        out = build_interactions_and_neighbors(
+            tree, geometry, theta, max_interactions_per_node, max_neighbors_per_leaf,
+            max_pair_queue, process_block, traversal_config, retry_logger, mac_type,
+            dehnen_radius_scale, pair_policy, policy_state, return_result=return_result,
+            return_compact_far_pairs=return_compact_far_pairs, return_interactions=True,
+            capacity_hint=capacity_hint, neighbor_positions=neighbor_positions,
+            strict_int32=strict_int32)
+        inter = out[0]
+        grouped = build_grouped_interactions_from_pairs(
+            tree, geometry, inter.sources, inter.targets, level_offsets=inter.level_offsets)
+        return (*out, grouped)
     if return_compact_far_pairs:
         # the compact pairs come from the result's level-major far arrays
         inter, nbrs, res = build_interactions_and_neighbors(

@@ -540,6 +540,22 @@ def resolve_tree_topology(tree):
     return tree if topo is None else topo
 
 
+FMM_CORE_REQUIRED_FIELDS = ("parent", "left_child", "right_child", "node_ranges",
+                            "num_particles", "use_morton_geometry")
+
+
+def require_fmm_core_topology(tree_or_topology) -> None:
+    """ValueError naming the FMM-core fields a topology lacks
+    (yggdrax/tree.py:135-142, 1085-1095, 1138-1145)."""
+    topo = resolve_tree_topology(tree_or_topology)
+    missing = [f for f in FMM_CORE_REQUIRED_FIELDS if not hasattr(topo, f)]
+    if missing:
+        tree_type = getattr(tree_or_topology, "tree_type", None)
+        prefix = f"tree_type='{tree_type}' " if tree_type is not None else ""
+        raise ValueError(f"{prefix}topology is missing FMM-core-required fields: "
+                         + ", ".join(missing))
+
+
 def device_topology(tree) -> DeviceTopology:
     """int32 device view of a tree or a bare topology."""
     dt = getattr(tree, "_device", None)
@@ -578,5 +594,6 @@ __all__ = [
     "MAX_TREE_LEVELS", "OctreeTopology", "OctreeTree", "RadixTree", "RadixTreeTopology",
     "RadixTreeWorkspace", "Tree", "TreeBuildConfig", "TreeBuildRequest", "available_tree_types",
     "build_octree", "build_tree", "device_topology", "infer_bounds", "morton_encode",
-    "register_tree_builder", "resolve_tree_topology", "tree_from_particles",
+    "register_tree_builder", "require_fmm_core_topology", "resolve_tree_topology",
+    "tree_from_particles",
 ]
