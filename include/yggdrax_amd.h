@@ -3,7 +3,9 @@
  * Every entry point takes caller-owned device pointers, plain sizes, a
  * caller-provided device workspace (size from the matching *_workspace_bytes
  * query) and a hipStream_t passed as void*. No entry point allocates device
- * memory or synchronises the stream; all launch asynchronously on `stream`.
+ * memory. All launch asynchronously on `stream`, except the few whose
+ * comment says they block on it because a data-dependent size decides what
+ * runs next (ygg_walk_bfs, ygg_walk_flat, ygg_grouped_classify).
  * Return value: YGG_OK or one of the YGG_ERR_* codes.
  *
  * Device indices are int32 (node ids, particle positions < 2^31); the host
@@ -218,17 +220,18 @@ int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_p
                          const float* bounds_min, const float* bounds_max, uint32_t* order,
                          int64_t* class_scan, int64_t* num_classes, void* ws, size_t ws_bytes,
                          void* stream);
-/* ygg_grouped_emit writes the class-major outputs from classify's order and
- * class_scan: class_sources/class_targets [num_pairs] in sorted order,
- * class_ids [num_pairs] in input order, class_offsets [num_classes + 1],
- * class_keys [num_classes][5] and class_displacements [num_classes][3]
- * (index * cell, float64 rounded to float32). */
-int ygg_grouped_emit(const void* sources, const void* targets, int64_t num_pairs, int idx_i64,
-                     const int32_t* node_level, const float* center, const float* bounds_min,
+/* ygg_grouped_emit writes the class-major outputs from classify's order,
+ * class_scan and what classify left in `ws` (the sorted packed keys and the
+ * node pairs; pass the same, unmodified workspace): class_sources and
+ * class_targets [num_pairs] in sorted order, class_ids [num_pairs] in input
+ * order, class_offsets [num_classes + 1], class_keys [num_classes][5] and
+ * class_displacements [num_classes][3] (index * cell, float64 rounded to
+ * float32). */
+int ygg_grouped_emit(int64_t num_pairs, int idx_i64, const float* bounds_min,
                      const float* bounds_max, const uint32_t* order, const int64_t* class_scan,
                      void* class_sources, void* class_targets, void* class_ids,
                      void* class_offsets, int32_t* class_keys, float* class_displacements,
-                     void* stream);
+                     void* ws, size_t ws_bytes, void* stream);
 
 /* ---- dual-tree traversal -------------------------------------------------- */
 

@@ -83,17 +83,16 @@ __device__ __forceinline__ void group_fields(const IdxT* __restrict__ src,
   }
 }
 
-__global__ void k_minmax_init(int32_t* mm) {
-  const int i = threadIdx.x;
-  if (i < 5) mm[i] = INT32_MAX;
-  else if (i < 10) mm[i] = INT32_MIN;
-}
+// Field-wise min (partials[b][0..4]) and max ([5..9]) of block b's pairs:
+// wavefront shuffles, then one LDS row per wavefront. No atomics, so the
+// reduction costs no cross-XCD traffic.
+constexpr int kMinmaxBlocks = 1024;
 
-// Field-wise min and max over all pairs: wavefront reduction, then one
-// atomic per field and wavefront into mm[0..4] (min) and mm[5..9] (max).
 template <typename IdxT>
-__global__ void k_group_minmax(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt,
-                               int64_t n, GroupGeom g, int32_t* __restrict__ mm) {
+__global__ void __launch_bounds__(256)
+k_group_minmax(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt, int64_t n,
+               GroupGeom g, int32_t* __restrict__ partials) {
+  __shared__ int32_t red[256 / YGG_WAVE][10];
   int32_t lo[5], hi[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
@@ -117,32 +116,89 @@ __global__ void k_group_minmax(const IdxT* __restrict__ src, const IdxT* __restr
       hi[k] = max(hi[k], __shfl_xor(hi[k], o));
     }
   }
+  const int wave = threadIdx.x / YGG_WAVE;
   if ((threadIdx.x & (YGG_WAVE - 1)) == 0) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      atomicMin(&mm[k], lo[k]);
-      atomicMax(&mm[5 + k], hi[k]);
+      red[wave][k] = lo[k];
+      red[wave][5 + k] = hi[k];
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    const int k = threadIdx.x;
+    int32_t v = red[0][k];
+    for (int w = 1; w < 256 / YGG_WAVE; ++w) v = k < 5 ? min(v, red[w][k]) : max(v, red[w][k]);
+    partials[10 * blockIdx.x + k] = v;
   }
 }
 
 // Field offsets and widths of the packed key; the fields are concatenated
 // most significant first (target level, source level, dx, dy, dz), each
 // minus its minimum, so unsigned order of the packed value is the
-// lexicographic order of the signed fields.
+// lexicographic order of the signed fields. total = sum of widths.
 struct Pack {
   int32_t min[5];
   int32_t width[5];
+  int32_t total;
 };
+
+__global__ void __launch_bounds__(256)
+k_group_pack_params(const int32_t* __restrict__ partials, int nblocks, Pack* __restrict__ pk) {
+  __shared__ int32_t red[256 / YGG_WAVE][10];
+  int32_t v[10];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    v[k] = INT32_MAX;
+    v[5 + k] = INT32_MIN;
+  }
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      v[k] = min(v[k], partials[10 * b + k]);
+      v[5 + k] = max(v[5 + k], partials[10 * b + 5 + k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    for (int o = YGG_WAVE / 2; o > 0; o >>= 1) {
+      v[k] = min(v[k], __shfl_xor(v[k], o));
+      v[5 + k] = max(v[5 + k], __shfl_xor(v[5 + k], o));
+    }
+  }
+  if ((threadIdx.x & (YGG_WAVE - 1)) == 0) {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) red[threadIdx.x / YGG_WAVE][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int k = 0; k < 5; ++k) {
+      int32_t lo = red[0][k], hi = red[0][5 + k];
+      for (int w = 1; w < 256 / YGG_WAVE; ++w) {
+        lo = min(lo, red[w][k]);
+        hi = max(hi, red[w][5 + k]);
+      }
+      const uint32_t span = (uint32_t)((int64_t)hi - (int64_t)lo);
+      const int width = span == 0 ? 0 : 32 - __clz((int)span);
+      pk->min[k] = lo;
+      pk->width[k] = width;
+      total += width;
+    }
+    pk->total = total;
+  }
+}
 
 template <typename IdxT>
 __global__ void k_group_pack(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt,
                              int64_t n, GroupGeom g, Pack pk, uint64_t* __restrict__ lo_out,
-                             uint64_t* __restrict__ hi_out) {
+                             uint64_t* __restrict__ hi_out, uint64_t* __restrict__ pairs) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     int32_t f[5];
     group_fields(src, tgt, i, g, f);
+    // node ids are below 2^31: one 8-byte record per pair for the emit gather
+    pairs[i] = ((uint64_t)(uint32_t)tgt[i] << 32) | (uint64_t)(uint32_t)src[i];
     uint64_t lo = 0, hi = 0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -170,47 +226,57 @@ __global__ void k_iota_u32(uint32_t* __restrict__ out, int64_t n) {
     out[i] = (uint32_t)i;
 }
 
-// 1 where sorted position i starts a class (its key differs from i - 1's).
-__global__ void k_group_flags(const uint32_t* __restrict__ order, const uint64_t* __restrict__ lo,
-                              const uint64_t* __restrict__ hi, int64_t n,
-                              int32_t* __restrict__ flags) {
+// 1 where sorted position i starts a class: its packed key (lo_s, hi_s,
+// already in sorted order) differs from position i - 1's.
+__global__ void k_group_flags(const uint64_t* __restrict__ lo_s, const uint64_t* __restrict__ hi_s,
+                              int64_t n, int32_t* __restrict__ flags) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t f = 1;
-    if (i > 0) {
-      const uint32_t p = order[i], q = order[i - 1];
-      f = (lo[p] != lo[q]) || (hi && hi[p] != hi[q]);
-    }
-    flags[i] = f;
-  }
+       i += (int64_t)gridDim.x * blockDim.x)
+    flags[i] = i == 0 || lo_s[i] != lo_s[i - 1] || (hi_s && hi_s[i] != hi_s[i - 1]);
 }
 
 // class_scan[i] = classes started before sorted position i, so position i
 // belongs to class class_scan[i + 1] - 1 and starts it iff the two differ.
+// A class head decodes its key fields from the sorted packed key.
 template <typename IdxT>
-__global__ void k_group_emit(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt,
-                             int64_t n, GroupGeom g, const uint32_t* __restrict__ order,
-                             const int64_t* __restrict__ scan, IdxT* __restrict__ csrc,
-                             IdxT* __restrict__ ctgt, IdxT* __restrict__ cids,
-                             IdxT* __restrict__ coff, int32_t* __restrict__ ckeys,
-                             float* __restrict__ cdisp) {
+__global__ void k_group_emit(const uint64_t* __restrict__ pairs, int64_t n,
+                             const float* __restrict__ bmin,
+                             const float* __restrict__ bmax, const Pack* __restrict__ pkp,
+                             const uint64_t* __restrict__ lo_s, const uint64_t* __restrict__ hi_s,
+                             const uint32_t* __restrict__ order, const int64_t* __restrict__ scan,
+                             IdxT* __restrict__ csrc, IdxT* __restrict__ ctgt,
+                             IdxT* __restrict__ cids, IdxT* __restrict__ coff,
+                             int32_t* __restrict__ ckeys, float* __restrict__ cdisp) {
+  const Pack pk = *pkp;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t p = order[i];
     const int64_t c = scan[i + 1] - 1;
-    csrc[i] = src[p];
-    ctgt[i] = tgt[p];
+    const uint64_t tp = pairs[p];
+    csrc[i] = (IdxT)(uint32_t)tp;
+    ctgt[i] = (IdxT)(uint32_t)(tp >> 32);
     cids[p] = (IdxT)c;
     if (scan[i + 1] != scan[i]) {
       coff[c] = (IdxT)i;
+      uint64_t lo = lo_s[i], hi = pk.total > 64 ? hi_s[i] : 0;
       int32_t f[5];
-      group_fields(src, tgt, p, g, f);
+#pragma unroll
+      for (int k = 4; k >= 0; --k) {
+        const int w = pk.width[k];
+        uint64_t bits = 0;
+        if (w > 0) {
+          bits = lo & ((1ull << w) - 1ull);
+          lo = (lo >> w) | (hi << (64 - w));
+          hi >>= w;
+        }
+        f[k] = (int32_t)((int64_t)pk.min[k] + (int64_t)bits);
+      }
 #pragma unroll
       for (int k = 0; k < 5; ++k) ckeys[5 * c + k] = f[k];
       // class displacement = index * target-level cell, rounded to float32
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const double ext = (double)g.bmax[a] - (double)g.bmin[a];
+        const double ext = (double)bmax[a] - (double)bmin[a];
         cdisp[3 * c + a] = (float)((double)f[2 + a] * cell_size(ext, f[0]));
       }
     }
@@ -218,17 +284,20 @@ __global__ void k_group_emit(const IdxT* __restrict__ src, const IdxT* __restric
   }
 }
 
-inline int bit_width_u32(uint32_t v) { return v == 0 ? 0 : 32 - __builtin_clz(v); }
-
 // The radix sort ranks at most 2^30 - 1 elements per call.
 constexpr int64_t kMaxGroupedPairs = (1ll << 30) - 1;
 
+// Workspace shared by the classify and emit calls. After classify, lo_s
+// holds the packed keys' low words and hi holds the high words, both in
+// sorted order, and pack the packing parameters.
 struct GroupWs {
-  int32_t* mm;
+  int32_t* partials;
+  Pack* pack;
   uint64_t* lo;
   uint64_t* hi;
-  uint64_t* hi_sorted_in;
-  uint64_t* keys_tmp;
+  uint64_t* hi_in;
+  uint64_t* lo_s;
+  uint64_t* pairs;
   uint32_t* order1;
   int32_t* flags;
   void* sort_ws;
@@ -237,22 +306,25 @@ struct GroupWs {
   size_t scan_bytes;
 };
 
-static size_t group_ws_layout(int64_t n, void* ws, size_t ws_bytes, GroupWs* out) {
+static bool group_ws_layout(int64_t n, void* ws, size_t ws_bytes, GroupWs* w, size_t* bytes) {
   Arena a(ws, ws_bytes);
-  GroupWs w;
-  w.mm = a.take<int32_t>(16);
-  w.lo = a.take<uint64_t>((size_t)n);
-  w.hi = a.take<uint64_t>((size_t)n);
-  w.hi_sorted_in = a.take<uint64_t>((size_t)n);
-  w.keys_tmp = a.take<uint64_t>((size_t)n);
-  w.order1 = a.take<uint32_t>((size_t)n);
-  w.flags = a.take<int32_t>((size_t)n);
-  w.sort_bytes = sort_workspace_bytes(n, 8, 64);
-  w.sort_ws = a.take<char>(w.sort_bytes);
-  w.scan_bytes = scan_workspace_bytes(n);
-  w.scan_ws = a.take<char>(w.scan_bytes);
-  if (out) *out = w;
-  return a.off;
+  w->partials = a.take<int32_t>(10 * kMinmaxBlocks);
+  w->pack = a.take<Pack>(1);
+  w->lo = a.take<uint64_t>((size_t)n);
+  w->hi = a.take<uint64_t>((size_t)n);
+  w->hi_in = a.take<uint64_t>((size_t)n);
+  w->lo_s = a.take<uint64_t>((size_t)n);
+  w->pairs = a.take<uint64_t>((size_t)n);
+  w->order1 = a.take<uint32_t>((size_t)n);
+  w->flags = a.take<int32_t>((size_t)n);
+  w->sort_bytes = sort_workspace_bytes(n, 8, 64);
+  w->sort_ws = a.take<char>(w->sort_bytes);
+  w->scan_bytes = scan_workspace_bytes(n);
+  w->scan_ws = a.take<char>(w->scan_bytes);
+  if (bytes) *bytes = a.off;
+  return w->partials && w->pack && w->lo && w->hi && w->hi_in && w->lo_s && w->pairs &&
+         w->order1 &&
+         w->flags && w->sort_ws && w->scan_ws;
 }
 
 template <typename IdxT>
@@ -260,52 +332,48 @@ int grouped_classify(const IdxT* src, const IdxT* tgt, int64_t n, const GroupGeo
                      uint32_t* order, int64_t* scan, int64_t* num_classes, void* ws,
                      size_t ws_bytes, hipStream_t st) {
   GroupWs w;
-  group_ws_layout(n, ws, ws_bytes, &w);
-  if (!w.mm || !w.lo || !w.hi || !w.hi_sorted_in || !w.keys_tmp || !w.order1 || !w.flags ||
-      !w.sort_ws || !w.scan_ws)
-    return YGG_ERR_WORKSPACE;
-  const int grid = grid_for(n, 256, 4096);
-  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, w.mm);
+  if (!group_ws_layout(n, ws, ws_bytes, &w, nullptr)) return YGG_ERR_WORKSPACE;
+  const int mgrid = grid_for(n, 256, kMinmaxBlocks);
+  hipLaunchKernelGGL(k_group_minmax<IdxT>, dim3(mgrid), dim3(256), 0, st, src, tgt, n, g,
+                     w.partials);
   YGG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_group_minmax<IdxT>, dim3(grid), dim3(256), 0, st, src, tgt, n, g, w.mm);
+  hipLaunchKernelGGL(k_group_pack_params, dim3(1), dim3(256), 0, st, w.partials, mgrid, w.pack);
   YGG_LAUNCH_CHECK();
-  int32_t mm[10];
-  YGG_CHECK(hipMemcpyAsync(mm, w.mm, sizeof(mm), hipMemcpyDeviceToHost, st));
-  YGG_CHECK(hipStreamSynchronize(st));
   Pack pk;
-  int W = 0;
-  for (int k = 0; k < 5; ++k) {
-    pk.min[k] = mm[k];
-    pk.width[k] = bit_width_u32((uint32_t)((int64_t)mm[5 + k] - (int64_t)mm[k]));
-    W += pk.width[k];
-  }
+  YGG_CHECK(hipMemcpyAsync(&pk, w.pack, sizeof(Pack), hipMemcpyDeviceToHost, st));
+  YGG_CHECK(hipStreamSynchronize(st));
+  const int W = pk.total;
   if (W > 128) return YGG_ERR_UNSUPPORTED;
   const int pgrid = grid_for(n, 256);
-  uint64_t* hi = W > 64 ? w.hi : nullptr;
   hipLaunchKernelGGL(k_group_pack<IdxT>, dim3(pgrid), dim3(256), 0, st, src, tgt, n, g, pk, w.lo,
-                     hi);
+                     W > 64 ? w.hi : nullptr, w.pairs);
   YGG_LAUNCH_CHECK();
   int rc;
-  if (W == 0) {
+  if (W == 0) {  // one class: identity order, all keys zero
     hipLaunchKernelGGL(k_iota_u32, dim3(pgrid), dim3(256), 0, st, order, n);
     YGG_LAUNCH_CHECK();
+    if ((rc = copy_u32((uint32_t*)w.lo_s, (const uint32_t*)w.lo, 2 * n, st))) return rc;
   } else if (W <= 64) {
-    if ((rc = sort_pairs_u64(w.lo, nullptr, w.keys_tmp, order, n, 0, W, w.sort_ws, w.sort_bytes,
+    if ((rc = sort_pairs_u64(w.lo, nullptr, w.lo_s, order, n, 0, W, w.sort_ws, w.sort_bytes,
                              st)))
       return rc;
   } else {
-    // stable LSD over two words: low word first, then the high word
-    if ((rc = sort_pairs_u64(w.lo, nullptr, w.keys_tmp, w.order1, n, 0, 64, w.sort_ws,
-                             w.sort_bytes, st)))
+    // stable LSD over two words: the low word, then the high word carrying
+    // the first order; then the low words gathered into sorted order
+    if ((rc = sort_pairs_u64(w.lo, nullptr, w.lo_s, w.order1, n, 0, 64, w.sort_ws, w.sort_bytes,
+                             st)))
       return rc;
     hipLaunchKernelGGL(k_gather_keys, dim3(pgrid), dim3(256), 0, st, w.hi, w.order1, n,
-                       w.hi_sorted_in);
+                       w.hi_in);
     YGG_LAUNCH_CHECK();
-    if ((rc = sort_pairs_u64(w.hi_sorted_in, w.order1, w.keys_tmp, order, n, 0, W - 64,
-                             w.sort_ws, w.sort_bytes, st)))
+    if ((rc = sort_pairs_u64(w.hi_in, w.order1, w.hi, order, n, 0, W - 64, w.sort_ws,
+                             w.sort_bytes, st)))
       return rc;
+    hipLaunchKernelGGL(k_gather_keys, dim3(pgrid), dim3(256), 0, st, w.lo, order, n, w.lo_s);
+    YGG_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_group_flags, dim3(pgrid), dim3(256), 0, st, order, w.lo, hi, n, w.flags);
+  hipLaunchKernelGGL(k_group_flags, dim3(pgrid), dim3(256), 0, st, w.lo_s,
+                     W > 64 ? w.hi : nullptr, n, w.flags);
   YGG_LAUNCH_CHECK();
   if ((rc = exclusive_scan_i32_to_i64(w.flags, nullptr, scan, n, w.scan_ws, w.scan_bytes, st)))
     return rc;
@@ -355,7 +423,10 @@ int ygg_dense_interactions(const void* offsets, const void* counts, const void* 
 
 size_t ygg_grouped_workspace_bytes(int64_t num_pairs) {
   if (num_pairs < 1) return 0;
-  return group_ws_layout(num_pairs, nullptr, 0, nullptr);
+  GroupWs w;
+  size_t bytes = 0;
+  group_ws_layout(num_pairs, nullptr, 0, &w, &bytes);
+  return bytes;
 }
 
 int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_pairs,
@@ -376,29 +447,32 @@ int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_p
                           class_scan, num_classes, ws, ws_bytes, st);
 }
 
-int ygg_grouped_emit(const void* sources, const void* targets, int64_t num_pairs, int idx_i64,
-                     const int32_t* node_level, const float* center, const float* bounds_min,
+int ygg_grouped_emit(int64_t num_pairs, int idx_i64, const float* bounds_min,
                      const float* bounds_max, const uint32_t* order, const int64_t* class_scan,
                      void* class_sources, void* class_targets, void* class_ids,
                      void* class_offsets, int32_t* class_keys, float* class_displacements,
-                     void* stream) {
-  if (num_pairs < 1 || !sources || !targets || !node_level || !center || !bounds_min ||
-      !bounds_max || !order || !class_scan || !class_sources || !class_targets || !class_ids ||
-      !class_offsets || !class_keys || !class_displacements)
+                     void* ws, size_t ws_bytes, void* stream) {
+  if (num_pairs < 1 || !bounds_min || !bounds_max || !order ||
+      !class_scan || !class_sources || !class_targets || !class_ids || !class_offsets ||
+      !class_keys || !class_displacements)
     return YGG_ERR_ARG;
-  GroupGeom g{node_level, center, bounds_min, bounds_max};
+  if (num_pairs > kMaxGroupedPairs) return YGG_ERR_UNSUPPORTED;
+  GroupWs w;
+  if (!group_ws_layout(num_pairs, ws, ws_bytes, &w, nullptr)) return YGG_ERR_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   dim3 grid(grid_for(num_pairs, 256));
   if (idx_i64)
-    hipLaunchKernelGGL(k_group_emit<int64_t>, grid, dim3(256), 0, st, (const int64_t*)sources,
-                       (const int64_t*)targets, num_pairs, g, order, class_scan,
-                       (int64_t*)class_sources, (int64_t*)class_targets, (int64_t*)class_ids,
-                       (int64_t*)class_offsets, class_keys, class_displacements);
+    hipLaunchKernelGGL(k_group_emit<int64_t>, grid, dim3(256), 0, st, w.pairs, num_pairs,
+                       bounds_min, bounds_max, w.pack,
+                       w.lo_s, w.hi, order, class_scan, (int64_t*)class_sources,
+                       (int64_t*)class_targets, (int64_t*)class_ids, (int64_t*)class_offsets,
+                       class_keys, class_displacements);
   else
-    hipLaunchKernelGGL(k_group_emit<int32_t>, grid, dim3(256), 0, st, (const int32_t*)sources,
-                       (const int32_t*)targets, num_pairs, g, order, class_scan,
-                       (int32_t*)class_sources, (int32_t*)class_targets, (int32_t*)class_ids,
-                       (int32_t*)class_offsets, class_keys, class_displacements);
+    hipLaunchKernelGGL(k_group_emit<int32_t>, grid, dim3(256), 0, st, w.pairs, num_pairs,
+                       bounds_min, bounds_max, w.pack,
+                       w.lo_s, w.hi, order, class_scan, (int32_t*)class_sources,
+                       (int32_t*)class_targets, (int32_t*)class_ids, (int32_t*)class_offsets,
+                       class_keys, class_displacements);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

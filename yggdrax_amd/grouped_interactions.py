@@ -103,11 +103,10 @@ def build_grouped_interactions_from_pairs(
     class_targets = torch.empty(n, **idx)
     class_ids = torch.empty(n, **idx)
     _lib.check(lib.ygg_grouped_emit(
-        src.data_ptr(), tgt.data_ptr(), n, i64, dt.node_level.data_ptr(), center.data_ptr(),
-        bmin.data_ptr(), bmax.data_ptr(), order.data_ptr(), scan.data_ptr(),
-        class_sources.data_ptr(), class_targets.data_ptr(), class_ids.data_ptr(),
-        class_offsets.data_ptr(), class_keys.data_ptr(), class_disp.data_ptr(), st),
-        "grouped emit")
+        n, i64, bmin.data_ptr(), bmax.data_ptr(),
+        order.data_ptr(), scan.data_ptr(), class_sources.data_ptr(), class_targets.data_ptr(),
+        class_ids.data_ptr(), class_offsets.data_ptr(), class_keys.data_ptr(),
+        class_disp.data_ptr(), ws.data_ptr(), ws.numel(), st), "grouped emit")
     return GroupedInteractionBuffers(class_keys, class_disp.to(geometry.center.dtype),
                                      class_offsets, class_sources, class_targets, class_ids,
                                      lo, level_nodes)
