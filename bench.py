@@ -16,8 +16,10 @@ the fill pass). `build_roofline` prices sort + build + geometry at 287
 compulsory bytes per point against HBM peak. The `learned` object compares
 the walk with learned capacities (one emission walk) against the plain
 count-then-fill walk on the same tree and checks the lists are identical.
-The CPU baseline is the oracle (C++, single thread) on the same 2^24-point
-workload (build, geometry and the counting walk).
+The `layouts` object times the downstream far-field layouts (level-major
+geometry, dense M2L buffers, displacement classes) on the 2^20-point
+Plummer workload. The CPU baseline is the oracle (C++, single thread) on
+the same 2^24-point workload (build, geometry and the counting walk).
 """
 
 from __future__ import annotations
@@ -311,6 +313,83 @@ def batched_leg(torch, steps, warmup):
                 sampled_systems_identical=same)
 
 
+def layouts_leg(ygg, torch, reps=3):
+    """Downstream far-field layouts on the 2^20-point Plummer workload
+    (build_tree leaf 32, dehnen theta 0.5): level-major geometry, dense M2L
+    buffers and displacement classes, each timed over `reps` calls on lists
+    already in HBM. The dense kernel is write-bound: its algorithmic bytes
+    are 21 B per (slot, entry) (8 B source, 12 B displacement, 1 B mask),
+    8 B per slot count and 8 B per far source read. Returns the figures and
+    the inputs the CPU baseline of the grouping reuses."""
+    from yggdrax_amd import dense_interactions as di, grouped_interactions as gi, inputs
+    pos_np, mass_np = inputs.plummer_sphere(1 << 20, 2)
+    tree, ps, _, _ = ygg.build_tree(torch.from_numpy(pos_np).cuda(),
+                                    torch.from_numpy(mass_np).cuda(), leaf_size=32,
+                                    return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    inter, _ = ygg.build_interactions_and_neighbors(tree, geom, theta=0.5, **WALK_KW)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        out = fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            out = fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps, out
+
+    t_lm, lm = timed(lambda: ygg.geometry_to_level_major(tree, geom))
+    t_d, d = timed(lambda: di.densify_interactions(tree, geom, inter))
+    slots = int(d.m2l_counts.numel())
+    k = int(d.m2l_sources.shape[2])
+    far = int(inter.sources.numel())
+    dense_bytes = slots * k * 21 + slots * 8 + far * 8
+    del d
+    torch.cuda.empty_cache()
+    t_g, g = timed(lambda: gi.build_grouped_interactions(tree, geom, inter))
+    classes = int(g.class_keys.shape[0])
+    del g
+    figures = dict(
+        workload="plummer 2^20 points, build_tree leaf 32, dehnen theta 0.5", far_pairs=far,
+        level_major_ms=round(t_lm, 4), level_major_shape=list(lm.node_indices.shape),
+        dense_ms=round(t_d, 3), dense_shape=[*lm.node_indices.shape, k],
+        dense_algorithmic_bytes=dense_bytes, dense_gbs=round(dense_bytes / t_d / 1e6, 1),
+        dense_frac_of_hbm_peak=round(dense_bytes / t_d / 1e6 / HBM_PEAK_GBS, 3),
+        grouped_ms=round(t_g, 3), grouped_classes=classes,
+        grouped_gpairs_s=round(far / t_g / 1e6, 3))
+    return figures, (pos_np, mass_np, tree, geom, inter)
+
+
+def layouts_cpu_baseline(torch, state, cpu_pairs=1 << 21):
+    """The oracle's NumPy grouping (the reference's own host algorithm) timed
+    on the first `cpu_pairs` far pairs of the layouts workload, with a check
+    that the GPU grouping of the same pairs is identical."""
+    import oracle
+    from yggdrax_amd import grouped_interactions as gi
+    pos_np, mass_np, tree, geom, inter = state
+    m = min(cpu_pairs, int(inter.sources.numel()))
+    src = inter.sources[:m].cpu().numpy()
+    tgt = inter.targets[:m].cpu().numpy()
+    ref = oracle.build_tree(pos_np, mass_np, leaf_size=32)
+    rgeom = oracle.compute_tree_geometry(ref, ref["positions_sorted"])
+    t0 = time.perf_counter()
+    exp = oracle.grouped_interactions_from_pairs(ref, rgeom, src, tgt)
+    t_cpu = time.perf_counter() - t0
+    gs = gi.build_grouped_interactions_from_pairs(tree, geom, inter.sources[:m],
+                                                  inter.targets[:m])
+    same = True
+    for f in gs._fields:
+        got = getattr(gs, f).cpu().numpy()
+        same &= bool(np.array_equal(got, np.asarray(exp[f]).astype(got.dtype)))
+    return dict(value=round(m / t_cpu / 1e6, 4), unit="Mpairs/s", cores=1, kind="port",
+                sample=f"oracle NumPy grouping (the reference's host algorithm) of the first "
+                       f"{m} far pairs, {t_cpu:.2f} s", gpu_identical=same)
+
+
 def cpu_baseline(sample_n, leaf, theta, pos=None, mass=None):
     import oracle
     from yggdrax_amd import inputs
@@ -341,6 +420,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-layouts", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scales the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -399,10 +479,17 @@ def main():
     if not args.no_batched:
         bat = batched_leg(torch, args.steps, args.warmup)
         _log(f"batched {bat}")
+    lay = lay_state = None
+    if not args.no_layouts:
+        lay, lay_state = layouts_leg(ygg, torch)
+        _log(f"layouts {lay}")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(1 << args.cpu_sample_log2n, args.leaf, args.theta, pos_np, mass_np)
         _log(f"cpu baseline {cpu}")
+        if lay is not None:
+            lay["grouped_cpu_baseline"] = layouts_cpu_baseline(torch, lay_state)
+            _log(f"layouts cpu baseline {lay['grouped_cpu_baseline']}")
     value = n / (ms * 1e-3) / 1e6
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpts/s", "n_gpus": 1,
@@ -413,7 +500,7 @@ def main():
                                f"uniform background, seed 3), build_octree leaf {args.leaf}, "
                                f"dehnen theta {args.theta}, count pass + exact fill, int64 outputs",
                    "parallelism": "single GPU"},
-        "stages": stages, "learned": lrn, "batched": bat, "roofline": roof,
+        "stages": stages, "learned": lrn, "batched": bat, "layouts": lay, "roofline": roof,
         "build_roofline": build_roof,
         "cpu_baseline": cpu,
     }

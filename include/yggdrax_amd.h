@@ -194,15 +194,19 @@ int ygg_geometry_level_major(const float* center, const float* half_extent, cons
  * level-major slot (l, j) and entry k < max_interactions, the k-th far source
  * of the slot's target node (or -1), its validity, and the target-minus-source
  * centre displacement (zeros when invalid); out_counts[l][j] is the target's
- * far count (0 for padding slots). offsets/counts/sources are the level-major
- * CSR far list in the index dtype (int64 when idx_i64, else int32). Outputs:
- * out_sources and out_mask [num_levels][max_nodes][max_interactions],
- * out_displacements [...][3], out_counts [num_levels][max_nodes]. */
+ * far count (0 for padding slots). offsets [num_nodes (+1)], counts
+ * [num_nodes] and sources [num_far] are the level-major CSR far list in the
+ * index dtype (int64 when idx_i64, else int32). Outputs: out_sources and
+ * out_mask [num_levels][max_nodes][max_interactions], out_displacements
+ * [...][3], out_counts [num_levels][max_nodes]. A row outside the sources or
+ * a source outside [0, num_nodes) is written as padding and sets
+ * *error_flag (device int32, cleared by the call) to 1. */
 int ygg_dense_interactions(const void* offsets, const void* counts, const void* sources,
-                           int idx_i64, const int32_t* nodes_by_level,
-                           const int32_t* level_offsets, int32_t num_levels, int64_t max_nodes,
-                           int64_t max_interactions, const float* center, void* out_sources,
-                           float* out_displacements, uint8_t* out_mask, void* out_counts,
+                           int idx_i64, int64_t num_nodes, int64_t num_far,
+                           const int32_t* nodes_by_level, const int32_t* level_offsets,
+                           int32_t num_levels, int64_t max_nodes, int64_t max_interactions,
+                           const float* center, void* out_sources, float* out_displacements,
+                           uint8_t* out_mask, void* out_counts, int32_t* error_flag,
                            void* stream);
 
 /* build_grouped_interactions_from_pairs (yggdrax/grouped_interactions.py:
@@ -213,10 +217,12 @@ int ygg_dense_interactions(const void* offsets, const void* counts, const void* 
  * pairs stably by key into `order` [num_pairs], writes class_scan
  * [num_pairs + 1] (classes started before each sorted position) and returns
  * the class count in *num_classes (host memory). It blocks on the stream.
- * num_pairs must be 1 .. 2^30 - 1. */
+ * num_pairs must be 1 .. 2^30 - 1; a pair naming a node outside
+ * [0, num_nodes) makes it return YGG_ERR_ARG before anything reads it. */
 size_t ygg_grouped_workspace_bytes(int64_t num_pairs);
 int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_pairs,
-                         int idx_i64, const int32_t* node_level, const float* center,
+                         int idx_i64, int64_t num_nodes, const int32_t* node_level,
+                         const float* center,
                          const float* bounds_min, const float* bounds_max, uint32_t* order,
                          int64_t* class_scan, int64_t* num_classes, void* ws, size_t ws_bytes,
                          void* stream);

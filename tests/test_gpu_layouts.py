@@ -163,3 +163,29 @@ def test_layouts_of_a_single_leaf_tree(ygg):
     assert g.class_ids.numel() == 0
     b = di.build_dense_interactions(tree, geom)
     assert torch.equal(b.m2l_counts, d.m2l_counts)
+
+
+def test_layouts_reject_lists_outside_the_tree(ygg):
+    """A far list naming a node outside the tree, or a CSR row reaching past
+    its sources, is refused with an error (the kernels bounds-check every
+    read and flag it) instead of being densified or grouped."""
+    from yggdrax_amd import dense_interactions as di
+    from yggdrax_amd import grouped_interactions as gi
+    (ref, rgeom, rint, _), (tree, _, geom, inter, _), _ = _run_both(ygg, "getting_started_radix")
+    T = tree.num_nodes
+    bad_src = inter.sources.clone()
+    bad_src[0] = T + 5
+    with pytest.raises(ValueError, match="not a CSR list"):
+        di.densify_interactions(tree, geom, inter._replace(sources=bad_src))
+    short = inter._replace(sources=inter.sources[:-1])
+    with pytest.raises(ValueError, match="not a CSR list"):
+        di.densify_interactions(tree, geom, short)
+    with pytest.raises(IndexError, match="out of range"):
+        gi.build_grouped_interactions_from_pairs(tree, geom, bad_src, inter.targets)
+    neg = inter.targets.clone()
+    neg[-1] = -1
+    with pytest.raises(IndexError, match="out of range"):
+        gi.build_grouped_interactions_from_pairs(tree, geom, inter.sources, neg)
+    # the untouched list still works after the refusals
+    g = gi.build_grouped_interactions(tree, geom, inter)
+    assert int(g.class_offsets[-1]) == inter.sources.numel()

@@ -116,9 +116,10 @@ _SIGS = {
     "ygg_octree_meta": (_INT, [_P, _P, _I64, ctypes.POINTER(OctreeMetaPtrs), _P, _SZ, _P]),
     "ygg_geometry_workspace_bytes": (_SZ, [_I64]),
     "ygg_geometry_level_major": (_INT, [_P] * 6 + [_I32, _I64] + [_P] * 5 + [_INT, _P]),
-    "ygg_dense_interactions": (_INT, [_P] * 3 + [_INT] + [_P] * 2 + [_I32, _I64, _I64] + [_P] * 6),
+    "ygg_dense_interactions": (_INT, [_P] * 3 + [_INT, _I64, _I64] + [_P] * 2 +
+                               [_I32, _I64, _I64] + [_P] * 7),
     "ygg_grouped_workspace_bytes": (_SZ, [_I64]),
-    "ygg_grouped_classify": (_INT, [_P, _P, _I64, _INT] + [_P] * 7 + [_P, _SZ, _P]),
+    "ygg_grouped_classify": (_INT, [_P, _P, _I64, _INT, _I64] + [_P] * 7 + [_P, _SZ, _P]),
     "ygg_grouped_emit": (_INT, [_I64, _INT] + [_P] * 10 + [_P, _SZ, _P]),
     "ygg_geometry": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I32, _P, _P, _P,
                             _P, _P, _SZ, _P]),

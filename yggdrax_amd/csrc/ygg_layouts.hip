@@ -14,14 +14,17 @@ namespace {
 
 // One thread per (slot, entry): consecutive threads write consecutive
 // entries of one slot, so every output stream is written coalesced and the
-// slot's CSR row is read once per wavefront.
+// slot's CSR row is read once per wavefront. A CSR row reaching outside the
+// num_far sources, or a source outside [0, num_nodes), is written as
+// padding and raises *err (checked by the caller after the launch).
 template <typename IdxT>
 __global__ void k_dense_m2l(const IdxT* __restrict__ offsets, const IdxT* __restrict__ counts,
-                            const IdxT* __restrict__ sources, const int32_t* __restrict__ nbl,
-                            const int32_t* __restrict__ loff, int num_levels, int64_t max_nodes,
-                            int64_t K, const float* __restrict__ center,
-                            IdxT* __restrict__ out_src, float* __restrict__ out_disp,
-                            uint8_t* __restrict__ out_mask, IdxT* __restrict__ out_counts) {
+                            const IdxT* __restrict__ sources, int64_t num_nodes, int64_t num_far,
+                            const int32_t* __restrict__ nbl, const int32_t* __restrict__ loff,
+                            int num_levels, int64_t max_nodes, int64_t K,
+                            const float* __restrict__ center, IdxT* __restrict__ out_src,
+                            float* __restrict__ out_disp, uint8_t* __restrict__ out_mask,
+                            IdxT* __restrict__ out_counts, int32_t* __restrict__ err) {
   const int64_t total = (int64_t)num_levels * max_nodes * K;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -31,9 +34,25 @@ __global__ void k_dense_m2l(const IdxT* __restrict__ offsets, const IdxT* __rest
     const int64_t s = slot - (int64_t)l * max_nodes;
     const int64_t lc = (int64_t)loff[l + 1] - loff[l];
     const int64_t node = s < lc ? (int64_t)nbl[loff[l] + s] : -1;
-    const int64_t cnt = node >= 0 ? (int64_t)counts[node] : 0;
-    const bool valid = j < cnt;
-    const int64_t src = valid ? (int64_t)sources[(int64_t)offsets[node] + j] : -1;
+    int64_t cnt = node >= 0 ? (int64_t)counts[node] : 0;
+    bool bad = cnt < 0;
+    if (bad) cnt = 0;
+    bool valid = j < cnt;
+    int64_t src = -1;
+    if (valid) {
+      const int64_t at = (int64_t)offsets[node] + j;
+      if (at < 0 || at >= num_far) {
+        bad = true;
+      } else {
+        src = (int64_t)sources[at];
+        if (src < 0 || src >= num_nodes) bad = true;
+      }
+      if (bad) {
+        valid = false;
+        src = -1;
+      }
+    }
+    if (bad) *err = 1;
     out_src[e] = (IdxT)src;
     out_mask[e] = valid ? 1 : 0;
 #pragma unroll
@@ -56,6 +75,7 @@ __device__ __forceinline__ double cell_size(double extent, int level) {
 }
 
 struct GroupGeom {
+  int64_t num_nodes;     // T
   const int32_t* level;  // node_level [T]
   const float* center;   // [T, 3]
   const float* bmin;     // [3]
@@ -88,10 +108,12 @@ __device__ __forceinline__ void group_fields(const IdxT* __restrict__ src,
 // reduction costs no cross-XCD traffic.
 constexpr int kMinmaxBlocks = 1024;
 
+// Pairs naming a node outside [0, T) are skipped and raise *err; the
+// classify call then fails before any kernel reads through them.
 template <typename IdxT>
 __global__ void __launch_bounds__(256)
 k_group_minmax(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt, int64_t n,
-               GroupGeom g, int32_t* __restrict__ partials) {
+               GroupGeom g, int32_t* __restrict__ partials, int32_t* __restrict__ err) {
   __shared__ int32_t red[256 / YGG_WAVE][10];
   int32_t lo[5], hi[5];
 #pragma unroll
@@ -101,6 +123,11 @@ k_group_minmax(const IdxT* __restrict__ src, const IdxT* __restrict__ tgt, int64
   }
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = (int64_t)tgt[i], s = (int64_t)src[i];
+    if (t < 0 || t >= g.num_nodes || s < 0 || s >= g.num_nodes) {
+      *err = 1;
+      continue;
+    }
     int32_t f[5];
     group_fields(src, tgt, i, g, f);
 #pragma unroll
@@ -141,10 +168,12 @@ struct Pack {
   int32_t min[5];
   int32_t width[5];
   int32_t total;
+  int32_t bad;  // a pair named a node outside [0, T)
 };
 
 __global__ void __launch_bounds__(256)
-k_group_pack_params(const int32_t* __restrict__ partials, int nblocks, Pack* __restrict__ pk) {
+k_group_pack_params(const int32_t* __restrict__ partials, int nblocks,
+                    const int32_t* __restrict__ err, Pack* __restrict__ pk) {
   __shared__ int32_t red[256 / YGG_WAVE][10];
   int32_t v[10];
 #pragma unroll
@@ -186,6 +215,7 @@ k_group_pack_params(const int32_t* __restrict__ partials, int nblocks, Pack* __r
       total += width;
     }
     pk->total = total;
+    pk->bad = *err;
   }
 }
 
@@ -292,6 +322,7 @@ constexpr int64_t kMaxGroupedPairs = (1ll << 30) - 1;
 // sorted order, and pack the packing parameters.
 struct GroupWs {
   int32_t* partials;
+  int32_t* err;
   Pack* pack;
   uint64_t* lo;
   uint64_t* hi;
@@ -309,6 +340,7 @@ struct GroupWs {
 static bool group_ws_layout(int64_t n, void* ws, size_t ws_bytes, GroupWs* w, size_t* bytes) {
   Arena a(ws, ws_bytes);
   w->partials = a.take<int32_t>(10 * kMinmaxBlocks);
+  w->err = a.take<int32_t>(1);
   w->pack = a.take<Pack>(1);
   w->lo = a.take<uint64_t>((size_t)n);
   w->hi = a.take<uint64_t>((size_t)n);
@@ -322,7 +354,7 @@ static bool group_ws_layout(int64_t n, void* ws, size_t ws_bytes, GroupWs* w, si
   w->scan_bytes = scan_workspace_bytes(n);
   w->scan_ws = a.take<char>(w->scan_bytes);
   if (bytes) *bytes = a.off;
-  return w->partials && w->pack && w->lo && w->hi && w->hi_in && w->lo_s && w->pairs &&
+  return w->partials && w->err && w->pack && w->lo && w->hi && w->hi_in && w->lo_s && w->pairs &&
          w->order1 &&
          w->flags && w->sort_ws && w->scan_ws;
 }
@@ -334,21 +366,24 @@ int grouped_classify(const IdxT* src, const IdxT* tgt, int64_t n, const GroupGeo
   GroupWs w;
   if (!group_ws_layout(n, ws, ws_bytes, &w, nullptr)) return YGG_ERR_WORKSPACE;
   const int mgrid = grid_for(n, 256, kMinmaxBlocks);
+  int rc;
+  if ((rc = fill_u32((uint32_t*)w.err, 0u, 1, st))) return rc;
   hipLaunchKernelGGL(k_group_minmax<IdxT>, dim3(mgrid), dim3(256), 0, st, src, tgt, n, g,
-                     w.partials);
+                     w.partials, w.err);
   YGG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_group_pack_params, dim3(1), dim3(256), 0, st, w.partials, mgrid, w.pack);
+  hipLaunchKernelGGL(k_group_pack_params, dim3(1), dim3(256), 0, st, w.partials, mgrid, w.err,
+                     w.pack);
   YGG_LAUNCH_CHECK();
   Pack pk;
   YGG_CHECK(hipMemcpyAsync(&pk, w.pack, sizeof(Pack), hipMemcpyDeviceToHost, st));
   YGG_CHECK(hipStreamSynchronize(st));
+  if (pk.bad) return YGG_ERR_ARG;
   const int W = pk.total;
   if (W > 128) return YGG_ERR_UNSUPPORTED;
   const int pgrid = grid_for(n, 256);
   hipLaunchKernelGGL(k_group_pack<IdxT>, dim3(pgrid), dim3(256), 0, st, src, tgt, n, g, pk, w.lo,
                      W > 64 ? w.hi : nullptr, w.pairs);
   YGG_LAUNCH_CHECK();
-  int rc;
   if (W == 0) {  // one class: identity order, all keys zero
     hipLaunchKernelGGL(k_iota_u32, dim3(pgrid), dim3(256), 0, st, order, n);
     YGG_LAUNCH_CHECK();
@@ -392,31 +427,34 @@ using namespace ygg;
 extern "C" {
 
 int ygg_dense_interactions(const void* offsets, const void* counts, const void* sources,
-                           int idx_i64, const int32_t* nodes_by_level,
-                           const int32_t* level_offsets, int32_t num_levels, int64_t max_nodes,
-                           int64_t max_interactions, const float* center, void* out_sources,
-                           float* out_displacements, uint8_t* out_mask, void* out_counts,
+                           int idx_i64, int64_t num_nodes, int64_t num_far,
+                           const int32_t* nodes_by_level, const int32_t* level_offsets,
+                           int32_t num_levels, int64_t max_nodes, int64_t max_interactions,
+                           const float* center, void* out_sources, float* out_displacements,
+                           uint8_t* out_mask, void* out_counts, int32_t* error_flag,
                            void* stream) {
-  if (num_levels < 1 || max_nodes < 1 || max_interactions < 1 || !offsets || !counts ||
-      !sources || !nodes_by_level || !level_offsets || !center || !out_sources ||
-      !out_displacements || !out_mask || !out_counts)
+  if (num_levels < 1 || max_nodes < 1 || max_interactions < 1 || num_nodes < 1 ||
+      num_far < 0 || !offsets || !counts || !sources || !nodes_by_level || !level_offsets ||
+      !center || !out_sources || !out_displacements || !out_mask || !out_counts || !error_flag)
     return YGG_ERR_ARG;
   if ((int64_t)num_levels * max_nodes > (1ll << 40) / max_interactions) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = fill_u32((uint32_t*)error_flag, 0u, 1, st))) return rc;
   const int64_t total = (int64_t)num_levels * max_nodes * max_interactions;
   dim3 g(grid_for(total, 256));
   if (idx_i64)
     hipLaunchKernelGGL(k_dense_m2l<int64_t>, g, dim3(256), 0, st, (const int64_t*)offsets,
-                       (const int64_t*)counts, (const int64_t*)sources, nodes_by_level,
-                       level_offsets, num_levels, max_nodes, max_interactions, center,
-                       (int64_t*)out_sources, out_displacements, out_mask,
-                       (int64_t*)out_counts);
+                       (const int64_t*)counts, (const int64_t*)sources, num_nodes, num_far,
+                       nodes_by_level, level_offsets, num_levels, max_nodes, max_interactions,
+                       center, (int64_t*)out_sources, out_displacements, out_mask,
+                       (int64_t*)out_counts, error_flag);
   else
     hipLaunchKernelGGL(k_dense_m2l<int32_t>, g, dim3(256), 0, st, (const int32_t*)offsets,
-                       (const int32_t*)counts, (const int32_t*)sources, nodes_by_level,
-                       level_offsets, num_levels, max_nodes, max_interactions, center,
-                       (int32_t*)out_sources, out_displacements, out_mask,
-                       (int32_t*)out_counts);
+                       (const int32_t*)counts, (const int32_t*)sources, num_nodes, num_far,
+                       nodes_by_level, level_offsets, num_levels, max_nodes, max_interactions,
+                       center, (int32_t*)out_sources, out_displacements, out_mask,
+                       (int32_t*)out_counts, error_flag);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
@@ -430,7 +468,8 @@ size_t ygg_grouped_workspace_bytes(int64_t num_pairs) {
 }
 
 int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_pairs,
-                         int idx_i64, const int32_t* node_level, const float* center,
+                         int idx_i64, int64_t num_nodes, const int32_t* node_level,
+                         const float* center,
                          const float* bounds_min, const float* bounds_max, uint32_t* order,
                          int64_t* class_scan, int64_t* num_classes, void* ws, size_t ws_bytes,
                          void* stream) {
@@ -438,7 +477,8 @@ int ygg_grouped_classify(const void* sources, const void* targets, int64_t num_p
       !bounds_max || !order || !class_scan || !num_classes)
     return YGG_ERR_ARG;
   if (num_pairs > kMaxGroupedPairs) return YGG_ERR_UNSUPPORTED;
-  GroupGeom g{node_level, center, bounds_min, bounds_max};
+  if (num_nodes < 1) return YGG_ERR_ARG;
+  GroupGeom g{num_nodes, node_level, center, bounds_min, bounds_max};
   hipStream_t st = (hipStream_t)stream;
   if (idx_i64)
     return grouped_classify((const int64_t*)sources, (const int64_t*)targets, num_pairs, g,

@@ -49,27 +49,26 @@ def densify_interactions(tree, geometry: TreeGeometry,
     T = dt.num_nodes
     if counts.numel() != T or offsets.numel() < T:
         raise ValueError(f"interaction counts/offsets do not describe the tree's {T} nodes")
-    if int(counts.sum()) > 0:
-        ends = offsets[:T] + counts
-        if int(counts.min()) < 0 or int(offsets[:T].min()) < 0 or int(
-                ends.max()) > sources.numel():
-            raise ValueError("interaction offsets/counts exceed the sources array")
-        if int(sources.min()) < 0 or int(sources.max()) >= T:
-            raise IndexError(f"interaction source index out of range [0, {T})")
-    if sources.numel() == 0:
+    num_far = int(sources.numel())
+    if num_far == 0:
         sources = torch.zeros(1, dtype=INDEX_DTYPE, device=dev)
     k = max(1, int(counts.max()) if counts.numel() else 0)
     m2l_sources = torch.empty((num_levels, max_nodes, k), dtype=INDEX_DTYPE, device=dev)
     m2l_mask = torch.empty((num_levels, max_nodes, k), dtype=torch.bool, device=dev)
     m2l_counts = torch.empty((num_levels, max_nodes), dtype=INDEX_DTYPE, device=dev)
     m2l_disp = torch.empty((num_levels, max_nodes, k, 3), dtype=torch.float32, device=dev)
+    err = torch.empty(1, dtype=torch.int32, device=dev)
     lib = _lib.load()
     _lib.check(lib.ygg_dense_interactions(
         offsets.data_ptr(), counts.data_ptr(), sources.data_ptr(),
-        int(INDEX_DTYPE == torch.int64), dt.nodes_by_level.data_ptr(),
+        int(INDEX_DTYPE == torch.int64), T, num_far, dt.nodes_by_level.data_ptr(),
         dt.level_offsets.data_ptr(), num_levels, max_nodes, k,
         geometry.center.contiguous().data_ptr(), m2l_sources.data_ptr(), m2l_disp.data_ptr(),
-        m2l_mask.data_ptr(), m2l_counts.data_ptr(), _lib.stream_ptr(dev)), "dense interactions")
+        m2l_mask.data_ptr(), m2l_counts.data_ptr(), err.data_ptr(), _lib.stream_ptr(dev)),
+        "dense interactions")
+    if int(err.item()):
+        raise ValueError("interaction offsets/counts/sources are not a CSR list over the tree's "
+                         f"{T} nodes")
     return DenseInteractionBuffers(level_geometry, m2l_sources, m2l_disp, m2l_mask, m2l_counts,
                                    interactions)
 

@@ -72,9 +72,6 @@ def build_grouped_interactions_from_pairs(
             torch.zeros((0, 3), dtype=geometry.center.dtype, device=dev),
             torch.zeros(1, **idx), torch.zeros(0, **idx), torch.zeros(0, **idx),
             torch.zeros(0, **idx), lo, level_nodes)
-    if (int(torch.minimum(src.min(), tgt.min())) < 0
-            or int(torch.maximum(src.max(), tgt.max())) >= dt.num_nodes):
-        raise IndexError(f"far-pair node index out of range [0, {dt.num_nodes})")
     bmin = torch.as_tensor(tree.bounds_min, device=dev)
     bmax = torch.as_tensor(tree.bounds_max, device=dev)
     if bmin.ndim > 1:
@@ -91,10 +88,13 @@ def build_grouped_interactions_from_pairs(
     scan = torch.empty(n + 1, dtype=torch.int64, device=dev)
     ws = _lib.workspace(lib.ygg_grouped_workspace_bytes(n), dev)
     num_classes = ctypes.c_int64(0)
-    _lib.check(lib.ygg_grouped_classify(
-        src.data_ptr(), tgt.data_ptr(), n, i64, dt.node_level.data_ptr(), center.data_ptr(),
-        bmin.data_ptr(), bmax.data_ptr(), order.data_ptr(), scan.data_ptr(),
-        ctypes.byref(num_classes), ws.data_ptr(), ws.numel(), st), "grouped classify")
+    rc = lib.ygg_grouped_classify(
+        src.data_ptr(), tgt.data_ptr(), n, i64, dt.num_nodes, dt.node_level.data_ptr(),
+        center.data_ptr(), bmin.data_ptr(), bmax.data_ptr(), order.data_ptr(), scan.data_ptr(),
+        ctypes.byref(num_classes), ws.data_ptr(), ws.numel(), st)
+    if rc == 1:  # YGG_ERR_ARG: a pair names a node outside the tree
+        raise IndexError(f"far-pair node index out of range [0, {dt.num_nodes})")
+    _lib.check(rc, "grouped classify")
     c = int(num_classes.value)
     class_keys = torch.empty((c, 5), dtype=torch.int32, device=dev)
     class_disp = torch.empty((c, 3), dtype=torch.float32, device=dev)
