@@ -239,6 +239,23 @@ int ygg_grouped_emit(int64_t num_pairs, int idx_i64, const float* bounds_min,
                      void* class_offsets, int32_t* class_keys, float* class_displacements,
                      void* ws, size_t ws_bytes, void* stream);
 
+/* ---- node mass moments ---------------------------------------------------- */
+
+/* compute_tree_mass_moments (yggdrax/tree_moments.py:686-746): mass_out[t]
+ * = sum of the masses of node t's particle range node_ranges[t] (inclusive)
+ * and center_out[t] = (sum m x) / mass, 0 where that mass is 0, over
+ * Morton-sorted positions [n][3] and masses [n]. The per-particle terms are
+ * m * x rounded to the product dtype; the sums are accumulated in float64
+ * (chunk sums of `chunk` particles in a segment tree plus the partial
+ * chunks of each range) and rounded once. Dtypes: mass_out has the masses'
+ * dtype, center_out is float64 if either input is (pos_f64, mass_f64 flags),
+ * else float32. */
+size_t ygg_tree_mass_moments_workspace_bytes(int64_t n, int64_t chunk);
+int ygg_tree_mass_moments(const void* pos_sorted, int pos_f64, const void* mass_sorted,
+                          int mass_f64, int64_t n, const int32_t* node_ranges, int64_t num_nodes,
+                          int64_t chunk, void* mass_out, void* center_out, void* ws,
+                          size_t ws_bytes, void* stream);
+
 /* ---- dual-tree traversal -------------------------------------------------- */
 
 #define YGG_MAC_BH 0

@@ -9,6 +9,7 @@ from .oracle import (  # noqa: F401
     build_octree,
     build_tree,
     compute_tree_geometry,
+    compute_tree_mass_moments,
     densify_interactions,
     geometry_to_level_major,
     grouped_interactions_from_pairs,

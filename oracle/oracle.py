@@ -8,9 +8,10 @@ capacity/retry driver (3840-4440) and output shaping (3561-3837), and
 the downstream far-field layouts: level-major geometry
 (yggdrax/_geometry_impl.py:342-433), dense M2L buffers
 (yggdrax/dense_interactions.py:35-187) and displacement classes
-(yggdrax/grouped_interactions.py:74-207). Outputs are dicts of numpy arrays whose keys are the reference's NamedTuple
-field names; indices are int64 as under the reference's default
-INDEX_DTYPE.
+(yggdrax/grouped_interactions.py:74-207), and per-node mass moments
+(yggdrax/tree_moments.py:686-746). Outputs are dicts of numpy arrays whose
+keys are the reference's NamedTuple field names; indices are int64 as under
+the reference's default INDEX_DTYPE.
 """
 
 from __future__ import annotations
@@ -269,6 +270,33 @@ def densify_interactions(topo, geom, inter):
             out["m2l_counts"][level, j] = c
             out["m2l_displacements"][level, j, :c] = lm["centers"][level, j] - center[row]
     return out
+
+
+def compute_tree_mass_moments(topo, positions_sorted, masses_sorted):
+    """compute_tree_mass_moments (tree_moments.py:686-746): prefix sums of
+    the masses and of the per-particle terms m * x (rounded to the product
+    dtype, as masses[:, None] * positions is), differenced at every node's
+    range, then center = moment / mass in the product dtype, 0 where the
+    mass is 0. The prefix sums run in 80-bit extended precision instead of
+    the reference's float dtype, so they approximate the exact node sums
+    (error <= n * 2^-64 * prefix magnitude) rather than reproducing the
+    float cumsum's rounding."""
+    pos = np.asarray(positions_sorted)
+    m = np.asarray(masses_sorted)
+    w = np.result_type(pos.dtype, m.dtype)
+    terms = m.astype(w)[:, None] * pos.astype(w)
+    ld = np.longdouble
+    mp = np.concatenate([np.zeros(1, ld), np.cumsum(m.astype(ld))])
+    wp = np.concatenate([np.zeros((1, 3), ld), np.cumsum(terms.astype(ld), axis=0)])
+    r = np.asarray(topo["node_ranges"], np.int64)
+    s, e = r[:, 0], r[:, 1] + 1
+    mass = (mp[e] - mp[s]).astype(m.dtype)
+    msum = (wp[e] - wp[s]).astype(w)
+    safe = np.where(mass == 0, 1, mass).astype(w)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        center = msum / safe[:, None]
+    center = np.where(mass[:, None] == 0, 0, center).astype(w)
+    return dict(mass=mass, center_of_mass=center)
 
 
 def _cell_size(extent, level):

@@ -662,3 +662,64 @@ def test_grouped_classes_reconstruct_the_pairs():
             want = keys[c][2 + a] * cell
             got = float(r["class_displacements"][c, a])
             assert abs(got - want) <= abs(want) * 2.0 ** -24
+
+
+# ---- per-node mass moments ---------------------------------------------------
+
+def _fsum_node(vals, s, e):
+    import math
+    return math.fsum(vals[s:e + 1].tolist())
+
+
+def test_mass_moments_reference_sample():
+    """tests/unit/test_tree_moments.py:18-40, 69-83, 135-160: three particles,
+    leaf size 1: leaves hold their particle's mass and position, the root the
+    total mass and the mass-weighted mean position."""
+    pos = np.array([[-0.5, -0.5, -0.5], [0.1, 0.0, 0.3], [0.6, 0.4, -0.2]], np.float32)
+    mass = np.array([1.0, 2.0, 3.0], np.float32)
+    t = oracle.build_tree(pos, mass, (np.full(3, -1.0, np.float32), np.ones(3, np.float32)),
+                          leaf_size=1)
+    ps, ms = t["positions_sorted"], t["masses_sorted"]
+    mm = oracle.compute_tree_mass_moments(t, ps, ms)
+    ni = int(t["num_internal_nodes"])
+    assert mm["mass"].dtype == np.float32 and mm["center_of_mass"].dtype == np.float32
+    assert np.array_equal(mm["mass"][ni:], ms)
+    assert np.allclose(mm["center_of_mass"][ni:], ps, rtol=1e-6, atol=0)
+    root = int(np.flatnonzero(np.asarray(t["parent"]) == -1)[0])
+    assert mm["mass"][root] == 6.0
+    want = (mass[:, None].astype(np.float64) * pos).sum(0) / 6.0
+    assert np.allclose(mm["center_of_mass"][root], want, rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_mass_moments_are_the_exact_node_sums(dtype):
+    """Brute force: every node's mass and moment is math.fsum of its terms
+    (the exactly rounded sum) to within one unit in the last place, and the
+    centre is the moment divided by the mass in the product dtype."""
+    pos, mass = inputs.plummer_sphere(3000, 8, a=0.3)
+    t = oracle.build_octree(pos, mass, leaf_size=8)
+    ps = t["positions_sorted"].astype(dtype)
+    ms = t["masses_sorted"].astype(dtype)
+    mm = oracle.compute_tree_mass_moments(t, ps, ms)
+    terms = ms[:, None] * ps
+    r = np.asarray(t["node_ranges"])
+    for node in range(0, r.shape[0], 7):
+        s, e = int(r[node, 0]), int(r[node, 1])
+        exact_m = dtype(_fsum_node(ms, s, e))
+        assert abs(mm["mass"][node] - exact_m) <= np.spacing(exact_m)
+        for a in range(3):
+            exact_x = dtype(_fsum_node(terms[:, a], s, e))
+            com = mm["center_of_mass"][node, a]
+            assert abs(com - exact_x / exact_m) <= 2 * np.spacing(abs(exact_x / exact_m)) + (
+                np.spacing(abs(exact_x)) / exact_m)
+
+
+def test_mass_moment_dtypes_follow_the_inputs():
+    """mass has the masses' dtype; the centre the product dtype."""
+    pos, mass = inputs.uniform_cube(200, 3)
+    t = oracle.build_tree(pos, mass, leaf_size=4)
+    ps, ms = t["positions_sorted"], t["masses_sorted"]
+    mm = oracle.compute_tree_mass_moments(t, ps, ms.astype(np.float64))
+    assert mm["mass"].dtype == np.float64 and mm["center_of_mass"].dtype == np.float64
+    mm = oracle.compute_tree_mass_moments(t, ps.astype(np.float64), ms)
+    assert mm["mass"].dtype == np.float32 and mm["center_of_mass"].dtype == np.float64

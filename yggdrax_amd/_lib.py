@@ -119,6 +119,9 @@ _SIGS = {
     "ygg_dense_interactions": (_INT, [_P] * 3 + [_INT, _I64, _I64] + [_P] * 2 +
                                [_I32, _I64, _I64] + [_P] * 7),
     "ygg_grouped_workspace_bytes": (_SZ, [_I64]),
+    "ygg_tree_mass_moments_workspace_bytes": (_SZ, [_I64, _I64]),
+    "ygg_tree_mass_moments": (_INT, [_P, _INT, _P, _INT, _I64, _P, _I64, _I64, _P, _P, _P, _SZ,
+                                     _P]),
     "ygg_grouped_classify": (_INT, [_P, _P, _I64, _INT, _I64] + [_P] * 7 + [_P, _SZ, _P]),
     "ygg_grouped_emit": (_INT, [_I64, _INT] + [_P] * 10 + [_P, _SZ, _P]),
     "ygg_geometry": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I32, _P, _P, _P,
