@@ -18,7 +18,8 @@ the walk with learned capacities (one emission walk) against the plain
 count-then-fill walk on the same tree and checks the lists are identical.
 The `layouts` object times the downstream far-field layouts (level-major
 geometry, dense M2L buffers, displacement classes) on the 2^20-point
-Plummer workload. The CPU baseline is the oracle (C++, single thread) on
+Plummer workload, the `moments` object the per-node mass moments of the
+benchmark's own tree. The CPU baseline is the oracle (C++, single thread) on
 the same 2^24-point workload (build, geometry and the counting walk).
 """
 
@@ -313,6 +314,30 @@ def batched_leg(torch, steps, warmup):
                 sampled_systems_identical=same)
 
 
+def moments_leg(ygg, torch, pos, mass, leaf, reps=5):
+    """Per-node mass moments (compute_tree_mass_moments) of the benchmark's
+    own tree, timed over `reps` calls. Algorithmic bytes: 16 B per particle
+    read (position and mass), 8 B per node range read, 16 B per node written
+    (mass and centre)."""
+    from yggdrax_amd import tree_moments as tm
+    tree, ps, ms, _ = ygg.build_octree(pos, mass, leaf_size=leaf, return_reordered=True)
+    tm.compute_tree_mass_moments(tree, ps, ms)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        tm.compute_tree_mass_moments(tree, ps, ms)
+    b.record()
+    b.synchronize()
+    ms_call = a.elapsed_time(b) / reps
+    n, t = int(ps.shape[0]), int(tree.num_nodes)
+    nbytes = 16 * n + 24 * t
+    return dict(ms=round(ms_call, 4), points=n, nodes=t, algorithmic_bytes=nbytes,
+                gbs=round(nbytes / ms_call / 1e6, 1),
+                frac_of_hbm_peak=round(nbytes / ms_call / 1e6 / HBM_PEAK_GBS, 3))
+
+
 def layouts_leg(ygg, torch, reps=3):
     """Downstream far-field layouts on the 2^20-point Plummer workload
     (build_tree leaf 32, dehnen theta 0.5): level-major geometry, dense M2L
@@ -479,6 +504,8 @@ def main():
     if not args.no_batched:
         bat = batched_leg(torch, args.steps, args.warmup)
         _log(f"batched {bat}")
+    mom = moments_leg(ygg, torch, pos, mass, args.leaf)
+    _log(f"moments {mom}")
     lay = lay_state = None
     if not args.no_layouts:
         lay, lay_state = layouts_leg(ygg, torch)
@@ -500,7 +527,8 @@ def main():
                                f"uniform background, seed 3), build_octree leaf {args.leaf}, "
                                f"dehnen theta {args.theta}, count pass + exact fill, int64 outputs",
                    "parallelism": "single GPU"},
-        "stages": stages, "learned": lrn, "batched": bat, "layouts": lay, "roofline": roof,
+        "stages": stages, "learned": lrn, "batched": bat, "layouts": lay, "moments": mom,
+        "roofline": roof,
         "build_roofline": build_roof,
         "cpu_baseline": cpu,
     }

@@ -46,9 +46,10 @@ struct Term {
 };
 
 // Chunk sums into the segment tree's bottom row seg[P + c], zeros for the
-// padding chunks c >= C. A group of g = 2^g_log2 lanes (g <= 64) takes one
-// chunk; the loop bound is uniform per block so every lane reaches the
-// shuffles.
+// padding chunks c >= C. A group of g = 2^g_log2 lanes (g <= 8) takes one
+// chunk, each lane summing every g-th particle of it, so a wavefront reads
+// 64 / g chunks and reduces with log2(g) shuffle steps. The loop bound is
+// uniform per block so every lane reaches the shuffles.
 template <typename P, typename M>
 __global__ void __launch_bounds__(256)
 k_chunk_sums(const P* __restrict__ pos, const M* __restrict__ mass, int64_t n, int64_t chunk,
@@ -86,7 +87,10 @@ __global__ void k_seg_level(Acc4* __restrict__ seg, int64_t h) {
   }
 }
 
-// The top levels h, h/2, ..., 1 by one block, a barrier between levels.
+// The top levels h, h/2, ..., 1 (h <= kTopLevel) by one block, a barrier
+// between levels.
+constexpr int64_t kTopLevel = 1024;
+
 __global__ void __launch_bounds__(1024) k_seg_top(Acc4* __restrict__ seg, int64_t h) {
   for (int64_t hh = h; hh >= 1; hh >>= 1) {
     for (int64_t i = hh + threadIdx.x; i < 2 * hh; i += blockDim.x) {
@@ -168,12 +172,12 @@ int mass_moments(const P* pos, const M* mass, int64_t n, const int32_t* ranges, 
   Acc4* seg = a.take<Acc4>((size_t)(2 * Pw));
   if (!seg) return YGG_ERR_WORKSPACE;
   int g_log2 = 0;
-  while ((1ll << g_log2) < chunk && g_log2 < 6) ++g_log2;
+  while ((1ll << g_log2) < chunk && g_log2 < 3) ++g_log2;
   hipLaunchKernelGGL((k_chunk_sums<P, M>), dim3(grid_for(Pw << g_log2, 256)), dim3(256), 0, st,
                      pos, mass, n, chunk, C, Pw, g_log2, seg);
   YGG_LAUNCH_CHECK();
   int64_t h = Pw >> 1;
-  for (; h > 1024; h >>= 1) {
+  for (; h > kTopLevel; h >>= 1) {
     hipLaunchKernelGGL(k_seg_level, dim3(grid_for(h, 256)), dim3(256), 0, st, seg, h);
     YGG_LAUNCH_CHECK();
   }
