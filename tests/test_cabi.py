@@ -73,7 +73,7 @@ def test_struct_layouts_match_header(tmp_path):
     from yggdrax_amd import _lib
     structs = {"YggWalkParams": _lib.WalkParams, "YggRadixTopology": _lib.RadixTopologyPtrs,
                "YggOctreeMeta": _lib.OctreeMetaPtrs, "YggFlatWalkParams": _lib.FlatWalkParams,
-               "YggBatch": _lib.Batch}
+               "YggBatch": _lib.Batch, "YggPolicyDesc": _lib.PolicyDesc}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "yggdrax_amd.h"',
              "int main(void) {"]
     for cname, py in structs.items():

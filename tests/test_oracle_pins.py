@@ -723,3 +723,20 @@ def test_mass_moment_dtypes_follow_the_inputs():
     assert mm["mass"].dtype == np.float64 and mm["center_of_mass"].dtype == np.float64
     mm = oracle.compute_tree_mass_moments(t, ps.astype(np.float64), ms)
     assert mm["mass"].dtype == np.float32 and mm["center_of_mass"].dtype == np.float64
+
+
+def test_pair_policy_descriptor_validation():
+    """Descriptor fields are checked when it is made (no device needed)."""
+    from yggdrax_amd import PairPolicyDescriptor as D
+    d = D(accept="distance", accept_dist_sq=0.5, tag="dist_sq", tag_thresholds=(0.1, 0.2),
+          tag_values=(1, 2, 3))
+    c = d.to_c()
+    assert (c.accept, c.near, c.tag_metric, c.num_thresholds) == (1, 0, 1, 2)
+    assert list(c.tag_values)[:3] == [1, 2, 3] and abs(c.thresholds[1] - 0.2) < 1e-7
+    for bad in (dict(accept="sometimes"), dict(near="far"), dict(tag="colour"),
+                dict(tag="dist_sq", tag_thresholds=(0.2, 0.1), tag_values=(1, 2, 3)),
+                dict(tag="dist_sq", tag_thresholds=(0.1,), tag_values=(1,)),
+                dict(tag="dist_sq", tag_thresholds=tuple(range(9)), tag_values=tuple(range(10))),
+                dict(tag="target_mod", tag_modulus=0)):
+        with pytest.raises(ValueError):
+            D(**bad)

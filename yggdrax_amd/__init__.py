@@ -25,6 +25,7 @@ from .interactions import (
     neighbors_for_leaf,
     set_default_dual_tree_config,
 )
+from .policy import PairPolicyDescriptor
 from .tree import (
     OctreeTopology,
     OctreeTree,
@@ -49,7 +50,7 @@ __all__ = [
     "build_interactions_and_neighbors", "interactions_for_node", "neighbors_for_leaf",
     "CompactTaggedFarPairs", "build_compact_far_pairs",
     "build_compact_far_pairs_and_leaf_neighbor_lists", "build_leaf_neighbor_lists",
-    "set_default_dual_tree_config", "OctreeTopology", "OctreeTree", "RadixTree",
+    "set_default_dual_tree_config", "PairPolicyDescriptor", "OctreeTopology", "OctreeTree", "RadixTree",
     "RadixTreeTopology", "Tree", "TreeBuildConfig", "available_tree_types", "build_octree",
     "build_tree", "infer_bounds", "morton_encode", "register_tree_builder",
     "resolve_tree_topology", "tree_from_particles",

@@ -75,6 +75,14 @@ class Batch(ctypes.Structure):
     ]
 
 
+class PolicyDesc(ctypes.Structure):
+    _fields_ = [
+        ("accept", _I32), ("near", _I32), ("tag_metric", _I32), ("num_thresholds", _I32),
+        ("accept_dist_sq", _F), ("tag_modulus", _I32), ("thresholds", _F * 8),
+        ("tag_values", _I32 * 9),
+    ]
+
+
 _BP = ctypes.POINTER(Batch)
 
 _SIGS = {
@@ -140,6 +148,8 @@ _SIGS = {
     "ygg_cap_counts": (_INT, [_P, _I64, _I32, _P, _P]),
     "ygg_far_output": (_INT, [_I64, _P, _P, _P, _I32, _P, _P, _P, _INT, _P, _P, _P, _P, _P]),
     "ygg_near_output": (_INT, [_I64, _P, _P, _I32, _P, _P, _P, _I64, _INT, _P, _P, _P]),
+    "ygg_policy_descriptor_actions": (_INT, [_P, _P, _I64, _I64, _INT, _F, _F,
+                                             ctypes.POINTER(PolicyDesc)] + [_P] * 5),
     "ygg_policy_geometry": (_INT, [_P, _P, _I64, _I64, _INT, _F, _F] + [_P] * 10 + [_P]),
     "ygg_policy_apply": (_INT, [_P, _P, _I64, _I64] + [_P] * 5 + [_I64] + [_P] * 4 + [_I64] +
                          [_P] * 2 + [_I64] + [_P] * 3),

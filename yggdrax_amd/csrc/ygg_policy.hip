@@ -5,6 +5,8 @@
 // two verdicts (accept only if both accept, near only if both say near, else
 // refine), appends far entries with their directed tags, near entries and
 // the refined child pairs (same split rule and child order as the MAC walk).
+// A policy given as a descriptor is evaluated on the device instead of on
+// the host (k_policy_descriptor).
 #include "ygg_common.h"
 #include "ygg_walk.h"
 
@@ -161,11 +163,103 @@ __global__ __launch_bounds__(256) void k_policy_apply(
   }
 }
 
+// ---- pair-policy descriptors ------------------------------------------------
+// A policy given as data (YggPolicyDesc) is evaluated here for both
+// directions of every pair, producing the same four arrays a host policy
+// callable would, for k_policy_apply. Floating-point steps follow
+// k_policy_geometry (dist_sq summed x, y, z without contraction).
+
+__device__ __forceinline__ int32_t desc_tag(const YggPolicyDesc& d, float d2, float et, float es,
+                                            int32_t target) {
+  float m;
+  switch (d.tag_metric) {
+    case YGG_TAG_DIST_SQ:
+      m = d2;
+      break;
+    case YGG_TAG_OPENING_RATIO: {
+      const float sum = et + es;
+      m = (sum * sum) / d2;
+      break;
+    }
+    case YGG_TAG_TARGET_MOD:
+      return d.tag_modulus > 0 ? target % d.tag_modulus : -1;
+    default:
+      return -1;
+  }
+  int k = 0;
+  for (int j = 0; j < d.num_thresholds; ++j) k += (m >= d.thresholds[j]) ? 1 : 0;
+  return d.tag_values[k];
+}
+
+__global__ void k_policy_descriptor(const NodeRec* __restrict__ nodes,
+                                    const uint64_t* __restrict__ wf, int64_t n, int NI,
+                                    int mac_type, float theta_sq, float theta_root,
+                                    YggPolicyDesc d, int32_t* __restrict__ act_f,
+                                    int32_t* __restrict__ act_r, int32_t* __restrict__ tag_f,
+                                    int32_t* __restrict__ tag_r) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t pr = wf[i];
+    const int32_t t = (int32_t)(pr >> 32), s = (int32_t)(uint32_t)pr;
+    const NodeRec nt = nodes[t], ns = nodes[s];
+    float dx = nt.cx - ns.cx, dy = nt.cy - ns.cy, dz = nt.cz - ns.cz;
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    const bool diff = t != s;
+    bool acc;
+    switch (d.accept) {
+      case YGG_ACCEPT_MAC: {
+        if (mac_type == YGG_MAC_ENGBLOM) {
+          float rb = fmaxf(nt.ext, ns.ext), rs = fminf(nt.ext, ns.ext);
+          float lhs = rb + theta_root * rs;
+          acc = diff && d2 > 0.0f && (lhs * lhs <= theta_sq * d2);
+        } else {
+          float sum = nt.ext + ns.ext;
+          acc = diff && d2 > 0.0f && (sum * sum <= theta_sq * d2);
+        }
+        break;
+      }
+      case YGG_ACCEPT_DISTANCE:
+        acc = diff && d2 >= d.accept_dist_sq;
+        break;
+      case YGG_ACCEPT_DIFFERENT:
+        acc = diff;
+        break;
+      default:
+        acc = false;
+    }
+    const bool near = d.near == YGG_NEAR_LEAF_PAIRS && !acc && diff && t >= NI && s >= NI;
+    const int32_t a = acc ? 0 : (near ? 1 : 2);
+    act_f[i] = a;
+    act_r[i] = a;
+    tag_f[i] = acc ? desc_tag(d, d2, nt.ext, ns.ext, t) : -1;
+    tag_r[i] = acc ? desc_tag(d, d2, ns.ext, nt.ext, s) : -1;
+  }
+}
+
 }  // namespace ygg
 
 using namespace ygg;
 
 extern "C" {
+
+int ygg_policy_descriptor_actions(const void* nodes, const uint64_t* pairs, int64_t n,
+                                  int64_t num_internal, int mac_type, float theta_sq,
+                                  float theta_root, const YggPolicyDesc* desc,
+                                  int32_t* actions_fwd, int32_t* actions_rev, int32_t* tags_fwd,
+                                  int32_t* tags_rev, void* stream) {
+  if (n < 0 || !nodes || !desc) return YGG_ERR_ARG;
+  if (desc->num_thresholds < 0 || desc->num_thresholds > YGG_POLICY_MAX_THRESHOLDS)
+    return YGG_ERR_ARG;
+  if (n == 0) return YGG_OK;
+  hipLaunchKernelGGL(k_policy_descriptor, dim3(grid_for(n, 256, 8192)), dim3(256), 0,
+                     (hipStream_t)stream, (const NodeRec*)nodes, pairs, n, (int)num_internal,
+                     mac_type, theta_sq, theta_root, *desc, actions_fwd, actions_rev, tags_fwd,
+                     tags_rev);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
 
 int ygg_policy_geometry(const void* nodes, const uint64_t* pairs, int64_t n, int64_t num_internal,
                         int mac_type, float theta_sq, float theta_root, int32_t* targets,

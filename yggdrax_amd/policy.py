@@ -7,9 +7,19 @@ is accepted only when both directions accept and is near only when both
 say near, otherwise it is refined. Accepted pairs carry the directed tags
 (forward tag on the target's row, reverse tag on the source's row), exposed
 in DualTreeWalkResult.interaction_tags.
+
+A policy can also be given as data, a ``PairPolicyDescriptor``: the same
+contract for the policies it can describe (accept by the built-in MAC, by
+distance or always; near for leaf pairs or never; tags by distance or
+opening-ratio buckets or target id modulo k), evaluated for both directions
+by a HIP kernel instead of on the host.
 """
 
 from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
 
 import torch
 
@@ -21,6 +31,69 @@ from .tree import device_topology, resolve_tree_topology
 
 ACTION_ACCEPT, ACTION_NEAR, ACTION_REFINE = 0, 1, 2
 
+_ACCEPT = {"mac": 0, "distance": 1, "different": 2, "never": 3}
+_NEAR = {"leaf_pairs": 0, "never": 1}
+_TAG = {"none": 0, "dist_sq": 1, "opening_ratio": 2, "target_mod": 3}
+MAX_TAG_THRESHOLDS = 8
+
+
+@dataclass(frozen=True)
+class PairPolicyDescriptor:
+    """A pair policy as data, evaluated by the walk's kernels in both
+    directions (the PairPolicy contract, _interactions_impl.py:731-810).
+
+    accept: "mac" (the built-in MAC of the call: mac_ok), "distance"
+      (different nodes with dist_sq >= accept_dist_sq), "different" (any two
+      different nodes) or "never".
+    near: "leaf_pairs" (two different leaves that are not accepted, the
+      default rule of _default_pair_actions, 694-711) or "never".
+    tag (accepted pairs only, -1 otherwise): "none"; "dist_sq" or
+      "opening_ratio" ((extent_target + extent_source)^2 / dist_sq), giving
+      tag_values[k] with k the number of tag_thresholds (ascending, at most
+      8) <= the metric; or "target_mod" (target node % tag_modulus).
+    All comparisons are in float32 on the same dist_sq and extents a policy
+    callable receives.
+    """
+
+    accept: str = "mac"
+    accept_dist_sq: float = 0.0
+    near: str = "leaf_pairs"
+    tag: str = "none"
+    tag_thresholds: Sequence[float] = ()
+    tag_values: Sequence[int] = (-1,)
+    tag_modulus: int = 1
+
+    def __post_init__(self):
+        if self.accept not in _ACCEPT:
+            raise ValueError(f"accept must be one of {sorted(_ACCEPT)}")
+        if self.near not in _NEAR:
+            raise ValueError(f"near must be one of {sorted(_NEAR)}")
+        if self.tag not in _TAG:
+            raise ValueError(f"tag must be one of {sorted(_TAG)}")
+        th = [float(x) for x in self.tag_thresholds]
+        if len(th) > MAX_TAG_THRESHOLDS:
+            raise ValueError(f"at most {MAX_TAG_THRESHOLDS} tag thresholds")
+        if any(b < a for a, b in zip(th, th[1:])):
+            raise ValueError("tag_thresholds must be ascending")
+        if self.tag in ("dist_sq", "opening_ratio") and len(self.tag_values) != len(th) + 1:
+            raise ValueError("tag_values needs one more entry than tag_thresholds")
+        if self.tag == "target_mod" and int(self.tag_modulus) < 1:
+            raise ValueError("tag_modulus must be >= 1")
+
+    def to_c(self) -> "_lib.PolicyDesc":
+        d = _lib.PolicyDesc()
+        d.accept, d.near, d.tag_metric = _ACCEPT[self.accept], _NEAR[self.near], _TAG[self.tag]
+        th = [float(x) for x in self.tag_thresholds]
+        d.num_thresholds = len(th)
+        d.accept_dist_sq = float(self.accept_dist_sq)
+        d.tag_modulus = int(self.tag_modulus)
+        for j, v in enumerate(th):
+            d.thresholds[j] = v
+        vals = list(self.tag_values) if self.tag in ("dist_sq", "opening_ratio") else [-1]
+        for j, v in enumerate(vals):
+            d.tag_values[j] = int(v)
+        return d
+
 def _lib_policy():
     return _lib.load()
 
@@ -28,7 +101,6 @@ def _lib_policy():
 def _policy_walk(ctx: _WalkContext, dt, pair_policy, policy_state, qcap, fcap, ncap):
     lib = _lib_policy()
     dev, st = ctx.dev, ctx.stream
-    idx = INDEX_DTYPE
     wf = [torch.empty(qcap, dtype=torch.int64, device=dev) for _ in range(2)]
     wf[0][0] = 0  # root pair (0, 0)
     far_k = torch.empty(fcap, dtype=torch.int32, device=dev)
@@ -39,40 +111,22 @@ def _policy_walk(ctx: _WalkContext, dt, pair_policy, policy_state, qcap, fcap, n
     counters = torch.zeros(2, dtype=torch.int64, device=dev)
     flags = torch.zeros(4, dtype=torch.int32, device=dev)
     out_size = torch.zeros(1, dtype=torch.int32, device=dev)
+    desc = pair_policy.to_c() if isinstance(pair_policy, PairPolicyDescriptor) else None
     size, gens, cur = 1, [1], 0
     while size > 0:
         if len(gens) > MAX_GENERATIONS:
             raise RuntimeError("pair-policy walk exceeded the generation limit")
         n = size
-        f32 = dict(dtype=torch.float32, device=dev)
-        t = torch.empty(n, dtype=torch.int32, device=dev)
-        s = torch.empty(n, dtype=torch.int32, device=dev)
-        ct = torch.empty((n, 3), **f32)
-        cs = torch.empty((n, 3), **f32)
-        d2 = torch.empty(n, **f32)
-        et = torch.empty(n, **f32)
-        es = torch.empty(n, **f32)
-        mok = torch.empty(n, dtype=torch.uint8, device=dev)
-        tl = torch.empty(n, dtype=torch.uint8, device=dev)
-        sl = torch.empty(n, dtype=torch.uint8, device=dev)
-        _lib.check(lib.ygg_policy_geometry(
-            ctx.nodes.data_ptr(), wf[cur].data_ptr(), n, dt.num_internal, ctx.mac,
-            ctx.theta_sq, ctx.theta_root, t.data_ptr(), s.data_ptr(), ct.data_ptr(),
-            cs.data_ptr(), d2.data_ptr(), et.data_ptr(), es.data_ptr(), mok.data_ptr(),
-            tl.data_ptr(), sl.data_ptr(), st), "policy geometry")
-        valid = torch.ones(n, dtype=torch.bool, device=dev)
-        ti, si = t.to(idx), s.to(idx)
-        same = ti == si
-        common = dict(valid_pairs=valid, mac_ok=mok.bool(), different_nodes=~same,
-                      same_node=same, dist_sq=d2)
-        fa, ftag = pair_policy(policy_state, target_leaf=tl.bool(), source_leaf=sl.bool(),
-                               target_nodes=ti, source_nodes=si, center_target=ct,
-                               center_source=cs, extent_target=et, extent_source=es, **common)
-        ra, rtag = pair_policy(policy_state, target_leaf=sl.bool(), source_leaf=tl.bool(),
-                               target_nodes=si, source_nodes=ti, center_target=cs,
-                               center_source=ct, extent_target=es, extent_source=et, **common)
-        conv = lambda a: torch.as_tensor(a, device=dev).to(torch.int32).reshape(n).contiguous()  # noqa
-        fa, ra, ftag, rtag = conv(fa), conv(ra), conv(ftag), conv(rtag)
+        if desc is not None:
+            fa, ra, ftag, rtag = (torch.empty(n, dtype=torch.int32, device=dev)
+                                  for _ in range(4))
+            _lib.check(lib.ygg_policy_descriptor_actions(
+                ctx.nodes.data_ptr(), wf[cur].data_ptr(), n, dt.num_internal, ctx.mac,
+                ctx.theta_sq, ctx.theta_root, ctypes.byref(desc), fa.data_ptr(),
+                ra.data_ptr(), ftag.data_ptr(), rtag.data_ptr(), st), "policy descriptor")
+        else:
+            fa, ra, ftag, rtag = _host_policy_actions(ctx, dt, pair_policy, policy_state,
+                                                      wf[cur], n)
         out_size.zero_()
         _lib.check(lib.ygg_policy_apply(
             ctx.nodes.data_ptr(), wf[cur].data_ptr(), n, dt.num_internal, fa.data_ptr(),
@@ -89,6 +143,43 @@ def _policy_walk(ctx: _WalkContext, dt, pair_policy, policy_state, qcap, fcap, n
     return dict(far_total=c[0], near_total=c[1], gen_sizes=gens, peak=max(gens),
                 far_keys=far_k, far_vals=far_v, far_tags=far_t, near_keys=near_k,
                 near_vals=near_v, decisions=(c[0] // 2, c[1] // 2, sum(gens) - (c[0] + c[1]) // 2))
+
+
+def _host_policy_actions(ctx: _WalkContext, dt, pair_policy, policy_state, pairs, n):
+    """One generation's geometry from the device, then the policy callable
+    in both directions on the host."""
+    lib = _lib_policy()
+    dev, st = ctx.dev, ctx.stream
+    idx = INDEX_DTYPE
+    f32 = dict(dtype=torch.float32, device=dev)
+    t = torch.empty(n, dtype=torch.int32, device=dev)
+    s = torch.empty(n, dtype=torch.int32, device=dev)
+    ct = torch.empty((n, 3), **f32)
+    cs = torch.empty((n, 3), **f32)
+    d2 = torch.empty(n, **f32)
+    et = torch.empty(n, **f32)
+    es = torch.empty(n, **f32)
+    mok = torch.empty(n, dtype=torch.uint8, device=dev)
+    tl = torch.empty(n, dtype=torch.uint8, device=dev)
+    sl = torch.empty(n, dtype=torch.uint8, device=dev)
+    _lib.check(lib.ygg_policy_geometry(
+        ctx.nodes.data_ptr(), pairs.data_ptr(), n, dt.num_internal, ctx.mac,
+        ctx.theta_sq, ctx.theta_root, t.data_ptr(), s.data_ptr(), ct.data_ptr(),
+        cs.data_ptr(), d2.data_ptr(), et.data_ptr(), es.data_ptr(), mok.data_ptr(),
+        tl.data_ptr(), sl.data_ptr(), st), "policy geometry")
+    valid = torch.ones(n, dtype=torch.bool, device=dev)
+    ti, si = t.to(idx), s.to(idx)
+    same = ti == si
+    common = dict(valid_pairs=valid, mac_ok=mok.bool(), different_nodes=~same,
+                  same_node=same, dist_sq=d2)
+    fa, ftag = pair_policy(policy_state, target_leaf=tl.bool(), source_leaf=sl.bool(),
+                           target_nodes=ti, source_nodes=si, center_target=ct,
+                           center_source=cs, extent_target=et, extent_source=es, **common)
+    ra, rtag = pair_policy(policy_state, target_leaf=sl.bool(), source_leaf=tl.bool(),
+                           target_nodes=si, source_nodes=ti, center_target=cs,
+                           center_source=ct, extent_target=es, extent_source=et, **common)
+    conv = lambda a: torch.as_tensor(a, device=dev).to(torch.int32).reshape(n).contiguous()  # noqa
+    return conv(fa), conv(ra), conv(ftag), conv(rtag)
 
 
 def build_with_pair_policy(tree, geometry, theta, mac_type, dehnen_radius_scale, pair_policy,
@@ -145,4 +236,5 @@ def build_with_pair_policy(tree, geometry, theta, mac_type, dehnen_radius_scale,
     return inter, nbrs
 
 
-__all__ = ["ACTION_ACCEPT", "ACTION_NEAR", "ACTION_REFINE", "build_with_pair_policy"]
+__all__ = ["ACTION_ACCEPT", "ACTION_NEAR", "ACTION_REFINE", "MAX_TAG_THRESHOLDS",
+           "PairPolicyDescriptor", "build_with_pair_policy"]
