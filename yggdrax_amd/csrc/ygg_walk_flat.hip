@@ -84,6 +84,12 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
   const int w = threadIdx.x / YGG_WAVE;
   __shared__ unsigned long long s_wave[kFlatThreads / YGG_WAVE];
   __shared__ unsigned long long s_base[3];
+  // Emission staging: a pair yields at most two far or two near entries, so
+  // each wavefront's entries of one chunk fit in 2 * 64 * kFlatPer slots,
+  // far entries first, then near ones. Each wavefront's entries occupy one
+  // contiguous range of each stream, written out by consecutive lanes.
+  constexpr int kStage = EMIT ? 2 * YGG_WAVE * kFlatPer : 1;
+  __shared__ uint2 s_stage[kFlatThreads / YGG_WAVE][kStage];
 
   for (int64_t ci = c_first; ci < c_end; ci += c_step) {
     const int64_t chunk = ci * kFlatChunk;
@@ -174,8 +180,11 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
     __syncthreads();
     const unsigned long long ex = wbase + incl - packed;
     int64_t cslot = (int64_t)s_base[0] + (int64_t)(ex & mask);
-    int64_t fslot = (int64_t)s_base[1] + (int64_t)((ex >> 21) & mask);
-    int64_t nslot = (int64_t)s_base[2] + (int64_t)((ex >> 42) & mask);
+    // wavefront-local offsets of this lane's far and near entries
+    const unsigned long long lex = incl - packed;
+    const unsigned wave_nf = (unsigned)((s_wave[w] >> 21) & mask);
+    unsigned lf = (unsigned)((lex >> 21) & mask);
+    unsigned ln = wave_nf + (unsigned)((lex >> 42) & mask);
     bool qover = false, fover = false, nover = false;
 #pragma unroll
     for (int k = 0; k < kFlatPer; ++k) {
@@ -183,28 +192,16 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
       if (m == 0) continue;
       if (m == 1) {
         if (EMIT) {
-          if (fslot + 1 < a.far_cap) {
-            a.far_keys[fslot] = x0[k];
-            a.far_vals[fslot] = ps[k];
-            a.far_keys[fslot + 1] = x1[k];
-            a.far_vals[fslot + 1] = pt[k];
-          } else {
-            fover = true;
-          }
+          s_stage[w][lf] = make_uint2(x0[k], ps[k]);
+          s_stage[w][lf + 1] = make_uint2(x1[k], pt[k]);
         }
-        fslot += 2;
+        lf += 2;
       } else if (m == 2) {
         if (EMIT) {
-          if (nslot + 1 < a.near_cap) {
-            a.near_keys[nslot] = x0[k];
-            a.near_vals[nslot] = ps[k];
-            a.near_keys[nslot + 1] = x1[k];
-            a.near_vals[nslot + 1] = pt[k];
-          } else {
-            nover = true;
-          }
+          s_stage[w][ln] = make_uint2(x0[k], ps[k]);
+          s_stage[w][ln + 1] = make_uint2(x1[k], pt[k]);
         }
-        nslot += 2;
+        ln += 2;
       } else {
         uint32_t ca[4], cb[4];
         int cnt;
@@ -237,10 +234,37 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         }
       }
     }
+    if (EMIT) {
+      __syncthreads();  // staged entries visible to the whole wavefront
+      const unsigned wave_nn = (unsigned)((s_wave[w] >> 42) & mask);
+      const unsigned long long wex = wbase;  // block-exclusive prefix of this wavefront
+      const int64_t fbase = (int64_t)s_base[1] + (int64_t)((wex >> 21) & mask);
+      const int64_t nbase = (int64_t)s_base[2] + (int64_t)((wex >> 42) & mask);
+      for (unsigned j = lane; j < wave_nf; j += YGG_WAVE) {
+        const int64_t slot = fbase + j;
+        if (slot < a.far_cap) {
+          const uint2 e = s_stage[w][j];
+          a.far_keys[slot] = e.x;
+          a.far_vals[slot] = e.y;
+        } else {
+          fover = true;
+        }
+      }
+      for (unsigned j = lane; j < wave_nn; j += YGG_WAVE) {
+        const int64_t slot = nbase + j;
+        if (slot < a.near_cap) {
+          const uint2 e = s_stage[w][wave_nf + j];
+          a.near_keys[slot] = e.x;
+          a.near_vals[slot] = e.y;
+        } else {
+          nover = true;
+        }
+      }
+    }
     if (qover) atomicOr(&a.flags[0], 1u);
     if (fover) atomicOr(&a.flags[1], 1u);
     if (nover) atomicOr(&a.flags[2], 1u);
-    __syncthreads();  // s_wave / s_base reuse
+    __syncthreads();  // s_wave / s_base / s_stage reuse
   }
 }
 
