@@ -376,6 +376,11 @@ typedef struct YggFlatWalkParams {
   int32_t* node_hist;         /* [num_nodes] or NULL */
   const int32_t* roots;       /* [num_roots] seed pairs (r, r) of a forest, or NULL: (root, root) */
   int64_t num_roots;
+  uint64_t* spread_counters;  /* optional [2 * spread_lanes]: the counting walk (emit 0) adds its
+                                 far/near totals here, one lane per group of workgroups, instead
+                                 of reserving per chunk; the walk then writes their sums to
+                                 counters[0..1]. NULL: totals go straight to counters. */
+  int64_t spread_lanes;
 } YggFlatWalkParams;
 
 int ygg_walk_flat(const YggFlatWalkParams* params, uint32_t* host_status,

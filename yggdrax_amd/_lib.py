@@ -63,7 +63,8 @@ class FlatWalkParams(ctypes.Structure):
         ("near_cap", _I64), ("nodes", _P), ("wf0", _P), ("wf1", _P), ("gen_sizes", _P),
         ("flags", _P), ("counters", _P), ("far_keys", _P), ("far_vals", _P),
         ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("xcd_groups", _I32),
-        ("node_hist", _P), ("roots", _P), ("num_roots", _I64),
+        ("node_hist", _P), ("roots", _P), ("num_roots", _I64), ("spread_counters", _P),
+        ("spread_lanes", _I64),
     ]
 
 

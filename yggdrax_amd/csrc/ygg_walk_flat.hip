@@ -43,6 +43,8 @@ struct FlatArgs {
   const int32_t* roots; // forest seeds (r, r), or null for the single root
   int64_t num_roots;
   int32_t xcd_groups;   // > 1: chunk ranges per XCD (see k_walk_flat)
+  unsigned long long* spread;  // counting walk: [2 * spread_lanes] far/near totals, or null
+  int32_t spread_lanes;
 };
 
 __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v) {
@@ -84,11 +86,17 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
   const int w = threadIdx.x / YGG_WAVE;
   __shared__ unsigned long long s_wave[kFlatThreads / YGG_WAVE];
   __shared__ unsigned long long s_base[3];
-  // Emission staging: a pair yields at most two far or two near entries, so
-  // each wavefront's entries of one chunk fit in 2 * 64 * kFlatPer slots,
-  // far entries first, then near ones. Each wavefront's entries occupy one
-  // contiguous range of each stream, written out by consecutive lanes.
-  constexpr int kStage = EMIT ? 2 * YGG_WAVE * kFlatPer : 1;
+  // counting walk with spread counters: this workgroup's far/near totals,
+  // added to one lane at the end instead of one reservation per chunk
+  const bool spread = !EMIT && a.spread != nullptr;
+  unsigned long long acc_f = 0, acc_n = 0;
+  // Output staging. Each wavefront's outputs of one chunk occupy one
+  // contiguous range of each stream (slots follow the block scan), so they
+  // are gathered in LDS and written out by consecutive lanes. A pair yields
+  // at most two far or two near entries: 2 * 64 * kFlatPer slots hold both
+  // (far first, then near). Child pairs (up to four per pair) go through the
+  // same buffer in rounds of kStage.
+  constexpr int kStage = 2 * YGG_WAVE * kFlatPer;
   __shared__ uint2 s_stage[kFlatThreads / YGG_WAVE][kStage];
 
   for (int64_t ci = c_first; ci < c_end; ci += c_step) {
@@ -174,12 +182,17 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
       const unsigned long long tc = btot & mask, tf = (btot >> 21) & mask,
                                tn = (btot >> 42) & mask;
       s_base[0] = tc ? atomicAdd(&a.gen_sizes[g + 1], (uint32_t)tc) : 0;
-      s_base[1] = tf ? atomicAdd(&a.counters[0], tf) : 0;
-      s_base[2] = tn ? atomicAdd(&a.counters[1], tn) : 0;
+      if (spread) {
+        acc_f += tf;
+        acc_n += tn;
+        s_base[1] = s_base[2] = 0;
+      } else {
+        s_base[1] = tf ? atomicAdd(&a.counters[0], tf) : 0;
+        s_base[2] = tn ? atomicAdd(&a.counters[1], tn) : 0;
+      }
     }
     __syncthreads();
     const unsigned long long ex = wbase + incl - packed;
-    int64_t cslot = (int64_t)s_base[0] + (int64_t)(ex & mask);
     // wavefront-local offsets of this lane's far and near entries
     const unsigned long long lex = incl - packed;
     const unsigned wave_nf = (unsigned)((s_wave[w] >> 21) & mask);
@@ -202,36 +215,6 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
           s_stage[w][ln + 1] = make_uint2(x1[k], pt[k]);
         }
         ln += 2;
-      } else {
-        uint32_t ca[4], cb[4];
-        int cnt;
-        if (m == 3) {
-          ca[0] = x0[k]; cb[0] = x0[k];
-          ca[1] = x0[k]; cb[1] = x1[k];
-          ca[2] = x1[k]; cb[2] = x1[k];
-          cnt = 3;
-        } else if (m == 4) {
-          ca[0] = x0[k]; cb[0] = x2[k];
-          ca[1] = x0[k]; cb[1] = x3[k];
-          ca[2] = x1[k]; cb[2] = x2[k];
-          ca[3] = x1[k]; cb[3] = x3[k];
-          cnt = 4;
-        } else if (m == 5) {
-          ca[0] = x0[k]; cb[0] = ps[k];
-          ca[1] = x1[k]; cb[1] = ps[k];
-          cnt = 2;
-        } else {
-          ca[0] = pt[k]; cb[0] = x2[k];
-          ca[1] = pt[k]; cb[1] = x3[k];
-          cnt = 2;
-        }
-        for (int j = 0; j < cnt; ++j, ++cslot) {
-          uint32_t lo = ca[j] < cb[j] ? ca[j] : cb[j], hi = ca[j] < cb[j] ? cb[j] : ca[j];
-          if (cslot < a.queue_cap)
-            wf_out[cslot] = ((uint64_t)lo << 32) | hi;
-          else
-            qover = true;
-        }
       }
     }
     if (EMIT) {
@@ -261,10 +244,88 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         }
       }
     }
+    // Child pairs, in rounds of kStage slots of the wavefront's range; the
+    // round count is the largest over the block's wavefronts so every thread
+    // reaches the same barriers.
+    {
+      unsigned max_nc = 0;
+#pragma unroll
+      for (int q = 0; q < kFlatThreads / YGG_WAVE; ++q)
+        max_nc = max(max_nc, (unsigned)(s_wave[q] & mask));
+      const unsigned wave_nc = (unsigned)(s_wave[w] & mask);
+      const int64_t cbase = (int64_t)s_base[0] + (int64_t)(wbase & mask);
+      const unsigned lc0 = (unsigned)(lex & mask);
+      for (unsigned r0 = 0; r0 < max_nc; r0 += kStage) {
+        if (EMIT || r0 > 0) __syncthreads();  // previous use of s_stage finished
+        unsigned lc = lc0;
+#pragma unroll
+        for (int k = 0; k < kFlatPer; ++k) {
+          const int m = mode[k];
+          if (m < 3) continue;
+          // children (a_j, b_j): mode 3 (t,t) split, 4 cross split,
+          // 5 split t only, 6 split s only
+          const uint32_t a0 = m == 6 ? pt[k] : x0[k];
+          const uint32_t b0 = m == 3 ? x0[k] : (m == 5 ? ps[k] : x2[k]);
+          const uint32_t a1 = m == 6 ? pt[k] : (m == 5 ? x1[k] : x0[k]);
+          const uint32_t b1 = m == 3 ? x1[k] : (m == 5 ? ps[k] : x3[k]);
+          const uint32_t a2 = x1[k];
+          const uint32_t b2 = m == 3 ? x1[k] : x2[k];
+          const uint32_t a3 = x1[k], b3 = x3[k];
+          const unsigned cnt = m == 3 ? 3u : (m == 4 ? 4u : 2u);
+          unsigned o = lc - r0;  // wraps when lc < r0: outside the window
+          if (o < (unsigned)kStage) s_stage[w][o] = make_uint2(min(a0, b0), max(a0, b0));
+          o = lc + 1 - r0;
+          if (o < (unsigned)kStage) s_stage[w][o] = make_uint2(min(a1, b1), max(a1, b1));
+          o = lc + 2 - r0;
+          if (cnt > 2 && o < (unsigned)kStage) s_stage[w][o] = make_uint2(min(a2, b2), max(a2, b2));
+          o = lc + 3 - r0;
+          if (cnt > 3 && o < (unsigned)kStage) s_stage[w][o] = make_uint2(min(a3, b3), max(a3, b3));
+          lc += cnt;
+        }
+        __syncthreads();
+        const unsigned n_r = wave_nc > r0 ? min(wave_nc - r0, (unsigned)kStage) : 0u;
+        for (unsigned j = lane; j < n_r; j += YGG_WAVE) {
+          const int64_t slot = cbase + r0 + j;
+          if (slot < a.queue_cap) {
+            const uint2 e = s_stage[w][j];
+            wf_out[slot] = ((uint64_t)e.x << 32) | e.y;
+          } else {
+            qover = true;
+          }
+        }
+      }
+    }
     if (qover) atomicOr(&a.flags[0], 1u);
     if (fover) atomicOr(&a.flags[1], 1u);
     if (nover) atomicOr(&a.flags[2], 1u);
     __syncthreads();  // s_wave / s_base / s_stage reuse
+  }
+  if (spread && threadIdx.x == 0 && (acc_f | acc_n)) {
+    const int lane_id = (int)(blockIdx.x % (unsigned)a.spread_lanes);
+    if (acc_f) atomicAdd(&a.spread[2 * lane_id], acc_f);
+    if (acc_n) atomicAdd(&a.spread[2 * lane_id + 1], acc_n);
+  }
+}
+
+// counters[0..1] = the sums of the spread lanes (one workgroup).
+__global__ void k_spread_sum(FlatArgs a) {
+  __shared__ unsigned long long s_sum[2][256];
+  unsigned long long f = 0, n = 0;
+  for (int l = threadIdx.x; l < a.spread_lanes; l += blockDim.x) {
+    f += a.spread[2 * l];
+    n += a.spread[2 * l + 1];
+  }
+  s_sum[0][threadIdx.x] = f;
+  s_sum[1][threadIdx.x] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tf = 0, tn = 0;
+    for (int i = 0; i < (int)blockDim.x; ++i) {
+      tf += s_sum[0][i];
+      tn += s_sum[1][i];
+    }
+    a.counters[0] = tf;
+    a.counters[1] = tn;
   }
 }
 
@@ -273,6 +334,8 @@ __global__ void k_flat_init(FlatArgs a, int root, int G) {
   for (int k = i; k <= G; k += gridDim.x * blockDim.x) a.gen_sizes[k] = 0;
   if (i < 4) a.flags[i] = 0;
   if (i < 2) a.counters[i] = 0;  // [0] far total, [1] near total
+  if (a.spread)
+    for (int k = i; k < 2 * a.spread_lanes; k += gridDim.x * blockDim.x) a.spread[k] = 0;
   if (a.roots) {
     for (int64_t k = i; k < a.num_roots; k += (int64_t)gridDim.x * blockDim.x) {
       const uint32_t r = (uint32_t)a.roots[k];
@@ -506,6 +569,9 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
   a.num_roots = p->roots ? p->num_roots : 1;
   a.xcd_groups = p->xcd_groups;
   if (p->roots && (p->num_roots < 1 || p->num_roots > p->queue_cap)) return YGG_ERR_ARG;
+  if (p->spread_counters && (p->spread_lanes < 1 || p->spread_lanes > 4096)) return YGG_ERR_ARG;
+  a.spread = (unsigned long long*)p->spread_counters;
+  a.spread_lanes = (int32_t)p->spread_lanes;
   const int G = p->max_generations;
   const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
   hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
@@ -535,6 +601,10 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
       }
     if ((fl[0] | fl[1] | fl[2]) || done >= 0) {
       *num_generations = done >= 0 ? done : end;
+      if (a.spread && !p->emit) {
+        hipLaunchKernelGGL(k_spread_sum, dim3(1), dim3(256), 0, st, a);
+        YGG_LAUNCH_CHECK();
+      }
       if (gen_ms) {
         for (int k = 0; k < G; ++k) gen_ms[k] = 0.f;
         for (int k = 0; k < end; ++k)
@@ -583,12 +653,16 @@ int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
   a.roots = p->roots;
   a.num_roots = p->roots ? p->num_roots : 1;
   a.xcd_groups = p->xcd_groups;
+  if (p->spread_counters && (p->spread_lanes < 1 || p->spread_lanes > 4096)) return YGG_ERR_ARG;
+  a.spread = (unsigned long long*)p->spread_counters;
+  a.spread_lanes = (int32_t)p->spread_lanes;
   const int G = p->max_generations;
   const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
   hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
                      G);
   const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
   for (int k = 0; k < G; ++k) launch_flat(a, k, p->emit != 0, p->pairs_per_thread, blocks, st);
+  if (a.spread && !p->emit) hipLaunchKernelGGL(k_spread_sum, dim3(1), dim3(256), 0, st, a);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

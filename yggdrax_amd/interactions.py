@@ -44,6 +44,9 @@ WALK_XCD_GROUPS = int(os.environ.get("YGGDRAX_WALK_XCD_GROUPS", "0"))
 WALK_PAIRS_PER_THREAD = int(os.environ.get("YGGDRAX_WALK_PAIRS_PER_THREAD", "8"))
 MAC_TYPES = {"bh": 0, "dehnen": 1, "engblom": 2}
 FLAT_GRID_BLOCKS = 2048
+# Counter lanes of the counting walk's far/near totals (workgroup b adds to
+# lane b % SPREAD_LANES).
+SPREAD_LANES = 64
 # Forces the attempt-by-attempt walk with per-row slots (the reference's
 # fixed-capacity method) instead of the flat walk plus ladder emulation.
 EXACT_ATTEMPTS = os.environ.get("YGGDRAX_EXACT_ATTEMPTS", "0") == "1"
@@ -282,6 +285,9 @@ class _WalkContext:
         Q = max(4, int(queue_cap))
         wf = torch.empty((2, Q), dtype=torch.int64, device=dev)
         counters = torch.empty(2, dtype=torch.int64, device=dev)
+        # the counting walk adds its totals into spread lanes instead of
+        # reserving per chunk on two contended addresses
+        spread = None if emit else torch.empty(2 * SPREAD_LANES, dtype=torch.int64, device=dev)
         fk = fv = nk = nv = None
         if emit:
             fk = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
@@ -305,6 +311,8 @@ class _WalkContext:
         p.hist_gen, p.node_hist = int(hist_gen), _lib.ptr(node_hist)
         p.xcd_groups = WALK_XCD_GROUPS
         p.pairs_per_thread = int(self.pairs_per_thread)
+        if spread is not None:
+            p.spread_counters, p.spread_lanes = spread.data_ptr(), SPREAD_LANES
         roots = getattr(self, "roots", None)
         if roots is not None:
             p.roots, p.num_roots = roots.data_ptr(), int(roots.numel())
