@@ -33,7 +33,8 @@ struct FlatArgs {
   uint64_t* wf1;
   uint32_t* gen_sizes;
   uint32_t* flags;                 // [0] queue, [1] far stream, [2] near stream
-  unsigned long long* counters;    // [0] far total, [1] near total
+  unsigned long long* counters;    // [0] far total, [1] near total (emission walk: [0]
+                                   // holds far << 32 | near until k_unpack_totals)
   uint32_t* far_keys;
   uint32_t* far_vals;
   uint32_t* near_keys;
@@ -186,6 +187,16 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         acc_f += tf;
         acc_n += tn;
         s_base[1] = s_base[2] = 0;
+      } else if (EMIT) {
+        // far (high 32 bits) and near (low 32 bits) running totals in one
+        // reservation; a field that would pass 2^32 marks the walk as
+        // overflowed (its result is discarded)
+        const unsigned long long add = (tf << 32) | tn;
+        const unsigned long long old = add ? atomicAdd(&a.counters[0], add) : 0ull;
+        s_base[1] = old >> 32;
+        s_base[2] = old & 0xFFFFFFFFull;
+        if ((old >> 32) + tf > 0xFFFFFFFFull || (old & 0xFFFFFFFFull) + tn > 0xFFFFFFFFull)
+          atomicOr(&a.flags[1], 1u);
       } else {
         s_base[1] = tf ? atomicAdd(&a.counters[0], tf) : 0;
         s_base[2] = tn ? atomicAdd(&a.counters[1], tn) : 0;
@@ -304,6 +315,16 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
     const int lane_id = (int)(blockIdx.x % (unsigned)a.spread_lanes);
     if (acc_f) atomicAdd(&a.spread[2 * lane_id], acc_f);
     if (acc_n) atomicAdd(&a.spread[2 * lane_id + 1], acc_n);
+  }
+}
+
+// The emission walk's packed running total (far << 32 | near) in
+// counters[0], unpacked into counters[0] (far) and counters[1] (near).
+__global__ void k_unpack_totals(FlatArgs a) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const unsigned long long c = a.counters[0];
+    a.counters[0] = c >> 32;
+    a.counters[1] = c & 0xFFFFFFFFull;
   }
 }
 
@@ -605,6 +626,10 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
         hipLaunchKernelGGL(k_spread_sum, dim3(1), dim3(256), 0, st, a);
         YGG_LAUNCH_CHECK();
       }
+      if (p->emit) {
+        hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a);
+        YGG_LAUNCH_CHECK();
+      }
       if (gen_ms) {
         for (int k = 0; k < G; ++k) gen_ms[k] = 0.f;
         for (int k = 0; k < end; ++k)
@@ -663,6 +688,7 @@ int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
   const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
   for (int k = 0; k < G; ++k) launch_flat(a, k, p->emit != 0, p->pairs_per_thread, blocks, st);
   if (a.spread && !p->emit) hipLaunchKernelGGL(k_spread_sum, dim3(1), dim3(256), 0, st, a);
+  if (p->emit) hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
