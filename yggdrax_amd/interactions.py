@@ -573,6 +573,35 @@ def _sort_entries(keys, vals, n, rows):
     return ko, vo, off
 
 
+_SIDE_STREAMS: dict = {}
+
+
+def _side_stream(dev) -> torch.cuda.Stream:
+    """A second stream of the device (default priority) for independent work."""
+    key = torch.device(dev).index
+    st = _SIDE_STREAMS.get(key)
+    if st is None:
+        st = _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return st
+
+
+def _sort_far_and_near(fk, fv, nf, T, nk, nv, nn, L):
+    """The far and near row-key sorts are independent: the far one runs on a
+    side stream while the near one runs on the current stream, so their
+    passes (each limited by its tile look-back chain) overlap."""
+    dev = fk.device
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        far = _sort_entries(fk, fv, nf, T)
+    near = _sort_entries(nk, nv, nn, L)
+    main.wait_stream(side)
+    for t in far:
+        t.record_stream(main)
+    return far, near
+
+
 def _emulate_ladder(lad: _Ladder, peak: int, max_far: int, max_near: int):
     """Outcome of the reference's retry loop from full-walk statistics.
 
@@ -651,8 +680,8 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                           near_cap=tot["near_total"])
         assert not (em["queue_overflow"] or em["far_overflow"] or em["near_overflow"])
     nf, nn = em["far_total"], em["near_total"]
-    fk, fv, far_off = _sort_entries(em["far_keys"], em["far_vals"], nf, T)
-    _, nv, near_off = _sort_entries(em["near_keys"], em["near_vals"], nn, L)
+    (fk, fv, far_off), (_, nv, near_off) = _sort_far_and_near(
+        em["far_keys"], em["far_vals"], nf, T, em["near_keys"], em["near_vals"], nn, L)
     mx = torch.stack([torch.diff(far_off).max(), torch.diff(near_off).max()]).cpu()
     max_far, max_near = int(mx[0]), int(mx[1])
     stats = dict(peak=em["peak"], gen_sizes=em["gen_sizes"], decisions=em["decisions"],
