@@ -17,10 +17,16 @@
 namespace ygg {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kTile = kSortThreads * kSortItems;  // 4096 keys
-constexpr int kWaves = kSortThreads / YGG_WAVE;      // 4
-constexpr int kWaveKeys = kTile / kWaves;            // 1024 keys per wave
+constexpr int kWaves = kSortThreads / YGG_WAVE;  // 4
+// Keys per thread of one tile. A tile of 256 * ITEMS keys leaves runs of
+// about tile / 2^BITS keys per digit for the scatter; 8-bit digits keep
+// 4096-key tiles (runs of 16), wider digits on 32-bit keys take 8192-key
+// tiles so a 9-bit digit keeps runs of 16 (64-bit keys stay at 16 items:
+// registers).
+template <typename K, int BITS>
+constexpr int sort_items() {
+  return (sizeof(K) == 4 && BITS >= 9) ? 32 : 16;
+}
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPre = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
@@ -81,7 +87,7 @@ __global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
   }
 }
 
-template <typename K, int BITS>
+template <typename K, int BITS, int ITEMS>
 __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
                                                   K* __restrict__ keys_out,
@@ -92,6 +98,9 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   constexpr int kBins = 1 << BITS;
   constexpr unsigned kMask = kBins - 1;
   constexpr int D = kBins / kSortThreads;  // digits owned per thread
+  constexpr int kSortItems = ITEMS;
+  constexpr int kTile = kSortThreads * ITEMS;
+  constexpr int kWaveKeys = kTile / kWaves;
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wave_hist[kWaves][kBins];
   __shared__ uint32_t s_digit_start[kBins];
@@ -251,15 +260,24 @@ int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
   return YGG_OK;
 }
 
-static inline int64_t num_tiles(int64_t n) { return (n + kTile - 1) / kTile; }
+static inline int64_t tile_keys(int key_bytes, int digit_bits) {
+  return (int64_t)kSortThreads * ((key_bytes == 4 && digit_bits >= 9) ? 32 : 16);
+}
+static inline int64_t num_tiles(int64_t n, int key_bytes, int digit_bits) {
+  const int64_t t = tile_keys(key_bytes, digit_bits);
+  return (n + t - 1) / t;
+}
 
 // 8-bit digits: a 4096-key tile then has runs of ~16 keys per digit, so the
-// scatter writes stay coalesced; measured on gfx950 at 1.5e8-3.3e8 u32
-// pairs, 10-bit digits (runs of ~4 keys) move each byte 2.7x slower, which
-// outweighs the pass saved. Wider digits remain available to the autotuner.
+// scatter writes stay coalesced. 32-bit keys with 9-bit digits use
+// 8192-key tiles (runs of ~16 again) and pay where they save a pass:
+// measured on gfx950, 3.3e8 (u32, u32) pairs with 18-bit keys sort in
+// 6.0 ms with two 9-bit passes against 7.85 ms with three 8-bit passes,
+// while 19-bit keys stay faster with three 8-bit passes (3.6 ms at 1.5e8)
+// than with 9-bit (three passes) or 10-bit digits (runs of ~8 keys).
+// Every width remains available to the autotuner.
 int default_digit_bits(int key_bytes, int range_bits) {
-  (void)key_bytes;
-  (void)range_bits;
+  if (key_bytes == 4 && passes_for(range_bits, 9) < passes_for(range_bits, 8)) return 9;
   return 8;
 }
 
@@ -268,7 +286,7 @@ size_t sort_workspace_bytes(int64_t n, int key_bytes, int range_bits, int digit_
   const int passes = passes_for(range_bits, digit_bits);
   const size_t bins = (size_t)1 << digit_bits;
   Arena a(nullptr, 0);
-  int64_t t = num_tiles(n);
+  int64_t t = num_tiles(n, key_bytes, digit_bits);
   a.take<char>((size_t)n * key_bytes);                              // alternate keys
   a.take<uint32_t>((size_t)n);                                      // alternate values
   a.take<uint32_t>((size_t)passes * bins);                          // histograms
@@ -281,8 +299,11 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
                            uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
                            size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre) {
   constexpr int kBins = 1 << BITS;
+  constexpr int kItems = sort_items<K, BITS>();
+  static_assert(kItems * kSortThreads == (sizeof(K) == 4 && BITS >= 9 ? 8192 : 4096),
+                "tile size must match tile_keys()");
   const int passes = passes_for(end_bit - begin_bit, BITS);
-  const int64_t tiles = num_tiles(n);
+  const int64_t tiles = num_tiles(n, (int)sizeof(K), BITS);
   Arena a(ws, ws_bytes);
   K* kalt = (K*)a.take<char>((size_t)n * sizeof(K));
   uint32_t* valt = a.take<uint32_t>((size_t)n);
@@ -310,7 +331,7 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
     bool to_out = ((passes - 1 - p) % 2) == 0;
     K* dk = to_out ? keys_out : kalt;
     uint32_t* dv = to_out ? vals_out : valt;
-    hipLaunchKernelGGL((k_onesweep<K, BITS>), dim3((unsigned)tiles), dim3(kSortThreads), 0, st,
+    hipLaunchKernelGGL((k_onesweep<K, BITS, kItems>), dim3((unsigned)tiles), dim3(kSortThreads), 0, st,
                        src_k, src_v, dk, dv, n, begin_bit + BITS * p, hist + p * kBins,
                        lookback + (int64_t)p * tiles * kBins, ctr + p);
     YGG_LAUNCH_CHECK();

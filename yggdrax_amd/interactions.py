@@ -552,13 +552,16 @@ class _FlatResult(NamedTuple):
     stats: dict
 
 
-def _sort_entries(keys, vals, n, rows):
+def _sort_entries(keys, vals, n, rows, exact_keys=False):
     """Stable radix sort of (row key, value) pairs by row key; the digit
-    width comes from the autotuner's plan for this size."""
+    width comes from the autotuner's plan for this size. Keys below `rows`
+    need (rows - 1).bit_length() bits (`exact_keys`); streams padded with
+    0xFFFFFFFF fillers sort on rows.bit_length() bits so every filler lands
+    after the last row."""
     from . import autotune
     lib = _lib.load()
     dev = keys.device
-    end_bit = max(1, int(rows).bit_length())
+    end_bit = max(1, (int(rows) - 1 if exact_keys else int(rows)).bit_length())
     ko = torch.empty(max(1, n), dtype=torch.int32, device=dev)
     vo = torch.empty(max(1, n), dtype=torch.int32, device=dev)
     if n > 0:
@@ -594,8 +597,8 @@ def _sort_far_and_near(fk, fv, nf, T, nk, nv, nn, L):
     side = _side_stream(dev)
     side.wait_stream(main)
     with torch.cuda.stream(side):
-        far = _sort_entries(fk, fv, nf, T)
-    near = _sort_entries(nk, nv, nn, L)
+        far = _sort_entries(fk, fv, nf, T, exact_keys=True)
+    near = _sort_entries(nk, nv, nn, L, exact_keys=True)
     main.wait_stream(side)
     for t in far:
         t.record_stream(main)
