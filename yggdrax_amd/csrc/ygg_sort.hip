@@ -42,15 +42,28 @@ __global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, 
   __shared__ uint32_t h[kMaxPasses * kBins];
   for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
-    const int64_t i = b0 + threadIdx.x;
-    const bool valid = i < n;
-    K k = valid ? keys[i] : (K)0;
-    for (int p = 0; p < passes; ++p) {
-      const int lo = begin_bit + BITS * p;
-      const int used = end_bit - lo < BITS ? end_bit - lo : BITS;
-      pass_hist_add<BITS>(&h[p * kBins], (unsigned)((k >> lo) & (kBins - 1)), valid, p, used);
+  // Four keys per thread per step, all loaded before any is counted, so
+  // each wavefront keeps four coalesced loads in flight. The step bound is
+  // uniform per block, so every lane reaches the ballots.
+  constexpr int kUnroll = 4;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * kUnroll;
+  for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x * kUnroll; b0 < n; b0 += step) {
+    K k[kUnroll];
+    bool valid[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
+      valid[j] = i < n;
+      k[j] = valid[j] ? keys[i] : (K)0;
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      for (int p = 0; p < passes; ++p) {
+        const int lo = begin_bit + BITS * p;
+        const int used = end_bit - lo < BITS ? end_bit - lo : BITS;
+        pass_hist_add<BITS>(&h[p * kBins], (unsigned)((k[j] >> lo) & (kBins - 1)), valid[j], p,
+                            used);
+      }
     }
   }
   __syncthreads();
