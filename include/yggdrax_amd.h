@@ -342,6 +342,39 @@ int ygg_near_output(int64_t num_leaves, const int64_t* offsets, const int32_t* c
                     const int32_t* leaf_position, int64_t max_count, int out_i64,
                     void* neighbors, void* neighbor_leaf_positions, void* stream);
 
+/* A pair policy given as data, evaluated on the device for both directions
+ * of every pair (the PairPolicy contract of _resolve_pair_actions,
+ * yggdrax/_interactions_impl.py:731-810, for the policies it can describe).
+ * accept: YGG_ACCEPT_MAC (the built-in MAC of the walk), _DISTANCE
+ * (different nodes with dist_sq >= accept_dist_sq), _DIFFERENT (any two
+ * different nodes), _NEVER. near: YGG_NEAR_LEAF_PAIRS (two different leaves
+ * not accepted, the default rule of _default_pair_actions, 694-711) or
+ * _NEVER. Accepted pairs get a tag, every other pair -1: tag_metric
+ * _DIST_SQ or _OPENING_RATIO ((e_t + e_s)^2 / dist_sq) gives
+ * tag_values[k], k = number of thresholds <= the metric; _TARGET_MOD gives
+ * target % tag_modulus; _NONE gives -1. */
+#define YGG_ACCEPT_MAC 0
+#define YGG_ACCEPT_DISTANCE 1
+#define YGG_ACCEPT_DIFFERENT 2
+#define YGG_ACCEPT_NEVER 3
+#define YGG_NEAR_LEAF_PAIRS 0
+#define YGG_NEAR_NEVER 1
+#define YGG_TAG_NONE 0
+#define YGG_TAG_DIST_SQ 1
+#define YGG_TAG_OPENING_RATIO 2
+#define YGG_TAG_TARGET_MOD 3
+#define YGG_POLICY_MAX_THRESHOLDS 8
+typedef struct YggPolicyDesc {
+  int32_t accept;
+  int32_t near;
+  int32_t tag_metric;
+  int32_t num_thresholds; /* 0 .. YGG_POLICY_MAX_THRESHOLDS */
+  float accept_dist_sq;
+  int32_t tag_modulus;
+  float thresholds[YGG_POLICY_MAX_THRESHOLDS]; /* ascending */
+  int32_t tag_values[YGG_POLICY_MAX_THRESHOLDS + 1];
+} YggPolicyDesc;
+
 /* Flat-emission walk: the same pair semantics as ygg_walk_bfs over the full
  * walk, with workgroup-aggregated appends to three streams: the next
  * wavefront, far entries (key = level-major rank of the target row, value =
@@ -381,6 +414,9 @@ typedef struct YggFlatWalkParams {
                                  of reserving per chunk; the walk then writes their sums to
                                  counters[0..1]. NULL: totals go straight to counters. */
   int64_t spread_lanes;
+  const YggPolicyDesc* policy; /* optional: accept/near rules of a pair-policy descriptor for both
+                                  walks (NULL: the built-in MAC, near for two different leaves) */
+  int32_t* far_tags;          /* emission walk with a policy: [far_cap] tag of each far entry */
 } YggFlatWalkParams;
 
 int ygg_walk_flat(const YggFlatWalkParams* params, uint32_t* host_status,
@@ -539,39 +575,6 @@ int ygg_batch_mlp_totals(const YggBatch* batch, const float* outputs, float scal
                          float scale_near, float scale_queue, double* totals, void* stream);
 
 /* ---- custom pair policies ------------------------------------------------- */
-
-/* A pair policy given as data, evaluated on the device for both directions
- * of every pair (the PairPolicy contract of _resolve_pair_actions,
- * yggdrax/_interactions_impl.py:731-810, for the policies it can describe).
- * accept: YGG_ACCEPT_MAC (the built-in MAC of the walk), _DISTANCE
- * (different nodes with dist_sq >= accept_dist_sq), _DIFFERENT (any two
- * different nodes), _NEVER. near: YGG_NEAR_LEAF_PAIRS (two different leaves
- * not accepted, the default rule of _default_pair_actions, 694-711) or
- * _NEVER. Accepted pairs get a tag, every other pair -1: tag_metric
- * _DIST_SQ or _OPENING_RATIO ((e_t + e_s)^2 / dist_sq) gives
- * tag_values[k], k = number of thresholds <= the metric; _TARGET_MOD gives
- * target % tag_modulus; _NONE gives -1. */
-#define YGG_ACCEPT_MAC 0
-#define YGG_ACCEPT_DISTANCE 1
-#define YGG_ACCEPT_DIFFERENT 2
-#define YGG_ACCEPT_NEVER 3
-#define YGG_NEAR_LEAF_PAIRS 0
-#define YGG_NEAR_NEVER 1
-#define YGG_TAG_NONE 0
-#define YGG_TAG_DIST_SQ 1
-#define YGG_TAG_OPENING_RATIO 2
-#define YGG_TAG_TARGET_MOD 3
-#define YGG_POLICY_MAX_THRESHOLDS 8
-typedef struct YggPolicyDesc {
-  int32_t accept;
-  int32_t near;
-  int32_t tag_metric;
-  int32_t num_thresholds; /* 0 .. YGG_POLICY_MAX_THRESHOLDS */
-  float accept_dist_sq;
-  int32_t tag_modulus;
-  float thresholds[YGG_POLICY_MAX_THRESHOLDS]; /* ascending */
-  int32_t tag_values[YGG_POLICY_MAX_THRESHOLDS + 1];
-} YggPolicyDesc;
 
 /* Both directions' actions (0 accept, 1 near, 2 refine) and tags of one
  * generation's pairs under a policy descriptor, for ygg_policy_apply. */

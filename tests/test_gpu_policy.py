@@ -155,3 +155,67 @@ def test_policy_descriptor_matches_callable_and_oracle(case):
     assert sorted(zip(parity.np_(hi.targets).tolist(), parity.np_(hi.sources).tolist(),
                       parity.np_(hres.interaction_tags).tolist())) == sorted(
         zip(parity.np_(inter.targets).tolist(), src.tolist(), tags.tolist()))
+
+
+def test_descriptor_default_rules_equal_plain_walk():
+    """The descriptor with the built-in rules (MAC accept, near leaf pairs,
+    no tag) runs the flat walk with its rule fields set and gives the plain
+    walk's lists exactly, every tag -1 (65536 clustered points, leaf 16)."""
+    import yggdrax_amd as ygg
+    pos, mass = inputs.clustered(1 << 16, 5)
+    tree, ps, _, _ = ygg.build_octree(torch.from_numpy(pos).cuda(),
+                                      torch.from_numpy(mass).cuda(), leaf_size=16,
+                                      return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    desc = ygg.PairPolicyDescriptor(accept="mac", near="leaf_pairs", tag="none")
+    inter, nbrs, res = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.5, mac_type="dehnen", pair_policy=desc, return_result=True)
+    plain, pn, pres = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.5, mac_type="dehnen", return_result=True)
+    torch.cuda.synchronize()
+    print("default descriptor far", inter.sources.numel(), flush=True)
+    assert torch.equal(inter.counts, plain.counts) and torch.equal(inter.offsets, plain.offsets)
+    assert torch.equal(inter.level_offsets, plain.level_offsets)
+    assert torch.equal(nbrs.counts, pn.counts) and torch.equal(nbrs.offsets, pn.offsets)
+    T = tree.num_nodes
+    key = lambda i: torch.sort(i.targets.long() * T + i.sources.long())[0]  # noqa: E731
+    assert torch.equal(key(inter), key(plain))
+    L = nbrs.counts.numel()
+    rows = torch.repeat_interleave(torch.arange(L, device=nbrs.counts.device), nbrs.counts)
+    nkey = lambda n: torch.sort(rows * T + n.neighbors.long())[0]  # noqa: E731
+    assert torch.equal(nkey(nbrs), nkey(pn))
+    assert bool(torch.all(res.interaction_tags == -1))
+    assert (res.accept_decisions, res.near_decisions, res.refine_decisions) == (
+        pres.accept_decisions, pres.near_decisions, pres.refine_decisions)
+
+
+@pytest.mark.parametrize("case", [0, 2])
+def test_descriptor_flat_walk_matches_callable_at_16k(case):
+    """Descriptor policies in the flat walk against the generation-by-generation
+    host callable path at 16384 Plummer points (leaf 16): identical far
+    (target, source, tag) multisets and near lists."""
+    import yggdrax_amd as ygg
+    desc, policy, state, _, _ = _descriptors()[case]
+    pos, mass = inputs.plummer_sphere(1 << 14, 9)
+    tree, ps, _, _ = ygg.build_tree(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
+                                    leaf_size=16, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    inter, nbrs, res = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.6, mac_type="dehnen", pair_policy=desc, return_result=True)
+    hi, hn, hres = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.6, mac_type="dehnen", pair_policy=policy, policy_state=state,
+        return_result=True)
+    torch.cuda.synchronize()
+    print("descriptor 16k", case, "far", inter.sources.numel(), flush=True)
+    assert torch.equal(hi.counts, inter.counts) and torch.equal(hi.offsets, inter.offsets)
+    assert torch.equal(hn.counts, nbrs.counts)
+    T = tree.num_nodes
+    k1 = (inter.targets.long() * T + inter.sources.long()) * 16 + res.interaction_tags.long()
+    k2 = (hi.targets.long() * T + hi.sources.long()) * 16 + hres.interaction_tags.long()
+    assert torch.equal(torch.sort(k1)[0], torch.sort(k2)[0])
+    L = nbrs.counts.numel()
+    rows = torch.repeat_interleave(torch.arange(L, device=nbrs.counts.device), nbrs.counts)
+    assert torch.equal(torch.sort(rows * T + nbrs.neighbors.long())[0],
+                       torch.sort(rows * T + hn.neighbors.long())[0])
+    assert (res.accept_decisions, res.near_decisions) == (hres.accept_decisions,
+                                                          hres.near_decisions)

@@ -169,28 +169,6 @@ __global__ __launch_bounds__(256) void k_policy_apply(
 // callable would, for k_policy_apply. Floating-point steps follow
 // k_policy_geometry (dist_sq summed x, y, z without contraction).
 
-__device__ __forceinline__ int32_t desc_tag(const YggPolicyDesc& d, float d2, float et, float es,
-                                            int32_t target) {
-  float m;
-  switch (d.tag_metric) {
-    case YGG_TAG_DIST_SQ:
-      m = d2;
-      break;
-    case YGG_TAG_OPENING_RATIO: {
-      const float sum = et + es;
-      m = (sum * sum) / d2;
-      break;
-    }
-    case YGG_TAG_TARGET_MOD:
-      return d.tag_modulus > 0 ? target % d.tag_modulus : -1;
-    default:
-      return -1;
-  }
-  int k = 0;
-  for (int j = 0; j < d.num_thresholds; ++j) k += (m >= d.thresholds[j]) ? 1 : 0;
-  return d.tag_values[k];
-}
-
 __global__ void k_policy_descriptor(const NodeRec* __restrict__ nodes,
                                     const uint64_t* __restrict__ wf, int64_t n, int NI,
                                     int mac_type, float theta_sq, float theta_root,

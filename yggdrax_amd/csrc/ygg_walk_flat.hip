@@ -46,6 +46,8 @@ struct FlatArgs {
   int32_t xcd_groups;   // > 1: chunk ranges per XCD (see k_walk_flat)
   unsigned long long* spread;  // counting walk: [2 * spread_lanes] far/near totals, or null
   int32_t spread_lanes;
+  YggPolicyDesc policy;  // accept / near rules (the built-in MAC and leaf pairs by default)
+  int32_t* far_tags;     // emission walk with tags: [far_cap]
 };
 
 __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v) {
@@ -58,8 +60,9 @@ __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v
   return v;
 }
 
-template <bool EMIT, int kFlatPer>
+template <bool EMIT, int kFlatPer, bool TAGS = false>
 __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
+  static_assert(!TAGS || EMIT, "tags are written by the emission walk");
   constexpr int kFlatChunk = kFlatThreads * kFlatPer;
   if (atomic_load_agent(&a.flags[0]) | atomic_load_agent(&a.flags[1]) |
       atomic_load_agent(&a.flags[2]))
@@ -99,6 +102,7 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
   // same buffer in rounds of kStage.
   constexpr int kStage = 2 * YGG_WAVE * kFlatPer;
   __shared__ uint2 s_stage[kFlatThreads / YGG_WAVE][kStage];
+  __shared__ int32_t s_tag[kFlatThreads / YGG_WAVE][TAGS ? 2 * YGG_WAVE * kFlatPer : 1];
 
   for (int64_t ci = c_first; ci < c_end; ci += c_step) {
     const int64_t chunk = ci * kFlatChunk;
@@ -136,12 +140,26 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
       if (a.node_hist && g == a.hist_gen) atomicAdd(&a.node_hist[t], 1);
       pt[k] = (uint32_t)t;
       ps[k] = (uint32_t)s;
-      if (ok) {
+      // accept / near rules (uniform per launch): the built-in MAC with
+      // near leaf pairs unless a policy descriptor says otherwise
+      bool acc = ok;
+      if (a.policy.accept == YGG_ACCEPT_DISTANCE)
+        acc = diff && d2 >= a.policy.accept_dist_sq;
+      else if (a.policy.accept == YGG_ACCEPT_DIFFERENT)
+        acc = diff;
+      else if (a.policy.accept == YGG_ACCEPT_NEVER)
+        acc = false;
+      const bool near_rule = a.policy.near == YGG_NEAR_LEAF_PAIRS;
+      if (acc) {
         mode[k] = 1;
         x0[k] = (uint32_t)nt.pad;  // level-major rank of t
         x1[k] = (uint32_t)ns.pad;
+        if (TAGS) {  // directed tags: (t <- s) and (s <- t)
+          x2[k] = (uint32_t)desc_tag(a.policy, d2, et, es, t);
+          x3[k] = (uint32_t)desc_tag(a.policy, d2, es, et, s);
+        }
         nf += 2;
-      } else if (!t_int && !s_int && diff) {
+      } else if (near_rule && !t_int && !s_int && diff) {
         mode[k] = 2;
         x0[k] = (uint32_t)nt.leafpos;
         x1[k] = (uint32_t)ns.leafpos;
@@ -218,6 +236,10 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         if (EMIT) {
           s_stage[w][lf] = make_uint2(x0[k], ps[k]);
           s_stage[w][lf + 1] = make_uint2(x1[k], pt[k]);
+          if (TAGS) {
+            s_tag[w][lf] = (int32_t)x2[k];
+            s_tag[w][lf + 1] = (int32_t)x3[k];
+          }
         }
         lf += 2;
       } else if (m == 2) {
@@ -240,6 +262,7 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
           const uint2 e = s_stage[w][j];
           a.far_keys[slot] = e.x;
           a.far_vals[slot] = e.y;
+          if (TAGS) a.far_tags[slot] = s_tag[w][j];
         } else {
           fover = true;
         }
@@ -534,7 +557,14 @@ extern "C" {
 
 static void launch_flat(const FlatArgs& a, int k, bool emit, int per, int blocks,
                         hipStream_t st) {
-  if (emit) {
+  if (emit && a.far_tags) {
+    if (per >= 16)
+      hipLaunchKernelGGL((k_walk_flat<true, 16, true>), dim3(blocks), dim3(256), 0, st, a, k);
+    else if (per >= 8)
+      hipLaunchKernelGGL((k_walk_flat<true, 8, true>), dim3(blocks), dim3(256), 0, st, a, k);
+    else
+      hipLaunchKernelGGL((k_walk_flat<true, 4, true>), dim3(blocks), dim3(256), 0, st, a, k);
+  } else if (emit) {
     if (per >= 16)
       hipLaunchKernelGGL((k_walk_flat<true, 16>), dim3(blocks), dim3(256), 0, st, a, k);
     else if (per >= 8)
@@ -549,6 +579,23 @@ static void launch_flat(const FlatArgs& a, int k, bool emit, int per, int blocks
     else
       hipLaunchKernelGGL((k_walk_flat<false, 4>), dim3(blocks), dim3(256), 0, st, a, k);
   }
+}
+
+// Accept / near rules and tag stream of the walk: a policy descriptor's, or
+// the built-in MAC with near leaf pairs and no tags.
+static int set_policy(const YggFlatWalkParams* p, FlatArgs* a) {
+  YggPolicyDesc d = {};
+  d.accept = YGG_ACCEPT_MAC;
+  d.near = YGG_NEAR_LEAF_PAIRS;
+  d.tag_metric = YGG_TAG_NONE;
+  if (p->policy) {
+    d = *p->policy;
+    if (d.num_thresholds < 0 || d.num_thresholds > YGG_POLICY_MAX_THRESHOLDS) return YGG_ERR_ARG;
+  }
+  a->policy = d;
+  a->far_tags = (p->emit && p->policy) ? p->far_tags : nullptr;
+  if (p->emit && p->policy && !p->far_tags) return YGG_ERR_ARG;
+  return YGG_OK;
 }
 
 static hipEvent_t g_flat_events[2 * 1025];
@@ -593,6 +640,7 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
   if (p->spread_counters && (p->spread_lanes < 1 || p->spread_lanes > 4096)) return YGG_ERR_ARG;
   a.spread = (unsigned long long*)p->spread_counters;
   a.spread_lanes = (int32_t)p->spread_lanes;
+  if (int rc = set_policy(p, &a)) return rc;
   const int G = p->max_generations;
   const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
   hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
@@ -681,6 +729,7 @@ int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
   if (p->spread_counters && (p->spread_lanes < 1 || p->spread_lanes > 4096)) return YGG_ERR_ARG;
   a.spread = (unsigned long long*)p->spread_counters;
   a.spread_lanes = (int32_t)p->spread_lanes;
+  if (int rc = set_policy(p, &a)) return rc;
   const int G = p->max_generations;
   const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
   hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,

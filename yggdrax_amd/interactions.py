@@ -278,8 +278,12 @@ class _WalkContext:
         self.kernel_log = WALK_TIMING["log"]
 
     def run_flat(self, *, emit: bool, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
-                 gen_hint: int = 0, hist_gen: int = -1, node_hist=None, grid_blocks: int = 0):
-        """One flat-emission walk over the whole tree (ygg_walk_flat)."""
+                 gen_hint: int = 0, hist_gen: int = -1, node_hist=None, grid_blocks: int = 0,
+                 policy=None):
+        """One flat-emission walk over the whole tree (ygg_walk_flat).
+
+        ``policy`` (a _lib.PolicyDesc) replaces the built-in accept / near
+        rules; the emission walk then also writes a tag per far entry."""
         dt = self.dt
         dev = self.dev
         Q = max(4, int(queue_cap))
@@ -288,10 +292,12 @@ class _WalkContext:
         # the counting walk adds its totals into spread lanes instead of
         # reserving per chunk on two contended addresses
         spread = None if emit else torch.empty(2 * SPREAD_LANES, dtype=torch.int64, device=dev)
-        fk = fv = nk = nv = None
+        fk = fv = nk = nv = ftag = None
         if emit:
             fk = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
             fv = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
+            if policy is not None:
+                ftag = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
             nk = torch.empty(max(1, near_cap), dtype=torch.int32, device=dev)
             nv = torch.empty(max(1, near_cap), dtype=torch.int32, device=dev)
         p = _lib.FlatWalkParams()
@@ -313,6 +319,9 @@ class _WalkContext:
         p.pairs_per_thread = int(self.pairs_per_thread)
         if spread is not None:
             p.spread_counters, p.spread_lanes = spread.data_ptr(), SPREAD_LANES
+        if policy is not None:
+            p.policy = ctypes.cast(ctypes.pointer(policy), ctypes.c_void_p)
+            p.far_tags = _lib.ptr(ftag)
         roots = getattr(self, "roots", None)
         if roots is not None:
             p.roots, p.num_roots = roots.data_ptr(), int(roots.numel())
@@ -337,7 +346,7 @@ class _WalkContext:
                     gen_sizes=raw_gens, peak=max(raw_gens + [1]),
                     queue_overflow=bool(int(hs[G + 1])), far_overflow=bool(int(hs[G + 2])),
                     near_overflow=bool(int(hs[G + 3])), far_keys=fk, far_vals=fv,
-                    near_keys=nk, near_vals=nv, queue_capacity=Q)
+                    near_keys=nk, near_vals=nv, far_tags=ftag, queue_capacity=Q)
 
     def run(self, *, mode: int, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
             far_row_start=None, far_row_cap=None, near_row_start=None, near_row_cap=None,

@@ -145,6 +145,28 @@ def _policy_walk(ctx: _WalkContext, dt, pair_policy, policy_state, qcap, fcap, n
                 near_vals=near_v, decisions=(c[0] // 2, c[1] // 2, sum(gens) - (c[0] + c[1]) // 2))
 
 
+def _descriptor_walk(ctx: _WalkContext, dt, desc: "PairPolicyDescriptor"):
+    """A descriptor policy evaluated inside the flat walk's own kernels: a
+    counting walk sizes the streams exactly, then the emission walk writes
+    far entries with their tags and near entries in one pass per
+    generation."""
+    from .interactions import _auto_pair_queue_candidates
+    T, NI = dt.num_nodes, dt.num_internal
+    c = desc.to_c()
+    qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
+    tot = ctx.run_flat(emit=False, queue_cap=qbig, policy=c)
+    if tot["queue_overflow"]:
+        raise RuntimeError("Count-pass traversal wavefront overflowed its conservative buffer.")
+    em = ctx.run_flat(emit=True, queue_cap=tot["peak"], far_cap=tot["far_total"],
+                      near_cap=tot["near_total"], policy=c)
+    if em["queue_overflow"] or em["far_overflow"] or em["near_overflow"]:
+        raise RuntimeError("pair-policy emission walk overflowed its counted streams")
+    return dict(far_total=em["far_total"], near_total=em["near_total"],
+                gen_sizes=em["gen_sizes"], peak=em["peak"], far_keys=em["far_keys"],
+                far_vals=em["far_vals"], far_tags=em["far_tags"], near_keys=em["near_keys"],
+                near_vals=em["near_vals"], decisions=em["decisions"])
+
+
 def _host_policy_actions(ctx: _WalkContext, dt, pair_policy, policy_state, pairs, n):
     """One generation's geometry from the device, then the policy callable
     in both directions on the host."""
@@ -187,7 +209,7 @@ def build_with_pair_policy(tree, geometry, theta, mac_type, dehnen_radius_scale,
                            max_neighbors_per_leaf=2048, max_pair_queue=None,
                            traversal_config=None, return_result=False):
     """build_interactions_and_neighbors with a custom pair policy."""
-    from .interactions import _empty_outputs, _sort_entries, _to_neighbors
+    from .interactions import _empty_outputs, _sort_far_and_near, _to_neighbors
 
     dt = device_topology(tree)
     topo = resolve_tree_topology(tree)
@@ -201,23 +223,28 @@ def build_with_pair_policy(tree, geometry, theta, mac_type, dehnen_radius_scale,
                   max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
                   traversal_config=traversal_config)
     T, L = dt.num_nodes, dt.num_leaves
-    qcap = T * DEFAULT_PAIR_QUEUE_MULTIPLIER
-    fcap = ncap = 16 * T + 1024
-    res = None
-    for _ in range(8):
-        res = _policy_walk(ctx, dt, pair_policy, policy_state, qcap, fcap, ncap)
-        if res is not None:
-            break
-        fcap, ncap = fcap * 4, ncap * 4
-    if res is None:
-        raise RuntimeError("pair-policy walk did not fit its buffers")
+    if isinstance(pair_policy, PairPolicyDescriptor):
+        res = _descriptor_walk(ctx, dt, pair_policy)
+    else:
+        qcap = T * DEFAULT_PAIR_QUEUE_MULTIPLIER
+        fcap = ncap = 16 * T + 1024
+        res = None
+        for _ in range(8):
+            res = _policy_walk(ctx, dt, pair_policy, policy_state, qcap, fcap, ncap)
+            if res is not None:
+                break
+            fcap, ncap = fcap * 4, ncap * 4
+        if res is None:
+            raise RuntimeError("pair-policy walk did not fit its buffers")
     nf, nn = res["far_total"], res["near_total"]
-    fk, perm, far_off = _sort_entries(res["far_keys"], torch.arange(
-        max(1, nf), dtype=torch.int32, device=ctx.dev)[:max(1, nf)], nf, T)
+    # far entries sort with their stream positions, which then gather the
+    # sources and tags
+    iota = torch.arange(max(1, nf), dtype=torch.int32, device=ctx.dev)
+    (fk, perm, far_off), (_, nv, near_off) = _sort_far_and_near(
+        res["far_keys"], iota, nf, T, res["near_keys"], res["near_vals"], nn, L)
     pl = perm[:nf].long()
     fv = res["far_vals"][pl]
     tags = res["far_tags"][pl]
-    _, nv, near_off = _sort_entries(res["near_keys"], res["near_vals"], nn, L)
     mx = torch.stack([torch.diff(far_off).max(), torch.diff(near_off).max()]).cpu()
     stats = dict(peak=res["peak"], gen_sizes=res["gen_sizes"], decisions=res["decisions"],
                  max_far=int(mx[0]), max_near=int(mx[1]), far_total=nf, near_total=nn)
