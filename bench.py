@@ -196,7 +196,12 @@ def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
             and _same_lists(torch, (pn.offsets, pn.counts, pn.neighbors),
                             (ln.offsets, ln.counts, ln.neighbors), L))
     nf, nn = int(pi.sources.numel()), int(pn.neighbors.numel())
+    # slot memory the fixed-capacity method would need on this tree: one row
+    # of the largest count per node (far) and per leaf (near), int32 entries
+    max_far, max_near = int(pi.counts.max()), int(pn.counts.max())
     return dict(plain_walk_ms=round(t_plain, 3), learned_walk_ms=round(t_learn, 3),
+                max_far_per_node=max_far, max_near_per_leaf=max_near,
+                fixed_capacity_slot_bytes=4 * (T * max_far + L * max_near),
                 walk_speedup=round(t_plain / t_learn, 3), identical_lists=same,
                 fallback_walks=fallbacks, hint=hint, true_far=nf, true_near=nn,
                 far_overalloc=round(hint["far"] / max(1, nf), 3),
@@ -208,7 +213,11 @@ def batched_leg(torch, steps, warmup):
     (PCG64 seed 4), build_octree leaf 32, dehnen theta 0.6. Times the batched
     pipeline (one set of launches and one forest walk for all systems)
     against the same systems run one call at a time through the single-system
-    path, and checks a sample of systems' lists are identical."""
+    path, and against the reference's fixed-capacity method (per-row slots of
+    2048 far and 2048 near entries, its automatic pair-queue ladder, retry on
+    overflow) run
+    one call per system, and checks a sample of systems' lists are
+    identical across all of them."""
     from yggdrax_amd import batch, inputs
     import yggdrax_amd as ygg
     pos_np, mass_np, offsets = inputs.plummer_batch(1024, 4)
@@ -232,12 +241,18 @@ def batched_leg(torch, steps, warmup):
         return bt, batch.build_interactions_and_neighbors_batched(bt, g, 0.6, capacity_hint=hint,
                                                                   **kw), hint
 
-    def run_one(k):
+    def run_one(k, **walk):
         p0, p1 = int(offsets[k]), int(offsets[k + 1])
         t, ps, _, _ = ygg.build_octree(P[p0:p1], M[p0:p1], leaf_size=32, return_reordered=True)
         g = ygg.compute_tree_geometry(t, ps)
         return ygg.build_interactions_and_neighbors(t, g, theta=0.6, neighbor_positions=False,
-                                                    **kw)
+                                                    **kw, **walk)
+
+    # the reference's fixed-capacity method: per-row slots of the given
+    # capacities and its retry ladder on overflow (a retry logger selects
+    # the attempt-by-attempt walk), one call per system
+    fixed = dict(max_interactions_per_node=2048, max_neighbors_per_leaf=2048,
+                 retry_logger=lambda event: None)
 
     for _ in range(max(1, warmup)):
         run_batched()
@@ -272,9 +287,21 @@ def batched_leg(torch, steps, warmup):
     outs = [run_one(k) for k in range(len(offsets) - 1)]
     torch.cuda.synchronize()
     t_s = (time.perf_counter() - t0) * 1e3
+    run_one(0, **fixed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fouts = [run_one(k, **fixed) for k in range(len(offsets) - 1)]
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) * 1e3
     same = True
     for k in range(0, len(offsets) - 1, 97):
         si, sn = outs[k]
+        fi, fn = fouts[k]
+        same &= bool(torch.equal(si.counts, fi.counts) and torch.equal(sn.counts, fn.counts)
+                     and _same_lists(torch, (si.offsets, si.counts, si.sources),
+                                     (fi.offsets, fi.counts, fi.sources), si.counts.numel())
+                     and _same_lists(torch, (sn.offsets, sn.counts, sn.neighbors),
+                                     (fn.offsets, fn.counts, fn.neighbors), sn.counts.numel()))
         bi, bn = lists.system(k)
         li, ln = llists.system(k)
         gi, gn = glists.system(k)
@@ -309,6 +336,8 @@ def batched_leg(torch, steps, warmup):
                 per_system_loop_ms=round(t_s, 3), speedup_vs_per_system=round(t_s / t_b, 2),
                 learned_speedup_vs_per_system=round(t_s / t_l, 2),
                 learned_speedup_vs_batched_plain=round(t_b / t_l, 3),
+                fixed_capacity_per_system_ms=round(t_f, 3),
+                learned_speedup_vs_fixed_capacity=round(t_f / t_l, 2),
                 far_entries=int(lists.stats["far_total"]),
                 near_entries=int(lists.stats["near_total"]), fallback_systems=len(bt.fallback),
                 sampled_systems_identical=same)
