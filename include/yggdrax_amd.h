@@ -89,6 +89,19 @@ int ygg_reorder(const float* pos, const float* mass, const uint32_t* perm, int64
 int ygg_reorder_scatter(const float* pos, const float* mass, const uint32_t* perm, int64_t n,
                         float* pos_sorted, float* mass_sorted, int32_t* inverse, void* stream);
 
+/* int32 -> int64 widening of up to YGG_WIDEN_MAX_ITEMS arrays in one launch:
+ * dst[i] = src[i] for i < n of every item. It produces the public
+ * topology's int64 index arrays (yggdrax/dtypes.py:12-40 index policy) from
+ * the kernels' int32 arrays. Items with n == 0 are skipped; src and dst of
+ * one item must not overlap. */
+#define YGG_WIDEN_MAX_ITEMS 32
+typedef struct YggWidenItem {
+  const int32_t* src;
+  int64_t* dst;
+  int64_t n;
+} YggWidenItem;
+int ygg_widen_i32_to_i64(const YggWidenItem* items, int count, void* stream);
+
 /* ---- radix (LBVH) topology ---------------------------------------------- */
 
 /* Output arrays of the adaptive radix build for L = ceil(n/leaf_size) leaves

@@ -84,9 +84,16 @@ class PolicyDesc(ctypes.Structure):
     ]
 
 
+class WidenItem(ctypes.Structure):
+    _fields_ = [("src", _P), ("dst", _P), ("n", _I64)]
+
+
+WIDEN_MAX_ITEMS = 32
+
 _BP = ctypes.POINTER(Batch)
 
 _SIGS = {
+    "ygg_widen_i32_to_i64": (_INT, [_P, _INT, _P]),
     "ygg_walk_flat_fixed": (_INT, [ctypes.POINTER(FlatWalkParams), _P]),
     "ygg_batch_infer_bounds": (_INT, [_BP, _P, _P, _P]),
     "ygg_batch_morton_sort_workspace_bytes": (_SZ, [_BP]),

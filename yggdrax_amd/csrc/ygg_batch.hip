@@ -69,8 +69,7 @@ __global__ __launch_bounds__(256) void k_b_encode_hist(Seg sg, const float* __re
       codes[i] = c;
     }
 #pragma unroll
-    for (int p = 0; p < 8; ++p)
-      pass_hist_add<8>(&h[p * 256], (unsigned)((c >> (8 * p)) & 0xFF), valid, p, p == 7 ? 7 : 8);
+    for (int p = 0; p < 8; ++p) morton_hist_add(h, c, valid, p);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x)

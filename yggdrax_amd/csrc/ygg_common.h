@@ -107,11 +107,11 @@ __device__ __forceinline__ void wave_hist_add_bits(uint32_t* h, unsigned d, bool
   if (valid && mbcnt64(peers) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
 }
 
-// Histogram add for the digit of one radix pass. The lowest pass of Morton
-// codes and row keys is close to uniform, so one LDS atomic per lane is
-// cheapest there; higher passes repeat digits across a wave and are
-// aggregated with ballots, over only the `used` low bits the key range can
-// set (the top pass of a 19-bit key has 3).
+// Histogram add for the digit of one radix pass. Pass 0 adds with one LDS
+// atomic per lane; later passes aggregate repeated digits across the wave
+// with ballots first, over only the `used` low bits the key range can set
+// (the top pass of a 63-bit Morton code has 7). Used for the top passes of
+// Morton codes, whose digits repeat within a cluster.
 template <int BITS>
 __device__ __forceinline__ void pass_hist_add(uint32_t* h, unsigned d, bool valid, int pass,
                                               int used) {

@@ -395,6 +395,32 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
                           leaf_position, pos_sorted, mass_sorted, inv, return_reordered)
 
 
+def _widen_index_arrays(arrays: dict) -> dict:
+    """INDEX_DTYPE versions of int32 device arrays: contiguous int32 GPU
+    tensors are widened to int64 together in one kernel launch
+    (ygg_widen_i32_to_i64); anything else goes through .to()."""
+    idx = INDEX_DTYPE
+    out, items = {}, []
+    for name, t in arrays.items():
+        if (idx == torch.int64 and t.dtype == torch.int32 and t.is_cuda and t.is_contiguous()
+                and len(items) < _lib.WIDEN_MAX_ITEMS):
+            out[name] = torch.empty(t.shape, dtype=torch.int64, device=t.device)
+            items.append((t, out[name]))
+        else:
+            out[name] = t.to(idx)
+    if items:
+        arr = (_lib.WidenItem * len(items))()
+        for j, (src, dst) in enumerate(items):
+            n = src.numel()
+            arr[j].src = src.data_ptr() if n else None
+            arr[j].dst = dst.data_ptr() if n else None
+            arr[j].n = n
+        dev = items[0][0].device
+        _lib.check(_lib.load().ygg_widen_i32_to_i64(arr, len(items), _lib.stream_ptr(dev)),
+                   "widen index arrays")
+    return out
+
+
 def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, max_level,
                    malformed, oct_num_levels, leaf_indices, leaf_position, pos_sorted,
                    mass_sorted, inv, return_reordered):
@@ -407,20 +433,33 @@ def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, 
     T = 2 * L - 1
     NI = L - 1
     idx = INDEX_DTYPE
+    wide = dict(parent=rad["parent"], left_child=rad["left_child"],
+                right_child=rad["right_child"], particle_indices=perm,
+                node_ranges=rad["node_ranges"], node_level=rad["node_level"],
+                level_offsets=rad["level_offsets"], nodes_by_level=rad["nodes_by_level"])
+    if octree:
+        wide.update({k: octa[k] for k in (
+            "oct_parent", "oct_children", "oct_child_counts", "oct_node_depths",
+            "oct_node_ranges", "oct_nodes_by_level", "oct_level_offsets", "oct_leaf_nodes",
+            "radix_node_to_oct")})
+        wide["radix_leaf_to_oct"] = octa["radix_node_to_oct"][NI:]
+    if inv is not None:
+        wide["inverse_permutation"] = inv
+    wide = _widen_index_arrays(wide)
     fields = dict(
-        parent=rad["parent"].to(idx),
-        left_child=rad["left_child"].to(idx),
-        right_child=rad["right_child"].to(idx),
+        parent=wide["parent"],
+        left_child=wide["left_child"],
+        right_child=wide["right_child"],
         left_is_leaf=rad["left_is_leaf"].view(torch.bool),
         right_is_leaf=rad["right_is_leaf"].view(torch.bool),
-        particle_indices=perm.to(idx),
+        particle_indices=wide["particle_indices"],
         morton_codes=sorted_codes.view(torch.uint64),
-        node_ranges=rad["node_ranges"].to(idx),
+        node_ranges=wide["node_ranges"],
         num_particles=n,
         num_internal_nodes=NI,
-        node_level=rad["node_level"].to(idx),
-        level_offsets=rad["level_offsets"].to(idx),
-        nodes_by_level=rad["nodes_by_level"].to(idx),
+        node_level=wide["node_level"],
+        level_offsets=wide["level_offsets"],
+        nodes_by_level=wide["nodes_by_level"],
         num_levels=num_levels,
         bounds_min=b6[:3].clone(),
         bounds_max=b6[3:].clone(),
@@ -431,21 +470,21 @@ def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, 
     )
     if octree:
         fields.update(
-            oct_parent=octa["oct_parent"].to(idx),
-            oct_children=octa["oct_children"].to(idx),
-            oct_child_counts=octa["oct_child_counts"].to(idx),
+            oct_parent=wide["oct_parent"],
+            oct_children=wide["oct_children"],
+            oct_child_counts=wide["oct_child_counts"],
             oct_child_mask=octa["oct_child_mask"].view(torch.bool),
             oct_valid_mask=octa["oct_valid_mask"].view(torch.bool),
             oct_node_codes=octa["oct_node_codes"].view(torch.uint64),
-            oct_node_depths=octa["oct_node_depths"].to(idx),
-            oct_node_ranges=octa["oct_node_ranges"].to(idx),
-            oct_nodes_by_level=octa["oct_nodes_by_level"].to(idx),
-            oct_level_offsets=octa["oct_level_offsets"].to(idx),
+            oct_node_depths=wide["oct_node_depths"],
+            oct_node_ranges=wide["oct_node_ranges"],
+            oct_nodes_by_level=wide["oct_nodes_by_level"],
+            oct_level_offsets=wide["oct_level_offsets"],
             oct_num_levels=int(oct_num_levels),
             oct_leaf_mask=octa["oct_leaf_mask"].view(torch.bool),
-            oct_leaf_nodes=octa["oct_leaf_nodes"].to(idx),
-            radix_node_to_oct=octa["radix_node_to_oct"].to(idx),
-            radix_leaf_to_oct=octa["radix_node_to_oct"][NI:].to(idx),
+            oct_leaf_nodes=wide["oct_leaf_nodes"],
+            radix_node_to_oct=wide["radix_node_to_oct"],
+            radix_leaf_to_oct=wide["radix_leaf_to_oct"],
         )
         topo = OctreeTopology(**fields)
     else:
@@ -460,7 +499,7 @@ def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, 
     cls = OctreeTree if octree else RadixTree
     tree = cls(topology=topo, build_mode="adaptive", positions_sorted=pos_sorted,
                masses_sorted=mass_sorted,
-               inverse_permutation=None if inv is None else inv.to(idx), _device=dt)
+               inverse_permutation=wide.get("inverse_permutation"), _device=dt)
     if return_reordered:
         return tree, pos_sorted, mass_sorted, tree.inverse_permutation
     return tree
