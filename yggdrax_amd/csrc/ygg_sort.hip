@@ -18,15 +18,21 @@ namespace ygg {
 
 constexpr int kSortThreads = 256;
 constexpr int kWaves = kSortThreads / YGG_WAVE;  // 4
-// Keys per thread of one tile. A tile of 256 * ITEMS keys leaves runs of
-// about tile / 2^BITS keys per digit for the scatter; 8-bit digits keep
-// 4096-key tiles (runs of 16), wider digits on 32-bit keys take 8192-key
-// tiles so a 9-bit digit keeps runs of 16 (64-bit keys stay at 16 items:
-// registers).
+// Tile shape of one onesweep pass: threads per workgroup, 16 keys per
+// thread. A tile leaves runs of about tile / 2^BITS keys per digit for the
+// scatter, and runs shorter than 16 keys cost more than anything else here
+// (9-bit digits on 4096-key tiles sort 3.3e8 pairs in 6.8 ms against
+// 4.5 ms on 8192-key tiles, gfx950). 8-bit digits and 64-bit keys use
+// 256 threads (4096 keys). 32-bit keys with 9-bit digits use 1024 threads:
+// 16384-key tiles at 4 waves per SIMD (16-bit per-wave counters keep the
+// workgroup in 151 KB of LDS); 3.3e8 pairs with 18-bit keys sort in 4.29 ms
+// against 5.37 ms with 256 threads of 32 keys (2 waves per SIMD). 10- and
+// 11-bit digits take 512 threads (8192 keys) to fit LDS.
 template <typename K, int BITS>
-constexpr int sort_items() {
-  return (sizeof(K) == 4 && BITS >= 9) ? 32 : 16;
+constexpr int sort_threads() {
+  return sizeof(K) == 4 ? (BITS == 9 ? 1024 : (BITS >= 10 ? 512 : 256)) : 256;
 }
+constexpr int kSortItems = 16;
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPre = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
@@ -101,8 +107,8 @@ __global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
   }
 }
 
-template <typename K, int BITS, int ITEMS>
-__global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
+template <typename K, int BITS, int ITEMS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
                                                   K* __restrict__ keys_out,
                                                   uint32_t* __restrict__ vals_out, int64_t n,
@@ -111,15 +117,19 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
                                                   uint32_t* __restrict__ tile_counter) {
   constexpr int kBins = 1 << BITS;
   constexpr unsigned kMask = kBins - 1;
-  constexpr int D = kBins / kSortThreads;  // digits owned per thread
-  constexpr int kSortItems = ITEMS;
-  constexpr int kTile = kSortThreads * ITEMS;
-  constexpr int kWaveKeys = kTile / kWaves;
+  // thread t < kBins / D owns digits [t * D, t * D + D)
+  constexpr int D = kBins >= THREADS ? kBins / THREADS : 1;
+  constexpr int kW = THREADS / YGG_WAVE;
+  constexpr int kTile = THREADS * ITEMS;
+  constexpr int kWaveKeys = kTile / kW;
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_wave_hist[kWaves][kBins];
+  // per-wave digit counts, then their exclusive prefix over waves; 16 bits
+  // hold any count of a tile of up to 65535 keys
+  using HistT = typename std::conditional<(THREADS > 256), uint16_t, uint32_t>::type;
+  __shared__ HistT s_wave_hist[kW][kBins];
   __shared__ uint32_t s_digit_start[kBins];
   __shared__ uint32_t s_global[kBins];
-  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint32_t s_wsum[kW];
   // Keys, then values, are staged through one buffer (two scatter phases),
   // with each slot's digit kept beside it: less LDS per tile, more tiles
   // resident per CU.
@@ -134,45 +144,48 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   const int w = tid / YGG_WAVE;
   const unsigned lane = lane_id();
   if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-  for (int i = tid; i < kWaves * kBins; i += kSortThreads) (&s_wave_hist[0][0])[i] = 0;
+  for (int i = tid; i < kW * kBins; i += THREADS) (&s_wave_hist[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile = s_tile;
   const int64_t base = tile * kTile;
   const int64_t wbase = base + (int64_t)w * kWaveKeys;
 
-  K key[kSortItems];
-  uint32_t val[kSortItems];
-  uint32_t rank[kSortItems];
+  K key[ITEMS];
+  uint32_t val[ITEMS];
+  uint32_t rank[ITEMS];
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     int64_t idx = wbase + k * YGG_WAVE + lane;
     bool ok = idx < n;
     key[k] = ok ? keys_in[idx] : (K)~(K)0;
     val[k] = ok ? (vals_in ? vals_in[idx] : (uint32_t)idx) : 0u;
   }
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     unsigned d = (unsigned)((key[k] >> shift) & kMask);
     uint64_t peers = wave_match_bits<BITS>(d, ~0ull);
     unsigned below = mbcnt64(peers);
     unsigned cnt = (unsigned)__popcll(peers);
     unsigned before = s_wave_hist[w][d];
     rank[k] = before + below;
-    if (below == 0) s_wave_hist[w][d] = before + cnt;
+    if (below == 0) s_wave_hist[w][d] = (HistT)(before + cnt);
   }
   __syncthreads();
   // Thread `tid` owns digits [tid * D, tid * D + D).
+  const bool owner = tid * D < kBins;
   unsigned tot[D];
   unsigned tsum = 0;
   uint32_t* lb = lookback + tile * kBins;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
+    tot[j] = 0;
+    if (!owner) continue;
     const unsigned d = (unsigned)tid * D + j;
     unsigned total = 0;
 #pragma unroll
-    for (int q = 0; q < kWaves; ++q) {
+    for (int q = 0; q < kW; ++q) {
       unsigned c = s_wave_hist[q][d];
-      s_wave_hist[q][d] = total;  // exclusive prefix over waves
+      s_wave_hist[q][d] = (HistT)total;  // exclusive prefix over waves
       total += c;
     }
     tot[j] = total;
@@ -186,7 +199,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   for (int j = 0; j < D; ++j) {
     const unsigned d = (unsigned)tid * D + j;
     unsigned e = 0;
-    if (tile > 0) {
+    if (owner && tile > 0) {
       int64_t pt = tile - 1;
       while (true) {
         uint32_t v = atomic_load_agent(&lookback[pt * kBins + d]);
@@ -209,6 +222,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   run += inc - tsum;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
+    if (!owner) break;
     const unsigned d = (unsigned)tid * D + j;
     s_digit_start[d] = run;
     s_global[d] = hist_scan[d] + excl_lb[j] - run;
@@ -216,7 +230,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     unsigned d = (unsigned)((key[k] >> shift) & kMask);
     unsigned pos = s_digit_start[d] + s_wave_hist[w][d] + rank[k];
     rank[k] = pos;  // tile slot, reused for the values
@@ -229,7 +243,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
   __syncthreads();
   const int64_t valid = (n - base) < kTile ? (n - base) : kTile;
   if (!kTwoPhase) {
-    for (int p = tid; p < valid; p += kSortThreads) {
+    for (int p = tid; p < valid; p += THREADS) {
       const K kk = s_buf[p];
       const int64_t g = (int64_t)s_global[(unsigned)((kk >> shift) & kMask)] + p;
       keys_out[g] = kk;
@@ -237,14 +251,14 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ keys_in,
     }
     return;
   }
-  for (int p = tid; p < valid; p += kSortThreads)
+  for (int p = tid; p < valid; p += THREADS)
     keys_out[(int64_t)s_global[s_dig[p]] + p] = s_buf[p];
   __syncthreads();
   uint32_t* s_vb = reinterpret_cast<uint32_t*>(s_buf);
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) s_vb[rank[k]] = val[k];
+  for (int k = 0; k < ITEMS; ++k) s_vb[rank[k]] = val[k];
   __syncthreads();
-  for (int p = tid; p < valid; p += kSortThreads)
+  for (int p = tid; p < valid; p += THREADS)
     vals_out[(int64_t)s_global[s_dig[p]] + p] = s_vb[p];
 }
 
@@ -275,7 +289,9 @@ int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
 }
 
 static inline int64_t tile_keys(int key_bytes, int digit_bits) {
-  return (int64_t)kSortThreads * ((key_bytes == 4 && digit_bits >= 9) ? 32 : 16);
+  const int threads =
+      key_bytes == 4 ? (digit_bits == 9 ? 1024 : (digit_bits >= 10 ? 512 : 256)) : 256;
+  return (int64_t)threads * kSortItems;
 }
 static inline int64_t num_tiles(int64_t n, int key_bytes, int digit_bits) {
   const int64_t t = tile_keys(key_bytes, digit_bits);
@@ -284,11 +300,11 @@ static inline int64_t num_tiles(int64_t n, int key_bytes, int digit_bits) {
 
 // 8-bit digits: a 4096-key tile then has runs of ~16 keys per digit, so the
 // scatter writes stay coalesced. 32-bit keys with 9-bit digits use
-// 8192-key tiles (runs of ~16 again) and pay where they save a pass:
-// measured on gfx950, 3.3e8 (u32, u32) pairs with 18-bit keys sort in
-// 6.0 ms with two 9-bit passes against 7.85 ms with three 8-bit passes,
-// while 19-bit keys stay faster with three 8-bit passes (3.6 ms at 1.5e8)
-// than with 9-bit (three passes) or 10-bit digits (runs of ~8 keys).
+// 16384-key tiles (runs of ~32) and pay where they save a pass: measured
+// on gfx950, 3.3e8 (u32, u32) pairs with 18-bit keys sort in 4.3 ms with
+// two 9-bit passes, while 19-bit keys keep three 8-bit passes (3.2 ms at
+// 1.5e8). 64-bit Morton codes keep 8-bit digits (2^24 keys: 1.49 ms,
+// against 1.72 ms with 9-bit and 2.8-4.8 ms with 10- or 11-bit digits).
 // Every width remains available to the autotuner.
 int default_digit_bits(int key_bytes, int range_bits) {
   if (key_bytes == 4 && passes_for(range_bits, 9) < passes_for(range_bits, 8)) return 9;
@@ -313,11 +329,11 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
                            uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
                            size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre) {
   constexpr int kBins = 1 << BITS;
-  constexpr int kItems = sort_items<K, BITS>();
-  static_assert(kItems * kSortThreads == (sizeof(K) == 4 && BITS >= 9 ? 8192 : 4096),
-                "tile size must match tile_keys()");
+  constexpr int kThreads = sort_threads<K, BITS>();
+  constexpr int kItems = kSortItems;
   const int passes = passes_for(end_bit - begin_bit, BITS);
   const int64_t tiles = num_tiles(n, (int)sizeof(K), BITS);
+  if (tile_keys((int)sizeof(K), BITS) != (int64_t)kThreads * kItems) return YGG_ERR_ARG;
   Arena a(ws, ws_bytes);
   K* kalt = (K*)a.take<char>((size_t)n * sizeof(K));
   uint32_t* valt = a.take<uint32_t>((size_t)n);
@@ -345,7 +361,7 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
     bool to_out = ((passes - 1 - p) % 2) == 0;
     K* dk = to_out ? keys_out : kalt;
     uint32_t* dv = to_out ? vals_out : valt;
-    hipLaunchKernelGGL((k_onesweep<K, BITS, kItems>), dim3((unsigned)tiles), dim3(kSortThreads), 0, st,
+    hipLaunchKernelGGL((k_onesweep<K, BITS, kItems, kThreads>), dim3((unsigned)tiles), dim3(kThreads), 0, st,
                        src_k, src_v, dk, dv, n, begin_bit + BITS * p, hist + p * kBins,
                        lookback + (int64_t)p * tiles * kBins, ctr + p);
     YGG_LAUNCH_CHECK();
