@@ -23,14 +23,17 @@ constexpr int kWaves = kSortThreads / YGG_WAVE;  // 4
 // scatter, and runs shorter than 16 keys cost more than anything else here
 // (9-bit digits on 4096-key tiles sort 3.3e8 pairs in 6.8 ms against
 // 4.5 ms on 8192-key tiles, gfx950). 8-bit digits and 64-bit keys use
-// 256 threads (4096 keys). 32-bit keys with 9-bit digits use 1024 threads:
-// 16384-key tiles at 4 waves per SIMD (16-bit per-wave counters keep the
-// workgroup in 151 KB of LDS); 3.3e8 pairs with 18-bit keys sort in 4.29 ms
-// against 5.37 ms with 256 threads of 32 keys (2 waves per SIMD). 10- and
-// 11-bit digits take 512 threads (8192 keys) to fit LDS.
+// 256 threads (4096 keys). 32-bit keys with 9- or 10-bit digits use 1024
+// threads: 16384-key tiles at 4 waves per SIMD, with 16-bit per-wave
+// counters, and for 10-bit digits keys and values staged through one
+// buffer, so the workgroup fits 151 / 139 KB of LDS. 3.3e8 pairs with
+// 18-bit keys sort in 4.3 ms against 5.37 ms with 256 threads of 32 keys
+// (2 waves per SIMD); 1.5e8 pairs with 19-bit keys sort in 2.31 ms with two
+// 10-bit passes against 3.23 ms with three 8-bit passes. 11-bit digits
+// take 512 threads (8192 keys) to fit LDS.
 template <typename K, int BITS>
 constexpr int sort_threads() {
-  return sizeof(K) == 4 ? (BITS == 9 ? 1024 : (BITS >= 10 ? 512 : 256)) : 256;
+  return sizeof(K) == 4 ? (BITS == 9 || BITS == 10 ? 1024 : (BITS >= 11 ? 512 : 256)) : 256;
 }
 constexpr int kSortItems = 16;
 constexpr uint32_t kFlagAgg = 1u << 30;
@@ -133,9 +136,10 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys
   // Keys, then values, are staged through one buffer (two scatter phases),
   // with each slot's digit kept beside it: less LDS per tile, more tiles
   // resident per CU.
-  // (64-bit keys only: for 32-bit keys one buffer each is faster.)
+  // (64-bit keys, and 32-bit keys where two buffers would not fit: for
+  // 32-bit keys one buffer each is faster.)
   using DigT = typename std::conditional<(BITS <= 8), uint8_t, uint16_t>::type;
-  constexpr bool kTwoPhase = sizeof(K) == 8;
+  constexpr bool kTwoPhase = sizeof(K) == 8 || (THREADS == 1024 && BITS >= 10);
   __shared__ K s_buf[kTile];
   __shared__ DigT s_dig[kTwoPhase ? kTile : 1];
   __shared__ uint32_t s_vals[kTwoPhase ? 1 : kTile];
@@ -290,7 +294,8 @@ int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
 
 static inline int64_t tile_keys(int key_bytes, int digit_bits) {
   const int threads =
-      key_bytes == 4 ? (digit_bits == 9 ? 1024 : (digit_bits >= 10 ? 512 : 256)) : 256;
+      key_bytes == 4 ? (digit_bits == 9 || digit_bits == 10 ? 1024 : (digit_bits >= 11 ? 512 : 256))
+                     : 256;
   return (int64_t)threads * kSortItems;
 }
 static inline int64_t num_tiles(int64_t n, int key_bytes, int digit_bits) {
@@ -300,15 +305,21 @@ static inline int64_t num_tiles(int64_t n, int key_bytes, int digit_bits) {
 
 // 8-bit digits: a 4096-key tile then has runs of ~16 keys per digit, so the
 // scatter writes stay coalesced. 32-bit keys with 9-bit digits use
-// 16384-key tiles (runs of ~32) and pay where they save a pass: measured
-// on gfx950, 3.3e8 (u32, u32) pairs with 18-bit keys sort in 4.3 ms with
-// two 9-bit passes, while 19-bit keys keep three 8-bit passes (3.2 ms at
-// 1.5e8). 64-bit Morton codes keep 8-bit digits (2^24 keys: 1.49 ms,
-// against 1.72 ms with 9-bit and 2.8-4.8 ms with 10- or 11-bit digits).
-// Every width remains available to the autotuner.
+// 16384-key tiles (runs of ~32 and ~16) and pay where they save a pass, so
+// 32-bit keys take the fewest passes among 8, 9 and 10 bits (the narrowest
+// on ties). Measured on gfx950: 3.3e8 (u32, u32) pairs with 18-bit keys sort
+// in 4.4 ms with two 9-bit passes (4.9 ms with two 10-bit passes); 1.5e8
+// pairs with 19-bit keys in 2.31 ms with two 10-bit passes against 3.0 ms
+// (three 9-bit) and 3.23 ms (three 8-bit). 64-bit Morton codes keep 8-bit
+// digits (2^24 keys: 1.49 ms, against 1.72 ms with 9-bit and 2.8-4.8 ms
+// with 10- or 11-bit digits). Every width remains available to the
+// autotuner.
 int default_digit_bits(int key_bytes, int range_bits) {
-  if (key_bytes == 4 && passes_for(range_bits, 9) < passes_for(range_bits, 8)) return 9;
-  return 8;
+  if (key_bytes != 4) return 8;
+  int best = 8;
+  for (int b = 9; b <= 10; ++b)
+    if (passes_for(range_bits, b) < passes_for(range_bits, best)) best = b;
+  return best;
 }
 
 size_t sort_workspace_bytes(int64_t n, int key_bytes, int range_bits, int digit_bits) {
