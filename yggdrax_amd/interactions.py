@@ -329,10 +329,10 @@ class _WalkContext:
         gen_ms = (ctypes.c_float * MAX_GENERATIONS)() if self.timing else None
         _lib.check(self.lib.ygg_walk_flat(ctypes.byref(p), self.host_status.data_ptr(),
                                           ctypes.byref(ngen), gen_ms, self.stream), "walk_flat")
-        hs = self.host_status
         G = MAX_GENERATIONS
-        raw_gens = [int(hs[k]) & 0xFFFFFFFF for k in range(min(ngen.value + 2, G + 1))]
-        c = [int(v) for v in counters.cpu()]
+        hs = self.host_status[:G + 4].tolist()  # pinned words the walk wrote before returning
+        raw_gens = [v & 0xFFFFFFFF for v in hs[:min(ngen.value + 2, G + 1)]]
+        c = counters.tolist()
         # every evaluated pair is exactly one of accept (2 far entries),
         # near (2 near entries) or refine
         pairs = sum(min(v, Q) for v in raw_gens)
@@ -344,8 +344,8 @@ class _WalkContext:
                                         gen_sizes=raw_gens, far_total=c[0], near_total=c[1]))
         return dict(far_total=c[0], near_total=c[1], decisions=dec,
                     gen_sizes=raw_gens, peak=max(raw_gens + [1]),
-                    queue_overflow=bool(int(hs[G + 1])), far_overflow=bool(int(hs[G + 2])),
-                    near_overflow=bool(int(hs[G + 3])), far_keys=fk, far_vals=fv,
+                    queue_overflow=bool(hs[G + 1]), far_overflow=bool(hs[G + 2]),
+                    near_overflow=bool(hs[G + 3]), far_keys=fk, far_vals=fv,
                     near_keys=nk, near_vals=nv, far_tags=ftag, queue_capacity=Q)
 
     def run(self, *, mode: int, queue_cap: int, far_cap: int = 0, near_cap: int = 0,

@@ -377,22 +377,36 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
         _lib.check(reorder(pos.data_ptr(), _lib.ptr(m), perm.data_ptr(), n,
                            pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(), st),
                    "reorder")
-    # One host read: level count, depth flags and octree level count.
-    scal = rad["scalars"].cpu()
-    if int(scal[2]) != 0:  # levels beyond the fast walk: exact pointer doubling
+    # The int64 index arrays are widened before the host waits for the
+    # build, so that launch is queued while the scalars are read.
+    wide = _widen_fields(rad, octa if octree else None, perm, inv)
+    # One host read: level count, depth flags and the octree level count.
+    scal, oscal = _read_scalars(rad["scalars"], octa["scalars"] if octree else None)
+    if scal[2] != 0:  # levels beyond the fast walk: exact pointer doubling
         _lib.check(lib.ygg_levels_pointer_doubling(ctypes.byref(rptr), T, rws.data_ptr(),
                                                    rws.numel(), st), "levels")
-        scal = rad["scalars"].cpu()
-    num_levels = int(scal[0])
-    max_level = int(scal[1])
-    malformed = bool(int(scal[3])) or bool(int(scal[2]))
+        wide.update(_widen_index_arrays(
+            {k: rad[k] for k in ("node_level", "level_offsets", "nodes_by_level")}))
+        scal, _ = _read_scalars(rad["scalars"], None)
+    num_levels = scal[0]
+    max_level = scal[1]
+    malformed = bool(scal[3]) or bool(scal[2])
     if malformed and max_level >= 65536:
         raise RuntimeError("tree depth exceeds 65535 levels (cyclic parent links from "
                            "duplicate Morton codes)")
-    oct_num_levels = int(octa["scalars"].cpu()[0]) if octree else 0
+    oct_num_levels = oscal[0] if octree else 0
     return _assemble_tree(rad, octa if octree else None, perm, sorted_codes, b6, n, leaf_size,
                           num_levels, max_level, malformed, oct_num_levels, leaf_indices,
-                          leaf_position, pos_sorted, mass_sorted, inv, return_reordered)
+                          leaf_position, pos_sorted, mass_sorted, inv, return_reordered,
+                          wide=wide)
+
+
+def _read_scalars(radix_scalars: torch.Tensor, octree_scalars: Optional[torch.Tensor]):
+    """Both builds' int32 scalar words in one device-to-host copy."""
+    parts = [radix_scalars] + ([octree_scalars] if octree_scalars is not None else [])
+    host = torch.cat(parts).tolist() if len(parts) > 1 else radix_scalars.tolist()
+    nr = radix_scalars.numel()
+    return host[:nr], host[nr:]
 
 
 def _widen_index_arrays(arrays: dict) -> dict:
@@ -421,23 +435,14 @@ def _widen_index_arrays(arrays: dict) -> dict:
     return out
 
 
-def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, max_level,
-                   malformed, oct_num_levels, leaf_indices, leaf_position, pos_sorted,
-                   mass_sorted, inv, return_reordered):
-    """Public topology (reference field names and dtypes) and the kernels'
-    int32 view from the build's device arrays; also used for one system's
-    slices of a batched build."""
-    octree = octa is not None
-    dev = rad["parent"].device
-    L = int(rad["leaf_codes"].shape[0])
-    T = 2 * L - 1
-    NI = L - 1
-    idx = INDEX_DTYPE
+def _widen_fields(rad, octa, perm, inv) -> dict:
+    """INDEX_DTYPE copies of every index array of the public topology."""
+    NI = int(rad["leaf_codes"].shape[0]) - 1
     wide = dict(parent=rad["parent"], left_child=rad["left_child"],
                 right_child=rad["right_child"], particle_indices=perm,
                 node_ranges=rad["node_ranges"], node_level=rad["node_level"],
                 level_offsets=rad["level_offsets"], nodes_by_level=rad["nodes_by_level"])
-    if octree:
+    if octa is not None:
         wide.update({k: octa[k] for k in (
             "oct_parent", "oct_children", "oct_child_counts", "oct_node_depths",
             "oct_node_ranges", "oct_nodes_by_level", "oct_level_offsets", "oct_leaf_nodes",
@@ -445,7 +450,24 @@ def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, 
         wide["radix_leaf_to_oct"] = octa["radix_node_to_oct"][NI:]
     if inv is not None:
         wide["inverse_permutation"] = inv
-    wide = _widen_index_arrays(wide)
+    return _widen_index_arrays(wide)
+
+
+def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, max_level,
+                   malformed, oct_num_levels, leaf_indices, leaf_position, pos_sorted,
+                   mass_sorted, inv, return_reordered, wide: Optional[dict] = None):
+    """Public topology (reference field names and dtypes) and the kernels'
+    int32 view from the build's device arrays; also used for one system's
+    slices of a batched build. `wide` holds the already widened index
+    arrays (_widen_fields), computed here when absent."""
+    octree = octa is not None
+    dev = rad["parent"].device
+    L = int(rad["leaf_codes"].shape[0])
+    T = 2 * L - 1
+    NI = L - 1
+    idx = INDEX_DTYPE
+    if wide is None:
+        wide = _widen_fields(rad, octa, perm, inv)
     fields = dict(
         parent=wide["parent"],
         left_child=wide["left_child"],
