@@ -219,3 +219,51 @@ def test_descriptor_flat_walk_matches_callable_at_16k(case):
                        torch.sort(rows * T + hn.neighbors.long())[0])
     assert (res.accept_decisions, res.near_decisions) == (hres.accept_decisions,
                                                           hres.near_decisions)
+
+
+def all_near_policy(state, *, valid_pairs, mac_ok, different_nodes, target_leaf, source_leaf,
+                    same_node, target_nodes, source_nodes, center_target, center_source,
+                    dist_sq, extent_target, extent_source):
+    """Nothing accepted: every pair of different leaves is near."""
+    near = valid_pairs & target_leaf & source_leaf & different_nodes
+    return _where(near, NEAR, REFINE), _where(near, -1, -1)
+
+
+def test_descriptor_never_accept_gives_all_leaf_pairs_near():
+    """accept="never" with near leaf pairs: no far entries, and every leaf's
+    near row holds every other leaf (oracle policy walk with the same rule,
+    256 uniform points, leaf 8)."""
+    import yggdrax_amd as ygg
+    pos, mass = inputs.uniform_cube(256, 29)
+    ref = oracle.build_tree(pos, mass, leaf_size=8)
+    rgeom = oracle.compute_tree_geometry(ref, ref["positions_sorted"])
+    far, near, _ = O.walk_with_policy(ref, rgeom, 0.6, "dehnen", 1.0, all_near_policy, None)
+    tree, ps, _, _ = ygg.build_tree(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
+                                    leaf_size=8, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    desc = ygg.PairPolicyDescriptor(accept="never", near="leaf_pairs")
+    inter, nbrs = ygg.build_interactions_and_neighbors(tree, geom, theta=0.6,
+                                                       mac_type="dehnen", pair_policy=desc)
+    torch.cuda.synchronize()
+    L = nbrs.counts.numel()
+    print("never-accept near", nbrs.neighbors.numel(), "leaves", L, flush=True)
+    assert inter.sources.numel() == 0 and all(len(r) == 0 for r in far)
+    assert nbrs.neighbors.numel() == L * (L - 1)
+    noff, nb = parity.np_(nbrs.offsets), parity.np_(nbrs.neighbors)
+    for r in range(len(near)):
+        assert sorted(nb[noff[r]:noff[r + 1]].tolist()) == sorted(near[r])
+
+
+def test_descriptor_never_accept_never_near_is_empty():
+    import yggdrax_amd as ygg
+    pos, mass = inputs.uniform_cube(512, 31)
+    tree, ps, _, _ = ygg.build_tree(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
+                                    leaf_size=8, return_reordered=True)
+    geom = ygg.compute_tree_geometry(tree, ps)
+    desc = ygg.PairPolicyDescriptor(accept="never", near="never")
+    inter, nbrs, res = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.6, mac_type="dehnen", pair_policy=desc, return_result=True)
+    torch.cuda.synchronize()
+    assert inter.sources.numel() == 0 and nbrs.neighbors.numel() == 0
+    assert int(inter.counts.sum()) == 0 and int(nbrs.counts.sum()) == 0
+    assert res.accept_decisions == 0 and res.near_decisions == 0 and res.refine_decisions > 0
