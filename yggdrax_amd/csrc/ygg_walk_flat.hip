@@ -526,27 +526,34 @@ __global__ void k_b_far_level_offsets(Seg sg, const int64_t* __restrict__ off,
   out[i] = (OutT)(off[n0 + level_offsets[i]] - off[n0]);
 }
 
+// One wavefront per near row (batch leaf j): the neighbours of a leaf lie
+// in its own system, found once per row instead of once per entry (0.80 ms
+// against 1.75 ms for 1.6e8 entries in rows of ~400 over 1024 Plummer
+// systems, gfx950; far rows of ~73 entries gain nothing from this). Lane 0
+// writes the row's local offset; the first threads also write each
+// system's end offset.
 template <typename OutT>
-__global__ void k_b_near_localize(Seg sg, const int64_t* __restrict__ off, int64_t n,
-                                  OutT* __restrict__ neighbors, OutT* __restrict__ offsets_out) {
+__global__ __launch_bounds__(256) void k_b_near_localize(Seg sg, const int64_t* __restrict__ off,
+                                                         int64_t n, OutT* __restrict__ neighbors,
+                                                         OutT* __restrict__ offsets_out) {
   const int64_t Ltot = sg.leaf[sg.S];
   const int64_t total = off[Ltot] < n ? off[Ltot] : n;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = (int64_t)neighbors[e];
-    neighbors[e] = (OutT)(v - sg.node_off(sg.of_node(v)));
+  const unsigned lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / YGG_WAVE);
+  for (int64_t j = (int64_t)blockIdx.x * (blockDim.x / YGG_WAVE) + threadIdx.x / YGG_WAVE;
+       j < Ltot; j += nw) {
+    const int s = sg.of_leaf(j);
+    const int64_t n0 = sg.node_off(s);
+    const int64_t b = off[j] < total ? off[j] : total;
+    const int64_t e = off[j + 1] < total ? off[j + 1] : total;
+    for (int64_t k = b + lane; k < e; k += YGG_WAVE)
+      neighbors[k] = (OutT)((int64_t)neighbors[k] - n0);
+    if (lane == 0) offsets_out[j + s] = (OutT)(off[j] - off[sg.leaf_off(s)]);
   }
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < Ltot + sg.S;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    if (j < Ltot) {
-      const int s = sg.of_leaf(j);
-      offsets_out[j + s] = (OutT)(off[j] - off[sg.leaf_off(s)]);
-    } else {
-      const int s = (int)(j - Ltot);
-      offsets_out[sg.leaf_off(s + 1) + s] =
-          (OutT)(off[sg.leaf_off(s + 1)] - off[sg.leaf_off(s)]);
-    }
-  }
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < sg.S;
+       s += (int64_t)gridDim.x * blockDim.x)
+    offsets_out[sg.leaf_off((int)s + 1) + s] =
+        (OutT)(off[sg.leaf_off((int)s + 1)] - off[sg.leaf_off((int)s)]);
 }
 
 }  // namespace ygg
@@ -857,8 +864,9 @@ int ygg_batch_near_localize(const YggBatch* b, const int64_t* row_offsets, int64
   if (!b || b->num_systems < 1 || n < 0) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const Seg sg = seg_of(b);
-  const int64_t work = (n > b->num_leaves + b->num_systems) ? n : b->num_leaves + b->num_systems;
-  dim3 g(grid_for(work, 256, 16384));
+  // one wavefront per leaf row; the grid also covers the system end offsets
+  const int64_t rows = b->num_leaves > 1 ? b->num_leaves : 1;
+  dim3 g(grid_for(rows, 256 / YGG_WAVE, 16384));
   if (out_i64)
     hipLaunchKernelGGL(k_b_near_localize<int64_t>, g, dim3(256), 0, st, sg, row_offsets, n,
                        (int64_t*)neighbors, (int64_t*)near_offsets_out);
