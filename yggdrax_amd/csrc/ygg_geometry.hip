@@ -156,6 +156,10 @@ __global__ __launch_bounds__(256) void k_seg_query(const Box8* __restrict__ seg,
 
 // Batched: one wave per segment-tree leaf slot j < P over the batch's
 // concatenated leaves (system of j by the leaf offsets).
+// LANES lanes per leaf: 32 when leaves hold at most 32 points, so a
+// wavefront reduces two leaves at once (every shuffle stays within its
+// half), else 64.
+template <int LANES>
 __global__ __launch_bounds__(256) void k_b_seg_leaves(Seg sg, const float* __restrict__ pos,
                                                       const int32_t* __restrict__ ranges,
                                                       int64_t Ltot, int leaf_cap, int64_t P,
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(256) void k_b_seg_leaves(Seg sg, const float* __res
                                                       float* __restrict__ half,
                                                       float* __restrict__ radius,
                                                       float* __restrict__ maxext) {
-  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE;
-  const unsigned lane = lane_id();
+  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+  const unsigned lane = lane_id() % LANES;
   if (j >= P) return;
   if (j >= Ltot) {
     if (lane == 0) seg[P + j] = empty_box();
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void k_b_seg_leaves(Seg sg, const float* __res
   const int cnt = min(ranges[2 * node + 1] - ranges[2 * node] + 1, leaf_cap);
   float lo[3] = {INFINITY, INFINITY, INFINITY};
   float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int m = lane; m < cnt; m += YGG_WAVE) {
+  for (int m = lane; m < cnt; m += LANES) {
     const float* p = pos + 3 * (a + m);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(256) void k_b_seg_leaves(Seg sg, const float* __res
     }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int off = LANES / 2; off > 0; off >>= 1) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       lo[d] = fminf(lo[d], __shfl_xor(lo[d], off, 64));
@@ -455,9 +459,14 @@ int ygg_batch_geometry(const YggBatch* b, const float* pos_sorted, const int32_t
   Box8* seg = a.take<Box8>((size_t)(2 * P));
   if (!seg) return YGG_ERR_WORKSPACE;
   const int lc = (int)(leaf_cap > 0x7FFFFFFF ? 0x7FFFFFFF : leaf_cap);
-  hipLaunchKernelGGL(k_b_seg_leaves, dim3(grid_for(P * YGG_WAVE, 256)), dim3(256), 0, st, sg,
-                     pos_sorted, node_ranges, Ltot, lc, P, seg, center, half_extent, radius,
-                     max_extent);
+  if (lc <= 32)
+    hipLaunchKernelGGL(k_b_seg_leaves<32>, dim3(grid_for(P * 32, 256)), dim3(256), 0, st, sg,
+                       pos_sorted, node_ranges, Ltot, lc, P, seg, center, half_extent, radius,
+                       max_extent);
+  else
+    hipLaunchKernelGGL(k_b_seg_leaves<64>, dim3(grid_for(P * YGG_WAVE, 256)), dim3(256), 0, st,
+                       sg, pos_sorted, node_ranges, Ltot, lc, P, seg, center, half_extent, radius,
+                       max_extent);
   for (int64_t S = P; S > 1; S /= 512) {
     int blocks = (int)((S / 2 + 255) / 256);
     hipLaunchKernelGGL(k_seg_up, dim3(blocks), dim3(256), 0, st, seg, (int)S);
