@@ -502,7 +502,7 @@ def build_interactions_and_neighbors_batched(trees: BatchedTrees, geom: BatchedG
     dt = ctx.dt
     lad = _ladder(dt, mac_type, max_interactions_per_node=None, max_neighbors_per_leaf=2048,
                   max_pair_queue=None, traversal_config=None)
-    fr = _run_walk_flat(ctx, dt, lad, sizes=capacity_hint, strict_int32=False)
+    fr = _run_walk_flat(ctx, dt, lad, sizes=capacity_hint, strict_int32=False, need_max=False)
     out = _lists_from_streams(trees, _LayoutIndex(lay, geom.center.device), fr.far_keys,
                               fr.far_vals, fr.near_vals, fr.far_off, fr.near_off,
                               fr.stats["far_total"], fr.stats["near_total"], fr.stats)

@@ -650,11 +650,14 @@ def _emulate_ladder(lad: _Ladder, peak: int, max_far: int, max_near: int):
 
 
 def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
-                   sizes: Optional[dict] = None, strict_int32: bool = True
-                   ) -> Optional[_FlatResult]:
+                   sizes: Optional[dict] = None, strict_int32: bool = True,
+                   need_max: bool = True) -> Optional[_FlatResult]:
     """Full walk with flat emission. `sizes` (queue, far, near capacities)
     skips the counting walk; on any stream overflow the counting walk runs
-    and the emission walk is repeated at exact sizes."""
+    and the emission walk is repeated at exact sizes. The largest far and
+    near row counts (one host read after the sorts) are taken when
+    `need_max`, or when the int32 guard or the retry-ladder emulation needs
+    them; otherwise stats hold None for them."""
     T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
     qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
     from . import autotune
@@ -694,8 +697,10 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     nf, nn = em["far_total"], em["near_total"]
     (fk, fv, far_off), (_, nv, near_off) = _sort_far_and_near(
         em["far_keys"], em["far_vals"], nf, T, em["near_keys"], em["near_vals"], nn, L)
-    mx = torch.stack([torch.diff(far_off).max(), torch.diff(near_off).max()]).cpu()
-    max_far, max_near = int(mx[0]), int(mx[1])
+    max_far = max_near = None
+    if need_max or strict_int32 or not lad.use_count:
+        mx = torch.stack([torch.diff(far_off).max(), torch.diff(near_off).max()]).tolist()
+        max_far, max_near = int(mx[0]), int(mx[1])
     stats = dict(peak=em["peak"], gen_sizes=em["gen_sizes"], decisions=em["decisions"],
                  max_far=max_far, max_near=max_near, far_total=nf, near_total=nn)
     if lad.use_count:
@@ -1031,7 +1036,7 @@ def build_interactions_and_neighbors(
         fr = None
         if retry_logger is None and not EXACT_ATTEMPTS:
             fr = _run_walk_flat(ctx, dt, _ladder(dt, mac_type, **kw), sizes=capacity_hint,
-                                strict_int32=strict_int32)
+                                strict_int32=strict_int32, need_max=neighbor_positions)
         if fr is not None:
             inter, nbrs = _flat_outputs(fr, dt, topo, neighbor_positions)
             raw = fr
