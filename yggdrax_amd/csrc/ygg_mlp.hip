@@ -35,46 +35,115 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
   return (uint16_t)(r >> 16);
 }
 
-// ---- inference: one thread per sample, weights staged once per block in LDS.
+// ---- inference on the matrix cores --------------------------------------
+// Each wavefront runs 16 samples at a time through the four layers with
+// 16x16x16 bf16 MFMAs (fp32 accumulate). Operand layouts of
+// v_mfma_f32_16x16x16_bf16 on wave64 (lane l, j = 0..3):
+//   A[m][k]: lane l holds A[l % 16][4 (l / 16) + j]
+//   B[k][n]: lane l holds B[4 (l / 16) + j][l % 16]
+//   D[m][n]: lane l holds D[4 (l / 16) + j][l % 16]
+// With A = a 16-output slice of W (m = output, k = input) and B = the
+// activations (k = input feature, n = sample), a layer's D tile t is, lane
+// for lane, the B operand of k-step t of the next layer: no data moves
+// between layers. The weights are exactly bf16; each fp32 activation is
+// split into bf16 hi + lo (round to nearest even, twice), and both halves
+// are multiplied, so the products keep ~2^-17 relative accuracy and the
+// outputs match fp32 arithmetic to ~1e-6. Weight fragments live in
+// registers for the whole kernel.
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_bf16(const f32x4_t& v, bf16x4_t& hi, bf16x4_t& lo) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint16_t h = f32_to_bf16(v[j]);
+    hi[j] = (short)h;
+    lo[j] = (short)f32_to_bf16(v[j] - bf16_to_f32(h));
+  }
+}
+
+__device__ __forceinline__ f32x4_t mfma_split(const bf16x4_t& a, const f32x4_t& act,
+                                              f32x4_t acc) {
+  bf16x4_t hi, lo;
+  split_bf16(act, hi, lo);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, hi, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, lo, acc, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4_t relu4(f32x4_t v) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = fmaxf(v[j], 0.0f);
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ wbf,
                                                    const float* __restrict__ x, int64_t n,
                                                    float* __restrict__ y) {
-  __shared__ float w[kParams];
-  for (int i = threadIdx.x; i < kParams; i += blockDim.x) w[i] = bf16_to_f32(wbf[i]);
-  __syncthreads();
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    float in[kIn];
+  const unsigned l = lane_id();
+  const int r = (int)(l % 16), q = (int)(l / 16);
+  auto w = [&](int off) { return (short)wbf[off]; };
+  auto wf = [&](int off) { return bf16_to_f32(wbf[off]); };
+  bf16x4_t a1[4], a2[4][4], a3[4][4], a4[4];
+  f32x4_t b1[4], b2[4], b3[4], b4;
 #pragma unroll
-    for (int i = 0; i < kIn; ++i) in[i] = x[s * kIn + i];
-    float h1[kH], h2[kH];
+  for (int t = 0; t < 4; ++t) {
 #pragma unroll
-    for (int o = 0; o < kH; ++o) {
-      float acc = w[kOffB1 + o];
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * q + j;
+      a1[t][j] = k < kIn ? w(kOffW1 + (16 * t + r) * kIn + k) : (short)0;
+      b1[t][j] = wf(kOffB1 + 16 * t + k);
+      b2[t][j] = wf(kOffB2 + 16 * t + k);
+      b3[t][j] = wf(kOffB3 + 16 * t + k);
 #pragma unroll
-      for (int i = 0; i < kIn; ++i) acc += w[kOffW1 + o * kIn + i] * in[i];
-      h1[o] = fmaxf(acc, 0.0f);
+      for (int kk = 0; kk < 4; ++kk) {
+        a2[t][kk][j] = w(kOffW2 + (16 * t + r) * kH + 16 * kk + k);
+        a3[t][kk][j] = w(kOffW3 + (16 * t + r) * kH + 16 * kk + k);
+      }
+    }
+  }
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a4[kk][j] = r < kOut ? w(kOffW4 + r * kH + 16 * kk + 4 * q + j) : (short)0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b4[j] = 4 * q + j < kOut ? wf(kOffB4 + 4 * q + j) : 0.0f;
+
+  // wave-uniform loop over 16-sample tiles (every lane reaches every MFMA)
+  const int64_t tiles = (n + 15) / 16;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / YGG_WAVE);
+  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x / YGG_WAVE) + threadIdx.x / YGG_WAVE;
+       tile < tiles; tile += nw) {
+    const int64_t smp = tile * 16 + r;
+    const bool ok = smp < n;
+    f32x4_t in;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * q + j;
+      in[j] = (ok && k < kIn) ? x[smp * kIn + k] : 0.0f;
+    }
+    f32x4_t h1[4], h2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) h1[t] = relu4(mfma_split(a1[t], in, b1[t]));
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4_t acc = b2[t];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) acc = mfma_split(a2[t][kk], h1[kk], acc);
+      h2[t] = relu4(acc);
     }
 #pragma unroll
-    for (int o = 0; o < kH; ++o) {
-      float acc = w[kOffB2 + o];
+    for (int t = 0; t < 4; ++t) {
+      f32x4_t acc = b3[t];
 #pragma unroll
-      for (int i = 0; i < kH; ++i) acc += w[kOffW2 + o * kH + i] * h1[i];
-      h2[o] = fmaxf(acc, 0.0f);
+      for (int kk = 0; kk < 4; ++kk) acc = mfma_split(a3[t][kk], h2[kk], acc);
+      h1[t] = relu4(acc);
     }
+    f32x4_t out = b4;
 #pragma unroll
-    for (int o = 0; o < kH; ++o) {
-      float acc = w[kOffB3 + o];
+    for (int kk = 0; kk < 4; ++kk) out = mfma_split(a4[kk], h1[kk], out);
+    if (q == 0 && ok) {
 #pragma unroll
-      for (int i = 0; i < kH; ++i) acc += w[kOffW3 + o * kH + i] * h2[i];
-      h1[o] = fmaxf(acc, 0.0f);
-    }
-#pragma unroll
-    for (int o = 0; o < kOut; ++o) {
-      float acc = w[kOffB4 + o];
-#pragma unroll
-      for (int i = 0; i < kH; ++i) acc += w[kOffW4 + o * kH + i] * h1[i];
-      y[s * kOut + o] = acc;
+      for (int j = 0; j < kOut; ++j) y[smp * kOut + j] = out[j];
     }
   }
 }
@@ -302,8 +371,9 @@ int ygg_mlp_forward(const uint16_t* params_bf16, const float* features, int64_t 
                     float* outputs, void* stream) {
   if (n < 0 || !params_bf16) return YGG_ERR_ARG;
   if (n == 0) return YGG_OK;
-  hipLaunchKernelGGL(k_mlp_infer, dim3(grid_for(n, 256, 4096)), dim3(256), 0,
-                     (hipStream_t)stream, params_bf16, features, n, outputs);
+  // one wavefront per 16-sample tile, four per workgroup
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid_for((n + 15) / 16, 256 / YGG_WAVE, 4096)), dim3(256),
+                     0, (hipStream_t)stream, params_bf16, features, n, outputs);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
