@@ -77,6 +77,19 @@ int ygg_morton_sort_reorder(const float* pos, const float* mass, int64_t n, cons
                             uint64_t* sorted_codes, uint32_t* perm, float* pos_sorted,
                             float* mass_sorted, void* inverse, int inverse_i64, void* ws,
                             size_t ws_bytes, void* stream);
+/* The same outputs, with the gather and the inverse on a second stream:
+ * the encode and the code sort run on `stream`; `reorder_stream` waits for
+ * them (a HIP event) and then writes pos_sorted, mass_sorted and inverse.
+ * Work queued on `stream` after this call (the radix build, which reads
+ * only the sorted codes and perm) overlaps the reorder. The caller orders
+ * every reader of the reordered outputs after `reorder_stream` and keeps
+ * `ws` and `perm` alive until `reorder_stream` passes this point. Same
+ * workspace as ygg_morton_sort_reorder. */
+int ygg_morton_sort_reorder_async(const float* pos, const float* mass, int64_t n,
+                                  const float* bounds6, uint64_t* sorted_codes, uint32_t* perm,
+                                  float* pos_sorted, float* mass_sorted, void* inverse,
+                                  int inverse_i64, void* ws, size_t ws_bytes, void* stream,
+                                  void* reorder_stream);
 
 /* Gathers of the Morton-sorted particles (return_reordered of
  * yggdrax/_tree_impl.py:1045-1050): pos_sorted[i] = pos[perm[i]],

@@ -123,6 +123,8 @@ _SIGS = {
     "ygg_morton_sort": (_INT, [_P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "ygg_morton_sort_reorder_workspace_bytes": (_SZ, [_I64]),
     "ygg_morton_sort_reorder": (_INT, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _INT, _P, _SZ, _P]),
+    "ygg_morton_sort_reorder_async": (_INT, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _INT, _P, _SZ,
+                                             _P, _P]),
     "ygg_reorder": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _P]),
     "ygg_reorder_scatter": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _P]),
     "ygg_radix_build_workspace_bytes": (_SZ, [_I64, _I64]),

@@ -680,12 +680,11 @@ size_t ygg_morton_sort_reorder_workspace_bytes(int64_t n) {
   return align_up(sizeof(float4) * (size_t)n) + ygg_morton_sort_workspace_bytes(n);
 }
 
-int ygg_morton_sort_reorder(const float* pos, const float* mass, int64_t n, const float* bounds6,
-                            uint64_t* sorted_codes, uint32_t* perm, float* pos_sorted,
-                            float* mass_sorted, void* inverse, int inverse_i64, void* ws,
-                            size_t ws_bytes, void* stream) {
-  if (n < 1 || !pos || !pos_sorted) return YGG_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
+// Encode + histograms + pack, then the code sort, on `st`; returns the
+// packed records' address in the workspace through *packed_out.
+static int morton_sort_pack(const float* pos, const float* mass, int64_t n, const float* bounds6,
+                            uint64_t* sorted_codes, uint32_t* perm, void* ws, size_t ws_bytes,
+                            hipStream_t st, const float4** packed_out) {
   Arena a(ws, ws_bytes);
   float4* packed = a.take<float4>((size_t)n);
   uint64_t* codes = a.take<uint64_t>((size_t)n);
@@ -700,8 +699,15 @@ int ygg_morton_sort_reorder(const float* pos, const float* mass, int64_t n, cons
   YGG_LAUNCH_CHECK();
   if ((rc = sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, 64, sws, rest, st, hist)))
     return rc;
+  *packed_out = packed;
+  return YGG_OK;
+}
+
+static int reorder_from_packed(const float4* packed, const uint32_t* perm, int64_t n,
+                               float* pos_sorted, float* mass_sorted, void* inverse,
+                               int inverse_i64, hipStream_t st) {
   hipLaunchKernelGGL(k_reorder_packed, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, packed,
-                     perm, n, pos_sorted, mass ? mass_sorted : nullptr);
+                     perm, n, pos_sorted, mass_sorted);
   if (inverse && inverse_i64)
     hipLaunchKernelGGL(k_inverse_perm<int64_t>, dim3(grid_for(n, 256)), dim3(256), 0, st, perm, n,
                        (int64_t*)inverse);
@@ -710,6 +716,39 @@ int ygg_morton_sort_reorder(const float* pos, const float* mass, int64_t n, cons
                        (int32_t*)inverse);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
+}
+
+int ygg_morton_sort_reorder(const float* pos, const float* mass, int64_t n, const float* bounds6,
+                            uint64_t* sorted_codes, uint32_t* perm, float* pos_sorted,
+                            float* mass_sorted, void* inverse, int inverse_i64, void* ws,
+                            size_t ws_bytes, void* stream) {
+  if (n < 1 || !pos || !pos_sorted) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const float4* packed = nullptr;
+  int rc = morton_sort_pack(pos, mass, n, bounds6, sorted_codes, perm, ws, ws_bytes, st, &packed);
+  if (rc) return rc;
+  return reorder_from_packed(packed, perm, n, pos_sorted, mass ? mass_sorted : nullptr, inverse,
+                             inverse_i64, st);
+}
+
+int ygg_morton_sort_reorder_async(const float* pos, const float* mass, int64_t n,
+                                  const float* bounds6, uint64_t* sorted_codes, uint32_t* perm,
+                                  float* pos_sorted, float* mass_sorted, void* inverse,
+                                  int inverse_i64, void* ws, size_t ws_bytes, void* stream,
+                                  void* reorder_stream) {
+  if (n < 1 || !pos || !pos_sorted || !reorder_stream) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream, rst = (hipStream_t)reorder_stream;
+  const float4* packed = nullptr;
+  int rc = morton_sort_pack(pos, mass, n, bounds6, sorted_codes, perm, ws, ws_bytes, st, &packed);
+  if (rc) return rc;
+  hipEvent_t sorted;
+  YGG_CHECK(hipEventCreateWithFlags(&sorted, hipEventDisableTiming));
+  const hipError_t e1 = hipEventRecord(sorted, st);
+  const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(rst, sorted, 0) : e1;
+  (void)hipEventDestroy(sorted);  // released once the recorded work completes
+  if (e2 != hipSuccess) return YGG_ERR_HIP;
+  return reorder_from_packed(packed, perm, n, pos_sorted, mass ? mass_sorted : nullptr, inverse,
+                             inverse_i64, rst);
 }
 
 int ygg_widen_i32_to_i64(const YggWidenItem* items, int count, void* stream) {

@@ -332,16 +332,23 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
     pos_sorted = mass_sorted = inv = m = None
     if masses is not None:
         m = torch.as_tensor(masses, dtype=torch.float32, device=dev).contiguous()
+    reorder_side = None
     if return_reordered and not REORDER_SCATTER:
-        # sort and gather in one call (packed (x, y, z, m) records)
+        # sort on this stream; the gather of packed (x, y, z, m) records and
+        # the inverse run on a side stream, under the radix build below,
+        # which needs only the sorted codes
+        from .interactions import _side_stream
         pos_sorted = torch.empty_like(pos)
         mass_sorted = torch.empty_like(m) if m is not None else None
         inv = torch.empty(n, dtype=INDEX_DTYPE, device=dev)
         ws = _lib.workspace(lib.ygg_morton_sort_reorder_workspace_bytes(n), dev)
-        _lib.check(lib.ygg_morton_sort_reorder(
+        reorder_side = _side_stream(dev)
+        _lib.check(lib.ygg_morton_sort_reorder_async(
             pos.data_ptr(), _lib.ptr(m), n, b6.data_ptr(), sorted_codes.data_ptr(),
             perm.data_ptr(), pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(),
-            int(INDEX_DTYPE == torch.int64), ws.data_ptr(), ws.numel(), st), "morton_sort")
+            int(INDEX_DTYPE == torch.int64), ws.data_ptr(), ws.numel(), st,
+            reorder_side.cuda_stream), "morton_sort")
+        ws.record_stream(reorder_side)  # the side stream still reads the packed records
     else:
         ws = _lib.workspace(lib.ygg_morton_sort_workspace_bytes(n), dev)
         _lib.check(lib.ygg_morton_sort(pos.data_ptr(), n, b6.data_ptr(), sorted_codes.data_ptr(),
@@ -377,6 +384,8 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
         _lib.check(reorder(pos.data_ptr(), _lib.ptr(m), perm.data_ptr(), n,
                            pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(), st),
                    "reorder")
+    if reorder_side is not None:  # reordered particles and inverse are ready after this
+        torch.cuda.current_stream(dev).wait_stream(reorder_side)
     # The int64 index arrays are widened before the host waits for the
     # build, so that launch is queued while the scalars are read.
     wide = _widen_fields(rad, octa if octree else None, perm, inv)
