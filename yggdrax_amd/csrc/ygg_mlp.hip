@@ -46,19 +46,24 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 // activations (k = input feature, n = sample), a layer's D tile t is, lane
 // for lane, the B operand of k-step t of the next layer: no data moves
 // between layers. The weights are exactly bf16; each fp32 activation is
-// split into bf16 hi + lo (round to nearest even, twice), and both halves
-// are multiplied, so the products keep ~2^-17 relative accuracy and the
-// outputs match fp32 arithmetic to ~1e-6. Weight fragments live in
-// registers for the whole kernel.
+// split into bf16 hi + lo (split_bf16) and both halves are multiplied, so
+// the products keep ~2^-16 relative accuracy and the outputs match fp32
+// arithmetic to ~1e-5. Weight fragments live in registers for the whole
+// kernel.
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+#ifndef MLP_INFER_BLOCKS
+#define MLP_INFER_BLOCKS 256
+#endif
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
+// hi = v truncated to bf16, so v - hi is exact in fp32; lo = that remainder
+// rounded to nearest even. |v - hi - lo| <= 2^-9 |v - hi| < 2^-16 |v|.
 __device__ __forceinline__ void split_bf16(const f32x4_t& v, bf16x4_t& hi, bf16x4_t& lo) {
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const uint16_t h = f32_to_bf16(v[j]);
-    hi[j] = (short)h;
-    lo[j] = (short)f32_to_bf16(v[j] - bf16_to_f32(h));
+    const uint32_t u = __float_as_uint(v[j]);
+    hi[j] = (short)(u >> 16);
+    lo[j] = (short)f32_to_bf16(v[j] - __uint_as_float(u & 0xFFFF0000u));
   }
 }
 
@@ -108,19 +113,28 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ 
 #pragma unroll
   for (int j = 0; j < 4; ++j) b4[j] = 4 * q + j < kOut ? wf(kOffB4 + 4 * q + j) : 0.0f;
 
-  // wave-uniform loop over 16-sample tiles (every lane reaches every MFMA)
+  // Wave-uniform loop over 16-sample tiles (every lane reaches every
+  // MFMA). At one wavefront per SIMD nothing else hides a load, so the next
+  // tile's features are loaded while this tile runs.
   const int64_t tiles = (n + 15) / 16;
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x / YGG_WAVE);
-  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x / YGG_WAVE) + threadIdx.x / YGG_WAVE;
-       tile < tiles; tile += nw) {
-    const int64_t smp = tile * 16 + r;
-    const bool ok = smp < n;
-    f32x4_t in;
+  auto load_tile = [&](int64_t tl) {
+    f32x4_t v;
+    const int64_t sm = tl * 16 + r;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = 4 * q + j;
-      in[j] = (ok && k < kIn) ? x[smp * kIn + k] : 0.0f;
+      v[j] = (sm < n && k < kIn) ? x[sm * kIn + k] : 0.0f;
     }
+    return v;
+  };
+  int64_t tile = (int64_t)blockIdx.x * (blockDim.x / YGG_WAVE) + threadIdx.x / YGG_WAVE;
+  f32x4_t in_next = load_tile(tile);
+  for (; tile < tiles; tile += nw) {
+    const int64_t smp = tile * 16 + r;
+    const bool ok = smp < n;
+    const f32x4_t in = in_next;
+    if (tile + nw < tiles) in_next = load_tile(tile + nw);
     f32x4_t h1[4], h2[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) h1[t] = relu4(mfma_split(a1[t], in, b1[t]));
@@ -371,9 +385,11 @@ int ygg_mlp_forward(const uint16_t* params_bf16, const float* features, int64_t 
                     float* outputs, void* stream) {
   if (n < 0 || !params_bf16) return YGG_ERR_ARG;
   if (n == 0) return YGG_OK;
-  // one wavefront per 16-sample tile, four per workgroup
-  hipLaunchKernelGGL(k_mlp_infer, dim3(grid_for((n + 15) / 16, 256 / YGG_WAVE, 4096)), dim3(256),
-                     0, (hipStream_t)stream, params_bf16, features, n, outputs);
+  // 16-sample tiles, four wavefronts per workgroup, at most 256 workgroups:
+  // one wavefront per SIMD (the kernel's occupancy), each loading the weight
+  // fragments once and then looping over many tiles
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid_for((n + 15) / 16, 256 / YGG_WAVE, MLP_INFER_BLOCKS)),
+                     dim3(256), 0, (hipStream_t)stream, params_bf16, features, n, outputs);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
