@@ -53,7 +53,9 @@ __global__ __launch_bounds__(256) void k_b_encode_hist(Seg sg, const float* __re
                                                        uint64_t* __restrict__ codes,
                                                        uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[8 * 256];
+  __shared__ int32_t s_pt[kSegLdsCap];
   for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) h[i] = 0;
+  const bool staged = seg_stage(s_pt, sg.S, [&](int s) { return (int32_t)sg.pt_off(s); });
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < N; b0 += stride) {
@@ -61,7 +63,7 @@ __global__ __launch_bounds__(256) void k_b_encode_hist(Seg sg, const float* __re
     const bool valid = i < N;
     uint64_t c = 0;
     if (valid) {
-      const int s = sg.of_point(i);
+      const int s = staged ? seg_find_lds(s_pt, sg.S, i) : sg.of_point(i);
       const float* b = b6 + 6 * s;
       float lo[3] = {b[0], b[1], b[2]};
       float w[3] = {b[3] - b[0], b[4] - b[1], b[5] - b[2]};
@@ -77,11 +79,16 @@ __global__ __launch_bounds__(256) void k_b_encode_hist(Seg sg, const float* __re
 }
 
 // key[i] = system of the point at sorted position i (a point index).
-__global__ void k_b_point_system(Seg sg, const uint32_t* __restrict__ idx, int64_t N,
-                                 uint32_t* __restrict__ key) {
+__global__ __launch_bounds__(256) void k_b_point_system(Seg sg, const uint32_t* __restrict__ idx,
+                                                        int64_t N, uint32_t* __restrict__ key) {
+  __shared__ int32_t s_pt[kSegLdsCap];
+  const bool staged = seg_stage(s_pt, sg.S, [&](int s) { return (int32_t)sg.pt_off(s); });
+  __syncthreads();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x)
-    key[i] = (uint32_t)sg.of_point(idx[i]);
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = idx[i];
+    key[i] = (uint32_t)(staged ? seg_find_lds(s_pt, sg.S, g) : sg.of_point(g));
+  }
 }
 
 __global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ idx,
@@ -563,7 +570,7 @@ int ygg_batch_morton_sort(const YggBatch* b, const float* pos, const float* boun
     if ((rc = copy_u32((uint32_t*)sorted_codes, (const uint32_t*)c1, 2 * N, st))) return rc;
     return copy_u32(perm, i1, N, st);
   }
-  hipLaunchKernelGGL(k_b_point_system, dim3(grid_for(N, 256, 8192)), dim3(256), 0, st, sg, i1, N,
+  hipLaunchKernelGGL(k_b_point_system, dim3(grid_for(N, 256, 4096)), dim3(256), 0, st, sg, i1, N,
                      sk);
   YGG_LAUNCH_CHECK();
   if ((rc = sort_pairs_u32(sk, i1, sk2, perm, N, 0, bits_for(b->num_systems - 1), sws, rest, st)))

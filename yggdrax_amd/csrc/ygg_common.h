@@ -240,6 +240,30 @@ struct Seg {
   }
 };
 
+// Block-local copy of one per-system offset table (entries 0..S-1) for
+// binary searches in LDS; kernels stage it when S <= kSegLdsCap and search
+// the global table otherwise. Same result as Seg::find.
+constexpr int kSegLdsCap = 4096;
+
+template <class F>
+__device__ __forceinline__ bool seg_stage(int32_t* s_tab, int S, F off) {
+  if (S > kSegLdsCap) return false;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) s_tab[i] = off(i);
+  return true;
+}
+
+__device__ __forceinline__ int seg_find_lds(const int32_t* s_tab, int S, int64_t g) {
+  int lo = 0, hi = S - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int64_t)s_tab[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---- primitives implemented in ygg_sort.hip --------------------------------
 // Stable LSD radix sort of (key, u32 value) pairs on key bits [begin, end)
 // with digit_bits-wide digits (8..11; 0 = default_digit_bits).

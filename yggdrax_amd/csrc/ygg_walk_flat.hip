@@ -483,15 +483,21 @@ __global__ __launch_bounds__(256) void k_near_dense(int L, const int64_t* __rest
 }
 
 // ---- batched (forest) lists -> per-system lists ---------------------------------
+// Per far entry, the target's system (largest s with node_off(s) <= t) by a
+// binary search over the node offsets, staged in LDS (seg_stage).
 template <typename OutT>
-__global__ void k_b_far_localize_entries(Seg sg, const int64_t* __restrict__ off, int64_t rows,
-                                         int64_t n, OutT* __restrict__ sources,
-                                         OutT* __restrict__ targets) {
+__global__ __launch_bounds__(256) void k_b_far_localize_entries(
+    Seg sg, const int64_t* __restrict__ off, int64_t rows, int64_t n,
+    OutT* __restrict__ sources, OutT* __restrict__ targets) {
+  __shared__ int32_t s_noff[kSegLdsCap];
+  const bool staged = seg_stage(s_noff, sg.S, [&](int s) { return (int32_t)sg.node_off(s); });
+  __syncthreads();
   const int64_t total = off[rows] < n ? off[rows] : n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t t = (int64_t)targets[e];
-    const int64_t n0 = sg.node_off(sg.of_node(t));
+    const int64_t n0 = staged ? s_noff[seg_find_lds(s_noff, sg.S, t)]
+                              : sg.node_off(sg.of_node(t));
     targets[e] = (OutT)(t - n0);
     sources[e] = (OutT)((int64_t)sources[e] - n0);
   }
@@ -836,7 +842,8 @@ int ygg_batch_far_localize(const YggBatch* b, const int64_t* row_offsets,
   const Seg sg = seg_of(b);
   const int64_t T = 2 * b->num_leaves - b->num_systems;
   const int64_t rows = T > b->num_systems ? T : b->num_systems;
-  dim3 ge(grid_for(n > 0 ? n : 1, 256, 16384)), gr(grid_for(rows, 256)),
+  // entries: grid-stride over 2048 workgroups, each staging the node offsets once
+  dim3 ge(grid_for(n > 0 ? n : 1, 256, 2048)), gr(grid_for(rows, 256)),
       gl(grid_for(65 * (int64_t)b->num_systems, 256));
   if (out_i64) {
     if (n > 0)
