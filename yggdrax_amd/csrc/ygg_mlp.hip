@@ -67,14 +67,6 @@ __device__ __forceinline__ void split_bf16(const f32x4_t& v, bf16x4_t& hi, bf16x
   }
 }
 
-__device__ __forceinline__ f32x4_t mfma_split(const bf16x4_t& a, const f32x4_t& act,
-                                              f32x4_t acc) {
-  bf16x4_t hi, lo;
-  split_bf16(act, hi, lo);
-  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, hi, acc, 0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, lo, acc, 0, 0, 0);
-}
-
 __device__ __forceinline__ f32x4_t relu4(f32x4_t v) {
 #pragma unroll
   for (int j = 0; j < 4; ++j) v[j] = fmaxf(v[j], 0.0f);
@@ -135,26 +127,47 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ 
     const bool ok = smp < n;
     const f32x4_t in = in_next;
     if (tile + nw < tiles) in_next = load_tile(tile + nw);
-    f32x4_t h1[4], h2[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) h1[t] = relu4(mfma_split(a1[t], in, b1[t]));
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      f32x4_t acc = b2[t];
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) acc = mfma_split(a2[t][kk], h1[kk], acc);
-      h2[t] = relu4(acc);
-    }
+    // Every activation tile is split once and feeds the four output tiles;
+    // k-steps run outermost, so consecutive MFMAs write different
+    // accumulators and pipeline.
+    bf16x4_t hi[4], lo[4];
+    f32x4_t acc[4];
+    split_bf16(in, hi[0], lo[0]);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      f32x4_t acc = b3[t];
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) acc = mfma_split(a3[t][kk], h2[kk], acc);
-      h1[t] = relu4(acc);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t], hi[0], b1[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t], lo[0], acc[t], 0, 0, 0);
     }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) split_bf16(relu4(acc[kk]), hi[kk], lo[kk]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = b2[t];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t][kk], hi[kk], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t][kk], lo[kk], acc[t], 0, 0, 0);
+      }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) split_bf16(relu4(acc[kk]), hi[kk], lo[kk]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = b3[t];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[t][kk], hi[kk], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[t][kk], lo[kk], acc[t], 0, 0, 0);
+      }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) split_bf16(relu4(acc[kk]), hi[kk], lo[kk]);
     f32x4_t out = b4;
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) out = mfma_split(a4[kk], h1[kk], out);
+    for (int kk = 0; kk < 4; ++kk) {
+      out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a4[kk], hi[kk], out, 0, 0, 0);
+      out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a4[kk], lo[kk], out, 0, 0, 0);
+    }
     if (q == 0 && ok) {
 #pragma unroll
       for (int j = 0; j < kOut; ++j) y[smp * kOut + j] = out[j];
