@@ -21,13 +21,25 @@ __global__ __launch_bounds__(256) void k_b_bounds(Seg sg, const float* __restric
   const int s = blockIdx.x;
   const int64_t p0 = sg.pt_off(s), n = sg.n(s);
   uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+  // four points per thread and step, all loaded before any is reduced, so
+  // one workgroup keeps several loads in flight on a large system
+  constexpr int kU = 4;
+  for (int64_t i0 = threadIdx.x; i0 < n; i0 += kU * (int64_t)blockDim.x) {
+    float v[kU][3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      uint32_t o = f2ord(pos[3 * (p0 + i) + d]);
-      mn[d] = min(mn[d], o);
-      mx[d] = max(mx[d], o);
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = i0 + u * (int64_t)blockDim.x;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[u][d] = i < n ? pos[3 * (p0 + i) + d] : pos[3 * p0 + d];
     }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const uint32_t o = f2ord(v[u][d]);
+        mn[d] = min(mn[d], o);
+        mx[d] = max(mx[d], o);
+      }
   }
   __shared__ uint32_t smn[3], smx[3];
   if (threadIdx.x < 3) {
