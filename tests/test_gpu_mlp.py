@@ -24,10 +24,14 @@ def _batch(B, seed):
     return x, t, m
 
 
-def test_mlp_forward_matches_reference():
+@pytest.mark.parametrize("B", [1000, 3 * 1024 * 16 + 5])
+def test_mlp_forward_matches_reference(B):
+    """MFMA inference against the float32 reference. 1000 samples end in a
+    partial 16-sample tile; 49157 samples give each of the 1024 wavefronts
+    about three tiles (the tile loop and its feature prefetch)."""
     from yggdrax_amd import learned
     p = learned.init_params(3)
-    x, _, _ = _batch(1000, 1)
+    x, _, _ = _batch(B, 1)
     model = learned.CapacityModel(p)
     y = model.forward(torch.from_numpy(x).cuda()).cpu().numpy()
     ref, _ = mlp_ref.forward(p, x)
