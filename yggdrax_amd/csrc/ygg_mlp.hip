@@ -334,33 +334,56 @@ __global__ void k_b_mlp_features(Seg sg, int64_t Ttot, const int32_t* __restrict
 }
 
 // Batched totals: output 1 (near) counts each system's leaves.
-__global__ void k_b_mlp_totals(Seg sg, const float* __restrict__ y, int64_t n, float s0, float s1,
-                               float s2, double* __restrict__ tot) {
-  double a0 = 0, a1 = 0, a2 = 0;
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    const int s = sg.of_node(g);
-    a0 += (double)(s0 * exp2f(fminf(y[3 * g], kMaxLogRate)));
-    if (g - sg.node_off(s) >= sg.L(s) - 1)
-      a1 += (double)(s1 * exp2f(fminf(y[3 * g + 1], kMaxLogRate)));
-    a2 += (double)(s2 * exp2f(fminf(y[3 * g + 2], kMaxLogRate)));
-  }
+// Adds a workgroup's three partial sums into tot[0..2]: wavefront
+// shuffles, then one LDS row per wavefront and three atomics per workgroup
+// (same-address device atomics serialise across XCDs, so the grids of the
+// totals kernels are kept at kTotalsBlocks workgroups).
+constexpr int kTotalsBlocks = 128;
+
+__device__ __forceinline__ void block_add3(double a0, double a1, double a2, double* tot) {
+  __shared__ double red[256 / YGG_WAVE][3];
   for (int off = 32; off > 0; off >>= 1) {
     a0 += __shfl_xor(a0, off, 64);
     a1 += __shfl_xor(a1, off, 64);
     a2 += __shfl_xor(a2, off, 64);
   }
+  const int w = threadIdx.x / YGG_WAVE;
   if (lane_id() == 0) {
-    atomicAdd(&tot[0], a0);
-    atomicAdd(&tot[1], a1);
-    atomicAdd(&tot[2], a2);
+    red[w][0] = a0;
+    red[w][1] = a1;
+    red[w][2] = a2;
   }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = 0;
+    for (int q = 0; q < (int)(blockDim.x / YGG_WAVE); ++q) v += red[q][threadIdx.x];
+    atomicAdd(&tot[threadIdx.x], v);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_b_mlp_totals(Seg sg, const float* __restrict__ y,
+                                                      int64_t n, float s0, float s1, float s2,
+                                                      double* __restrict__ tot) {
+  __shared__ int32_t s_noff[kSegLdsCap];
+  const bool staged = seg_stage(s_noff, sg.S, [&](int s) { return (int32_t)sg.node_off(s); });
+  __syncthreads();
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int s = staged ? seg_find_lds(s_noff, sg.S, g) : sg.of_node(g);
+    a0 += (double)(s0 * exp2f(fminf(y[3 * g], kMaxLogRate)));
+    if (g - sg.node_off(s) >= sg.L(s) - 1)
+      a1 += (double)(s1 * exp2f(fminf(y[3 * g + 1], kMaxLogRate)));
+    a2 += (double)(s2 * exp2f(fminf(y[3 * g + 2], kMaxLogRate)));
+  }
+  block_add3(a0, a1, a2, tot);
 }
 
 // Sum of scale_j 2^y_j per output over [0, n) (stream totals), y row-major
 // [n][3]; output 1 (near) counts leaves only.
-__global__ void k_mlp_totals(const float* __restrict__ y, int64_t n, int64_t NI, float s0,
-                             float s1, float s2, double* __restrict__ tot) {
+__global__ __launch_bounds__(256) void k_mlp_totals(const float* __restrict__ y, int64_t n,
+                                                    int64_t NI, float s0, float s1, float s2,
+                                                    double* __restrict__ tot) {
   double a0 = 0, a1 = 0, a2 = 0;
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
@@ -368,16 +391,7 @@ __global__ void k_mlp_totals(const float* __restrict__ y, int64_t n, int64_t NI,
     if (s >= NI) a1 += (double)(s1 * exp2f(fminf(y[3 * s + 1], kMaxLogRate)));
     a2 += (double)(s2 * exp2f(fminf(y[3 * s + 2], kMaxLogRate)));
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    a0 += __shfl_xor(a0, off, 64);
-    a1 += __shfl_xor(a1, off, 64);
-    a2 += __shfl_xor(a2, off, 64);
-  }
-  if (lane_id() == 0) {
-    atomicAdd(&tot[0], a0);
-    atomicAdd(&tot[1], a1);
-    atomicAdd(&tot[2], a2);
-  }
+  block_add3(a0, a1, a2, tot);
 }
 
 // far[k] = s0 2^y[k][0] (the first pass's far-count prediction).
@@ -496,7 +510,7 @@ int ygg_mlp_totals(const float* outputs, int64_t num_nodes, float scale_far, flo
   if (num_nodes < 1 || !outputs || !totals) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)totals, 6);
-  hipLaunchKernelGGL(k_mlp_totals, dim3(grid_for(num_nodes, 256, 1024)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_mlp_totals, dim3(grid_for(num_nodes, 256, kTotalsBlocks)), dim3(256), 0, st,
                      outputs, num_nodes, (num_nodes - 1) / 2, scale_far, scale_near, scale_queue,
                      totals);
   YGG_LAUNCH_CHECK();
@@ -541,7 +555,7 @@ int ygg_batch_mlp_totals(const YggBatch* b, const float* outputs, float scale_fa
   hipStream_t st = (hipStream_t)stream;
   const int64_t T = 2 * b->num_leaves - b->num_systems;
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)totals, 6);
-  hipLaunchKernelGGL(k_b_mlp_totals, dim3(grid_for(T, 256, 1024)), dim3(256), 0, st, mlp_seg(b),
+  hipLaunchKernelGGL(k_b_mlp_totals, dim3(grid_for(T, 256, kTotalsBlocks)), dim3(256), 0, st, mlp_seg(b),
                      outputs, T, scale_far, scale_near, scale_queue, totals);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
