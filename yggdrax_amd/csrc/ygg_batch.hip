@@ -166,7 +166,11 @@ __global__ __launch_bounds__(256) void k_b_karras(Seg sg, const uint64_t* __rest
 }
 
 // Levels by walking up (capped at 255), per-system level histograms and
-// scalars [8 s + k]: 1 max_level, 2 deep, 3 malformed, 4 roots.
+// scalars [8 s + k]: 1 max_level, 2 deep, 3 malformed, 4 roots. A
+// system's nodes are contiguous, so most wavefronts lie in one system:
+// those reduce the max level and the root count over their lanes and add
+// them with one atomic each, instead of one atomic per node on the same
+// per-system word.
 __global__ __launch_bounds__(256) void k_b_levels(Seg sg, const uint32_t* __restrict__ penc,
                                                   int64_t Ttot, const int32_t* __restrict__ left,
                                                   const int32_t* __restrict__ right,
@@ -175,34 +179,56 @@ __global__ __launch_bounds__(256) void k_b_levels(Seg sg, const uint32_t* __rest
                                                   uint32_t* __restrict__ lcount,
                                                   int32_t* __restrict__ scal) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= Ttot) return;
-  const int s = sg.of_node(g);
-  const int n0 = sg.node_off(s), ni0 = sg.ni_off(s);
-  const int NI = sg.L(s) - 1;
-  const int k = (int)(g - n0);
-  const uint32_t* pe = penc + n0;
-  const int32_t p = dec_parent(pe[k]);
-  parent[g] = p;
-  int d = 0;
-  int32_t q = p;
-  bool deep = false;
-  while (q >= 0) {
-    ++d;
-    if (d > 255) {
-      deep = true;
-      break;
+  const bool valid = g < Ttot;
+  int s = 0, d = 0;
+  bool root = false;
+  if (valid) {
+    s = sg.of_node(g);
+    const int n0 = sg.node_off(s), ni0 = sg.ni_off(s);
+    const int NI = sg.L(s) - 1;
+    const int k = (int)(g - n0);
+    const uint32_t* pe = penc + n0;
+    const int32_t p = dec_parent(pe[k]);
+    parent[g] = p;
+    int32_t q = p;
+    bool deep = false;
+    while (q >= 0) {
+      ++d;
+      if (d > 255) {
+        deep = true;
+        break;
+      }
+      q = dec_parent(pe[q]);
     }
-    q = dec_parent(pe[q]);
+    level[g] = d;
+    int32_t* sc = scal + 8 * (int64_t)s;
+    if (deep) atomicOr((unsigned*)&sc[2], 1u);
+    if (d < 64) atomicAdd(&lcount[64 * (int64_t)s + d], 1u);
+    root = p < 0;
+    if (k < NI) {
+      if (dec_parent(pe[left[ni0 + k]]) != k || dec_parent(pe[right[ni0 + k]]) != k)
+        atomicOr((unsigned*)&sc[3], 1u);
+    }
   }
-  level[g] = d;
-  int32_t* sc = scal + 8 * (int64_t)s;
-  if (deep) atomicOr((unsigned*)&sc[2], 1u);
-  if (d < 64) atomicAdd(&lcount[64 * (int64_t)s + d], 1u);
-  atomicMax(&sc[1], d);
-  if (p < 0) atomicAdd(&sc[4], 1);
-  if (k < NI) {
-    if (dec_parent(pe[left[ni0 + k]]) != k || dec_parent(pe[right[ni0 + k]]) != k)
-      atomicOr((unsigned*)&sc[3], 1u);
+  // every lane reaches the ballots (no early return above)
+  const uint64_t vmask = __ballot(valid);
+  if (vmask == 0) return;
+  const int lead = __ffsll((unsigned long long)vmask) - 1;
+  const int s0 = __shfl(s, lead, 64);
+  if (__ballot(valid && s == s0) == vmask) {
+    int dm = valid ? d : 0, rc = (valid && root) ? 1 : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      dm = max(dm, __shfl_xor(dm, off, 64));
+      rc += __shfl_xor(rc, off, 64);
+    }
+    if ((int)lane_id() == lead) {
+      atomicMax(&scal[8 * (int64_t)s0 + 1], dm);
+      if (rc) atomicAdd(&scal[8 * (int64_t)s0 + 4], rc);
+    }
+  } else if (valid) {
+    atomicMax(&scal[8 * (int64_t)s + 1], d);
+    if (root) atomicAdd(&scal[8 * (int64_t)s + 4], 1);
   }
 }
 
