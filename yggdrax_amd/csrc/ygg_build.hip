@@ -313,8 +313,12 @@ __global__ __launch_bounds__(256) void k_levels_walk(const uint32_t* __restrict_
                                                      int32_t* __restrict__ level,
                                                      uint32_t* __restrict__ lcount,
                                                      int32_t* __restrict__ scal) {
+  // level histogram and max level reduced in LDS first: one global atomic
+  // per workgroup on the max-level word instead of one per node
   __shared__ uint32_t h[64];
+  __shared__ int s_maxd;
   if (threadIdx.x < 64) h[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_maxd = 0;
   __syncthreads();
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < T) {
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256) void k_levels_walk(const uint32_t* __restrict_
     level[k] = d;
     if (deep) atomicOr((unsigned*)&scal[2], 1u);
     if (d < 64) atomicAdd(&h[d], 1u);
-    atomicMax(&scal[1], d);
+    atomicMax(&s_maxd, d);
     if (p < 0) atomicAdd(&scal[4], 1);
     if (k < NI) {
       if (dec_parent(penc[left[k]]) != k || dec_parent(penc[right[k]]) != k)
@@ -343,6 +347,7 @@ __global__ __launch_bounds__(256) void k_levels_walk(const uint32_t* __restrict_
   }
   __syncthreads();
   if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&lcount[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x == 0 && k - (int)threadIdx.x < T) atomicMax(&scal[1], s_maxd);
 }
 
 __global__ void k_level_offsets(const uint32_t* __restrict__ lcount, int32_t* __restrict__ off,
@@ -468,8 +473,12 @@ __global__ __launch_bounds__(256) void k_oct_ranges(
     const uint8_t* __restrict__ valid, const uint64_t* __restrict__ rc,
     const int32_t* __restrict__ node_ranges, int L, int M, int32_t* __restrict__ oranges,
     uint32_t* __restrict__ lcount, uint32_t* __restrict__ skey, int32_t* __restrict__ scal) {
+  // depth histogram and max depth reduced in LDS, then one global atomic
+  // per workgroup on the max-depth word
   __shared__ uint32_t h[22];
+  __shared__ int s_maxd;
   if (threadIdx.x < 22) h[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_maxd = INT32_MIN;
   __syncthreads();
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < M) {
@@ -491,7 +500,7 @@ __global__ __launch_bounds__(256) void k_oct_ranges(
       oranges[2 * k] = node_ranges[2 * (NI + f)];
       oranges[2 * k + 1] = node_ranges[2 * (NI + l) + 1];
       atomicAdd(&h[min(max(d, 0), 21)], 1u);
-      atomicMax(&scal[3], d);
+      atomicMax(&s_maxd, d);
     } else {
       oranges[2 * k] = 0;
       oranges[2 * k + 1] = -1;
@@ -500,6 +509,7 @@ __global__ __launch_bounds__(256) void k_oct_ranges(
   }
   __syncthreads();
   if (threadIdx.x < 22 && h[threadIdx.x]) atomicAdd(&lcount[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x == 0 && s_maxd != INT32_MIN) atomicMax(&scal[3], s_maxd);
 }
 
 __global__ void k_oct_level_offsets(const uint32_t* __restrict__ lcount,
