@@ -355,35 +355,54 @@ __global__ __launch_bounds__(256) void k_b_oct_ranges(
     const int32_t* __restrict__ node_ranges, int64_t Mtot, int32_t* __restrict__ oranges,
     uint32_t* __restrict__ lcount, uint32_t* __restrict__ skey, int32_t* __restrict__ scal) {
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= Mtot) return;
-  const int s = sg.of_node(g);
-  const int L = sg.L(s);
-  const bool v = valid[g];
-  const int d = depths[g];
-  const uint64_t c = codes[g];
-  const uint64_t du = (uint64_t)max(d, 0);
-  const uint64_t bs = 63ull - 3ull * du;
-  const uint64_t bss = bs < 62ull ? bs : 62ull;
-  const uint64_t mask = (1ull << bss) - 1ull;
-  const uint64_t endc = v ? (c | mask) : 0ull;
-  const uint64_t* r = rc + sg.leaf_off(s);
-  int f = fixed_round_search(r, L, c, true);
-  int l = fixed_round_search(r, L, endc, false) - 1;
-  const int bound = max(L - 1, 0);
-  f = min(max(f, 0), bound);
-  l = min(max(l, 0), bound);
-  const int64_t n0 = sg.node_off(s);
-  const int64_t NI = L - 1;
-  if (v) {
-    oranges[2 * g] = node_ranges[2 * (n0 + NI + f)];
-    oranges[2 * g + 1] = node_ranges[2 * (n0 + NI + l) + 1];
-    atomicAdd(&lcount[22 * (int64_t)s + min(max(d, 0), 21)], 1u);
-    atomicMax(&scal[4 * (int64_t)s + 3], d);
-  } else {
-    oranges[2 * g] = 0;
-    oranges[2 * g + 1] = -1;
+  const bool in = g < Mtot;
+  int s = 0, d = 0;
+  bool v = false;
+  if (in) {
+    s = sg.of_node(g);
+    const int L = sg.L(s);
+    v = valid[g];
+    d = depths[g];
+    const uint64_t c = codes[g];
+    const uint64_t du = (uint64_t)max(d, 0);
+    const uint64_t bs = 63ull - 3ull * du;
+    const uint64_t bss = bs < 62ull ? bs : 62ull;
+    const uint64_t mask = (1ull << bss) - 1ull;
+    const uint64_t endc = v ? (c | mask) : 0ull;
+    const uint64_t* r = rc + sg.leaf_off(s);
+    int f = fixed_round_search(r, L, c, true);
+    int l = fixed_round_search(r, L, endc, false) - 1;
+    const int bound = max(L - 1, 0);
+    f = min(max(f, 0), bound);
+    l = min(max(l, 0), bound);
+    const int64_t n0 = sg.node_off(s);
+    const int64_t NI = L - 1;
+    if (v) {
+      oranges[2 * g] = node_ranges[2 * (n0 + NI + f)];
+      oranges[2 * g + 1] = node_ranges[2 * (n0 + NI + l) + 1];
+      atomicAdd(&lcount[22 * (int64_t)s + min(max(d, 0), 21)], 1u);
+    } else {
+      oranges[2 * g] = 0;
+      oranges[2 * g + 1] = -1;
+    }
+    skey[g] = ((uint32_t)s << 5) | (v ? (uint32_t)d : 22u);
   }
-  skey[g] = ((uint32_t)s << 5) | (v ? (uint32_t)d : 22u);
+  // Max depth per system: reduced over the wavefront when all its valid
+  // octree nodes lie in one system (one atomic), else one per node. Every
+  // lane reaches the ballots.
+  const bool contrib = in && v;
+  const uint64_t cmask = __ballot(contrib);
+  if (cmask == 0) return;
+  const int lead = __ffsll((unsigned long long)cmask) - 1;
+  const int s0 = __shfl(s, lead, 64);
+  if (__ballot(contrib && s == s0) == cmask) {
+    int dm = contrib ? d : INT32_MIN;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dm = max(dm, __shfl_xor(dm, off, 64));
+    if ((int)lane_id() == lead) atomicMax(&scal[4 * (int64_t)s0 + 3], dm);
+  } else if (contrib) {
+    atomicMax(&scal[4 * (int64_t)s + 3], d);
+  }
 }
 
 __global__ void k_b_oct_level_offsets(Seg sg, const uint32_t* __restrict__ lcount,
