@@ -34,8 +34,6 @@ def _load():
     return ctypes.CDLL(_LIB_PATH)
 
 
-lib = _load()
-
 _P = ctypes.c_void_p
 _I = ctypes.c_int64
 
@@ -44,13 +42,20 @@ def _ptr(a: np.ndarray):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-lib.ygo_walk_run.restype = ctypes.c_void_p
-lib.ygo_walk_run.argtypes = [_I, _I, _P, _P, _P, _P, _P, _I, ctypes.c_float,
-                             ctypes.c_int, _I, _I, _I, ctypes.c_int, ctypes.c_int]
-lib.ygo_walk_sizes.argtypes = [_P, _P]
-lib.ygo_walk_copy.argtypes = [_P, _P, _P, _P, _P, _P]
-lib.ygo_walk_free.argtypes = [_P]
-lib.ygo_geometry.restype = ctypes.c_int
+def configure(handle):
+    """Declare the ctypes signatures this driver relies on for a loaded
+    oracle library (the default one, or another build of the same source)."""
+    handle.ygo_walk_run.restype = ctypes.c_void_p
+    handle.ygo_walk_run.argtypes = [_I, _I, _P, _P, _P, _P, _P, _I, ctypes.c_float,
+                                    ctypes.c_int, _I, _I, _I, ctypes.c_int, ctypes.c_int]
+    handle.ygo_walk_sizes.argtypes = [_P, _P]
+    handle.ygo_walk_copy.argtypes = [_P, _P, _P, _P, _P, _P]
+    handle.ygo_walk_free.argtypes = [_P]
+    handle.ygo_geometry.restype = ctypes.c_int
+    return handle
+
+
+lib = configure(_load())
 
 MAC_CODES = {"bh": 0, "dehnen": 1, "engblom": 2}
 
