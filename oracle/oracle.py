@@ -486,8 +486,17 @@ NEAR_ERR = "Neighbor list capacity exceeded; increase max_neighbors_per_leaf and
 
 def run_walk_with_retries(topo, geom, theta, mac_type="bh", dehnen_radius_scale=1.0, *,
                           max_interactions_per_node=None, max_neighbors_per_leaf=2048,
-                          max_pair_queue=None, traversal_config=None, events=None):
-    """_run_dual_tree_walk_raw (3840-4440), eager mode, default pair policy."""
+                          max_pair_queue=None, traversal_config=None, events=None,
+                          queue_cache=None):
+    """_run_dual_tree_walk_raw (3840-4440), eager mode, default pair policy.
+
+    ``events`` collects the retry events the reference hands to its
+    retry_logger (4031-4052, 4339-4381) as (status, attempt, queue_capacity,
+    interaction_capacity, far_pair_count, near_pair_count) tuples; the pair
+    counts are the attempt's stored entries. ``queue_cache`` (a dict) plays
+    the module-level queue-capacity cache (245-293, 3927-3952) for calls with
+    a traversal_config. The returned raw dict carries ``path`` ("count" or
+    "fixed") and, on the fixed path, the successful ``attempt_caps``."""
     T = topo["parent"].shape[0]
     NI = (T - 1) // 2
     L = T - NI
@@ -495,12 +504,18 @@ def run_walk_with_retries(topo, geom, theta, mac_type="bh", dehnen_radius_scale=
         raise ValueError("dehnen_radius_scale must be > 0")
     kw = dict(theta=theta, mac_type=mac_type, dehnen_radius_scale=dehnen_radius_scale)
     cfg = traversal_config
+    cache_key = None
     if cfg is not None:
         qmax = max(4, int(cfg.max_pair_queue))
         inter = [int(cfg.max_interactions_per_node)]
         nbr = _neighbor_capacity_candidates(int(cfg.max_neighbors_per_leaf), L)
         user_q = True
-        qc = _queue_candidates_bounded(qmax, int(cfg.process_block))
+        cache_key = (T, NI, inter[0], int(cfg.max_neighbors_per_leaf), qmax,
+                     int(cfg.process_block), True, True, str(mac_type))
+        if queue_cache is not None and cache_key in queue_cache:
+            qc = [int(queue_cache[cache_key])]
+        else:
+            qc = _queue_candidates_bounded(qmax, int(cfg.process_block))
         for cap in _auto_pair_queue_candidates(T, NI):
             if cap > qc[-1]:
                 qc.append(cap)
@@ -527,30 +542,40 @@ def run_walk_with_retries(topo, geom, theta, mac_type="bh", dehnen_radius_scale=
             raise RuntimeError("Dual-tree traversal pair totals exceed signed int32 capacity; "
                                "refine the tree or use a lower-memory traversal strategy.")
         cnt["max_wf"] = int(max(1, cnt["gen_sizes"].max(initial=1)))
+        # The exact-length fill that follows (4226-4290) runs with a queue of
+        # 1.5x the observed peak, so its first attempt succeeds and emits no
+        # event; its lists are the counting walk's.
+        cnt["path"] = "count"
         return cnt
+
+    def emit(status, attempt, Q, K, res):
+        if events is not None:
+            events.append((status, attempt, Q, K, int(res["far_counts"].sum()),
+                           int(res["near_counts"].sum())))
 
     result = None
     success = False
+    attempt = 0
     for Q in qc:
         queue_retry = False
         for K in inter:
             for Kn in nbr:
+                attempt += 1
                 result = walk(topo, geom, queue_cap=Q, far_cap=K, near_cap=Kn, **kw)
                 result["attempt_caps"] = (Q, K, Kn)
+                result["path"] = "fixed"
                 if result["queue_overflow"]:
-                    if events is not None:
-                        events.append(("queue_overflow", Q, K))
+                    emit("queue_overflow", attempt, Q, K, result)
                     queue_retry = True
                     break
                 if result["far_overflow"]:
                     raise RuntimeError(FAR_ERR)
                 if result["near_overflow"]:
-                    if events is not None:
-                        events.append(("neighbor_overflow", Q, K))
+                    emit("neighbor_overflow", attempt, Q, K, result)
                     continue
                 success = True
-                if events is not None and len(events) > 0:
-                    events.append(("success", Q, K))
+                if attempt > 1:
+                    emit("success", attempt, Q, K, result)
                 break
             if queue_retry or success:
                 break
@@ -559,6 +584,8 @@ def run_walk_with_retries(topo, geom, theta, mac_type="bh", dehnen_radius_scale=
         if queue_retry and not success:
             continue
         if success:
+            if queue_cache is not None and cache_key is not None:
+                queue_cache[cache_key] = Q
             break
     else:
         if result["queue_overflow"]:
@@ -570,6 +597,55 @@ def run_walk_with_retries(topo, geom, theta, mac_type="bh", dehnen_radius_scale=
     if result["near_overflow"]:
         raise RuntimeError(NEAR_ERR)
     return result
+
+
+def to_result(raw, topo):
+    """The raw DualTreeWalkResult the reference returns with
+    return_result=True (_raw_to_result, 4606-4627).
+
+    Fixed-capacity path (_dual_tree_walk_impl, 1787-1905): interaction_offsets
+    is the node-order exclusive cumsum of the counts while the sources and
+    targets are laid out level-major (quirk B9), padded with -1 to
+    T * max_interactions_per_node; tags are all -1; neighbour indices are in
+    native leaf-row order, padded with -1 to L * max_neighbors_per_leaf.
+    Count path (_dual_tree_walk_compact_fill_impl, 2942-3558): node-order
+    offsets and node-order rows of exact length. Decision counters are those
+    of the returned attempt; the flags are False after a successful call."""
+    T = topo["parent"].shape[0]
+    fc = np.asarray(raw["far_counts"], np.int64)
+    nc = np.asarray(raw["near_counts"], np.int64)
+    L = nc.shape[0]
+    F, M = int(fc.sum()), int(nc.sum())
+    node_off = np.concatenate([[0], np.cumsum(fc)]).astype(np.int64)
+    near_off = np.concatenate([[0], np.cumsum(nc)]).astype(np.int64)
+    fe = np.asarray(raw["far_entries"], np.int64)
+    ne = np.asarray(raw["near_entries"], np.int64)
+    node_tgt = np.repeat(np.arange(T, dtype=np.int64), fc)
+    if raw["path"] == "fixed":
+        _, K, Kn = raw["attempt_caps"]
+        nbl = np.asarray(topo["nodes_by_level"], np.int64)
+        src = np.full(max(T * K, 1), -1, np.int64)
+        tgt = np.full(max(T * K, 1), -1, np.int64)
+        o = 0
+        for node in nbl:
+            c = int(fc[node])
+            src[o:o + c] = fe[node_off[node]:node_off[node] + c]
+            tgt[o:o + c] = node
+            o += c
+        tags = np.full(max(T * K, 1), -1, np.int64)
+        nbrs = np.full(max(L * Kn, 1), -1, np.int64)
+        nbrs[:M] = ne
+    else:
+        src, tgt = fe.copy(), node_tgt
+        tags = np.full(F, -1, np.int64)
+        nbrs = ne.copy()
+    leaf_indices, _, _, _ = leaf_ordering(topo)
+    return dict(interaction_offsets=node_off, interaction_sources=src, interaction_targets=tgt,
+                interaction_tags=tags, interaction_counts=fc, neighbor_offsets=near_off,
+                neighbor_indices=nbrs, neighbor_counts=nc, leaf_indices=leaf_indices,
+                far_pair_count=F, near_pair_count=M, queue_overflow=False, far_overflow=False,
+                near_overflow=False, accept_decisions=raw["accept_decisions"],
+                near_decisions=raw["near_decisions"], refine_decisions=raw["refine_decisions"])
 
 
 def to_interactions(raw, topo):
@@ -635,13 +711,15 @@ def to_neighbors(raw, topo):
 def build_interactions_and_neighbors(topo, geom, theta=0.5, max_interactions_per_node=None,
                                      max_neighbors_per_leaf=2048, max_pair_queue=None,
                                      process_block=None, traversal_config=None,
-                                     mac_type="bh", dehnen_radius_scale=1.0, events=None):
-    """_interactions_impl.py:6032-6144 with return_result=False."""
+                                     mac_type="bh", dehnen_radius_scale=1.0, events=None,
+                                     queue_cache=None):
+    """_interactions_impl.py:6032-6144; the third element is the walk's raw
+    outputs (to_result gives the reference's DualTreeWalkResult)."""
     raw = run_walk_with_retries(
         topo, geom, theta, mac_type, dehnen_radius_scale,
         max_interactions_per_node=max_interactions_per_node,
         max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
-        traversal_config=traversal_config, events=events)
+        traversal_config=traversal_config, events=events, queue_cache=queue_cache)
     return to_interactions(raw, topo), to_neighbors(raw, topo), raw
 
 

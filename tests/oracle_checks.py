@@ -242,3 +242,82 @@ def mapping_case():
     return np.array([[0.05, 0.10, 0.15], [0.18, 0.21, 0.22], [0.31, 0.30, 0.35],
                      [0.44, 0.41, 0.48], [0.55, 0.62, 0.58], [0.68, 0.70, 0.66],
                      [0.82, 0.84, 0.79], [0.93, 0.92, 0.95]], np.float32)
+
+
+def bfs_generations(t, g, theta, mac, scale=1.0):
+    """A plain-Python generation-by-generation dual-tree walk (the visiting
+    rule of _interactions_impl.py:1131-1785: MAC per pair, accept, near leaf
+    pairs, split rule 1444-1454, children sorted as (min, max)), written
+    independently of the oracle's C++ walk. Returns, per generation, the
+    number of pairs, the per-node far and per-leaf near entries it adds
+    (both directions) and the number of child pairs it pushes."""
+    from collections import Counter
+
+    from oracle import oracle as O
+    NI = int(t["num_internal_nodes"])
+    ext = O.mac_extents(g, mac, scale)
+    c = np.asarray(g["center"], np.float32)
+    th2 = np.float32(np.float32(theta) * np.float32(theta))
+    Lc, Rc = t["left_child"], t["right_child"]
+    wave = [(0, 0)]
+    gens = []
+    while wave:
+        far, near = Counter(), Counter()
+        nxt = []
+        for a, b in wave:
+            d = c[a] - c[b]
+            d2 = np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+            ea, eb = ext[a], ext[b]
+            if mac == "engblom":
+                rb, rs = max(ea, eb), min(ea, eb)
+                lhs = np.float32(rb + np.float32(np.sqrt(th2)) * rs)
+                ok = a != b and d2 > 0 and lhs * lhs <= th2 * d2
+            else:
+                s = np.float32(ea + eb)
+                ok = a != b and d2 > 0 and s * s <= th2 * d2
+            if ok:
+                far[a] += 1
+                far[b] += 1
+                continue
+            if a >= NI and b >= NI:
+                if a != b:
+                    near[a] += 1
+                    near[b] += 1
+                continue
+            same = a == b
+            st = a < NI and (same or b >= NI or ea >= eb)
+            ss = b < NI and (same or a >= NI or eb > ea)
+            if st and ss and same:
+                kids = [(Lc[a], Lc[a]), (Lc[a], Rc[a]), (Rc[a], Rc[a])]
+            elif st and ss:
+                kids = [(x, y) for x in (Lc[a], Rc[a]) for y in (Lc[b], Rc[b])]
+            elif st:
+                kids = [(Lc[a], b), (Rc[a], b)]
+            else:
+                kids = [(a, Lc[b]), (a, Rc[b])]
+            nxt.extend((min(int(x), int(y)), max(int(x), int(y))) for x, y in kids)
+        gens.append(dict(pairs=len(wave), far=far, near=near, pushes=len(nxt)))
+        wave = nxt
+    return gens
+
+
+def simulate_attempt(gens, Q, K, Kn):
+    """One fixed-capacity attempt over the generations: entries beyond a
+    row's capacity are dropped and raise its flag, pushes beyond the queue
+    raise the queue flag, and the walk stops after the first generation
+    that raised any flag. Returns (queue, far, near) flags and the stored
+    far and near totals."""
+    from collections import Counter
+    fc, nc = Counter(), Counter()
+    qo = fo = no = False
+    for gen in gens:
+        for k, v in gen["far"].items():
+            fo |= fc[k] + v > K
+            fc[k] = min(fc[k] + v, K)
+        for k, v in gen["near"].items():
+            no |= nc[k] + v > Kn
+            nc[k] = min(nc[k] + v, Kn)
+        qo |= gen["pushes"] > Q
+        if qo or fo or no:
+            break
+    return (qo, fo, no), sum(fc.values()), sum(nc.values())

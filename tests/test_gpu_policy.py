@@ -68,10 +68,12 @@ def test_policy_walk_parity(policy, state, n, leaf):
         tree, geom, theta=0.6, mac_type="dehnen", pair_policy=policy, policy_state=state,
         return_result=True)
     torch.cuda.synchronize()
-    src = parity.np_(inter.sources)
+    # count path: the raw result holds node-order rows at node-order offsets
+    src = parity.np_(res.interaction_sources)
     tags = parity.np_(res.interaction_tags)
-    off = parity.np_(inter.offsets)
-    cnt = parity.np_(inter.counts)
+    off = parity.np_(res.interaction_offsets)
+    cnt = parity.np_(res.interaction_counts)
+    assert np.array_equal(cnt, parity.np_(inter.counts))
     for node in range(len(far)):
         got = sorted(zip(src[off[node]:off[node] + cnt[node]].tolist(),
                          tags[off[node]:off[node] + cnt[node]].tolist()))
@@ -81,7 +83,7 @@ def test_policy_walk_parity(policy, state, n, leaf):
     for r in range(len(near)):
         assert sorted(nb[noff[r]:noff[r + 1]].tolist()) == sorted(near[r])
     if policy is target_parity_policy:
-        t = parity.np_(inter.targets)
+        t = parity.np_(res.interaction_targets)
         assert np.all(np.isin(tags, [0, 1])) and np.array_equal(tags, t % 2)
     if policy is mac_policy:  # identical to the built-in MAC walk
         plain, pn = ygg.build_interactions_and_neighbors(tree, geom, theta=0.6, mac_type="dehnen")
@@ -137,8 +139,8 @@ def test_policy_descriptor_matches_callable_and_oracle(case):
     inter, nbrs, res = ygg.build_interactions_and_neighbors(
         tree, geom, theta=0.6, mac_type="dehnen", pair_policy=desc, return_result=True)
     torch.cuda.synchronize()
-    src, tags = parity.np_(inter.sources), parity.np_(res.interaction_tags)
-    off, cnt = parity.np_(inter.offsets), parity.np_(inter.counts)
+    src, tags = parity.np_(res.interaction_sources), parity.np_(res.interaction_tags)
+    off, cnt = parity.np_(res.interaction_offsets), parity.np_(res.interaction_counts)
     print("descriptor", case, "far", int(cnt.sum()), flush=True)
     for node in range(len(far)):
         got = sorted(zip(src[off[node]:off[node] + cnt[node]].tolist(),
@@ -152,9 +154,10 @@ def test_policy_descriptor_matches_callable_and_oracle(case):
         return_result=True)
     assert torch.equal(hi.counts, inter.counts) and torch.equal(hi.offsets, inter.offsets)
     assert torch.equal(hn.counts, nbrs.counts)
-    assert sorted(zip(parity.np_(hi.targets).tolist(), parity.np_(hi.sources).tolist(),
+    assert sorted(zip(parity.np_(hres.interaction_targets).tolist(),
+                      parity.np_(hres.interaction_sources).tolist(),
                       parity.np_(hres.interaction_tags).tolist())) == sorted(
-        zip(parity.np_(inter.targets).tolist(), src.tolist(), tags.tolist()))
+        zip(parity.np_(res.interaction_targets).tolist(), src.tolist(), tags.tolist()))
 
 
 def test_descriptor_default_rules_equal_plain_walk():
@@ -210,8 +213,10 @@ def test_descriptor_flat_walk_matches_callable_at_16k(case):
     assert torch.equal(hi.counts, inter.counts) and torch.equal(hi.offsets, inter.offsets)
     assert torch.equal(hn.counts, nbrs.counts)
     T = tree.num_nodes
-    k1 = (inter.targets.long() * T + inter.sources.long()) * 16 + res.interaction_tags.long()
-    k2 = (hi.targets.long() * T + hi.sources.long()) * 16 + hres.interaction_tags.long()
+    k1 = ((res.interaction_targets.long() * T + res.interaction_sources.long()) * 16
+          + res.interaction_tags.long())
+    k2 = ((hres.interaction_targets.long() * T + hres.interaction_sources.long()) * 16
+          + hres.interaction_tags.long())
     assert torch.equal(torch.sort(k1)[0], torch.sort(k2)[0])
     L = nbrs.counts.numel()
     rows = torch.repeat_interleave(torch.arange(L, device=nbrs.counts.device), nbrs.counts)

@@ -632,17 +632,24 @@ class BatchedGraph:
     of the same layout (e.g. every step of a simulation) with one launch and
     one small status read.
 
-    Stream capacities and the generation count are fixed at capture: the
-    exact totals of a first eager run (sized by the capacity model when one
-    is given), times `slack`. A replay whose walk
-    overflows, does not finish within the captured generations, or meets a
-    system that needs the single-system path returns the eager pipeline's
-    result instead (`last_replay_fell_back`), so results never depend on the
-    capacities."""
+    Stream capacities and the generation count are fixed at capture. With a
+    capacity model the far, near and queue capacities are the model's
+    predicted totals for the capture inputs; without one they are the exact
+    totals of a first eager run times `slack`. `capacities` ({"far", "near",
+    "queue"}) and `generations` override both. The generation count is the
+    eager run's times `slack`, plus 4. A replay whose walk overflows, does
+    not finish within the captured generations, or meets a system that needs
+    the single-system path returns the eager pipeline's result instead
+    (`last_replay_fell_back`), so results never depend on the capacities.
+
+    A replay that does not fall back returns the graph's own output buffers:
+    the next replay overwrites them in place, so a caller that keeps a
+    result across replays clones its tensors first."""
 
     def __init__(self, positions, masses, offsets, *, leaf_size: int = 32, theta: float = 0.6,
                  mac_type: str = "dehnen", dehnen_radius_scale: float = 1.0, model=None,
-                 slack: float = 1.1, grid_blocks: int = FLAT_GRID_BLOCKS):
+                 slack: float = 1.1, grid_blocks: int = FLAT_GRID_BLOCKS,
+                 capacities: Optional[dict] = None, generations: Optional[int] = None):
         from . import autotune
         _lib.require_device(positions)
         dev = positions.device
@@ -660,10 +667,16 @@ class BatchedGraph:
         # sized row sorts then do only `slack` times the necessary work.
         lists = self._eager(model)
         gens = sum(1 for v in lists.stats["gen_sizes"] if v > 0)
-        caps = {"far": lists.stats["far_total"], "near": lists.stats["near_total"],
-                "queue": lists.stats["peak"]}
-        self.caps = {k: int(v * slack) + 1024 for k, v in caps.items()}
-        self.generations = int(gens * slack) + 4
+        if capacities is not None:
+            self.caps = {k: max(1, int(capacities[k])) for k in ("far", "near", "queue")}
+        elif self.hint is not None:
+            self.caps = {k: max(1, int(self.hint[k])) for k in ("far", "near", "queue")}
+        else:
+            caps = {"far": lists.stats["far_total"], "near": lists.stats["near_total"],
+                    "queue": lists.stats["peak"]}
+            self.caps = {k: int(v * slack) + 1024 for k, v in caps.items()}
+        self.generations = (max(1, int(generations)) if generations is not None
+                            else int(gens * slack) + 4)
         was = autotune.enabled()
         autotune.enable(False)
         try:
@@ -707,10 +720,14 @@ class BatchedGraph:
         return dict(trees=bt, geom=geom, lists=lists, walk=w, status=status)
 
     def replay(self, positions=None, masses=None):
-        """(trees, geometry, lists) for the current (or given) inputs."""
+        """(trees, geometry, lists) for the current (or given) inputs; the
+        results alias the graph's buffers (see the class notes)."""
+        if masses is not None and self.mass is None:
+            raise ValueError("this graph was captured without masses; masses cannot be "
+                             "replayed")
         if positions is not None:
             self.pos.copy_(positions)
-        if masses is not None and self.mass is not None:
+        if masses is not None:
             self.mass.copy_(torch.as_tensor(masses, dtype=torch.float32))
         self.graph.replay()
         S = self.lay.num_systems

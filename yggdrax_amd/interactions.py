@@ -286,7 +286,9 @@ class _WalkContext:
         rules; the emission walk then also writes a tag per far entry."""
         dt = self.dt
         dev = self.dev
-        Q = max(4, int(queue_cap))
+        roots = getattr(self, "roots", None)
+        # generation 0 holds every root pair, so the queue holds at least those
+        Q = max(4, int(queue_cap), 0 if roots is None else int(roots.numel()))
         wf = torch.empty((2, Q), dtype=torch.int64, device=dev)
         counters = torch.empty(2, dtype=torch.int64, device=dev)
         # the counting walk adds its totals into spread lanes instead of
@@ -322,7 +324,6 @@ class _WalkContext:
         if policy is not None:
             p.policy = ctypes.cast(ctypes.pointer(policy), ctypes.c_void_p)
             p.far_tags = _lib.ptr(ftag)
-        roots = getattr(self, "roots", None)
         if roots is not None:
             p.roots, p.num_roots = roots.data_ptr(), int(roots.numel())
         ngen = ctypes.c_int32(0)
@@ -617,11 +618,11 @@ def _sort_far_and_near(fk, fv, nf, T, nk, nv, nn, L):
 def _emulate_ladder(lad: _Ladder, peak: int, max_far: int, max_near: int):
     """Outcome of the reference's retry loop from full-walk statistics.
 
-    Returns the queue capacity of the successful attempt, raises the
-    reference's error, or returns None when the outcome depends on the
-    generation at which two overflows happen (then the attempt-by-attempt
-    walk decides)."""
-    success_q = None
+    Returns the (queue, interaction, neighbour) capacities of the successful
+    attempt, raises the reference's error, or returns None when the outcome
+    depends on the generation at which two overflows happen (then the
+    attempt-by-attempt walk decides)."""
+    success = None
     for Q in lad.queue:
         q_over = peak > Q
         for K in lad.inter:
@@ -634,19 +635,19 @@ def _emulate_ladder(lad: _Ladder, peak: int, max_far: int, max_near: int):
                 n_over = max_near > Kn
                 if q_over or n_over:
                     continue
-                success_q = Q
+                success = (Q, K, Kn)
                 break
-            if success_q is not None:
+            if success is not None:
                 break
-        if success_q is not None:
+        if success is not None:
             break
-    if success_q is None:
+    if success is None:
         if peak > lad.queue[-1]:
             raise RuntimeError(QUEUE_ERR_USER if lad.user_q else QUEUE_ERR_AUTO)
         raise RuntimeError(NEAR_ERR)
     if lad.cache_key is not None:
-        _store_cached_queue_capacity(lad.cache_key, success_q)
-    return success_q
+        _store_cached_queue_capacity(lad.cache_key, success[0])
+    return success
 
 
 def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
@@ -710,10 +711,11 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                 "host (would overflow int32). Enable local refinement or pass explicit "
                 "capacities (max_neighbors_per_leaf / max_interactions_per_node).")
     else:
-        q = _emulate_ladder(lad, em["peak"], max_far, max_near)
-        if q is None:
+        caps = _emulate_ladder(lad, em["peak"], max_far, max_near)
+        if caps is None:
             return None
-        stats["queue_capacity"] = q
+        stats["queue_capacity"] = caps[0]
+        stats["attempt_caps"] = caps
     return _FlatResult(fk, fv, nv, far_off, near_off, stats)
 
 
@@ -805,6 +807,7 @@ def fill_from_counts(ctx: _WalkContext, dt: DeviceTopology, far_caps: torch.Tens
         if not queue_candidates or queue_candidates[0] != q_suggest:
             queue_candidates = [q_suggest, *queue_candidates]
         queue_candidates = list(dict.fromkeys(queue_candidates))
+    k_suggest = max(1, int(far_caps.max())) if T else 1
     result = None
     for attempt, Q in enumerate(queue_candidates, start=1):
         o = ctx.run(mode=2, queue_cap=Q, far_row_start=far_start, far_row_cap=far_caps,
@@ -814,10 +817,10 @@ def fill_from_counts(ctx: _WalkContext, dt: DeviceTopology, far_caps: torch.Tens
         result = _RawWalk(o, _capped(o.far_counts, 0, far_caps), _capped(o.near_counts, 0, near_caps),
                           far_caps, near_caps)
         if o.queue_overflow:
-            _emit(retry_logger, "queue_overflow", attempt, Q, int(far_caps.max()), result)
+            _emit(retry_logger, "queue_overflow", attempt, Q, k_suggest, result)
             continue
         if attempt > 1:
-            _emit(retry_logger, "success", attempt, Q, int(far_caps.max()), result)
+            _emit(retry_logger, "success", attempt, Q, k_suggest, result)
         break
     assert result is not None
     if result.outcome.queue_overflow:
@@ -926,33 +929,78 @@ def _to_neighbors(raw: Optional[_RawWalk], dt: DeviceTopology) -> NodeNeighborLi
         target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev), target_block_size=0)
 
 
-def _to_result(raw: Optional[_RawWalk], dt: DeviceTopology, inter: NodeInteractionList,
-               nbrs: NodeNeighborList) -> DualTreeWalkResult:
+def _node_order_rows(values_lm: torch.Tensor, inter: NodeInteractionList,
+                     node_off: torch.Tensor) -> torch.Tensor:
+    """Rows of a level-major far array rearranged into node order."""
+    F = values_lm.numel()
+    if F == 0:
+        return values_lm.clone()
+    tgt = inter.targets.long()
+    rank = torch.arange(F, dtype=torch.int64, device=values_lm.device)
+    pos = node_off[tgt].long() + (rank - inter.offsets[tgt].long())
+    out = torch.empty_like(values_lm)
+    out[pos] = values_lm
+    return out
+
+
+def _padded(values: torch.Tensor, length: int) -> torch.Tensor:
+    out = torch.full((max(int(length), 1),), -1, dtype=values.dtype, device=values.device)
+    out[:values.numel()] = values
+    return out
+
+
+def _to_result(raw, dt: DeviceTopology, inter: NodeInteractionList, nbrs: NodeNeighborList,
+               tags: Optional[torch.Tensor] = None) -> DualTreeWalkResult:
+    """The reference's raw DualTreeWalkResult (_raw_to_result, 4606-4627).
+
+    Fixed-capacity path (the caller gave capacities): interaction_offsets is
+    the node-order exclusive cumsum of the counts while sources, targets and
+    tags are level-major (quirk B9, 1787-1847), padded with -1 to
+    T * max_interactions_per_node; neighbour indices in native leaf-row
+    order padded with -1 to L * max_neighbors_per_leaf (1206-1236). Count
+    path: node-order offsets with node-order rows of exact length
+    (_dual_tree_walk_compact_fill_impl, 3089-3111, 3540-3558). ``tags`` are
+    level-major far tags (-1 without them)."""
     dev = dt.parent.device
     idx = INDEX_DTYPE
-    T = dt.num_nodes
+    T, L = dt.num_nodes, dt.num_leaves
     counts = inter.counts
     node_off = torch.zeros(T + 1, dtype=idx, device=dev)
     node_off[1:] = torch.cumsum(counts, 0)
+    caps = None
     if isinstance(raw, _FlatResult):
         dec = raw.stats["decisions"]
-        o = WalkOutcome(None, None, None, None, (0, None), (0, None), raw.stats["gen_sizes"],
-                        False, False, False, dec, raw.stats.get("queue_capacity", 0))
+        caps = raw.stats.get("attempt_caps")
+    elif raw is not None:
+        dec = raw.outcome.decisions
+        if raw.outcome.far_layout[0] > 0:
+            caps = (raw.outcome.queue_capacity, raw.outcome.far_layout[0],
+                    raw.outcome.near_layout[0])
     else:
-        o = raw.outcome if raw is not None else None
+        dec = (0, 0, 0)
+    F = int(inter.sources.numel())
+    if tags is None:
+        tags = torch.full((F,), -1, dtype=idx, device=dev)
+    tags = tags.to(idx)
+    if caps is not None:
+        _, K, Kn = caps
+        src, tgt = _padded(inter.sources, T * K), _padded(inter.targets, T * K)
+        tags = _padded(tags, T * K)
+        nb = _padded(nbrs.neighbors, L * Kn)
+    elif F:
+        src = _node_order_rows(inter.sources, inter, node_off)
+        tgt = _node_order_rows(inter.targets, inter, node_off)
+        tags = _node_order_rows(tags, inter, node_off)
+        nb = nbrs.neighbors
+    else:
+        src, tgt, nb = inter.sources, inter.targets, nbrs.neighbors
     return DualTreeWalkResult(
-        interaction_offsets=node_off, interaction_sources=inter.sources,
-        interaction_targets=inter.targets,
-        interaction_tags=torch.full((inter.sources.numel(),), -1, dtype=idx, device=dev),
-        interaction_counts=counts, neighbor_offsets=nbrs.offsets,
-        neighbor_indices=nbrs.neighbors, neighbor_counts=nbrs.counts,
-        leaf_indices=nbrs.leaf_indices, far_pair_count=int(inter.sources.numel()),
-        near_pair_count=int(nbrs.neighbors.numel()),
-        queue_overflow=bool(o.queue_overflow) if o else False,
-        far_overflow=bool(o.far_overflow) if o else False,
-        near_overflow=bool(o.near_overflow) if o else False,
-        accept_decisions=o.decisions[0] if o else 0, near_decisions=o.decisions[1] if o else 0,
-        refine_decisions=o.decisions[2] if o else 0)
+        interaction_offsets=node_off, interaction_sources=src, interaction_targets=tgt,
+        interaction_tags=tags, interaction_counts=counts, neighbor_offsets=nbrs.offsets,
+        neighbor_indices=nb, neighbor_counts=nbrs.counts, leaf_indices=nbrs.leaf_indices,
+        far_pair_count=F, near_pair_count=int(nbrs.neighbors.numel()),
+        queue_overflow=False, far_overflow=False, near_overflow=False,
+        accept_decisions=int(dec[0]), near_decisions=int(dec[1]), refine_decisions=int(dec[2]))
 
 
 def build_interactions_and_neighbors(
@@ -1054,9 +1102,21 @@ def build_interactions_and_neighbors(
 
 
 def _compact_far_pairs(inter: NodeInteractionList, res) -> CompactTaggedFarPairs:
+    """Compact pairs in the level-major order of ``inter``. The result's tags
+    are level-major on the fixed-capacity path and node-order on the count
+    path; where the two row orders differ, each level-major entry reads its
+    tag at its node-order position."""
     n = int(inter.sources.numel())
-    tags = res.interaction_tags[:n] if res is not None else torch.full(
-        (n,), -1, dtype=INDEX_DTYPE, device=inter.sources.device)
+    dev = inter.sources.device
+    if res is None:
+        tags = torch.full((n,), -1, dtype=INDEX_DTYPE, device=dev)
+    elif n == 0 or torch.equal(res.interaction_targets[:n], inter.targets):
+        tags = res.interaction_tags[:n]
+    else:
+        tgt = inter.targets.long()
+        rank = torch.arange(n, dtype=torch.int64, device=dev)
+        pos = res.interaction_offsets[tgt].long() + (rank - inter.offsets[tgt].long())
+        tags = res.interaction_tags[pos]
     return CompactTaggedFarPairs(inter.sources, inter.targets, tags.to(INDEX_DTYPE),
                                  torch.tensor(n, dtype=INDEX_DTYPE,
                                               device=inter.sources.device))

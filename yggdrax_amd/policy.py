@@ -249,17 +249,16 @@ def build_with_pair_policy(tree, geometry, theta, mac_type, dehnen_radius_scale,
     stats = dict(peak=res["peak"], gen_sizes=res["gen_sizes"], decisions=res["decisions"],
                  max_far=int(mx[0]), max_near=int(mx[1]), far_total=nf, near_total=nn)
     if not lad.use_count:
-        q = _emulate_ladder(lad, res["peak"], stats["max_far"], stats["max_near"])
-        if q is None:
+        caps = _emulate_ladder(lad, res["peak"], stats["max_far"], stats["max_near"])
+        if caps is None:
             raise RuntimeError("overflow order of this capacity configuration needs the "
                                "per-generation policy walk with capacities")
-        stats["queue_capacity"] = q
+        stats["queue_capacity"] = caps[0]
+        stats["attempt_caps"] = caps
     fr = _FlatResult(fk, fv, nv, far_off, near_off, stats)
     inter, nbrs = _flat_outputs(fr, dt, topo)
     if return_result:
-        r = _to_result(fr, dt, inter, nbrs)
-        r = r._replace(interaction_tags=tags.to(INDEX_DTYPE))
-        return inter, nbrs, r
+        return inter, nbrs, _to_result(fr, dt, inter, nbrs, tags=tags)
     return inter, nbrs
 
 
