@@ -38,3 +38,27 @@ def test_c1_fixture_known_totals():
     off, cnt, src = z["far_offsets"], z["far_counts"], z["far_sources"]
     pairs = {(t, int(s)) for t in range(len(cnt)) for s in src[off[t]:off[t] + cnt[t]]}
     assert all((s, t) in pairs for t, s in pairs)
+
+
+def test_fullsize_fixtures_reproduce_on_sampled_systems():
+    """The digest fixtures of the full-size configurations come from the
+    oracle: the 2^20 Plummer system and sampled systems of both batches are
+    recomputed here and must match; the 2^24 fixture's totals are the
+    clustered walk's."""
+    from tests.golden import fullsize as F
+    from tests.golden import make_fullsize as MF
+    from yggdrax_amd import inputs
+    g = os.path.join(os.path.dirname(FIXTURE))
+    c2 = F.load(os.path.join(g, MF.FILES["c2"]))
+    assert not F.compare(MF.compute("c2")[0], c2, 0)
+    for name, picks in (("c4", (0, 17, 511, 1023)), ("c5", (3, 255))):
+        num, seed = MF.BATCHES[name]
+        fx = F.load(os.path.join(g, MF.FILES[name]))
+        assert all(v.shape == (num,) for v in fx.values())
+        pos, mass, off = inputs.plummer_batch(num, seed)
+        for k in picks:
+            a, b = int(off[k]), int(off[k + 1])
+            got = F.oracle_system(pos[a:b], mass[a:b], octree=True, leaf=32, theta=0.6)
+            assert not F.compare(got, fx, k), (name, k)
+    c3 = F.load(os.path.join(g, MF.FILES["c3"]))
+    assert int(c3["far_total"][0]) == 147891858 and int(c3["near_total"][0]) == 329099386

@@ -1,6 +1,8 @@
 """The benchmark workload at full size (2^24 clustered points, build_octree
-leaf 64, dehnen theta 0.5), checked through properties that do not need the
-oracle to redo the walk:
+leaf 64, dehnen theta 0.5): every output field against the oracle's digest
+fixture (tests/golden/c3_clustered_2p24.npz, written on the CPU by
+tests/golden/make_fullsize.py), the learned-capacity run against the same
+fixture, and properties that hold whatever the oracle says:
 
 * the CSR lists tile their entry arrays exactly;
 * far and near lists are symmetric (the walk emits both directions);
@@ -14,14 +16,20 @@ oracle to redo the walk:
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 import torch
 
 from tests import parity
+from tests.golden import fullsize as FS
 from yggdrax_amd import inputs
 
 pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                       "c3_clustered_2p24.npz")
 
 
 def _ancestors(parent, node):
@@ -31,25 +39,63 @@ def _ancestors(parent, node):
     return out
 
 
-def test_clustered_2p24_list_properties():
+@pytest.fixture(scope="module")
+def c3():
     import yggdrax_amd as ygg
-    theta = 0.5
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    print("[fullsize] c3 clustered 2^24", flush=True)
     pos, mass = inputs.clustered(1 << 24, 3)
     tree, ps, _, _ = ygg.build_octree(torch.from_numpy(pos).cuda(),
                                       torch.from_numpy(mass).cuda(), leaf_size=64,
                                       return_reordered=True)
     geom = ygg.compute_tree_geometry(tree, ps)
-    inter, nbrs = ygg.build_interactions_and_neighbors(
-        tree, geom, theta=theta, mac_type="dehnen", neighbor_positions=False,
-        strict_int32=False)
+    inter, nbrs, res = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.5, mac_type="dehnen", neighbor_positions=False,
+        strict_int32=False, return_result=True)
     torch.cuda.synchronize()
+    return tree, geom, inter, nbrs, res
+
+
+def test_clustered_2p24_every_field_matches_oracle(c3):
+    tree, geom, inter, nbrs, res = c3
+    fx = FS.load(FIXTURE)
+    dec = (res.accept_decisions, res.near_decisions, res.refine_decisions)
+    got = FS.gpu_system(tree, geom, inter, nbrs, octree=True, decisions=dec)
+    print("[fullsize] far", got["far_total"], "near", got["near_total"], flush=True)
+    bad = FS.compare(got, fx, 0)
+    assert not bad, bad
+
+
+def test_clustered_2p24_learned_capacity_matches_oracle(c3):
+    """The one-pass walk sized by the capacity model: no counting walk, and
+    every list field equal to the oracle's."""
+    import yggdrax_amd as ygg
+    from yggdrax_amd import learned
+    from yggdrax_amd.interactions import FLAT_STATS
+    tree, geom, _, _, _ = c3
+    model = learned.CapacityModel.load()
+    hint = model.capacity_hint(tree, geom)
+    c0 = FLAT_STATS["count_walks"]
+    inter, nbrs = ygg.build_interactions_and_neighbors(
+        tree, geom, theta=0.5, mac_type="dehnen", neighbor_positions=False,
+        strict_int32=False, capacity_hint=hint)
+    torch.cuda.synchronize()
+    assert FLAT_STATS["count_walks"] == c0, "the learned hint fell back to counting"
+    got = FS.gpu_system(tree, geom, inter, nbrs, octree=True)
+    bad = FS.compare(got, FS.load(FIXTURE), 0)
+    assert not bad, bad
+
+
+def test_clustered_2p24_list_properties(c3):
+    tree, geom, inter, nbrs, _ = c3
+    theta = 0.5
     T = tree.num_nodes
     L = tree.num_leaves
     NI = T - L
     F = inter.sources.numel()
     N = nbrs.neighbors.numel()
     print("far", F, "near", N, flush=True)
-    assert F == 147891858 and N == 329099386  # the benchmark's totals
 
     # CSR tiling: far rows are placed level-major, near rows in native order
     cnt = inter.counts.long()
