@@ -9,18 +9,19 @@ outputs in the reference's layout (int64 indices). Metric: Mpts/s =
 N / wall time per step.
 
 Prints ONE JSON line. The roofline object is for the dominant kernel (the
-walk's generation kernel); its algorithmic bytes use the per-pair model
-2 x 32 B node records + 8 B queue read per evaluated pair, 8 B per pushed
-child pair and 4 B per emitted list entry (an atomic slot plus the entry in
-the fill pass). `build_roofline` prices sort + build + geometry at 287
-compulsory bytes per point against HBM peak. The `learned` object compares
+walk's generation kernel, emission pass); its algorithmic bytes use the
+per-pair model 2 x 28 B node records + 8 B queue read per evaluated pair,
+8 B per pushed child pair and 4 B per emitted list entry, with the measured
+HBM traffic of the same kernel beside it. `build_roofline` prices sort +
+build + geometry at 287 compulsory bytes per point against HBM peak. The `learned` object compares
 the walk with learned capacities (one emission walk) against the plain
 count-then-fill walk on the same tree and checks the lists are identical.
 The `layouts` object times the downstream far-field layouts (level-major
 geometry, dense M2L buffers, displacement classes) on the 2^20-point
 Plummer workload, the `moments` object the per-node mass moments of the
-benchmark's own tree. The CPU baseline is the oracle (C++, single thread) on
-the same 2^24-point workload (build, geometry and the counting walk).
+benchmark's own tree. The CPU baseline is the oracle (C++) on the same
+2^24-point workload (build, geometry and the walk with every list entry), on
+one thread and on every host thread available (OpenMP), with the CPU model.
 """
 
 from __future__ import annotations
@@ -108,9 +109,7 @@ def walk_roofline(ygg, torch, pos, mass, leaf, theta):
     pairs = acc + near + ref
     pushes = sum(rec["gen_sizes"][1:])
     entries = rec["far_total"] + rec["near_total"]
-    # per pair: pair read 8 B + two 32 B node records; 8 B per pushed child
-    # pair; 8 B (row key + value) per emitted entry
-    total_bytes = pairs * (8 + 2 * 32) + pushes * 8 + entries * 8
+    total_bytes = (pairs * PAIR_BYTES + pushes * PUSH_BYTES + entries * ENTRY_BYTES)
     gen_ms = [m for m, g in zip(rec["gen_ms"], rec["gen_sizes"]) if g > 0]
     total_ms = sum(gen_ms)
     launches = len(gen_ms)
@@ -125,6 +124,12 @@ def walk_roofline(ygg, torch, pos, mass, leaf, theta):
                 pairs=pairs, pairs_per_s=round(pairs / (total_ms * 1e-3)),
                 bytes_per_pair=round(total_bytes / max(1, pairs), 2),
                 emit_walk_ms=round(total_ms, 4), count_walk_ms=round(count_ms, 4))
+
+
+# Algorithmic bytes of the walk: per evaluated pair two 28 B node records
+# (centre 12, extent 4, children 8, leaf row 4) and the 8 B queue read; 8 B
+# per pushed child pair; 4 B per emitted directed list entry (int32).
+PAIR_BYTES, PUSH_BYTES, ENTRY_BYTES = 2 * 28 + 8, 8, 4
 
 
 # Compulsory bytes per point of sort + build + geometry with 8-bit radix
@@ -208,25 +213,101 @@ def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
                 near_overalloc=round(hint["near"] / max(1, nn), 3))
 
 
+_MIX_C = 0x9E3779B97F4A7C15 - (1 << 64)
+_MIX_1 = 0xBF58476D1CE4E5B9 - (1 << 64)
+_MIX_2 = 0x94D049BB133111EB - (1 << 64)
+
+
+def _mix(z):
+    """splitmix64 finaliser on int64 bits (logical shifts by masking)."""
+    z = z + _MIX_C
+    z = (z ^ ((z >> 30) & ((1 << 34) - 1))) * _MIX_1
+    z = (z ^ ((z >> 27) & ((1 << 37) - 1))) * _MIX_2
+    return z ^ ((z >> 31) & ((1 << 33) - 1))
+
+
+def _entry_hash(torch, system, row, value, salt):
+    return _mix(_mix((system * (1 << 32) + row) * _MIX_C + salt) ^ value.long())
+
+
+def _batch_digests(torch, lists):
+    """Per-system digests [S, 4] of a BatchedLists: far (target, source)
+    entries, near (leaf row, neighbour) entries, far counts, near counts;
+    order-free within rows. Two results are equal system k exactly when row
+    k of their digests is (up to 64-bit hash collisions)."""
+    lay = lists.layout
+    dev = lists.sources.device
+    S = lay.num_systems
+    out = torch.zeros((S, 4), dtype=torch.int64, device=dev)
+    node_sys = torch.repeat_interleave(torch.arange(S, device=dev),
+                                       torch.from_numpy(np.diff(lay.node_off)).to(dev))
+    leaf_sys = torch.repeat_interleave(torch.arange(S, device=dev),
+                                       torch.from_numpy(np.diff(lay.leaf_off)).to(dev))
+    node_loc = torch.arange(node_sys.numel(), device=dev) - torch.from_numpy(
+        lay.node_off[:-1]).to(dev)[node_sys]
+    leaf_loc = torch.arange(leaf_sys.numel(), device=dev) - torch.from_numpy(
+        lay.leaf_off[:-1]).to(dev)[leaf_sys]
+    fb = torch.from_numpy(np.asarray(lists.far_base, np.int64)).to(dev)
+    nb = torch.from_numpy(np.asarray(lists.near_base, np.int64)).to(dev)
+    e = torch.arange(lists.sources.numel(), device=dev)
+    esys = torch.searchsorted(fb, e, right=True) - 1
+    out[:, 0].index_add_(0, esys, _entry_hash(torch, esys, lists.targets.long(),
+                                              lists.sources, 1))
+    ncnt = lists.near_counts.long()
+    nrow = torch.repeat_interleave(leaf_loc, ncnt)
+    nsys = torch.repeat_interleave(leaf_sys, ncnt)
+    out[:, 1].index_add_(0, nsys, _entry_hash(torch, nsys, nrow, lists.neighbors, 2))
+    out[:, 2].index_add_(0, node_sys, _entry_hash(torch, node_sys, node_loc,
+                                                  lists.far_counts, 3))
+    out[:, 3].index_add_(0, leaf_sys, _entry_hash(torch, leaf_sys, leaf_loc,
+                                                  lists.near_counts, 4))
+    del nb
+    return out
+
+
+def _single_digests(torch, outs):
+    """The same digests for single-system results (inter, nbrs), one row per
+    system."""
+    rows = []
+    for k, (inter, nbrs) in enumerate(outs):
+        dev = inter.counts.device
+        kk = torch.full((1,), k, dtype=torch.int64, device=dev)
+        T, L = inter.counts.numel(), nbrs.counts.numel()
+        nrow = torch.repeat_interleave(torch.arange(L, device=dev), nbrs.counts.long())
+        rows.append(torch.stack([
+            _entry_hash(torch, kk, inter.targets.long(), inter.sources, 1).sum(),
+            _entry_hash(torch, kk, nrow, nbrs.neighbors, 2).sum(),
+            _entry_hash(torch, kk, torch.arange(T, device=dev), inter.counts, 3).sum(),
+            _entry_hash(torch, kk, torch.arange(L, device=dev), nbrs.counts, 4).sum()]))
+    return torch.stack(rows)
+
+
+def _mismatches(torch, a, b):
+    return int((a != b).any(dim=1).sum())
+
+
 def batched_leg(torch, steps, warmup):
     """Batched workload: 1024 Plummer systems, N_i log-uniform in [256, 65536]
-    (PCG64 seed 4), build_octree leaf 32, dehnen theta 0.6. Times the batched
-    pipeline (one set of launches and one forest walk for all systems)
-    against the same systems run one call at a time through the single-system
-    path, and against the reference's fixed-capacity method (per-row slots of
-    2048 far and 2048 near entries, its automatic pair-queue ladder, retry on
-    overflow) run
-    one call per system, and checks a sample of systems' lists are
-    identical across all of them."""
+    (PCG64 seed 4), build_octree leaf 32, dehnen theta 0.6. Times, on the same
+    systems: the plain batched pipeline (counting walk, then the emission
+    walk, over the forest), the learned batched pipeline (capacity model,
+    then one emission walk) -- the equal-batching comparison of the learned
+    capacities --, its HIP-graph capture replayed on new positions every step
+    with stream capacities from the capacity model, one single-system call
+    per system, and the reference's fixed-capacity method (per-row slots of
+    2048 far and 2048 near entries, its automatic pair-queue ladder, retry
+    on overflow) one call per system. Every system's lists are compared
+    across all of them (per-system digests)."""
     from yggdrax_amd import batch, inputs
     import yggdrax_amd as ygg
     pos_np, mass_np, offsets = inputs.plummer_batch(1024, 4)
     P = torch.from_numpy(pos_np).cuda()
     M = torch.from_numpy(mass_np).cuda()
+    S = len(offsets) - 1
     kw = dict(mac_type="dehnen")
 
-    def run_batched():
-        bt = batch.build_octree_batched(P, M, offsets, leaf_size=32)
+    def run_batched(p=P):
+        bt = batch.build_octree_batched(p, M, offsets, leaf_size=32)
         g = batch.compute_tree_geometry_batched(bt)
         return bt, batch.build_interactions_and_neighbors_batched(bt, g, 0.6, **kw)
 
@@ -271,56 +352,45 @@ def batched_leg(torch, steps, warmup):
     torch.cuda.synchronize()
     t_l = (time.perf_counter() - t0) / steps * 1e3
     learned_fallbacks = I.FLAT_STATS["count_walks"] - c0
-    # the same learned batched pipeline captured once as a HIP graph, replayed
+    # the learned batched pipeline captured once as a HIP graph (stream
+    # capacities from the capacity model), replayed on new positions: a
+    # small random displacement of every point per step, inputs in HBM
     graph = batch.BatchedGraph(P, M, offsets, leaf_size=32, theta=0.6, model=model, **kw)
-    graph.replay()
+    rng = np.random.default_rng(404)
+    moved = [P + torch.from_numpy(rng.normal(0.0, 1e-4, pos_np.shape).astype(np.float32)).cuda()
+             for _ in range(steps)]
+    graph.replay(moved[0])
     torch.cuda.synchronize()
+    fell_back = 0
     t0 = time.perf_counter()
-    for _ in range(steps):
-        _, _, glists = graph.replay()
+    for k in range(steps):
+        _, _, glists = graph.replay(moved[k])
+        fell_back += int(graph.last_replay_fell_back)
     torch.cuda.synchronize()
     t_g = (time.perf_counter() - t0) / steps * 1e3
-    graph_fell_back = graph.last_replay_fell_back
+    g_dig = _batch_digests(torch, glists)
+    _, moved_lists = run_batched(moved[-1])
+    moved_dig = _batch_digests(torch, moved_lists)
     run_one(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    outs = [run_one(k) for k in range(len(offsets) - 1)]
+    outs = [run_one(k) for k in range(S)]
     torch.cuda.synchronize()
     t_s = (time.perf_counter() - t0) * 1e3
     run_one(0, **fixed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fouts = [run_one(k, **fixed) for k in range(len(offsets) - 1)]
+    fouts = [run_one(k, **fixed) for k in range(S)]
     torch.cuda.synchronize()
     t_f = (time.perf_counter() - t0) * 1e3
-    same = True
-    for k in range(0, len(offsets) - 1, 97):
-        si, sn = outs[k]
-        fi, fn = fouts[k]
-        same &= bool(torch.equal(si.counts, fi.counts) and torch.equal(sn.counts, fn.counts)
-                     and _same_lists(torch, (si.offsets, si.counts, si.sources),
-                                     (fi.offsets, fi.counts, fi.sources), si.counts.numel())
-                     and _same_lists(torch, (sn.offsets, sn.counts, sn.neighbors),
-                                     (fn.offsets, fn.counts, fn.neighbors), sn.counts.numel()))
-        bi, bn = lists.system(k)
-        li, ln = llists.system(k)
-        gi, gn = glists.system(k)
-        same &= bool(torch.equal(bi.counts, gi.counts) and torch.equal(bn.counts, gn.counts)
-                     and _same_lists(torch, (bi.offsets, bi.counts, bi.sources),
-                                     (gi.offsets, gi.counts, gi.sources), bi.counts.numel())
-                     and _same_lists(torch, (bn.offsets, bn.counts, bn.neighbors),
-                                     (gn.offsets, gn.counts, gn.neighbors), bn.counts.numel()))
-        same &= bool(torch.equal(bi.counts, li.counts) and torch.equal(bn.counts, ln.counts)
-                     and _same_lists(torch, (bi.offsets, bi.counts, bi.sources),
-                                     (li.offsets, li.counts, li.sources), bi.counts.numel())
-                     and _same_lists(torch, (bn.offsets, bn.counts, bn.neighbors),
-                                     (ln.offsets, ln.counts, ln.neighbors), bn.counts.numel()))
-        same &= bool(torch.equal(si.counts, bi.counts) and torch.equal(si.offsets, bi.offsets)
-                     and torch.equal(sn.counts, bn.counts) and torch.equal(sn.offsets, bn.offsets)
-                     and _same_lists(torch, (si.offsets, si.counts, si.sources),
-                                     (bi.offsets, bi.counts, bi.sources), si.counts.numel())
-                     and _same_lists(torch, (sn.offsets, sn.counts, sn.neighbors),
-                                     (bn.offsets, bn.counts, bn.neighbors), sn.counts.numel()))
+    b_dig = _batch_digests(torch, lists)
+    l_dig = _batch_digests(torch, llists)
+    s_dig = _single_digests(torch, outs)
+    f_dig = _single_digests(torch, fouts)
+    mism = dict(learned_vs_plain_batched=_mismatches(torch, l_dig, b_dig),
+                graph_replay_vs_eager_on_new_positions=_mismatches(torch, g_dig, moved_dig),
+                per_system_vs_batched=_mismatches(torch, s_dig, b_dig),
+                fixed_capacity_vs_per_system=_mismatches(torch, f_dig, s_dig))
     n = int(offsets[-1])
     return dict(workload="1024 Plummer systems, N_i log-uniform [256, 65536] (seed 4), "
                          "build_octree leaf 32, dehnen theta 0.6",
@@ -328,19 +398,23 @@ def batched_leg(torch, steps, warmup):
                 learned_batched_ms=round(t_l, 3),
                 learned_batched_mpts_s=round(n / t_l / 1e3, 3),
                 learned_fallback_walks=learned_fallbacks,
+                learned_speedup_vs_batched_plain=round(t_b / t_l, 3),
                 graph_replay_ms=round(t_g, 3), graph_replay_mpts_s=round(n / t_g / 1e3, 3),
-                graph_replay_fell_back=graph_fell_back,
-                graph_speedup_vs_per_system=round(t_s / t_g, 2),
+                graph_replay_positions="new positions every step (N(0, 1e-4) displacement)",
+                graph_capacities=dict(graph.caps, generations=graph.generations,
+                                      source="capacity model"),
+                graph_replays_fell_back=fell_back,
                 learned_far_overalloc=round(hint["far"] / max(1, lists.stats["far_total"]), 3),
                 learned_near_overalloc=round(hint["near"] / max(1, lists.stats["near_total"]), 3),
                 per_system_loop_ms=round(t_s, 3), speedup_vs_per_system=round(t_s / t_b, 2),
                 learned_speedup_vs_per_system=round(t_s / t_l, 2),
-                learned_speedup_vs_batched_plain=round(t_b / t_l, 3),
+                graph_speedup_vs_per_system=round(t_s / t_g, 2),
                 fixed_capacity_per_system_ms=round(t_f, 3),
                 learned_speedup_vs_fixed_capacity=round(t_f / t_l, 2),
                 far_entries=int(lists.stats["far_total"]),
                 near_entries=int(lists.stats["near_total"]), fallback_systems=len(bt.fallback),
-                sampled_systems_identical=same)
+                systems_compared=S, list_mismatches=mism,
+                all_systems_identical=not any(mism.values()))
 
 
 def moments_leg(ygg, torch, pos, mass, leaf, reps=5):
@@ -444,22 +518,56 @@ def layouts_cpu_baseline(torch, state, cpu_pairs=1 << 21):
                        f"{m} far pairs, {t_cpu:.2f} s", gpu_identical=same)
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(int(env), n) if env and env.isdigit() else n)
+
+
 def cpu_baseline(sample_n, leaf, theta, pos=None, mass=None):
+    """The oracle (C++) on the same clustered workload: build_octree,
+    compute_tree_geometry and the walk with every list entry, on one thread
+    and on every host thread this process may use (OpenMP; the outputs do
+    not depend on the thread count)."""
     import oracle
+    from oracle import oracle as O
     from yggdrax_amd import inputs
     if pos is None or pos.shape[0] != sample_n:
         pos, mass = inputs.clustered(sample_n, 3)
-    t0 = time.perf_counter()
-    topo = oracle.build_octree(pos, mass, leaf_size=leaf)
-    geom = oracle.compute_tree_geometry(topo, topo["positions_sorted"])
-    from oracle import oracle as O
-    # the oracle's count walk (no int32 guard needed: it only counts)
-    cap = max(topo["parent"].shape[0] * 128, 1 << 16)
-    O.walk(topo, geom, theta, "dehnen", queue_cap=cap, far_cap=1 << 62, near_cap=1 << 62)
-    dt = time.perf_counter() - t0
-    return dict(value=round(sample_n / dt / 1e6, 4), unit="Mpts/s", cores=1, kind="port",
-                sample=f"oracle (C++, one thread) build_octree + geometry + counting walk on "
-                       f"{sample_n} points of the same clustered workload (seed 3), {dt:.1f} s")
+
+    def run(threads):
+        O.lib.ygo_set_threads(threads)
+        t0 = time.perf_counter()
+        topo = oracle.build_octree(pos, mass, leaf_size=leaf)
+        geom = oracle.compute_tree_geometry(topo, topo["positions_sorted"])
+        cap = max(topo["parent"].shape[0] * 128, 1 << 16)
+        w = O.walk(topo, geom, theta, "dehnen", queue_cap=cap, far_cap=1 << 62,
+                   near_cap=1 << 62)
+        dt = time.perf_counter() - t0
+        return dt, int(w["far_counts"].sum()) + int(w["near_counts"].sum())
+
+    threads = _host_threads()
+    d1, e1 = run(1)
+    dn, en = run(threads)
+    O.lib.ygo_set_threads(threads)
+    model = _cpu_model()
+    what = (f"oracle (C++) build_octree + geometry + walk with every list entry on {sample_n} "
+            f"points of the same clustered workload (seed 3); {model}")
+    return dict(value=round(sample_n / dn / 1e6, 4), unit="Mpts/s", cores=threads, kind="port",
+                sample=f"{what}, {threads} threads, {dn:.1f} s", cpu_model=model,
+                single_thread=dict(value=round(sample_n / d1 / 1e6, 4), unit="Mpts/s", cores=1,
+                                   seconds=round(d1, 2)),
+                entries=en, same_entries_1_and_n_threads=bool(e1 == en))
 
 
 def main():
@@ -522,6 +630,10 @@ def main():
     _log(f"stages {stages}")
     roof = walk_roofline(ygg, torch, pos, mass, args.leaf, args.theta)
     roof["traffic"] = measured_traffic(args.log2n, args.leaf, args.theta)
+    if roof["traffic"]:
+        hbm = roof["traffic"] / (roof["avg_launch_ms"] * 1e-3) / 1e9
+        roof["hbm_measured_gbs"] = round(hbm, 1)
+        roof["hbm_measured_frac"] = round(hbm / HBM_PEAK_GBS, 4)
     _log(f"roofline {roof}")
     build_roof = build_roofline(n, stages)
     _log(f"build roofline {build_roof}")

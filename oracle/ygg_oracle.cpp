@@ -17,6 +17,10 @@
 #include <cstring>
 #include <numeric>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#include <parallel/algorithm>
+#endif
 
 namespace {
 
@@ -96,8 +100,13 @@ inline int64_t jax_searchsorted(const uint64_t* a, int64_t n, uint64_t q,
 template <class K>
 void stable_argsort(const K* keys, int64_t n, int64_t* perm) {
   for (int64_t i = 0; i < n; ++i) perm[i] = i;
-  std::stable_sort(perm, perm + n,
-                   [keys](int64_t a, int64_t b) { return keys[a] < keys[b]; });
+  auto less = [keys](int64_t a, int64_t b) { return keys[a] < keys[b]; };
+#ifdef _OPENMP
+  // a stable sort has one result, whatever the thread count
+  __gnu_parallel::stable_sort(perm, perm + n, less);
+#else
+  std::stable_sort(perm, perm + n, less);
+#endif
 }
 
 }  // namespace
@@ -514,13 +523,194 @@ struct WalkResult {
   int64_t decisions[3];  // accept, near, refine
 };
 
+}  // extern "C"
+
+namespace {
+
+// One evaluated pair's outcome inside a generation chunk.
+struct Acc { int64_t t, s; };
+struct ChunkOut {
+  std::vector<Acc> acc, nea;          // accepted and near pairs, in pair order
+  std::vector<int64_t> ct, cs;        // child pairs (min, max), in pair order
+  int64_t dec[3] = {0, 0, 0};
+};
+
+// Evaluates pairs [p0, p1) of a generation (MAC, split rule 1444-1454,
+// children as (min, max)).
+void walk_chunk(int64_t p0, int64_t p1, const std::vector<int64_t>& wt,
+                const std::vector<int64_t>& ws, int64_t NI, const int64_t* left,
+                const int64_t* right, const float* center, const float* mac_ext,
+                float theta_sq, int mac_type, ChunkOut& o) {
+  for (int64_t p = p0; p < p1; ++p) {
+    int64_t t = wt[p], s = ws[p];
+    float dx = center[t * 3 + 0] - center[s * 3 + 0];
+    float dy = center[t * 3 + 1] - center[s * 3 + 1];
+    float dz = center[t * 3 + 2] - center[s * 3 + 2];
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    float et = mac_ext[t], es = mac_ext[s];
+    bool diff = t != s;
+    bool ok;
+    if (mac_type == 2) {  // engblom (384-393)
+      float rb = std::max(et, es), rs = std::min(et, es);
+      float tt = std::sqrt(theta_sq);
+      float lhs = rb + tt * rs;
+      ok = diff && d2 > 0.0f && (lhs * lhs <= theta_sq * d2);
+    } else {  // bh / dehnen (368-382)
+      float es2 = et + es;
+      ok = diff && d2 > 0.0f && (es2 * es2 <= theta_sq * d2);
+    }
+    bool t_int = t < NI, s_int = s < NI;
+    bool near = !ok && !t_int && !s_int && diff;
+    if (ok) {
+      o.dec[0] += 1;
+      o.acc.push_back({t, s});
+      continue;
+    }
+    if (near) {
+      o.dec[1] += 1;
+      o.nea.push_back({t, s});
+      continue;
+    }
+    o.dec[2] += 1;
+    bool same = !diff;
+    bool split_t = t_int && (same || !s_int || et >= es);  // 1444-1454
+    bool split_s = s_int && (same || !t_int || es > et);
+    int64_t c[4][2];
+    int nc = 0;
+    if (split_t && split_s) {
+      int64_t tl = left[t], tr = right[t];
+      if (same) {
+        c[0][0] = tl; c[0][1] = tl;
+        c[1][0] = tl; c[1][1] = tr;
+        c[2][0] = tr; c[2][1] = tr;
+        nc = 3;
+      } else {
+        int64_t sl = left[s], sr = right[s];
+        c[0][0] = tl; c[0][1] = sl;
+        c[1][0] = tl; c[1][1] = sr;
+        c[2][0] = tr; c[2][1] = sl;
+        c[3][0] = tr; c[3][1] = sr;
+        nc = 4;
+      }
+    } else if (split_t) {
+      c[0][0] = left[t]; c[0][1] = s;
+      c[1][0] = right[t]; c[1][1] = s;
+      nc = 2;
+    } else if (split_s) {
+      c[0][0] = t; c[0][1] = left[s];
+      c[1][0] = t; c[1][1] = right[s];
+      nc = 2;
+    }
+    for (int k = 0; k < nc; ++k) {
+      int64_t a = c[k][0], b = c[k][1];
+      if (a < 0 || b < 0) continue;
+      o.ct.push_back(std::min(a, b));
+      o.cs.push_back(std::max(a, b));
+    }
+  }
+}
+
+// Appends one generation's directed entries to their rows in the serial
+// order of the reference's two scatters (1464-1566 far, 1568-1672 near):
+// every forward entry (row = pair's first node) in pair order, then every
+// backward entry. A row whose count reaches `cap` drops the entry and raises
+// the overflow flag. Rows are owned by buckets (row % kBuckets), each bucket
+// takes its entries in that same order, so the rows come out exactly as a
+// sequential loop would write them, with any number of threads.
+bool append_rows(const std::vector<ChunkOut>& outs, bool near, const int64_t* leaf_pos,
+                 int64_t cap, std::vector<int64_t>& counts,
+                 std::vector<std::vector<int64_t>>& rows) {
+  constexpr int kBuckets = 64;
+  const int64_t nch = (int64_t)outs.size();
+  const int64_t nseg = 2 * nch;  // forward segments, then backward ones
+  auto seg_list = [&](int64_t g) -> const std::vector<Acc>& {
+    const ChunkOut& o = outs[g % nch];
+    return near ? o.nea : o.acc;
+  };
+  auto row_val = [&](int64_t g, const Acc& e, int64_t& row, int64_t& val) {
+    const bool fwd = g < nch;
+    int64_t r = fwd ? e.t : e.s;
+    val = fwd ? e.s : e.t;
+    row = near ? leaf_pos[r] : r;
+  };
+  std::vector<int64_t> cnt((size_t)nseg * kBuckets, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < nseg; ++g) {
+    for (const Acc& e : seg_list(g)) {
+      int64_t row, val;
+      row_val(g, e, row, val);
+      cnt[(size_t)g * kBuckets + (size_t)(row % kBuckets)] += 1;
+    }
+  }
+  std::vector<int64_t> off((size_t)nseg * kBuckets), bstart(kBuckets + 1, 0);
+  int64_t run = 0;
+  for (int b = 0; b < kBuckets; ++b) {
+    bstart[b] = run;
+    for (int64_t g = 0; g < nseg; ++g) {
+      off[(size_t)g * kBuckets + b] = run;
+      run += cnt[(size_t)g * kBuckets + b];
+    }
+  }
+  bstart[kBuckets] = run;
+  std::vector<Acc> buf((size_t)run);
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < nseg; ++g) {
+    int64_t* o = &off[(size_t)g * kBuckets];
+    for (const Acc& e : seg_list(g)) {
+      int64_t row, val;
+      row_val(g, e, row, val);
+      buf[(size_t)o[row % kBuckets]++] = {row, val};
+    }
+  }
+  int over = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : over)
+  for (int b = 0; b < kBuckets; ++b) {
+    for (int64_t i = bstart[b]; i < bstart[b + 1]; ++i) {
+      const Acc& e = buf[(size_t)i];
+      int64_t& c = counts[e.t];
+      if (c < cap) {
+        rows[e.t].push_back(e.s);
+        ++c;
+      } else {
+        over = 1;
+      }
+    }
+  }
+  return over != 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Threads of the OpenMP regions (the outputs do not depend on it).
+void ygo_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+int ygo_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 // _interactions_impl.py:1131-1785 (_dual_tree_walk_impl) restated as a
 // generation-synchronous loop. mac_type: 0 = bh, 1 = dehnen, 2 = engblom.
 // mac_ext[T] is the per-node MAC extent (_build_mac_extents, 993-1061).
 // leaf_pos[T] maps a leaf node to its native leaf row (-1 for internal).
 // Rows hold entries in emission order; capacities follow the reference:
 // an entry whose slot reaches the cap is dropped and raises the flag, and
-// the walk stops after the generation in which any flag was raised.
+// the walk stops after the generation in which any flag was raised. Pairs
+// of a generation are evaluated in chunks on OpenMP threads and merged in
+// chunk order, so the result is the sequential one for any thread count.
 void* ygo_walk_run(int64_t T, int64_t NI, const int64_t* left,
                    const int64_t* right, const float* center,
                    const float* mac_ext, const int64_t* leaf_pos, int64_t root,
@@ -543,110 +733,38 @@ void* ygo_walk_run(int64_t T, int64_t NI, const int64_t* left,
   const int64_t Q = std::max<int64_t>(queue_cap, 4);
   std::vector<int64_t> wt{root}, ws{root};
   bool qo = false, fo = false, no = false;
-  struct Acc { int64_t t, s; };
-  std::vector<Acc> acc, nea;
+  std::vector<ChunkOut> outs;
   while (!wt.empty() && !qo && !fo && !no) {
     r->gen_sizes.push_back((int64_t)wt.size());
-    acc.clear();
-    nea.clear();
-    std::vector<int64_t> nt, ns;
+    const int64_t n = (int64_t)wt.size();
+    const int64_t nch = std::max<int64_t>(1, std::min<int64_t>(n / 2048, 512));
+    outs.assign((size_t)nch, ChunkOut());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < nch; ++c)
+      walk_chunk(n * c / nch, n * (c + 1) / nch, wt, ws, NI, left, right, center, mac_ext,
+                 theta_sq, mac_type, outs[(size_t)c]);
     int64_t pushes = 0;
-    for (size_t p = 0; p < wt.size(); ++p) {
-      int64_t t = wt[p], s = ws[p];
-      float dx = center[t * 3 + 0] - center[s * 3 + 0];
-      float dy = center[t * 3 + 1] - center[s * 3 + 1];
-      float dz = center[t * 3 + 2] - center[s * 3 + 2];
-      float d2 = dx * dx;
-      d2 = d2 + dy * dy;
-      d2 = d2 + dz * dz;
-      float et = mac_ext[t], es = mac_ext[s];
-      bool diff = t != s;
-      bool ok;
-      if (mac_type == 2) {  // engblom (384-393)
-        float rb = std::max(et, es), rs = std::min(et, es);
-        float tt = std::sqrt(theta_sq);
-        float lhs = rb + tt * rs;
-        ok = diff && d2 > 0.0f && (lhs * lhs <= theta_sq * d2);
-      } else {  // bh / dehnen (368-382)
-        float es2 = et + es;
-        ok = diff && d2 > 0.0f && (es2 * es2 <= theta_sq * d2);
-      }
-      bool t_int = t < NI, s_int = s < NI;
-      bool near = !ok && !t_int && !s_int && diff;
-      if (ok) {
-        r->decisions[0] += 1;
-        acc.push_back({t, s});
-        continue;
-      }
-      if (near) {
-        r->decisions[1] += 1;
-        nea.push_back({t, s});
-        continue;
-      }
-      r->decisions[2] += 1;
-      bool same = !diff;
-      bool split_t = t_int && (same || !s_int || et >= es);  // 1444-1454
-      bool split_s = s_int && (same || !t_int || es > et);
-      int64_t c[4][2];
-      int nc = 0;
-      if (split_t && split_s) {
-        int64_t tl = left[t], tr = right[t];
-        if (same) {
-          c[0][0] = tl; c[0][1] = tl;
-          c[1][0] = tl; c[1][1] = tr;
-          c[2][0] = tr; c[2][1] = tr;
-          nc = 3;
-        } else {
-          int64_t sl = left[s], sr = right[s];
-          c[0][0] = tl; c[0][1] = sl;
-          c[1][0] = tl; c[1][1] = sr;
-          c[2][0] = tr; c[2][1] = sl;
-          c[3][0] = tr; c[3][1] = sr;
-          nc = 4;
-        }
-      } else if (split_t) {
-        c[0][0] = left[t]; c[0][1] = s;
-        c[1][0] = right[t]; c[1][1] = s;
-        nc = 2;
-      } else if (split_s) {
-        c[0][0] = t; c[0][1] = left[s];
-        c[1][0] = t; c[1][1] = right[s];
-        nc = 2;
-      }
-      for (int k = 0; k < nc; ++k) {
-        int64_t a = c[k][0], b = c[k][1];
-        if (a < 0 || b < 0) continue;
-        if (pushes < Q) {
-          nt.push_back(std::min(a, b));
-          ns.push_back(std::max(a, b));
-        } else {
-          qo = true;
-        }
-        ++pushes;
+    for (auto& o : outs) {
+      for (int k = 0; k < 3; ++k) r->decisions[k] += o.dec[k];
+      pushes += (int64_t)o.ct.size();
+    }
+    // the first Q child pairs in pair order enter the next generation
+    std::vector<int64_t> nt, ns;
+    const int64_t kept = std::min(pushes, Q);
+    if (pushes > Q) qo = true;
+    nt.reserve((size_t)kept);
+    ns.reserve((size_t)kept);
+    for (auto& o : outs) {
+      for (size_t k = 0; k < o.ct.size() && (int64_t)nt.size() < kept; ++k) {
+        nt.push_back(o.ct[k]);
+        ns.push_back(o.cs[k]);
       }
     }
-    if (collect_far) {  // 1464-1566: all forward entries, then all backward
-      for (auto& e : acc) {
-        int64_t& cnt = r->far_counts[e.t];
-        if (cnt < far_cap) { r->far_rows[e.t].push_back(e.s); ++cnt; } else fo = true;
-      }
-      for (auto& e : acc) {
-        int64_t& cnt = r->far_counts[e.s];
-        if (cnt < far_cap) { r->far_rows[e.s].push_back(e.t); ++cnt; } else fo = true;
-      }
-    }
-    if (collect_near) {  // 1568-1672
-      for (auto& e : nea) {
-        int64_t row = leaf_pos[e.t];
-        int64_t& cnt = r->near_counts[row];
-        if (cnt < near_cap) { r->near_rows[row].push_back(e.s); ++cnt; } else no = true;
-      }
-      for (auto& e : nea) {
-        int64_t row = leaf_pos[e.s];
-        int64_t& cnt = r->near_counts[row];
-        if (cnt < near_cap) { r->near_rows[row].push_back(e.t); ++cnt; } else no = true;
-      }
-    }
+    if (collect_far && append_rows(outs, false, leaf_pos, far_cap, r->far_counts, r->far_rows))
+      fo = true;
+    if (collect_near &&
+        append_rows(outs, true, leaf_pos, near_cap, r->near_counts, r->near_rows))
+      no = true;
     wt.swap(nt);
     ws.swap(ns);
   }

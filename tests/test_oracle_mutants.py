@@ -45,8 +45,8 @@ def _build(name, tmpdir):
     with open(path, "w") as f:
         f.write(src.replace(old, new))
     so = os.path.join(tmpdir, f"libygg_oracle_{name}.so")
-    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-ffp-contract=off", "-shared",
-                    "-o", so, path], check=True)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off",
+                    "-shared", "-o", so, path], check=True)
     return name, so
 
 
