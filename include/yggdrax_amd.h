@@ -28,6 +28,7 @@ extern "C" {
 #define YGG_ERR_HIP 2
 #define YGG_ERR_WORKSPACE 3
 #define YGG_ERR_UNSUPPORTED 4
+#define YGG_ERR_TOO_LARGE 5 /* input beyond a fixed maximum (ygg_morton_sort_noalloc) */
 
 /* ---- primitives ---------------------------------------------------------- */
 
@@ -67,6 +68,22 @@ int ygg_morton_encode(const float* pos, int64_t n, const float* bounds6, uint64_
 size_t ygg_morton_sort_workspace_bytes(int64_t n);
 int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t* sorted_codes,
                     uint32_t* perm, void* ws, size_t ws_bytes, void* stream);
+
+/* ygg_morton_sort without a workspace argument and without any allocation:
+ * no counterpart in the reference. It is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its workspace is a file-scope device
+ * array of YGG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends;
+ * n above YGG_NOALLOC_MAX_POINTS returns YGG_ERR_TOO_LARGE. The stream must
+ * belong to the current device (at most YGG_NOALLOC_MAX_DEVICES devices).
+ * Every call on one GPU shares that GPU's array, so calls that overlap on
+ * one GPU (other streams or threads) race; calls ordered on one stream do
+ * not. Outputs equal ygg_morton_sort's. */
+#define YGG_NOALLOC_MAX_POINTS (1 << 20)
+#define YGG_NOALLOC_SCRATCH_BYTES (24u << 20)
+#define YGG_NOALLOC_MAX_DEVICES 64
+int ygg_morton_sort_noalloc(const float* pos, int64_t n, const float* bounds6,
+                            uint64_t* sorted_codes, uint32_t* perm, void* stream);
 
 /* ygg_morton_sort plus the reordered particles in one call: the encode
  * kernel also packs (x, y, z, m) so the gather reads one 16-byte record per

@@ -23,7 +23,8 @@ _F = ctypes.c_float
 _INT = ctypes.c_int
 
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "workspace too small",
-          4: "unsupported size"}
+          4: "unsupported size",
+          5: "input larger than the fixed maximum of this entry point"}
 
 
 class RadixTopologyPtrs(ctypes.Structure):
@@ -121,6 +122,7 @@ _SIGS = {
     "ygg_morton_encode": (_INT, [_P, _I64, _P, _P, _P]),
     "ygg_morton_sort_workspace_bytes": (_SZ, [_I64]),
     "ygg_morton_sort": (_INT, [_P, _I64, _P, _P, _P, _P, _SZ, _P]),
+    "ygg_morton_sort_noalloc": (_INT, [_P, _I64, _P, _P, _P, _P]),
     "ygg_morton_sort_reorder_workspace_bytes": (_SZ, [_I64]),
     "ygg_morton_sort_reorder": (_INT, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _INT, _P, _SZ, _P]),
     "ygg_morton_sort_reorder_async": (_INT, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _INT, _P, _SZ,

@@ -6,6 +6,8 @@
 #include "ygg_common.h"
 #include "ygg_build_dev.h"
 
+#include <mutex>
+
 namespace ygg {
 
 // ---- bounds (bounds.py:29-33) ----------------------------------------------
@@ -684,6 +686,43 @@ int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t*
                      bounds6, codes, hist);
   YGG_LAUNCH_CHECK();
   return sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, 64, sws, rest, st, hist);
+}
+
+// ---- ygg_morton_sort_noalloc: the same encode + sort with its workspace in
+// a file-scope device array, for callers that cannot allocate. The array's
+// name is a host-side stand-in: its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device on that device's first call
+// and kept, and only that address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[YGG_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[YGG_NOALLOC_MAX_DEVICES];
+
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  YGG_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= YGG_NOALLOC_MAX_DEVICES) return YGG_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    YGG_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return YGG_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return YGG_OK;
+}
+
+int ygg_morton_sort_noalloc(const float* pos, int64_t n, const float* bounds6,
+                            uint64_t* sorted_codes, uint32_t* perm, void* stream) {
+  if (n < 1 || !pos || !bounds6 || !sorted_codes || !perm) return YGG_ERR_ARG;
+  if (n > YGG_NOALLOC_MAX_POINTS) return YGG_ERR_TOO_LARGE;
+  const size_t need = ygg_morton_sort_workspace_bytes(n);
+  if (need > (size_t)YGG_NOALLOC_SCRATCH_BYTES) return YGG_ERR_TOO_LARGE;
+  void* ws = nullptr;
+  int rc = scratch_for_current_device(&ws);
+  if (rc) return rc;
+  return ygg_morton_sort(pos, n, bounds6, sorted_codes, perm, ws, need, stream);
 }
 
 size_t ygg_morton_sort_reorder_workspace_bytes(int64_t n) {
