@@ -247,21 +247,22 @@ def _batch_digests(torch, lists):
         lay.node_off[:-1]).to(dev)[node_sys]
     leaf_loc = torch.arange(leaf_sys.numel(), device=dev) - torch.from_numpy(
         lay.leaf_off[:-1]).to(dev)[leaf_sys]
+    # valid entries lead the (possibly capacity-sized) entry arrays, system
+    # after system
+    fcnt, ncnt = lists.far_counts.long(), lists.near_counts.long()
+    nf, nn = int(fcnt.sum()), int(ncnt.sum())
     fb = torch.from_numpy(np.asarray(lists.far_base, np.int64)).to(dev)
-    nb = torch.from_numpy(np.asarray(lists.near_base, np.int64)).to(dev)
-    e = torch.arange(lists.sources.numel(), device=dev)
+    e = torch.arange(nf, device=dev)
     esys = torch.searchsorted(fb, e, right=True) - 1
-    out[:, 0].index_add_(0, esys, _entry_hash(torch, esys, lists.targets.long(),
-                                              lists.sources, 1))
-    ncnt = lists.near_counts.long()
+    out[:, 0].index_add_(0, esys, _entry_hash(torch, esys, lists.targets[:nf].long(),
+                                              lists.sources[:nf], 1))
     nrow = torch.repeat_interleave(leaf_loc, ncnt)
     nsys = torch.repeat_interleave(leaf_sys, ncnt)
-    out[:, 1].index_add_(0, nsys, _entry_hash(torch, nsys, nrow, lists.neighbors, 2))
+    out[:, 1].index_add_(0, nsys, _entry_hash(torch, nsys, nrow, lists.neighbors[:nn], 2))
     out[:, 2].index_add_(0, node_sys, _entry_hash(torch, node_sys, node_loc,
                                                   lists.far_counts, 3))
     out[:, 3].index_add_(0, leaf_sys, _entry_hash(torch, leaf_sys, leaf_loc,
                                                   lists.near_counts, 4))
-    del nb
     return out
 
 
