@@ -206,9 +206,13 @@ int ygg_octree_meta(const uint64_t* sorted_codes, const int32_t* node_ranges, in
  * per leaf (max_leaf_size). level_sync = 0 takes each internal box as a
  * segment-tree range reduction over its leaves (valid for well-formed trees,
  * YggRadixTopology.scalars[3] == 0); level_sync = 1 runs the reference's
- * level-by-level pass using node_level and max_level. */
+ * level-by-level pass using node_level and max_level. particle_radius
+ * (float32 [n] in sorted order, or NULL) bounds each particle as the cube
+ * [p - r, p + r] instead of the point p (the particle_radius argument,
+ * yggdrax/_geometry_impl.py:107-112). */
 size_t ygg_geometry_workspace_bytes(int64_t num_nodes);
-int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int32_t* left,
+int ygg_geometry(const float* pos_sorted, const float* particle_radius,
+                 const int32_t* node_ranges, const int32_t* left,
                  const int32_t* right, const int32_t* parent, const int32_t* node_level,
                  int64_t num_nodes, int64_t leaf_size, int64_t leaf_cap, int level_sync,
                  int32_t max_level,
@@ -380,6 +384,26 @@ int ygg_far_output(int64_t num_nodes, const int32_t* nodes_by_level, const int64
 /* Near list CSR in native leaf order plus the dense -1 padded
  * neighbor_leaf_positions[L][max_count] (_result_to_neighbors,
  * yggdrax/_interactions_impl.py:3640-3837). */
+/* Near-field target blocks (_build_target_blocks in _result_to_neighbors,
+ * yggdrax/_interactions_impl.py:3689-3770, enabled there by
+ * YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE > 0): each native leaf row of the near
+ * CSR list (near_offsets[L + 1]) is cut into ceil(count / block_size)
+ * blocks of block_size consecutive neighbours. ygg_target_block_offsets
+ * writes the int64 exclusive prefix of the blocks per row
+ * (block_offsets[L + 1]; block_offsets[L] = number of blocks B). Then
+ * ygg_target_blocks writes block_leaf_ids[B] (the row of each block),
+ * block_source_leaf_ids[B * block_size] (native leaf row of each neighbour,
+ * 0 past the row's end) and valid_mask[B * block_size]; neighbors and the
+ * outputs are int64 (in_i64 / out_i64) or int32. */
+size_t ygg_target_block_offsets_workspace_bytes(int64_t num_leaves);
+int ygg_target_block_offsets(int64_t num_leaves, const int64_t* near_offsets, int64_t block_size,
+                             int64_t* block_offsets, void* ws, size_t ws_bytes, void* stream);
+int ygg_target_blocks(int64_t num_leaves, const int64_t* near_offsets, const void* neighbors,
+                      int in_i64, const int32_t* leaf_position, int64_t block_size,
+                      const int64_t* block_offsets, int64_t num_blocks, int out_i64,
+                      void* block_leaf_ids, void* block_source_leaf_ids, uint8_t* valid_mask,
+                      void* stream);
+
 int ygg_near_output(int64_t num_leaves, const int64_t* offsets, const int32_t* counts,
                     int32_t fixed_cap, const int64_t* row_start, const int32_t* entries,
                     const int32_t* leaf_position, int64_t max_count, int out_i64,

@@ -199,9 +199,14 @@ def build_octree(positions, masses=None, bounds=None, *, leaf_size=8):
     return topo
 
 
-def compute_tree_geometry(topo, positions_sorted, max_leaf_size=None):
-    """geometry.py:18-75 -> _geometry_impl.py:176-338."""
+def compute_tree_geometry(topo, positions_sorted, max_leaf_size=None, particle_radius=None):
+    """geometry.py:18-75 -> _geometry_impl.py:176-338; particle_radius (sorted
+    order) bounds each particle as [p - r, p + r] (107-112)."""
     pos = np.ascontiguousarray(positions_sorted, dtype=np.float32)
+    prad = (None if particle_radius is None
+            else np.ascontiguousarray(particle_radius, dtype=np.float32))
+    if prad is not None and prad.shape != (pos.shape[0],):
+        raise ValueError("particle_radius must have shape (n_particles,)")
     T = topo["parent"].shape[0]
     cap = max_leaf_size if max_leaf_size is not None else topo.get("leaf_size")
     if cap is None:
@@ -213,7 +218,8 @@ def compute_tree_geometry(topo, positions_sorted, max_leaf_size=None):
     NI = (T - 1) // 2
     left = np.ascontiguousarray(topo["left_child"], np.int64) if NI else np.zeros(1, np.int64)
     right = np.ascontiguousarray(topo["right_child"], np.int64) if NI else np.zeros(1, np.int64)
-    st = lib.ygo_geometry(_ptr(pos), _ptr(np.ascontiguousarray(topo["node_ranges"], np.int64)),
+    st = lib.ygo_geometry(_ptr(pos), None if prad is None else _ptr(prad),
+                          _ptr(np.ascontiguousarray(topo["node_ranges"], np.int64)),
                           _ptr(left), _ptr(right),
                           _ptr(np.ascontiguousarray(topo["parent"], np.int64)), _I(T), _I(int(cap)),
                           _ptr(center), _ptr(half), _ptr(radius), _ptr(maxext))
@@ -681,8 +687,9 @@ def to_interactions(raw, topo):
                 level_offsets=obl[level_idx], target_levels=levels)
 
 
-def to_neighbors(raw, topo):
-    """_result_to_neighbors (3640-3837) with the default target block size 0."""
+def to_neighbors(raw, topo, target_block_size=None):
+    """_result_to_neighbors (3640-3837); the target block size defaults to
+    YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE as in the reference (0 = none)."""
     counts = raw["near_counts"]
     L = counts.shape[0]
     leaf_indices, _, po, p2n = leaf_ordering(topo)
@@ -699,13 +706,48 @@ def to_neighbors(raw, topo):
         valid = cols[None, :] < counts[:, None]
         idx = np.where(valid, offsets[:-1, None] + cols[None, :], 0)
         nlp = np.where(valid, lookup[nbrs[idx]] if nbrs.size else -1, -1)
+    k = target_block_size
+    if k is None:
+        try:
+            k = max(0, int(os.environ.get("YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE", "0")))
+        except ValueError:
+            k = 0
+    blocks = target_blocks(counts, offsets, nlp.astype(np.int64), k)
     return dict(offsets=offsets, neighbors=nbrs, leaf_indices=leaf_indices, counts=counts,
                 particle_order_leaf_indices=po, particle_order_to_native_leaf=p2n,
                 neighbor_leaf_positions=nlp.astype(np.int64),
-                target_block_leaf_ids=np.zeros(0, np.int64),
-                target_block_source_leaf_ids=np.zeros((0, 0), np.int64),
-                target_block_valid_mask=np.zeros((0, 0), bool),
-                target_block_offsets=np.zeros(L + 1, np.int64), target_block_size=0)
+                target_block_leaf_ids=blocks[0], target_block_source_leaf_ids=blocks[1],
+                target_block_valid_mask=blocks[2], target_block_offsets=blocks[3],
+                target_block_size=int(k))
+
+
+def target_blocks(counts, offsets, nlp, k):
+    """_build_target_blocks (_interactions_impl.py:3689-3770): row r's
+    neighbour columns in blocks of k, from the dense neighbour-leaf-position
+    matrix; invalid slots hold 0."""
+    L = counts.shape[0]
+    if k <= 0 or L == 0:
+        return (np.zeros(0, np.int64), np.zeros((0, 0), np.int64), np.zeros((0, 0), bool),
+                np.zeros(L + 1, np.int64))
+    max_nbr = nlp.shape[1]
+    if max_nbr == 0:
+        return (np.zeros(0, np.int64), np.zeros((0, k), np.int64), np.zeros((0, k), bool),
+                np.zeros(L + 1, np.int64))
+    per = (counts + (k - 1)) // k
+    boff = np.concatenate([[0], np.cumsum(per)]).astype(np.int64)
+    B = int(boff[-1])
+    ids = np.arange(B, dtype=np.int64)
+    row = np.searchsorted(boff[1:], ids, side="right")
+    local = ids - boff[row]
+    start = offsets[row] + local * k
+    stop = offsets[row + 1]
+    j = np.arange(k, dtype=np.int64)
+    edge_valid = (start[:, None] + j[None, :]) < stop[:, None]
+    lidx = local[:, None] * k + j[None, :]
+    lvalid = lidx < max_nbr
+    src = nlp[row[:, None], np.where(edge_valid & lvalid, lidx, 0)]
+    ok = edge_valid & lvalid & (src >= 0)
+    return row.astype(np.int64), np.where(ok, src, 0).astype(np.int64), ok, boff
 
 
 def build_interactions_and_neighbors(topo, geom, theta=0.5, max_interactions_per_node=None,

@@ -427,10 +427,11 @@ void ygo_octree_meta(const uint64_t* sorted_codes, const int64_t* node_ranges,
 }
 
 // _geometry_impl.py:176-338 (compute_tree_geometry) for positional leaf
-// bounds. T = 2L-1 nodes, NI = L-1 internal. Outputs center[T*3],
+// bounds. T = 2L-1 nodes, NI = L-1 internal. prad[n] (sorted order) or
+// NULL is the per-particle radius. Outputs center[T*3],
 // half[T*3], radius[T], max_extent[T]. Returns 0, or 1 if the depth pointer
 // doubling did not converge (the reference would loop forever there).
-int ygo_geometry(const float* pos_sorted, const int64_t* ranges,
+int ygo_geometry(const float* pos_sorted, const float* prad, const int64_t* ranges,
                  const int64_t* left, const int64_t* right,
                  const int64_t* parent, int64_t T, int64_t leaf_cap,
                  float* center, float* half, float* radius, float* maxext) {
@@ -444,8 +445,11 @@ int ygo_geometry(const float* pos_sorted, const int64_t* ranges,
       for (int64_t m = 0; m < leaf_cap; ++m) {
         if (m >= cnt) continue;
         float v = pos_sorted[(a + m) * 3 + d];
-        lo = std::min(lo, v);
-        hi = std::max(hi, v);
+        // 107-112: with particle_radius each point is the cube [v - r, v + r]
+        float vl = prad ? v - prad[a + m] : v;
+        float vh = prad ? v + prad[a + m] : v;
+        lo = std::min(lo, vl);
+        hi = std::max(hi, vh);
       }
       mn[k * 3 + d] = lo;
       mx[k * 3 + d] = hi;

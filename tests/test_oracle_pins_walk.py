@@ -147,3 +147,39 @@ def test_raw_result_layout_and_round_trip(path):
         assert res["neighbor_indices"].shape == (M,)
         assert np.all(np.diff(res["interaction_targets"]) >= 0)
     assert np.all(res["interaction_tags"] == -1)
+
+
+def check_target_blocks(counts, nlp, k, rows, src, valid, boff):
+    """Invariants of near-field target blocks with block size k: row r has
+    ceil(count_r / k) blocks, consecutive in block order; every block but the
+    row's last is full; the valid slots of a row's blocks, read in order,
+    are exactly the row's neighbour leaf positions; invalid slots hold 0."""
+    L = len(counts)
+    per = [-(-int(c) // k) for c in counts]
+    assert list(boff) == [0] + list(np.cumsum(per))
+    assert len(rows) == boff[-1] == src.shape[0] == valid.shape[0]
+    for r in range(L):
+        blk = range(int(boff[r]), int(boff[r + 1]))
+        assert all(int(rows[b]) == r for b in blk)
+        got = [int(src[b, j]) for b in blk for j in range(k) if valid[b, j]]
+        assert got == [int(v) for v in nlp[r, :counts[r]]]
+        for q, b in enumerate(blk):
+            n_valid = int(valid[b].sum())
+            assert n_valid == (k if q < len(blk) - 1 else int(counts[r]) - k * (len(blk) - 1))
+    assert np.all(src[~valid] == 0)
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 64])
+def test_target_blocks_invariants(k):
+    t, g, theta, mac = _system("octree", lambda: inputs.plummer_sphere(3000, 4, a=0.4), 8,
+                               0.5, "dehnen")
+    _, nbrs, raw = oracle.build_interactions_and_neighbors(t, g, theta=theta, mac_type=mac)
+    nb = O.to_neighbors(raw, t, target_block_size=k)
+    assert nb["target_block_size"] == k
+    check_target_blocks(nb["counts"], nb["neighbor_leaf_positions"], k,
+                        nb["target_block_leaf_ids"], nb["target_block_source_leaf_ids"],
+                        nb["target_block_valid_mask"], nb["target_block_offsets"])
+    off = O.to_neighbors(raw, t, target_block_size=0)
+    assert off["target_block_leaf_ids"].shape == (0,)
+    assert off["target_block_offsets"].shape == (t["parent"].shape[0]
+                                                - t["num_internal_nodes"] + 1,)

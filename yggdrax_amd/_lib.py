@@ -123,6 +123,10 @@ _SIGS = {
     "ygg_morton_sort_workspace_bytes": (_SZ, [_I64]),
     "ygg_morton_sort": (_INT, [_P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "ygg_morton_sort_noalloc": (_INT, [_P, _I64, _P, _P, _P, _P]),
+    "ygg_target_block_offsets_workspace_bytes": (_SZ, [_I64]),
+    "ygg_target_block_offsets": (_INT, [_I64, _P, _I64, _P, _P, _SZ, _P]),
+    "ygg_target_blocks": (_INT, [_I64, _P, _P, _INT, _P, _I64, _P, _I64, _INT, _P, _P, _P,
+                                 _P]),
     "ygg_morton_sort_reorder_workspace_bytes": (_SZ, [_I64]),
     "ygg_morton_sort_reorder": (_INT, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _INT, _P, _SZ, _P]),
     "ygg_morton_sort_reorder_async": (_INT, [_P, _P, _I64, _P, _P, _P, _P, _P, _P, _INT, _P, _SZ,
@@ -144,7 +148,7 @@ _SIGS = {
                                      _P]),
     "ygg_grouped_classify": (_INT, [_P, _P, _I64, _INT, _I64] + [_P] * 7 + [_P, _SZ, _P]),
     "ygg_grouped_emit": (_INT, [_I64, _INT] + [_P] * 10 + [_P, _SZ, _P]),
-    "ygg_geometry": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I32, _P, _P, _P,
+    "ygg_geometry": (_INT, [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _INT, _I32, _P, _P, _P,
                             _P, _P, _SZ, _P]),
     "ygg_radix_leaf_order": (_INT, [_I64, _P, _P, _P]),
     "ygg_walk_prepare_nodes": (_INT, [_P, _P, _P, _P, _P, _P, _P, _I64, _INT, _F, _P, _P]),

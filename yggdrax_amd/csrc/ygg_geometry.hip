@@ -56,9 +56,10 @@ __device__ __forceinline__ void merge_into(Box8& a, const Box8& b) {
 // One wave per segment-tree leaf slot j < P: real leaves reduce their
 // particles, padding slots get the empty box.
 __global__ __launch_bounds__(256) void k_seg_leaves(
-    const float* __restrict__ pos, const int32_t* __restrict__ ranges, int T, int NI,
-    int leaf_cap, int P, Box8* __restrict__ seg, float* __restrict__ center,
-    float* __restrict__ half, float* __restrict__ radius, float* __restrict__ maxext) {
+    const float* __restrict__ pos, const float* __restrict__ prad,
+    const int32_t* __restrict__ ranges, int T, int NI, int leaf_cap, int P,
+    Box8* __restrict__ seg, float* __restrict__ center, float* __restrict__ half,
+    float* __restrict__ radius, float* __restrict__ maxext) {
   const int L = T - NI;
   int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE;
   const unsigned lane = lane_id();
@@ -74,11 +75,13 @@ __global__ __launch_bounds__(256) void k_seg_leaves(
   float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int m = lane; m < cnt; m += YGG_WAVE) {
     const float* p = pos + 3 * ((int64_t)a + m);
+    // a particle with a radius is bounded as the cube [p - r, p + r]
+    const float r = prad ? prad[(int64_t)a + m] : 0.0f;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       float v = p[d];
-      lo[d] = fminf(lo[d], v);
-      hi[d] = fmaxf(hi[d], v);
+      lo[d] = fminf(lo[d], prad ? v - r : v);
+      hi[d] = fmaxf(hi[d], prad ? v + r : v);
     }
   }
 #pragma unroll
@@ -295,20 +298,22 @@ __global__ void k_geom_level(const int32_t* __restrict__ left, const int32_t* __
   }
 }
 
-__global__ void k_geom_leaf_only(const float* __restrict__ pos, const int32_t* __restrict__ ranges,
-                                 int T, int NI, int leaf_cap, float* __restrict__ bmin,
-                                 float* __restrict__ bmax) {
+__global__ void k_geom_leaf_only(const float* __restrict__ pos, const float* __restrict__ prad,
+                                 const int32_t* __restrict__ ranges, int T, int NI, int leaf_cap,
+                                 float* __restrict__ bmin, float* __restrict__ bmax) {
   int64_t node = NI + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= T) return;
   int a = ranges[2 * node];
   int cnt = min(ranges[2 * node + 1] - a + 1, leaf_cap);
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int m = 0; m < cnt; ++m)
+  for (int m = 0; m < cnt; ++m) {
+    const float r = prad ? prad[(int64_t)a + m] : 0.0f;
     for (int d = 0; d < 3; ++d) {
       float v = pos[3 * ((int64_t)a + m) + d];
-      lo[d] = fminf(lo[d], v);
-      hi[d] = fmaxf(hi[d], v);
+      lo[d] = fminf(lo[d], prad ? v - r : v);
+      hi[d] = fmaxf(hi[d], prad ? v + r : v);
     }
+  }
   for (int d = 0; d < 3; ++d) {
     bmin[3 * node + d] = lo[d];
     bmax[3 * node + d] = hi[d];
@@ -353,7 +358,8 @@ size_t ygg_geometry_workspace_bytes(int64_t num_nodes) {
   return a.off;
 }
 
-int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int32_t* left,
+int ygg_geometry(const float* pos_sorted, const float* particle_radius,
+                 const int32_t* node_ranges, const int32_t* left,
                  const int32_t* right, const int32_t* parent, const int32_t* node_level,
                  int64_t num_nodes, int64_t leaf_size, int64_t leaf_cap, int level_sync,
                  int32_t max_level,
@@ -377,7 +383,8 @@ int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int3
     const int P = (int)seg_width(T);
     int64_t threads = (int64_t)P * YGG_WAVE;
     hipLaunchKernelGGL(k_seg_leaves, dim3(grid_for(threads, 256)), dim3(256), 0, st, pos_sorted,
-                       node_ranges, T, NI, lc, P, seg, center, half_extent, radius, max_extent);
+                       particle_radius, node_ranges, T, NI, lc, P, seg, center, half_extent,
+                       radius, max_extent);
     for (int S = P; S > 1; S /= 512) {
       int blocks = (S / 2 + 255) / 256;
       hipLaunchKernelGGL(k_seg_up, dim3(blocks), dim3(256), 0, st, seg, S);
@@ -392,7 +399,7 @@ int ygg_geometry(const float* pos_sorted, const int32_t* node_ranges, const int3
   hipLaunchKernelGGL(k_geom_zero_internal, dim3(grid_for(3 * (int64_t)NI + 1, 256)), dim3(256), 0,
                      st, NI, bmin, bmax);
   hipLaunchKernelGGL(k_geom_leaf_only, dim3(grid_for(L, 256)), dim3(256), 0, st, pos_sorted,
-                     node_ranges, T, NI, lc, bmin, bmax);
+                     particle_radius, node_ranges, T, NI, lc, bmin, bmax);
   for (int level = max_level - 1; level >= 0; --level) {
     for (int phase = 0; phase < 2; ++phase)
       hipLaunchKernelGGL(k_geom_level, dim3(grid_for(NI + 1, 256)), dim3(256), 0, st, left, right,

@@ -311,6 +311,48 @@ __global__ void k_min_cap_arr(int32_t* __restrict__ cnt, int64_t n, const int32_
   if (k < n && cnt[k] > cap[k]) cnt[k] = cap[k];
 }
 
+// Near-field target blocks (_build_target_blocks,
+// yggdrax/_interactions_impl.py:3689-3770): every native leaf row is cut
+// into ceil(count / k) blocks of k consecutive entries of its CSR row.
+__global__ void k_target_blocks_per_row(int64_t L, const int64_t* __restrict__ off,
+                                        int64_t k, int32_t* __restrict__ blocks) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= L) return;
+  const int64_t c = off[r + 1] - off[r];
+  blocks[r] = (int32_t)((c + k - 1) / k);
+}
+
+// One thread per (block, slot): the block's row is the last row whose first
+// block is <= b (upper bound over block_offsets[1..L]); slot j holds the
+// native leaf row of the row's neighbour local * k + j, or 0 and invalid
+// past the row's end.
+template <typename OUT, typename IN>
+__global__ __launch_bounds__(256) void k_target_blocks(
+    int64_t L, const int64_t* __restrict__ off, const IN* __restrict__ nbrs,
+    const int32_t* __restrict__ leaf_position, int64_t k,
+    const int64_t* __restrict__ boff, int64_t B, OUT* __restrict__ block_row,
+    OUT* __restrict__ block_src, uint8_t* __restrict__ valid) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * k) return;
+  const int64_t b = i / k, j = i - b * k;
+  int64_t lo = 0, hi = L;  // first r in [0, L) with boff[r + 1] > b
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (boff[mid + 1] > b)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  const int64_t r = lo;
+  const int64_t e = off[r] + (b - boff[r]) * k + j;
+  const bool ok = e < off[r + 1];
+  const int32_t src = ok ? leaf_position[(int64_t)nbrs[e]] : 0;
+  const bool v = ok && src >= 0;
+  block_src[i] = (OUT)(v ? src : 0);
+  valid[i] = v ? 1 : 0;
+  if (j == 0) block_row[b] = (OUT)r;
+}
+
 }  // namespace ygg
 
 using namespace ygg;
@@ -494,6 +536,55 @@ int ygg_far_output(int64_t num_nodes, const int32_t* nodes_by_level, const int64
                        lm_offsets, counts, fixed_cap, row_start, entries, node_level,
                        (int32_t*)sources, (int32_t*)targets, (int32_t*)target_levels,
                        (int32_t*)node_offsets);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+size_t ygg_target_block_offsets_workspace_bytes(int64_t num_leaves) {
+  return align_up(sizeof(int32_t) * (size_t)(num_leaves + 1)) +
+         ygg::scan_workspace_bytes(num_leaves);
+}
+
+int ygg_target_block_offsets(int64_t num_leaves, const int64_t* near_offsets, int64_t block_size,
+                             int64_t* block_offsets, void* ws, size_t ws_bytes, void* stream) {
+  if (num_leaves < 1 || block_size < 1 || !near_offsets || !block_offsets) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  Arena a(ws, ws_bytes);
+  int32_t* blocks = a.take<int32_t>((size_t)num_leaves + 1);
+  const size_t rest = ygg::scan_workspace_bytes(num_leaves);
+  char* sws = a.take<char>(rest);
+  if (!blocks || !sws) return YGG_ERR_WORKSPACE;
+  hipLaunchKernelGGL(k_target_blocks_per_row, dim3(grid_for(num_leaves, 256)), dim3(256), 0, st,
+                     num_leaves, near_offsets, block_size, blocks);
+  YGG_LAUNCH_CHECK();
+  return ygg::exclusive_scan_i32_to_i64(blocks, nullptr, block_offsets, num_leaves, sws, rest,
+                                        st);
+}
+
+int ygg_target_blocks(int64_t num_leaves, const int64_t* near_offsets, const void* neighbors,
+                      int in_i64, const int32_t* leaf_position, int64_t block_size,
+                      const int64_t* block_offsets, int64_t num_blocks, int out_i64,
+                      void* block_leaf_ids, void* block_source_leaf_ids, uint8_t* valid_mask,
+                      void* stream) {
+  if (num_leaves < 1 || block_size < 1 || num_blocks < 0) return YGG_ERR_ARG;
+  if (num_blocks == 0) return YGG_OK;
+  const int64_t n = num_blocks * block_size;
+  dim3 grid(grid_for(n, 256));
+  hipStream_t st = (hipStream_t)stream;
+#define YGG_TB(OUT, IN)                                                                        \
+  hipLaunchKernelGGL((k_target_blocks<OUT, IN>), grid, dim3(256), 0, st, num_leaves,           \
+                     near_offsets, (const IN*)neighbors, leaf_position, block_size,            \
+                     block_offsets, num_blocks, (OUT*)block_leaf_ids,                          \
+                     (OUT*)block_source_leaf_ids, valid_mask)
+  if (out_i64 && in_i64)
+    YGG_TB(int64_t, int64_t);
+  else if (out_i64)
+    YGG_TB(int64_t, int32_t);
+  else if (in_i64)
+    YGG_TB(int32_t, int64_t);
+  else
+    YGG_TB(int32_t, int32_t);
+#undef YGG_TB
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

@@ -20,9 +20,11 @@ class TreeGeometry(NamedTuple):
 def compute_tree_geometry(tree, positions_sorted, *, max_leaf_size=None,
                           particle_radius=None) -> TreeGeometry:
     """Bounding boxes, centres, half extents (>= 1e-6), sphere radii and max
-    half extents of every node, computed on the GPU."""
-    if particle_radius is not None:
-        raise NotImplementedError("particle_radius is not supported by the GPU geometry kernel")
+    half extents of every node, computed on the GPU. ``particle_radius``
+    (per particle, in the sorted order of ``positions_sorted``) bounds each
+    particle as the cube [p - r, p + r] instead of the point p
+    (_geometry_impl.py:107-112), so every node's box covers the balls its
+    particles stand for."""
     lib = _lib.load()
     dt = device_topology(tree)
     pos = torch.as_tensor(positions_sorted)
@@ -34,6 +36,11 @@ def compute_tree_geometry(tree, positions_sorted, *, max_leaf_size=None,
     if pos.shape[0] != dt.num_particles:
         raise ValueError("positions_sorted must match tree.num_particles")
     pos = pos.contiguous()
+    prad = None
+    if particle_radius is not None:
+        prad = torch.as_tensor(particle_radius, device=pos.device).to(torch.float32).contiguous()
+        if prad.shape != (pos.shape[0],):
+            raise ValueError("particle_radius must have shape (n_particles,)")
     cap = max_leaf_size if max_leaf_size is not None else (dt.leaf_size or dt.num_particles)
     T = dt.num_nodes
     dev = pos.device
@@ -45,7 +52,8 @@ def compute_tree_geometry(tree, positions_sorted, *, max_leaf_size=None,
     left = dt.left if dt.left.numel() else None
     right = dt.right if dt.right.numel() else None
     _lib.check(lib.ygg_geometry(
-        pos.data_ptr(), dt.node_ranges.data_ptr(), _lib.ptr(left), _lib.ptr(right),
+        pos.data_ptr(), _lib.ptr(prad), dt.node_ranges.data_ptr(), _lib.ptr(left),
+        _lib.ptr(right),
         dt.parent.data_ptr(), dt.node_level.data_ptr(), T, int(dt.leaf_size), int(cap),
         int(dt.malformed),
         int(dt.max_level), center.data_ptr(), half.data_ptr(), radius.data_ptr(),

@@ -10,9 +10,11 @@ reference's host-side control: capacity ladders and retries
 
 from __future__ import annotations
 
+import contextvars
 import ctypes
 import logging
 import os
+import time
 from collections import OrderedDict
 from dataclasses import dataclass
 from typing import Callable, List, NamedTuple, Optional
@@ -51,6 +53,37 @@ SPREAD_LANES = 64
 # fixed-capacity method) instead of the flat walk plus ladder emulation.
 EXACT_ATTEMPTS = os.environ.get("YGGDRAX_EXACT_ATTEMPTS", "0") == "1"
 NODE_RECORD_BYTES = 32
+
+
+def _env_flag(name: str, default: bool = False) -> bool:
+    raw = os.environ.get(name)
+    if raw is None:
+        return bool(default)
+    return str(raw).strip().lower() in ("1", "true", "yes", "on")
+
+
+# Diagnostic lines of the traversal driver on stdout, as the reference's
+# YGGDRAX_TRAVERSAL_DIAGNOSTICS (_interactions_impl.py:401-413).
+_TRAVERSAL_DIAGNOSTICS = _env_flag("YGGDRAX_TRAVERSAL_DIAGNOSTICS", False)
+
+
+def _traversal_diag(message: str) -> None:
+    if _TRAVERSAL_DIAGNOSTICS:
+        print(f"[yggdrax.traversal] {message}", flush=True)
+
+
+# Stage-timing hook of build_compact_far_pairs_and_leaf_neighbor_lists
+# (timing_callback, _interactions_impl.py:5100-5102): set for the duration of
+# one call, called with (stage name, seconds).
+_STAGE_CALLBACK: "contextvars.ContextVar[Optional[Callable[[str, float], None]]]" = \
+    contextvars.ContextVar("ygg_stage_callback", default=None)
+
+
+def _record_stage(name: str, start: float, device) -> None:
+    cb = _STAGE_CALLBACK.get()
+    if cb is not None:
+        torch.cuda.synchronize(device)
+        cb(name, float(time.perf_counter() - start))
 
 
 @dataclass(frozen=True)
@@ -673,6 +706,8 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
             ctx.pairs_per_thread = old
     ctx.pairs_per_thread = autotune.walk_tile(T, with_tile, ctx.pairs_per_thread)
     em = None
+    t_stage = time.perf_counter()
+    c_walks = FLAT_STATS["count_walks"]
     if sizes is not None:
         FLAT_STATS["hinted_walks"] += 1
         em = ctx.run_flat(emit=True, queue_cap=sizes["queue"], far_cap=sizes["far"],
@@ -692,12 +727,21 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                 and max(tot["far_total"], tot["near_total"]) > (1 << 31) - 1):
             raise RuntimeError("Dual-tree traversal pair totals exceed signed int32 capacity; "
                                "refine the tree or use a lower-memory traversal strategy.")
+        _traversal_diag(f"count pass far_total={tot['far_total']} "
+                        f"near_total={tot['near_total']} max_wf={tot['peak']} "
+                        f"generations={len(tot['gen_sizes'])}")
+        _record_stage("dual_split_shared_count", t_stage, ctx.dev)
+        t_stage = time.perf_counter()
         em = ctx.run_flat(emit=True, queue_cap=tot["peak"], far_cap=tot["far_total"],
                           near_cap=tot["near_total"])
         assert not (em["queue_overflow"] or em["far_overflow"] or em["near_overflow"])
     nf, nn = em["far_total"], em["near_total"]
+    _traversal_diag(f"compact fill sizing queue={em['queue_capacity']} "
+                    f"total_far_pairs={nf} total_near_pairs={nn} "
+                    f"hinted={sizes is not None and FLAT_STATS['count_walks'] == c_walks}")
     (fk, fv, far_off), (_, nv, near_off) = _sort_far_and_near(
         em["far_keys"], em["far_vals"], nf, T, em["near_keys"], em["near_vals"], nn, L)
+    _record_stage("dual_split_shared_combined_fill", t_stage, ctx.dev)
     max_far = max_near = None
     if need_max or strict_int32 or not lad.use_count:
         mx = torch.stack([torch.diff(far_off).max(), torch.diff(near_off).max()]).tolist()
@@ -762,7 +806,7 @@ def _flat_outputs(fr: _FlatResult, dt: DeviceTopology, topo, neighbor_positions:
         target_block_source_leaf_ids=torch.zeros((0, 0), dtype=idx, device=dev),
         target_block_valid_mask=torch.zeros((0, 0), dtype=torch.bool, device=dev),
         target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev), target_block_size=0)
-    return inter, nbrs
+    return inter, _with_target_blocks(nbrs, dt)
 
 
 def _count_then_fill(ctx: _WalkContext, dt: DeviceTopology, retry_logger, collect_far,
@@ -893,6 +937,47 @@ def _target_block_size() -> int:
         return 0
 
 
+def _with_target_blocks(nbrs: NodeNeighborList, dt: DeviceTopology) -> NodeNeighborList:
+    """Fill the target-block fields of a near list (_build_target_blocks,
+    _interactions_impl.py:3689-3770) when YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE
+    is positive: each native leaf row is cut into ceil(count / k) blocks of
+    k consecutive neighbours, each slot holding the neighbour's native leaf
+    row (0 and invalid past the row's end)."""
+    k = _target_block_size()
+    if k <= 0:
+        return nbrs
+    lib = _lib.load()
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    L = dt.num_leaves
+    if L == 0 or int(nbrs.neighbors.numel()) == 0:
+        # no neighbour at all: the reference's empty blocks
+        return nbrs._replace(
+            target_block_leaf_ids=torch.zeros(0, dtype=idx, device=dev),
+            target_block_source_leaf_ids=torch.zeros((0, k if L else 0), dtype=idx, device=dev),
+            target_block_valid_mask=torch.zeros((0, k if L else 0), dtype=torch.bool, device=dev),
+            target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev), target_block_size=k)
+    st = _lib.stream_ptr(dev)
+    off = nbrs.offsets.to(torch.int64).contiguous()
+    boff = torch.empty(L + 1, dtype=torch.int64, device=dev)
+    ws = _lib.workspace(lib.ygg_target_block_offsets_workspace_bytes(L), dev)
+    _lib.check(lib.ygg_target_block_offsets(L, off.data_ptr(), k, boff.data_ptr(), ws.data_ptr(),
+                                            ws.numel(), st), "target block offsets")
+    B = int(boff[-1])
+    rows = torch.empty(B, dtype=idx, device=dev)
+    src = torch.empty((B, k), dtype=idx, device=dev)
+    valid = torch.empty((B, k), dtype=torch.uint8, device=dev)
+    nb = nbrs.neighbors.contiguous()
+    _lib.check(lib.ygg_target_blocks(L, off.data_ptr(), nb.data_ptr(),
+                                     int(nb.dtype == torch.int64), dt.leaf_position.data_ptr(), k,
+                                     boff.data_ptr(), B, int(idx == torch.int64),
+                                     rows.data_ptr(), src.data_ptr(), valid.data_ptr(), st),
+               "target blocks")
+    return nbrs._replace(target_block_leaf_ids=rows, target_block_source_leaf_ids=src,
+                         target_block_valid_mask=valid.bool(),
+                         target_block_offsets=boff.to(idx), target_block_size=k)
+
+
 def _to_neighbors(raw: Optional[_RawWalk], dt: DeviceTopology) -> NodeNeighborList:
     """_result_to_neighbors (3640-3837)."""
     lib = _lib.load()
@@ -916,17 +1001,14 @@ def _to_neighbors(raw: Optional[_RawWalk], dt: DeviceTopology) -> NodeNeighborLi
                                        dt.leaf_position.data_ptr(), maxc, int(idx == torch.int64),
                                        neighbors.data_ptr(), nlp.data_ptr(), _lib.stream_ptr(dev)),
                    "near_output")
-    k = _target_block_size()
-    if k > 0:
-        raise NotImplementedError("YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE > 0 is not supported")
-    return NodeNeighborList(
+    return _with_target_blocks(NodeNeighborList(
         offsets=off.to(idx), neighbors=neighbors, leaf_indices=leaf_indices,
         counts=counts.to(idx), particle_order_leaf_indices=po,
         particle_order_to_native_leaf=p2n, neighbor_leaf_positions=nlp,
         target_block_leaf_ids=torch.zeros(0, dtype=idx, device=dev),
         target_block_source_leaf_ids=torch.zeros((0, 0), dtype=idx, device=dev),
         target_block_valid_mask=torch.zeros((0, 0), dtype=torch.bool, device=dev),
-        target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev), target_block_size=0)
+        target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev), target_block_size=0), dt)
 
 
 def _node_order_rows(values_lm: torch.Tensor, inter: NodeInteractionList,
@@ -1070,13 +1152,20 @@ def build_interactions_and_neighbors(
             traversal_config=traversal_config, return_result=return_result)
     dt = device_topology(tree)
     topo = resolve_tree_topology(tree)
-    if _target_block_size() > 0:
-        raise NotImplementedError("YGGDRAX_NEARFIELD_TARGET_BLOCK_SIZE > 0 is not supported")
     if float(dehnen_radius_scale) <= 0.0:
         raise ValueError("dehnen_radius_scale must be > 0")
     kw = dict(max_interactions_per_node=max_interactions_per_node,
               max_neighbors_per_leaf=max_neighbors_per_leaf, max_pair_queue=max_pair_queue,
               traversal_config=traversal_config)
+    use_count = (max_interactions_per_node is None and max_pair_queue is None
+                 and traversal_config is None)
+    _traversal_diag(
+        f"walk start total_nodes={dt.num_nodes} num_internal={dt.num_internal} "
+        f"use_count_pass={use_count} "
+        f"max_interactions_per_node={max_interactions_per_node} "
+        f"max_neighbors_per_leaf={int(max_neighbors_per_leaf)} "
+        f"max_pair_queue={max_pair_queue} traversal_config={traversal_config} "
+        f"process_block={process_block} mac_type={mac_type}")
     if dt.num_internal == 0:
         inter, nbrs, raw = _empty_outputs(dt, topo), _to_neighbors(None, dt), None
     else:
@@ -1135,12 +1224,20 @@ def build_compact_far_pairs_and_leaf_neighbor_lists(
     """Compact far pairs and leaf neighbour lists from one walk
     (_interactions_impl.py:5819-5915). With ``compact_far_pair_capacity`` the
     arrays are padded with -1 to that length and a larger count raises the
-    reference's error (_interactions_impl.py:5346-5372)."""
-    _, nbrs, compact = build_interactions_and_neighbors(
-        tree, geometry, theta, max_interactions_per_node, max_neighbors_per_leaf,
-        max_pair_queue, process_block, traversal_config, retry_logger, mac_type,
-        dehnen_radius_scale, pair_policy, policy_state, return_compact_far_pairs=True,
-        return_interactions=False, capacity_hint=capacity_hint)
+    reference's error (_interactions_impl.py:5346-5372). ``timing_callback``
+    receives (stage, seconds) for the counting walk
+    ("dual_split_shared_count") and the emission walk with its row sorts
+    ("dual_split_shared_combined_fill"), the stages of the reference's shared
+    count pass (5100-5102, 5131-5296)."""
+    token = _STAGE_CALLBACK.set(timing_callback)
+    try:
+        _, nbrs, compact = build_interactions_and_neighbors(
+            tree, geometry, theta, max_interactions_per_node, max_neighbors_per_leaf,
+            max_pair_queue, process_block, traversal_config, retry_logger, mac_type,
+            dehnen_radius_scale, pair_policy, policy_state, return_compact_far_pairs=True,
+            return_interactions=False, capacity_hint=capacity_hint)
+    finally:
+        _STAGE_CALLBACK.reset(token)
     if compact_far_pair_capacity is not None:
         cap = int(compact_far_pair_capacity)
         n = int(compact.far_pair_count)
@@ -1194,11 +1291,14 @@ def build_leaf_neighbor_lists(tree, geometry: TreeGeometry, theta: float = 0.5,
 
 
 def interactions_for_node(data: NodeInteractionList, node: int) -> torch.Tensor:
+    """Source nodes of ``node``'s far list (_interactions_impl.py:6147-6154)."""
     start = int(data.offsets[node])
     return data.sources[start:start + int(data.counts[node])]
 
 
 def neighbors_for_leaf(data: NodeNeighborList, leaf_node: int) -> torch.Tensor:
+    """Neighbour leaf nodes of ``leaf_node``; ValueError when it is not a
+    leaf of the list (_interactions_impl.py:6157-6170)."""
     m = (data.leaf_indices == int(leaf_node)).nonzero()
     if m.numel() == 0:
         raise ValueError("leaf_node not present in neighbor list")
