@@ -284,6 +284,15 @@ int sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* k
 // Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:
 // out[n] = total). `gather`, if non-null, reads counts[gather[i]].
 size_t scan_workspace_bytes(int64_t n);
+size_t sort_rows_fused_workspace_bytes(int64_t n, int end_bit, int digit_bits);
+int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                        int64_t n, int64_t rows, int end_bit, int digit_bits,
+                        const int32_t* nbl, const int32_t* node_level, int out_i64,
+                        void* sources, void* targets, void* levels, void* ws, size_t ws_bytes,
+                        hipStream_t st);
+int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                         int64_t n, int end_bit, int digit_bits, int out_i64, void* neighbors,
+                         void* ws, size_t ws_bytes, hipStream_t st);
 int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int64_t* out,
                               int64_t n, void* ws, size_t ws_bytes, hipStream_t st);
 

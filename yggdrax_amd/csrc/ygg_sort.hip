@@ -110,11 +110,55 @@ __global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
   }
 }
 
-template <typename K, int BITS, int ITEMS, int THREADS>
+// Where a pass writes its sorted pairs: key(g, k) and val(g, v) for the
+// pair that lands at position g. Every pass but a fused last one writes the
+// keys and values themselves.
+template <typename K>
+struct EmitKV {
+  K* __restrict__ keys_out;
+  uint32_t* __restrict__ vals_out;
+  __device__ __forceinline__ void key(int64_t g, K k) const { keys_out[g] = k; }
+  __device__ __forceinline__ void val(int64_t g, uint32_t v) const { vals_out[g] = v; }
+};
+
+// Last pass of the far row-key sort fused with the level-major far list
+// (_result_to_interactions, _interactions_impl.py:3561-3637): the key is the
+// target's level-major rank, the value the source node. Writes the sorted
+// key (for the row offsets), the target node (nodes_by_level[key]) and its
+// level, and the source. Keys >= rows are unused stream capacity.
+template <typename OutT>
+struct EmitFarList {
+  uint32_t* __restrict__ keys_out;
+  const int32_t* __restrict__ nbl;
+  const int32_t* __restrict__ node_level;
+  OutT* __restrict__ sources;
+  OutT* __restrict__ targets;
+  OutT* __restrict__ levels;
+  uint32_t rows;
+  __device__ __forceinline__ void key(int64_t g, uint32_t k) const {
+    keys_out[g] = k;
+    if (k >= rows) return;
+    const int32_t node = nbl[k];
+    targets[g] = (OutT)node;
+    levels[g] = (OutT)node_level[node];
+  }
+  __device__ __forceinline__ void val(int64_t g, uint32_t v) const { sources[g] = (OutT)v; }
+};
+
+// Last pass of the near row-key sort fused with the near list: the key is
+// the native leaf row, the value the neighbour leaf node.
+template <typename OutT>
+struct EmitNearList {
+  uint32_t* __restrict__ keys_out;
+  OutT* __restrict__ neighbors;
+  __device__ __forceinline__ void key(int64_t g, uint32_t k) const { keys_out[g] = k; }
+  __device__ __forceinline__ void val(int64_t g, uint32_t v) const { neighbors[g] = (OutT)v; }
+};
+
+template <typename K, int BITS, int ITEMS, int THREADS, typename Emit>
 __global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys_in,
                                                   const uint32_t* __restrict__ vals_in,
-                                                  K* __restrict__ keys_out,
-                                                  uint32_t* __restrict__ vals_out, int64_t n,
+                                                  Emit em, int64_t n,
                                                   int shift, const uint32_t* __restrict__ hist_scan,
                                                   uint32_t* __restrict__ lookback,
                                                   uint32_t* __restrict__ tile_counter) {
@@ -250,20 +294,18 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys
     for (int p = tid; p < valid; p += THREADS) {
       const K kk = s_buf[p];
       const int64_t g = (int64_t)s_global[(unsigned)((kk >> shift) & kMask)] + p;
-      keys_out[g] = kk;
-      vals_out[g] = s_vals[p];
+      em.key(g, kk);
+      em.val(g, s_vals[p]);
     }
     return;
   }
-  for (int p = tid; p < valid; p += THREADS)
-    keys_out[(int64_t)s_global[s_dig[p]] + p] = s_buf[p];
+  for (int p = tid; p < valid; p += THREADS) em.key((int64_t)s_global[s_dig[p]] + p, s_buf[p]);
   __syncthreads();
   uint32_t* s_vb = reinterpret_cast<uint32_t*>(s_buf);
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) s_vb[rank[k]] = val[k];
   __syncthreads();
-  for (int p = tid; p < valid; p += THREADS)
-    vals_out[(int64_t)s_global[s_dig[p]] + p] = s_vb[p];
+  for (int p = tid; p < valid; p += THREADS) em.val((int64_t)s_global[s_dig[p]] + p, s_vb[p]);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
@@ -335,10 +377,11 @@ size_t sort_workspace_bytes(int64_t n, int key_bytes, int range_bits, int digit_
   return a.off;
 }
 
-template <typename K, int BITS>
+template <typename K, int BITS, typename Last = EmitKV<K>>
 static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_out,
                            uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
-                           size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre) {
+                           size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre,
+                           const Last* last = nullptr) {
   constexpr int kBins = 1 << BITS;
   constexpr int kThreads = sort_threads<K, BITS>();
   constexpr int kItems = kSortItems;
@@ -372,14 +415,88 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
     bool to_out = ((passes - 1 - p) % 2) == 0;
     K* dk = to_out ? keys_out : kalt;
     uint32_t* dv = to_out ? vals_out : valt;
-    hipLaunchKernelGGL((k_onesweep<K, BITS, kItems, kThreads>), dim3((unsigned)tiles), dim3(kThreads), 0, st,
-                       src_k, src_v, dk, dv, n, begin_bit + BITS * p, hist + p * kBins,
-                       lookback + (int64_t)p * tiles * kBins, ctr + p);
+    if (last && p == passes - 1) {
+      hipLaunchKernelGGL((k_onesweep<K, BITS, kItems, kThreads, Last>), dim3((unsigned)tiles),
+                         dim3(kThreads), 0, st, src_k, src_v, *last, n, begin_bit + BITS * p,
+                         hist + p * kBins, lookback + (int64_t)p * tiles * kBins, ctr + p);
+    } else {
+      hipLaunchKernelGGL((k_onesweep<K, BITS, kItems, kThreads, EmitKV<K>>), dim3((unsigned)tiles),
+                         dim3(kThreads), 0, st, src_k, src_v, EmitKV<K>{dk, dv}, n,
+                         begin_bit + BITS * p, hist + p * kBins,
+                         lookback + (int64_t)p * tiles * kBins, ctr + p);
+    }
     YGG_LAUNCH_CHECK();
     src_k = dk;
     src_v = dv;
   }
   return YGG_OK;
+}
+
+// A u32 row-key sort [0, end_bit) whose last pass writes through `last`
+// (the row lists) instead of keys and values. The passes before it ping-pong
+// between the sort workspace and (keys_out, vals_tmp); keys_out finally
+// holds the sorted keys. Workspace: n values, then the sort's own.
+size_t sort_rows_fused_workspace_bytes(int64_t n, int end_bit, int digit_bits) {
+  return align_up(sizeof(uint32_t) * (size_t)n) + sort_workspace_bytes(n, 4, end_bit, digit_bits);
+}
+
+template <typename Last>
+static int sort_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                           int64_t n, int end_bit, int digit_bits, void* ws, size_t ws_bytes,
+                           hipStream_t st, const Last& last) {
+  if (n <= 0) return YGG_OK;
+  if (n > (int64_t)kCountMask || end_bit < 1 || end_bit > 32 || keys_in == keys_out)
+    return YGG_ERR_ARG;
+  if (digit_bits == 0) digit_bits = default_digit_bits(4, end_bit);
+  Arena a(ws, ws_bytes);
+  uint32_t* vals_tmp = a.take<uint32_t>((size_t)n);
+  const size_t rest = sort_workspace_bytes(n, 4, end_bit, digit_bits);
+  char* sws = a.take<char>(rest);
+  if (!vals_tmp || !sws) return YGG_ERR_WORKSPACE;
+  switch (digit_bits) {
+    case 8:
+      return sort_pairs_bits<uint32_t, 8, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
+                                                end_bit, sws, rest, st, nullptr, &last);
+    case 9:
+      return sort_pairs_bits<uint32_t, 9, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
+                                                end_bit, sws, rest, st, nullptr, &last);
+    case 10:
+      return sort_pairs_bits<uint32_t, 10, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
+                                                 end_bit, sws, rest, st, nullptr, &last);
+    case 11:
+      return sort_pairs_bits<uint32_t, 11, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
+                                                 end_bit, sws, rest, st, nullptr, &last);
+    default:
+      return YGG_ERR_ARG;
+  }
+}
+
+int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                        int64_t n, int64_t rows, int end_bit, int digit_bits,
+                        const int32_t* nbl, const int32_t* node_level, int out_i64,
+                        void* sources, void* targets, void* levels, void* ws, size_t ws_bytes,
+                        hipStream_t st) {
+  if (out_i64) {
+    EmitFarList<int64_t> e{keys_out, nbl, node_level, (int64_t*)sources, (int64_t*)targets,
+                           (int64_t*)levels, (uint32_t)rows};
+    return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st,
+                           e);
+  }
+  EmitFarList<int32_t> e{keys_out, nbl, node_level, (int32_t*)sources, (int32_t*)targets,
+                         (int32_t*)levels, (uint32_t)rows};
+  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e);
+}
+
+int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                         int64_t n, int end_bit, int digit_bits, int out_i64, void* neighbors,
+                         void* ws, size_t ws_bytes, hipStream_t st) {
+  if (out_i64) {
+    EmitNearList<int64_t> e{keys_out, (int64_t*)neighbors};
+    return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st,
+                           e);
+  }
+  EmitNearList<int32_t> e{keys_out, (int32_t*)neighbors};
+  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e);
 }
 
 template <typename K>

@@ -482,6 +482,22 @@ __global__ __launch_bounds__(256) void k_near_dense(int L, const int64_t* __rest
   }
 }
 
+// Dense [L][maxc] neighbour leaf rows from the near list itself.
+template <typename OutT>
+__global__ __launch_bounds__(256) void k_near_dense_from_list(
+    int L, const int64_t* __restrict__ off, const OutT* __restrict__ nbrs,
+    const int32_t* __restrict__ leaf_position, int64_t maxc, OutT* __restrict__ nlp) {
+  const int64_t total = (int64_t)L * maxc;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / maxc, e = i - r * maxc;
+    const int64_t o = off[r];
+    OutT v = (OutT)-1;
+    if (o + e < off[r + 1]) v = (OutT)leaf_position[(int64_t)nbrs[o + e]];
+    __builtin_nontemporal_store(v, &nlp[i]);
+  }
+}
+
 // ---- batched (forest) lists -> per-system lists ---------------------------------
 // Per far entry, the target's system (largest s with node_off(s) <= t) by a
 // binary search over the node offsets, staged in LDS (seg_stage).
@@ -788,6 +804,84 @@ int ygg_far_lists_from_sorted(int64_t num_nodes, const uint32_t* keys, const uin
     hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int32_t*)counts, (int32_t*)node_offsets, counts32);
   }
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits) {
+  return ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits);
+}
+
+int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n, int64_t num_nodes,
+                       int end_bit, int digit_bits, const int32_t* nodes_by_level,
+                       const int32_t* node_level, int out_i64, uint32_t* sorted_keys,
+                       int64_t* lm_offsets, void* sources, void* targets, void* target_levels,
+                       void* counts, void* node_offsets, int32_t* counts32, void* ws,
+                       size_t ws_bytes, void* stream) {
+  if (num_nodes < 1 || n < 0 || !lm_offsets || !sorted_keys) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int T = (int)num_nodes;
+  int rc = ygg::sort_far_rows_fused(keys, vals, sorted_keys, n, num_nodes, end_bit, digit_bits,
+                                    nodes_by_level, node_level, out_i64, sources, targets,
+                                    target_levels, ws, ws_bytes, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_row_offsets, dim3(grid_for((int64_t)T + 1, 256)), dim3(256), 0, st,
+                     sorted_keys, n, T, lm_offsets);
+  dim3 gr(grid_for(T, 256));
+  if (out_i64)
+    hipLaunchKernelGGL(k_far_rows<int64_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
+                       (int64_t*)counts, (int64_t*)node_offsets, counts32);
+  else
+    hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
+                       (int32_t*)counts, (int32_t*)node_offsets, counts32);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits) {
+  return ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits);
+}
+
+int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n, int64_t num_leaves,
+                        int end_bit, int digit_bits, int out_i64, uint32_t* sorted_keys,
+                        int64_t* row_offsets, void* neighbors, void* counts, void* offsets,
+                        void* ws, size_t ws_bytes, void* stream) {
+  if (num_leaves < 1 || n < 0 || !row_offsets || !sorted_keys) return YGG_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int L = (int)num_leaves;
+  int rc = ygg::sort_near_rows_fused(keys, vals, sorted_keys, n, end_bit, digit_bits, out_i64,
+                                     neighbors, ws, ws_bytes, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_row_offsets, dim3(grid_for((int64_t)L + 1, 256)), dim3(256), 0, st,
+                     sorted_keys, n, L, row_offsets);
+  dim3 gr(grid_for((int64_t)L + 1, 256));
+  if (out_i64)
+    hipLaunchKernelGGL(k_near_rows<int64_t>, gr, dim3(256), 0, st, L, row_offsets,
+                       (int64_t*)counts, (int64_t*)offsets);
+  else
+    hipLaunchKernelGGL(k_near_rows<int32_t>, gr, dim3(256), 0, st, L, row_offsets,
+                       (int32_t*)counts, (int32_t*)offsets);
+  YGG_LAUNCH_CHECK();
+  return YGG_OK;
+}
+
+int ygg_near_dense_from_list(int64_t num_leaves, const int64_t* row_offsets,
+                             const void* neighbors, const int32_t* leaf_position,
+                             int64_t max_count, int out_i64, void* neighbor_leaf_positions,
+                             void* stream) {
+  if (num_leaves < 1 || max_count < 0) return YGG_ERR_ARG;
+  if (max_count == 0) return YGG_OK;
+  const int L = (int)num_leaves;
+  dim3 gd(grid_for((int64_t)L * max_count, 256, 32768));
+  hipStream_t st = (hipStream_t)stream;
+  if (out_i64)
+    hipLaunchKernelGGL(k_near_dense_from_list<int64_t>, gd, dim3(256), 0, st, L, row_offsets,
+                       (const int64_t*)neighbors, leaf_position, max_count,
+                       (int64_t*)neighbor_leaf_positions);
+  else
+    hipLaunchKernelGGL(k_near_dense_from_list<int32_t>, gd, dim3(256), 0, st, L, row_offsets,
+                       (const int32_t*)neighbors, leaf_position, max_count,
+                       (int32_t*)neighbor_leaf_positions);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

@@ -588,11 +588,72 @@ def _run_walk_raw(tree, geometry: TreeGeometry, theta: float, mac_type: str, *,
 
 class _FlatResult(NamedTuple):
     far_keys: torch.Tensor     # sorted level-major row keys (uint32 as int32)
-    far_vals: torch.Tensor
-    near_vals: torch.Tensor    # sorted by native leaf row
+    far_vals: Optional[torch.Tensor]
+    near_vals: Optional[torch.Tensor]    # sorted by native leaf row
     far_off: torch.Tensor      # int64 [T + 1] level-major row offsets
     near_off: torch.Tensor     # int64 [L + 1]
     stats: dict
+    # the list outputs, when the sorts wrote them directly (fused last pass)
+    inter: Optional["NodeInteractionList"] = None
+    near: Optional[tuple] = None       # (neighbors, counts, offsets)
+
+
+def _fused_lists(em: dict, dt: DeviceTopology):
+    """Far and near row-key sorts whose last pass writes the reference's
+    list layouts directly (ygg_far_lists_sort / ygg_near_lists_sort): the far
+    sort on a side stream, the near one on the current stream. Returns the
+    NodeInteractionList, the near (neighbors, counts, offsets) and both int64
+    row offset arrays."""
+    from . import autotune
+    lib = _lib.load()
+    dev = dt.parent.device
+    idx = INDEX_DTYPE
+    i64 = int(idx == torch.int64)
+    T, L = dt.num_nodes, dt.num_leaves
+    nf, nn = em["far_total"], em["near_total"]
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        fend = max(1, (T - 1).bit_length())
+        fbits = autotune.sort_digit_bits(4, fend, nf) if nf else 8
+        sources = torch.empty(nf, dtype=idx, device=dev)
+        targets = torch.empty(nf, dtype=idx, device=dev)
+        levels = torch.empty(nf, dtype=idx, device=dev)
+        counts = torch.empty(T, dtype=idx, device=dev)
+        counts32 = torch.empty(T, dtype=torch.int32, device=dev)
+        node_off = torch.empty(T + 1, dtype=idx, device=dev)
+        far_off = torch.empty(T + 1, dtype=torch.int64, device=dev)
+        fsorted = torch.empty(max(1, nf), dtype=torch.int32, device=dev)
+        fws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(nf, fend, fbits), dev)
+        _lib.check(lib.ygg_far_lists_sort(
+            em["far_keys"].data_ptr(), em["far_vals"].data_ptr(), nf, T, fend, fbits,
+            dt.nodes_by_level.data_ptr(), dt.node_level.data_ptr(), i64, fsorted.data_ptr(),
+            far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
+            counts.data_ptr(), node_off.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
+            fws.numel(), _lib.stream_ptr(dev)), "far lists sort")
+    nend = max(1, (L - 1).bit_length())
+    nbits = autotune.sort_digit_bits(4, nend, nn) if nn else 8
+    neighbors = torch.empty(nn, dtype=idx, device=dev)
+    ncounts = torch.empty(L, dtype=idx, device=dev)
+    noff = torch.empty(L + 1, dtype=idx, device=dev)
+    near_off = torch.empty(L + 1, dtype=torch.int64, device=dev)
+    nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
+    nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
+    _lib.check(lib.ygg_near_lists_sort(
+        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, L, nend, nbits, i64,
+        nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
+        noff.data_ptr(), nws.data_ptr(), nws.numel(), _lib.stream_ptr(dev)), "near lists sort")
+    main.wait_stream(side)
+    for t in (sources, targets, levels, counts, counts32, node_off, far_off, fsorted, fws):
+        t.record_stream(main)
+    nl = int(dt.num_levels)
+    level_offsets = far_off[dt.level_offsets[: nl + 1].long()].to(idx)
+    if nf == 0:
+        node_off = torch.zeros(T + 1, dtype=idx, device=dev)
+        level_offsets = torch.zeros(nl + 1, dtype=idx, device=dev)
+    inter = NodeInteractionList(node_off, sources, targets, counts, level_offsets, levels)
+    return inter, (neighbors, ncounts, noff), fsorted, far_off, near_off
 
 
 def _sort_entries(keys, vals, n, rows, exact_keys=False):
@@ -685,13 +746,14 @@ def _emulate_ladder(lad: _Ladder, peak: int, max_far: int, max_near: int):
 
 def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                    sizes: Optional[dict] = None, strict_int32: bool = True,
-                   need_max: bool = True) -> Optional[_FlatResult]:
+                   need_max: bool = True, fused_lists: bool = False) -> Optional[_FlatResult]:
     """Full walk with flat emission. `sizes` (queue, far, near capacities)
     skips the counting walk; on any stream overflow the counting walk runs
     and the emission walk is repeated at exact sizes. The largest far and
     near row counts (one host read after the sorts) are taken when
     `need_max`, or when the int32 guard or the retry-ladder emulation needs
-    them; otherwise stats hold None for them."""
+    them; otherwise stats hold None for them. With `fused_lists` the row
+    sorts write the list outputs themselves (result fields inter, near)."""
     T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
     qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
     from . import autotune
@@ -739,8 +801,13 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     _traversal_diag(f"compact fill sizing queue={em['queue_capacity']} "
                     f"total_far_pairs={nf} total_near_pairs={nn} "
                     f"hinted={sizes is not None and FLAT_STATS['count_walks'] == c_walks}")
-    (fk, fv, far_off), (_, nv, near_off) = _sort_far_and_near(
-        em["far_keys"], em["far_vals"], nf, T, em["near_keys"], em["near_vals"], nn, L)
+    inter = near = None
+    if fused_lists:
+        inter, near, fk, far_off, near_off = _fused_lists(em, dt)
+        fv = nv = None
+    else:
+        (fk, fv, far_off), (_, nv, near_off) = _sort_far_and_near(
+            em["far_keys"], em["far_vals"], nf, T, em["near_keys"], em["near_vals"], nn, L)
     _record_stage("dual_split_shared_combined_fill", t_stage, ctx.dev)
     max_far = max_near = None
     if need_max or strict_int32 or not lad.use_count:
@@ -760,7 +827,7 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
             return None
         stats["queue_capacity"] = caps[0]
         stats["attempt_caps"] = caps
-    return _FlatResult(fk, fv, nv, far_off, near_off, stats)
+    return _FlatResult(fk, fv, nv, far_off, near_off, stats, inter, near)
 
 
 def _flat_outputs(fr: _FlatResult, dt: DeviceTopology, topo, neighbor_positions: bool = True):
@@ -770,6 +837,25 @@ def _flat_outputs(fr: _FlatResult, dt: DeviceTopology, topo, neighbor_positions:
     T, L = dt.num_nodes, dt.num_leaves
     i64 = int(idx == torch.int64)
     st = _lib.stream_ptr(dev)
+    if fr.inter is not None:
+        neighbors, ncounts, noff = fr.near
+        maxc = fr.stats["max_near"] if neighbor_positions else 0
+        nlp = torch.empty((L, maxc), dtype=idx, device=dev)
+        if maxc > 0:
+            _lib.check(lib.ygg_near_dense_from_list(L, fr.near_off.data_ptr(),
+                                                    neighbors.data_ptr(),
+                                                    dt.leaf_position.data_ptr(), maxc, i64,
+                                                    nlp.data_ptr(), st), "near dense")
+        leaf_indices, po, p2n = _leaf_tables(dt)
+        return fr.inter, _with_target_blocks(NodeNeighborList(
+            offsets=noff, neighbors=neighbors, leaf_indices=leaf_indices, counts=ncounts,
+            particle_order_leaf_indices=po, particle_order_to_native_leaf=p2n,
+            neighbor_leaf_positions=nlp,
+            target_block_leaf_ids=torch.zeros(0, dtype=idx, device=dev),
+            target_block_source_leaf_ids=torch.zeros((0, 0), dtype=idx, device=dev),
+            target_block_valid_mask=torch.zeros((0, 0), dtype=torch.bool, device=dev),
+            target_block_offsets=torch.zeros(L + 1, dtype=idx, device=dev),
+            target_block_size=0), dt)
     nf = fr.stats["far_total"]
     nn = fr.stats["near_total"]
     sources = torch.empty(nf, dtype=idx, device=dev)
@@ -1173,7 +1259,8 @@ def build_interactions_and_neighbors(
         fr = None
         if retry_logger is None and not EXACT_ATTEMPTS:
             fr = _run_walk_flat(ctx, dt, _ladder(dt, mac_type, **kw), sizes=capacity_hint,
-                                strict_int32=strict_int32, need_max=neighbor_positions)
+                                strict_int32=strict_int32, need_max=neighbor_positions,
+                                fused_lists=True)
         if fr is not None:
             inter, nbrs = _flat_outputs(fr, dt, topo, neighbor_positions)
             raw = fr
