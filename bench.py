@@ -356,7 +356,9 @@ def batched_leg(torch, steps, warmup):
     # the learned batched pipeline captured once as a HIP graph (stream
     # capacities from the capacity model), replayed on new positions: a
     # small random displacement of every point per step, inputs in HBM
-    graph = batch.BatchedGraph(P, M, offsets, leaf_size=32, theta=0.6, model=model, **kw)
+    from yggdrax_amd import autotune
+    graph = batch.BatchedGraph(P, M, offsets, leaf_size=32, theta=0.6, model=model,
+                               tune=autotune.enabled(), **kw)
     rng = np.random.default_rng(404)
     moved = [P + torch.from_numpy(rng.normal(0.0, 1e-4, pos_np.shape).astype(np.float32)).cuda()
              for _ in range(steps)]
@@ -372,6 +374,9 @@ def batched_leg(torch, steps, warmup):
     g_dig = _batch_digests(torch, glists)
     _, moved_lists = run_batched(moved[-1])
     moved_dig = _batch_digests(torch, moved_lists)
+    # one call per system meets many tree sizes: no tuning inside its timing
+    tuning = autotune.enabled()
+    autotune.enable(False)
     run_one(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -384,6 +389,7 @@ def batched_leg(torch, steps, warmup):
     fouts = [run_one(k, **fixed) for k in range(S)]
     torch.cuda.synchronize()
     t_f = (time.perf_counter() - t0) * 1e3
+    autotune.enable(tuning)
     b_dig = _batch_digests(torch, lists)
     l_dig = _batch_digests(torch, llists)
     s_dig = _single_digests(torch, outs)
@@ -584,6 +590,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-layouts", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scales the HBM "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -603,8 +610,11 @@ def main():
         _log("copy probe: 3 copies of 1 GiB")
         return
     import yggdrax_amd as ygg
-    from yggdrax_amd import inputs
+    from yggdrax_amd import autotune, inputs
 
+    # plans (row-sort digit widths, walk grid and queue tile) are tuned in
+    # the untimed warm-up calls and applied in every timed one
+    autotune.enable(not args.no_autotune)
     n = 1 << args.log2n
     _log(f"generating clustered input n={n}")
     pos_np, mass_np = inputs.clustered(n, 3)
@@ -673,6 +683,7 @@ def main():
         "roofline": roof,
         "build_roofline": build_roof,
         "cpu_baseline": cpu,
+        "autotune": {"enabled": not args.no_autotune, "plans": autotune.plans()},
     }
     print(json.dumps(out), flush=True)
 

@@ -140,17 +140,6 @@ def test_radix_sort_every_digit_width(ygg, key_bytes, range_bits, n, digit_bits)
     assert np.array_equal(out, keys[perm])
 
 
-def test_autotuned_sort_plan_is_a_valid_width(ygg):
-    from yggdrax_amd import autotune
-    autotune.enable(True)
-    try:
-        bits = autotune.sort_digit_bits(4, 19, 1 << 18)
-    finally:
-        autotune.enable(False)
-    assert bits in autotune.SORT_DIGIT_BITS
-    assert autotune.plans()["sort/k4/r19/n19"] == bits
-
-
 def test_identical_points_stable(ygg):
     """tests/unit/test_tree.py:193-214: identical points keep input order."""
     pos = np.full((64, 3), 0.25, np.float32)

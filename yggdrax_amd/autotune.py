@@ -28,6 +28,8 @@ from . import _lib
 
 _LOCK = threading.Lock()
 _PLANS: Dict[str, int] = {}
+# measured milliseconds per candidate of every plan tuned in this process
+_TIMINGS: Dict[str, Dict[int, float]] = {}
 _ENABLED = os.environ.get("YGGDRAX_AUTOTUNE", "0") not in ("", "0", "false", "False")
 _CACHE = os.environ.get("YGGDRAX_AUTOTUNE_CACHE", "")
 
@@ -48,6 +50,19 @@ def enabled() -> bool:
 def plans() -> Dict[str, int]:
     with _LOCK:
         return dict(_PLANS)
+
+
+def timings() -> Dict[str, Dict[int, float]]:
+    """Candidate timings (ms) of the plans tuned in this process."""
+    with _LOCK:
+        return {k: dict(v) for k, v in _TIMINGS.items()}
+
+
+def clear() -> None:
+    """Forget every plan and timing (the defaults apply until retuned)."""
+    with _LOCK:
+        _PLANS.clear()
+        _TIMINGS.clear()
 
 
 def save(path: str) -> None:
@@ -88,9 +103,11 @@ def choose(key: str, candidates: Sequence[int], timer: Callable[[int], float],
             return _PLANS[key]
     if not _ENABLED:
         return default
-    best = min(candidates, key=timer)
+    measured = {int(c): float(timer(c)) for c in candidates}
+    best = min(measured, key=measured.get)
     with _LOCK:
         _PLANS[key] = int(best)
+        _TIMINGS[key] = measured
         if _CACHE:
             with open(_CACHE, "w") as f:
                 json.dump(_PLANS, f, indent=1, sort_keys=True)
@@ -130,6 +147,15 @@ def sort_digit_bits(key_bytes: int, range_bits: int, n: int) -> int:
     return choose(key, SORT_DIGIT_BITS, timer, default)
 
 
+def walk_plan(num_nodes: int, grid_default: int, tile_default: int) -> tuple:
+    """(grid blocks, queue tile) of the flat walk for trees of this size:
+    the tuned plans where they exist, else the defaults (no tuning here)."""
+    c = _size_class(num_nodes)
+    with _LOCK:
+        return (_PLANS.get(f"walk/grid/n{c}", grid_default),
+                _PLANS.get(f"walk/tile/n{c}", tile_default))
+
+
 def walk_tile(num_nodes: int, run_with: Callable[[int], None], default: int) -> int:
     """Queue tile of the flat walk (pairs per thread of a 256-thread chunk);
     `run_with(per)` runs one counting walk with that tile."""
@@ -144,5 +170,5 @@ def walk_grid_blocks(num_nodes: int, run_with: Callable[[int], None], default: i
     return choose(key, WALK_GRID_BLOCKS, lambda b: _time_ms(lambda: run_with(b), reps=1), default)
 
 
-__all__ = ["choose", "enable", "enabled", "load", "plans", "save", "sort_digit_bits",
-           "walk_grid_blocks", "walk_tile"]
+__all__ = ["choose", "clear", "enable", "enabled", "load", "plans", "save", "sort_digit_bits",
+           "timings", "walk_grid_blocks", "walk_plan", "walk_tile"]

@@ -571,7 +571,8 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
 
 
 def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_type: str,
-                dehnen_radius_scale: float, caps: dict, generations: int, grid_blocks: int):
+                dehnen_radius_scale: float, caps: dict, generations: int, grid_blocks: int,
+                pairs_per_thread: int):
     """Forest walk with fixed stream capacities and generation count and no
     host synchronisation (stream-capturable), followed by the row-key sorts
     over the full capacities (unused stream slots hold key 0xFFFFFFFF and
@@ -615,9 +616,9 @@ def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_ty
     p.near_keys, p.near_vals = nk.data_ptr(), nv.data_ptr()
     p.hist_gen, p.node_hist = -1, None
     p.roots, p.num_roots = roots.data_ptr(), S
-    from .interactions import WALK_PAIRS_PER_THREAD, WALK_XCD_GROUPS
+    from .interactions import WALK_XCD_GROUPS
     p.xcd_groups = WALK_XCD_GROUPS
-    p.pairs_per_thread = WALK_PAIRS_PER_THREAD
+    p.pairs_per_thread = int(pairs_per_thread)
     _lib.check(lib.ygg_walk_flat_fixed(ctypes.byref(p), st), "graph walk")
     fk2, fv2, far_off = _sort_entries(fk, fv, FC, T)
     _, nv2, near_off = _sort_entries(nk, nv, NC, Lt)
@@ -642,6 +643,11 @@ class BatchedGraph:
     the single-system path returns the eager pipeline's result instead
     (`last_replay_fell_back`), so results never depend on the capacities.
 
+    With `tune` the autotuner times its candidates during the eager run (the
+    forest walk's grid and queue tile, the row sorts' digit widths, also at
+    the graph's capacity-sized stream lengths) and the captured graph uses
+    the chosen plans (`plans`).
+
     A replay that does not fall back returns the graph's own output buffers:
     the next replay overwrites them in place, so a caller that keeps a
     result across replays clones its tensors first."""
@@ -649,8 +655,10 @@ class BatchedGraph:
     def __init__(self, positions, masses, offsets, *, leaf_size: int = 32, theta: float = 0.6,
                  mac_type: str = "dehnen", dehnen_radius_scale: float = 1.0, model=None,
                  slack: float = 1.1, grid_blocks: int = FLAT_GRID_BLOCKS,
-                 capacities: Optional[dict] = None, generations: Optional[int] = None):
+                 capacities: Optional[dict] = None, generations: Optional[int] = None,
+                 tune: bool = False):
         from . import autotune
+        from .interactions import WALK_PAIRS_PER_THREAD
         _lib.require_device(positions)
         dev = positions.device
         self.theta, self.mac_type, self.scale = float(theta), mac_type, float(dehnen_radius_scale)
@@ -665,7 +673,13 @@ class BatchedGraph:
         # totals, generation count and a warm-up of every kernel. The graph's
         # streams are sized from those totals times `slack`: the capacity-
         # sized row sorts then do only `slack` times the necessary work.
-        lists = self._eager(model)
+        was = autotune.enabled()
+        if tune:
+            autotune.enable(True)
+        try:
+            lists = self._eager(model)
+        finally:
+            autotune.enable(was)
         gens = sum(1 for v in lists.stats["gen_sizes"] if v > 0)
         if capacities is not None:
             self.caps = {k: max(1, int(capacities[k])) for k in ("far", "near", "queue")}
@@ -677,6 +691,18 @@ class BatchedGraph:
             self.caps = {k: int(v * slack) + 1024 for k, v in caps.items()}
         self.generations = (max(1, int(generations)) if generations is not None
                             else int(gens * slack) + 4)
+        T, Lt = self.lay.num_nodes, self.lay.num_leaves
+        if tune:
+            # the graph's row sorts run over the full stream capacities
+            autotune.enable(True)
+            try:
+                autotune.sort_digit_bits(4, max(1, T.bit_length()), self.caps["far"])
+                autotune.sort_digit_bits(4, max(1, Lt.bit_length()), self.caps["near"])
+            finally:
+                autotune.enable(was)
+        self.walk_grid, self.walk_tile = autotune.walk_plan(T, self.grid_blocks,
+                                                            WALK_PAIRS_PER_THREAD)
+        self.plans = autotune.plans()
         was = autotune.enabled()
         autotune.enable(False)
         try:
@@ -706,7 +732,7 @@ class BatchedGraph:
                             return_reordered=True, layout=self.lay, sync=False)
         geom = compute_tree_geometry_batched(bt)
         w = _graph_walk(bt, geom, self.theta, self.mac_type, self.scale, self.caps,
-                        self.generations, self.grid_blocks)
+                        self.generations, self.walk_grid, self.walk_tile)
         lists = _lists_from_streams(bt, self.li, w["far_keys"], w["far_vals"], w["near_vals"],
                                     w["far_off"], w["near_off"], self.caps["far"],
                                     self.caps["near"], {})

@@ -64,9 +64,9 @@ __global__ __launch_bounds__(256) void k_b_encode_hist(Seg sg, const float* __re
                                                        int64_t N, const float* __restrict__ b6,
                                                        uint64_t* __restrict__ codes,
                                                        uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[8 * 256];
+  __shared__ uint32_t h[kMortonHistWords];
   __shared__ int32_t s_pt[kSegLdsCap];
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) h[i] = 0;
+  for (int i = threadIdx.x; i < kMortonHistWords; i += blockDim.x) h[i] = 0;
   const bool staged = seg_stage(s_pt, sg.S, [&](int s) { return (int32_t)sg.pt_off(s); });
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -83,10 +83,10 @@ __global__ __launch_bounds__(256) void k_b_encode_hist(Seg sg, const float* __re
       codes[i] = c;
     }
 #pragma unroll
-    for (int p = 0; p < 8; ++p) morton_hist_add(h, c, valid, p);
+    for (int p = 0; p < kMortonPasses; ++p) morton_hist_add(h, c, valid, p);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x)
+  for (int i = threadIdx.x; i < kMortonHistWords; i += blockDim.x)
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
@@ -591,12 +591,12 @@ size_t ygg_batch_morton_sort_workspace_bytes(const YggBatch* b) {
   const int64_t N = b->num_points;
   Arena a(nullptr, 0);
   a.take<uint64_t>((size_t)N);      // codes
-  a.take<uint32_t>(8 * 256);        // histograms
+  a.take<uint32_t>(kMortonHistWords);        // histograms
   a.take<uint64_t>((size_t)N);      // code-sorted codes
   a.take<uint32_t>((size_t)N);      // code-sorted indices
   a.take<uint32_t>((size_t)N);      // system keys
   a.take<uint32_t>((size_t)N);      // system keys (sorted)
-  a.take<char>(ygg::sort_workspace_bytes(N, 8, 64));
+  a.take<char>(ygg::sort_workspace_bytes(N, 8, kMortonEndBit, kMortonDigitBits));
   return a.off;
 }
 
@@ -610,19 +610,20 @@ int ygg_batch_morton_sort(const YggBatch* b, const float* pos, const float* boun
   const Seg sg = make_seg(b);
   Arena a(ws, ws_bytes);
   uint64_t* codes = a.take<uint64_t>((size_t)N);
-  uint32_t* hist = a.take<uint32_t>(8 * 256);
+  uint32_t* hist = a.take<uint32_t>(kMortonHistWords);
   uint64_t* c1 = a.take<uint64_t>((size_t)N);
   uint32_t* i1 = a.take<uint32_t>((size_t)N);
   uint32_t* sk = a.take<uint32_t>((size_t)N);
   uint32_t* sk2 = a.take<uint32_t>((size_t)N);
-  size_t rest = ygg::sort_workspace_bytes(N, 8, 64);
+  size_t rest = ygg::sort_workspace_bytes(N, 8, kMortonEndBit, kMortonDigitBits);
   char* sws = a.take<char>(rest);
   if (!codes || !hist || !c1 || !i1 || !sk || !sk2 || !sws) return YGG_ERR_WORKSPACE;
-  if ((rc = fill_u32(hist, 0u, 8 * 256, st))) return rc;
+  if ((rc = fill_u32(hist, 0u, kMortonHistWords, st))) return rc;
   hipLaunchKernelGGL(k_b_encode_hist, dim3(grid_for(N, 256, 4096)), dim3(256), 0, st, sg, pos, N,
                      bounds6, codes, hist);
   YGG_LAUNCH_CHECK();
-  if ((rc = sort_pairs_u64(codes, nullptr, c1, i1, N, 0, 64, sws, rest, st, hist))) return rc;
+  if ((rc = sort_pairs_u64(codes, nullptr, c1, i1, N, 0, kMortonEndBit, sws, rest, st, hist,
+                            kMortonDigitBits))) return rc;
   if (b->num_systems == 1) {
     if ((rc = copy_u32((uint32_t*)sorted_codes, (const uint32_t*)c1, 2 * N, st))) return rc;
     return copy_u32(perm, i1, N, st);

@@ -132,8 +132,8 @@ __global__ __launch_bounds__(256) void k_encode_hist(const float* __restrict__ p
                                                      const float* __restrict__ b6,
                                                      uint64_t* __restrict__ codes,
                                                      uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[8 * 256];
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) h[i] = 0;
+  __shared__ uint32_t h[kMortonHistWords];
+  for (int i = threadIdx.x; i < kMortonHistWords; i += blockDim.x) h[i] = 0;
   __syncthreads();
   float lo[3] = {b6[0], b6[1], b6[2]};
   float w[3] = {b6[3] - b6[0], b6[4] - b6[1], b6[5] - b6[2]};
@@ -148,10 +148,10 @@ __global__ __launch_bounds__(256) void k_encode_hist(const float* __restrict__ p
       codes[i] = c;
     }
 #pragma unroll
-    for (int p = 0; p < 8; ++p) morton_hist_add(h, c, valid, p);
+    for (int p = 0; p < kMortonPasses; ++p) morton_hist_add(h, c, valid, p);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x)
+  for (int i = threadIdx.x; i < kMortonHistWords; i += blockDim.x)
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(256) void k_encode_hist_pack(const float* __restric
                                                           uint64_t* __restrict__ codes,
                                                           uint32_t* __restrict__ hist,
                                                           float4* __restrict__ packed) {
-  __shared__ uint32_t h[8 * 256];
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) h[i] = 0;
+  __shared__ uint32_t h[kMortonHistWords];
+  for (int i = threadIdx.x; i < kMortonHistWords; i += blockDim.x) h[i] = 0;
   __syncthreads();
   float lo[3] = {b6[0], b6[1], b6[2]};
   float w[3] = {b6[3] - b6[0], b6[4] - b6[1], b6[5] - b6[2]};
@@ -200,10 +200,10 @@ __global__ __launch_bounds__(256) void k_encode_hist_pack(const float* __restric
       packed[i] = make_float4(x, y, z, mass ? mass[i] : 0.0f);
     }
 #pragma unroll
-    for (int p = 0; p < 8; ++p) morton_hist_add(h, c, valid, p);
+    for (int p = 0; p < kMortonPasses; ++p) morton_hist_add(h, c, valid, p);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x)
+  for (int i = threadIdx.x; i < kMortonHistWords; i += blockDim.x)
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
@@ -666,8 +666,8 @@ int ygg_morton_encode(const float* pos, int64_t n, const float* bounds6, uint64_
 }
 
 size_t ygg_morton_sort_workspace_bytes(int64_t n) {
-  return align_up(sizeof(uint64_t) * (size_t)n) + align_up(sizeof(uint32_t) * 8 * 256) +
-         ygg::sort_workspace_bytes(n, 8, 64);
+  return align_up(sizeof(uint64_t) * (size_t)n) + align_up(sizeof(uint32_t) * kMortonHistWords) +
+         ygg::sort_workspace_bytes(n, 8, kMortonEndBit, kMortonDigitBits);
 }
 
 int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t* sorted_codes,
@@ -676,16 +676,17 @@ int ygg_morton_sort(const float* pos, int64_t n, const float* bounds6, uint64_t*
   hipStream_t st = (hipStream_t)stream;
   Arena a(ws, ws_bytes);
   uint64_t* codes = a.take<uint64_t>((size_t)n);
-  uint32_t* hist = a.take<uint32_t>(8 * 256);
-  size_t rest = ygg::sort_workspace_bytes(n, 8, 64);
+  uint32_t* hist = a.take<uint32_t>(kMortonHistWords);
+  size_t rest = ygg::sort_workspace_bytes(n, 8, kMortonEndBit, kMortonDigitBits);
   char* sws = a.take<char>(rest);
   if (!codes || !hist || !sws) return YGG_ERR_WORKSPACE;
-  int rc = fill_u32(hist, 0u, 8 * 256, st);
+  int rc = fill_u32(hist, 0u, kMortonHistWords, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_encode_hist, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, pos, n,
                      bounds6, codes, hist);
   YGG_LAUNCH_CHECK();
-  return sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, 64, sws, rest, st, hist);
+  return sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, kMortonEndBit, sws, rest, st, hist,
+                        kMortonDigitBits);
 }
 
 // ---- ygg_morton_sort_noalloc: the same encode + sort with its workspace in
@@ -737,16 +738,17 @@ static int morton_sort_pack(const float* pos, const float* mass, int64_t n, cons
   Arena a(ws, ws_bytes);
   float4* packed = a.take<float4>((size_t)n);
   uint64_t* codes = a.take<uint64_t>((size_t)n);
-  uint32_t* hist = a.take<uint32_t>(8 * 256);
-  size_t rest = ygg::sort_workspace_bytes(n, 8, 64);
+  uint32_t* hist = a.take<uint32_t>(kMortonHistWords);
+  size_t rest = ygg::sort_workspace_bytes(n, 8, kMortonEndBit, kMortonDigitBits);
   char* sws = a.take<char>(rest);
   if (!packed || !codes || !hist || !sws) return YGG_ERR_WORKSPACE;
-  int rc = fill_u32(hist, 0u, 8 * 256, st);
+  int rc = fill_u32(hist, 0u, kMortonHistWords, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_encode_hist_pack, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, pos,
                      mass, n, bounds6, codes, hist, packed);
   YGG_LAUNCH_CHECK();
-  if ((rc = sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, 64, sws, rest, st, hist)))
+  if ((rc = sort_pairs_u64(codes, nullptr, sorted_codes, perm, n, 0, kMortonEndBit, sws, rest, st, hist,
+                        kMortonDigitBits)))
     return rc;
   *packed_out = packed;
   return YGG_OK;

@@ -286,10 +286,8 @@ int sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* k
 size_t scan_workspace_bytes(int64_t n);
 size_t sort_rows_fused_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                        int64_t n, int64_t rows, int end_bit, int digit_bits,
-                        const int32_t* nbl, const int32_t* node_level, int out_i64,
-                        void* sources, void* targets, void* levels, void* ws, size_t ws_bytes,
-                        hipStream_t st);
+                        int64_t n, int end_bit, int digit_bits, int out_i64, void* sources,
+                        void* ws, size_t ws_bytes, hipStream_t st);
 int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                          int64_t n, int end_bit, int digit_bits, int out_i64, void* neighbors,
                          void* ws, size_t ws_bytes, hipStream_t st);

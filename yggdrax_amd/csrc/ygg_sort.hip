@@ -31,9 +31,18 @@ constexpr int kWaves = kSortThreads / YGG_WAVE;  // 4
 // (2 waves per SIMD); 1.5e8 pairs with 19-bit keys sort in 2.31 ms with two
 // 10-bit passes against 3.23 ms with three 8-bit passes. 11-bit digits
 // take 512 threads (8192 keys) to fit LDS.
+// 64-bit keys with 9-bit digits (the 63-bit Morton codes: 7 passes) take
+// 1024 threads of 12 keys: 12288-key tiles (runs of ~24 keys per digit) in
+// 150 KB of LDS; 2^24 clustered codes sort in 1.17 ms against 1.30 ms with
+// 8-bit digits on 4096-key tiles.
 template <typename K, int BITS>
 constexpr int sort_threads() {
-  return sizeof(K) == 4 ? (BITS == 9 || BITS == 10 ? 1024 : (BITS >= 11 ? 512 : 256)) : 256;
+  return sizeof(K) == 4 ? (BITS == 9 || BITS == 10 ? 1024 : (BITS >= 11 ? 512 : 256))
+                        : (BITS == 9 ? 1024 : 256);
+}
+template <typename K, int BITS>
+constexpr int sort_items() {
+  return (sizeof(K) == 8 && BITS == 9) ? 12 : 16;
 }
 constexpr int kSortItems = 16;
 constexpr uint32_t kFlagAgg = 1u << 30;
@@ -121,27 +130,17 @@ struct EmitKV {
   __device__ __forceinline__ void val(int64_t g, uint32_t v) const { vals_out[g] = v; }
 };
 
-// Last pass of the far row-key sort fused with the level-major far list
+// Last pass of the far row-key sort fused with the far list's sources
 // (_result_to_interactions, _interactions_impl.py:3561-3637): the key is the
 // target's level-major rank, the value the source node. Writes the sorted
-// key (for the row offsets), the target node (nodes_by_level[key]) and its
-// level, and the source. Keys >= rows are unused stream capacity.
+// key (targets, levels and row offsets follow from it in kernels with full
+// occupancy: dependent gathers here would stall the low-occupancy sort) and
+// the source.
 template <typename OutT>
 struct EmitFarList {
   uint32_t* __restrict__ keys_out;
-  const int32_t* __restrict__ nbl;
-  const int32_t* __restrict__ node_level;
   OutT* __restrict__ sources;
-  OutT* __restrict__ targets;
-  OutT* __restrict__ levels;
-  uint32_t rows;
-  __device__ __forceinline__ void key(int64_t g, uint32_t k) const {
-    keys_out[g] = k;
-    if (k >= rows) return;
-    const int32_t node = nbl[k];
-    targets[g] = (OutT)node;
-    levels[g] = (OutT)node_level[node];
-  }
+  __device__ __forceinline__ void key(int64_t g, uint32_t k) const { keys_out[g] = k; }
   __device__ __forceinline__ void val(int64_t g, uint32_t v) const { sources[g] = (OutT)v; }
 };
 
@@ -337,8 +336,9 @@ int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st) {
 static inline int64_t tile_keys(int key_bytes, int digit_bits) {
   const int threads =
       key_bytes == 4 ? (digit_bits == 9 || digit_bits == 10 ? 1024 : (digit_bits >= 11 ? 512 : 256))
-                     : 256;
-  return (int64_t)threads * kSortItems;
+                     : (digit_bits == 9 ? 1024 : 256);
+  const int items = (key_bytes == 8 && digit_bits == 9) ? 12 : kSortItems;
+  return (int64_t)threads * items;
 }
 static inline int64_t num_tiles(int64_t n, int key_bytes, int digit_bits) {
   const int64_t t = tile_keys(key_bytes, digit_bits);
@@ -384,7 +384,7 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
                            const Last* last = nullptr) {
   constexpr int kBins = 1 << BITS;
   constexpr int kThreads = sort_threads<K, BITS>();
-  constexpr int kItems = kSortItems;
+  constexpr int kItems = sort_items<K, BITS>();
   const int passes = passes_for(end_bit - begin_bit, BITS);
   const int64_t tiles = num_tiles(n, (int)sizeof(K), BITS);
   if (tile_keys((int)sizeof(K), BITS) != (int64_t)kThreads * kItems) return YGG_ERR_ARG;
@@ -472,18 +472,14 @@ static int sort_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uin
 }
 
 int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                        int64_t n, int64_t rows, int end_bit, int digit_bits,
-                        const int32_t* nbl, const int32_t* node_level, int out_i64,
-                        void* sources, void* targets, void* levels, void* ws, size_t ws_bytes,
-                        hipStream_t st) {
+                        int64_t n, int end_bit, int digit_bits, int out_i64, void* sources,
+                        void* ws, size_t ws_bytes, hipStream_t st) {
   if (out_i64) {
-    EmitFarList<int64_t> e{keys_out, nbl, node_level, (int64_t*)sources, (int64_t*)targets,
-                           (int64_t*)levels, (uint32_t)rows};
+    EmitFarList<int64_t> e{keys_out, (int64_t*)sources};
     return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st,
                            e);
   }
-  EmitFarList<int32_t> e{keys_out, nbl, node_level, (int32_t*)sources, (int32_t*)targets,
-                         (int32_t*)levels, (uint32_t)rows};
+  EmitFarList<int32_t> e{keys_out, (int32_t*)sources};
   return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e);
 }
 

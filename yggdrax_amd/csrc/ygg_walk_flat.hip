@@ -482,6 +482,25 @@ __global__ __launch_bounds__(256) void k_near_dense(int L, const int64_t* __rest
   }
 }
 
+// Targets and their levels of the level-major far list from its sorted row
+// keys (level-major ranks); keys >= rows are unused stream capacity.
+template <typename OutT>
+__global__ __launch_bounds__(256) void k_far_targets(const uint32_t* __restrict__ keys,
+                                                     int64_t n, int rows,
+                                                     const int32_t* __restrict__ nbl,
+                                                     const int32_t* __restrict__ node_level,
+                                                     OutT* __restrict__ targets,
+                                                     OutT* __restrict__ levels) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t key = keys[i];
+    if (key >= (uint32_t)rows) continue;
+    const int32_t node = nbl[key];
+    targets[i] = (OutT)node;
+    levels[i] = (OutT)node_level[node];
+  }
+}
+
 // Dense [L][maxc] neighbour leaf rows from the near list itself.
 template <typename OutT>
 __global__ __launch_bounds__(256) void k_near_dense_from_list(
@@ -821,19 +840,25 @@ int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n, in
   if (num_nodes < 1 || n < 0 || !lm_offsets || !sorted_keys) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int T = (int)num_nodes;
-  int rc = ygg::sort_far_rows_fused(keys, vals, sorted_keys, n, num_nodes, end_bit, digit_bits,
-                                    nodes_by_level, node_level, out_i64, sources, targets,
-                                    target_levels, ws, ws_bytes, st);
+  int rc = ygg::sort_far_rows_fused(keys, vals, sorted_keys, n, end_bit, digit_bits, out_i64,
+                                    sources, ws, ws_bytes, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_row_offsets, dim3(grid_for((int64_t)T + 1, 256)), dim3(256), 0, st,
                      sorted_keys, n, T, lm_offsets);
-  dim3 gr(grid_for(T, 256));
-  if (out_i64)
+  dim3 gr(grid_for(T, 256)), ge(grid_for(n > 0 ? n : 1, 256, 16384));
+  if (out_i64) {
+    if (n > 0)
+      hipLaunchKernelGGL(k_far_targets<int64_t>, ge, dim3(256), 0, st, sorted_keys, n, T,
+                         nodes_by_level, node_level, (int64_t*)targets, (int64_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int64_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int64_t*)counts, (int64_t*)node_offsets, counts32);
-  else
+  } else {
+    if (n > 0)
+      hipLaunchKernelGGL(k_far_targets<int32_t>, ge, dim3(256), 0, st, sorted_keys, n, T,
+                         nodes_by_level, node_level, (int32_t*)targets, (int32_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int32_t*)counts, (int32_t*)node_offsets, counts32);
+  }
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
