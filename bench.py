@@ -162,10 +162,13 @@ def _same_lists(torch, a, b, num_rows):
 
 
 def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
-    """Walk time of the plain path (counting walk, then the emission walk at
-    exact sizes) against the learned path (capacity MLP inference, then one
-    emission walk at the predicted sizes), on the same tree, with a check
-    that both produce the same lists."""
+    """Walk time of the three stream sizings on the same tree: the plain
+    call (one resumable emission walk from the default capacities), the
+    reference's count pass (counting walk, then the emission walk at exact
+    sizes) and the learned path (capacity MLP inference, then one emission
+    walk at the predicted sizes), with a check that all produce the same
+    lists, and the stream memory each one allocated."""
+    from yggdrax_amd import interactions as I
     from yggdrax_amd import learned
     model = learned.CapacityModel.load()
     tree, ps, _, _ = ygg.build_octree(pos, mass, leaf_size=leaf, return_reordered=True)
@@ -173,6 +176,13 @@ def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
 
     def plain():
         return ygg.build_interactions_and_neighbors(tree, geom, theta=theta, **WALK_KW)
+
+    def counted():
+        old, I.STREAM_SIZING = I.STREAM_SIZING, "count"
+        try:
+            return plain()
+        finally:
+            I.STREAM_SIZING = old
 
     def with_model():
         hint = model.capacity_hint(tree, geom)
@@ -188,29 +198,40 @@ def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps * 1e3, out
 
-    from yggdrax_amd import interactions as I
     t_plain, (pi, pn) = timed(plain)
-    runs0 = I.FLAT_STATS["count_walks"]
+    plain_walk = dict(I.LAST_WALK)
+    t_count, (ci, cn) = timed(counted)
+    runs0 = I.FLAT_STATS["hint_overflows"] + I.FLAT_STATS["hint_growths"]
     t_learn, ((li, ln), hint) = timed(with_model)
-    fallbacks = I.FLAT_STATS["count_walks"] - runs0
+    fallbacks = I.FLAT_STATS["hint_overflows"] + I.FLAT_STATS["hint_growths"] - runs0
     T = int(tree.num_nodes)
     L = int(pn.counts.numel())
-    same = (_same_lists(torch, (pi.offsets, pi.counts, pi.sources), (li.offsets, li.counts,
-                                                                      li.sources), T)
-            and torch.equal(pi.targets, li.targets)
-            and _same_lists(torch, (pn.offsets, pn.counts, pn.neighbors),
-                            (ln.offsets, ln.counts, ln.neighbors), L))
+
+    def same(a, b):
+        return (_same_lists(torch, (a[0].offsets, a[0].counts, a[0].sources),
+                            (b[0].offsets, b[0].counts, b[0].sources), T)
+                and torch.equal(a[0].targets, b[0].targets)
+                and _same_lists(torch, (a[1].offsets, a[1].counts, a[1].neighbors),
+                                (b[1].offsets, b[1].counts, b[1].neighbors), L))
     nf, nn = int(pi.sources.numel()), int(pn.neighbors.numel())
     # slot memory the fixed-capacity method would need on this tree: one row
     # of the largest count per node (far) and per leaf (near), int32 entries
     max_far, max_near = int(pi.counts.max()), int(pn.counts.max())
-    return dict(plain_walk_ms=round(t_plain, 3), learned_walk_ms=round(t_learn, 3),
+    # far and near stream bytes (32-bit key + 32-bit value per entry)
+    return dict(plain_walk_ms=round(t_plain, 3), count_pass_walk_ms=round(t_count, 3),
+                learned_walk_ms=round(t_learn, 3),
                 max_far_per_node=max_far, max_near_per_leaf=max_near,
                 fixed_capacity_slot_bytes=4 * (T * max_far + L * max_near),
-                walk_speedup=round(t_plain / t_learn, 3), identical_lists=same,
+                walk_speedup=round(t_plain / t_learn, 3),
+                walk_speedup_vs_count_pass=round(t_count / t_learn, 3),
+                identical_lists=same((pi, pn), (li, ln)) and same((pi, pn), (ci, cn)),
                 fallback_walks=fallbacks, hint=hint, true_far=nf, true_near=nn,
                 far_overalloc=round(hint["far"] / max(1, nf), 3),
-                near_overalloc=round(hint["near"] / max(1, nn), 3))
+                near_overalloc=round(hint["near"] / max(1, nn), 3),
+                plain_stream_bytes=8 * (plain_walk["far_capacity"] + plain_walk["near_capacity"]),
+                plain_stream_growths=plain_walk["stream_growths"],
+                learned_stream_bytes=8 * (hint["far"] + hint["near"]),
+                count_pass_stream_bytes=8 * (nf + nn))
 
 
 _MIX_C = 0x9E3779B97F4A7C15 - (1 << 64)
@@ -290,10 +311,12 @@ def _mismatches(torch, a, b):
 def batched_leg(torch, steps, warmup):
     """Batched workload: 1024 Plummer systems, N_i log-uniform in [256, 65536]
     (PCG64 seed 4), build_octree leaf 32, dehnen theta 0.6. Times, on the same
-    systems: the plain batched pipeline (counting walk, then the emission
-    walk, over the forest), the learned batched pipeline (capacity model,
-    then one emission walk) -- the equal-batching comparison of the learned
-    capacities --, its HIP-graph capture replayed on new positions every step
+    systems: the plain batched pipeline (one resumable emission walk over
+    the forest from the default stream capacities), the same with the
+    reference's count pass (counting walk, then the emission walk at exact
+    sizes), the learned batched pipeline (capacity model, then one emission
+    walk) -- the equal-batching comparisons of the learned capacities --,
+    its HIP-graph capture replayed on new positions every step
     with stream capacities from the capacity model, one single-system call
     per system, and the reference's fixed-capacity method (per-row slots of
     2048 far and 2048 near entries, its automatic pair-queue ladder, retry
@@ -344,15 +367,27 @@ def batched_leg(torch, steps, warmup):
         bt, lists = run_batched()
     torch.cuda.synchronize()
     t_b = (time.perf_counter() - t0) / steps * 1e3
+    plain_walk = dict(I.LAST_WALK)
+    sizing, I.STREAM_SIZING = I.STREAM_SIZING, "count"
+    try:
+        run_batched()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _, clists = run_batched()
+        torch.cuda.synchronize()
+        t_c = (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        I.STREAM_SIZING = sizing
     run_learned()
     torch.cuda.synchronize()
-    c0 = I.FLAT_STATS["count_walks"]
+    c0 = I.FLAT_STATS["hint_overflows"] + I.FLAT_STATS["hint_growths"]
     t0 = time.perf_counter()
     for _ in range(steps):
         _, llists, hint = run_learned()
     torch.cuda.synchronize()
     t_l = (time.perf_counter() - t0) / steps * 1e3
-    learned_fallbacks = I.FLAT_STATS["count_walks"] - c0
+    learned_fallbacks = I.FLAT_STATS["hint_overflows"] + I.FLAT_STATS["hint_growths"] - c0
     # the learned batched pipeline captured once as a HIP graph (stream
     # capacities from the capacity model), replayed on new positions: a
     # small random displacement of every point per step, inputs in HBM
@@ -394,7 +429,9 @@ def batched_leg(torch, steps, warmup):
     l_dig = _batch_digests(torch, llists)
     s_dig = _single_digests(torch, outs)
     f_dig = _single_digests(torch, fouts)
+    c_dig = _batch_digests(torch, clists)
     mism = dict(learned_vs_plain_batched=_mismatches(torch, l_dig, b_dig),
+                count_pass_vs_plain_batched=_mismatches(torch, c_dig, b_dig),
                 graph_replay_vs_eager_on_new_positions=_mismatches(torch, g_dig, moved_dig),
                 per_system_vs_batched=_mismatches(torch, s_dig, b_dig),
                 fixed_capacity_vs_per_system=_mismatches(torch, f_dig, s_dig))
@@ -404,8 +441,14 @@ def batched_leg(torch, steps, warmup):
                 points=n, batched_ms=round(t_b, 3), batched_mpts_s=round(n / t_b / 1e3, 3),
                 learned_batched_ms=round(t_l, 3),
                 learned_batched_mpts_s=round(n / t_l / 1e3, 3),
+                count_pass_batched_ms=round(t_c, 3),
                 learned_fallback_walks=learned_fallbacks,
                 learned_speedup_vs_batched_plain=round(t_b / t_l, 3),
+                learned_speedup_vs_batched_count_pass=round(t_c / t_l, 3),
+                plain_stream_bytes=8 * (plain_walk["far_capacity"]
+                                        + plain_walk["near_capacity"]),
+                plain_stream_growths=plain_walk["stream_growths"],
+                learned_stream_bytes=8 * (hint["far"] + hint["near"]),
                 graph_replay_ms=round(t_g, 3), graph_replay_mpts_s=round(n / t_g / 1e3, 3),
                 graph_replay_positions="new positions every step (N(0, 1e-4) displacement)",
                 graph_capacities=dict(graph.caps, generations=graph.generations,
@@ -677,7 +720,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"clustered 2^{args.log2n} points (64 Plummer clusters + 10% "
                                f"uniform background, seed 3), build_octree leaf {args.leaf}, "
-                               f"dehnen theta {args.theta}, count pass + exact fill, int64 outputs",
+                               f"dehnen theta {args.theta}, one resumable emission walk from the "
+                               f"default stream capacities, int64 outputs",
                    "parallelism": "single GPU"},
         "stages": stages, "learned": lrn, "batched": bat, "layouts": lay, "moments": mom,
         "roofline": roof,

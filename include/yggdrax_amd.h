@@ -452,6 +452,7 @@ typedef struct YggPolicyDesc {
  * flag (queue, far, near) and the walk stops after that generation.
  * host_status and gen_ms (optional per-generation kernel times from HIP
  * events on `stream`) are as for ygg_walk_bfs. */
+#define YGG_WALK_GEN_RECORD_WORDS 16
 typedef struct YggFlatWalkParams {
   int32_t num_nodes, num_internal, num_leaves, root;
   int32_t mac_type, emit, max_generations, grid_blocks;
@@ -484,6 +485,20 @@ typedef struct YggFlatWalkParams {
   const YggPolicyDesc* policy; /* optional: accept/near rules of a pair-policy descriptor for both
                                   walks (NULL: the built-in MAC, near for two different leaves) */
   int32_t* far_tags;          /* emission walk with a policy: [far_cap] tag of each far entry */
+  uint64_t* gen_fill;         /* optional, emission walk only: [max_generations *
+                                 YGG_WALK_GEN_RECORD_WORDS] per-generation records (entry counts
+                                 far << 32 | near, and the stream offsets the generation starts
+                                 at), which make the walk resumable.
+                                 Generation g writes after generations 0..g-1 and runs only if two
+                                 entries per pair fit the remaining far and near capacity; if not,
+                                 it writes nothing, flags[3] = g + 1, *num_generations = g, the
+                                 counters hold the totals of generations 0..g-1, and a call with
+                                 start_generation = g and larger streams (their first
+                                 counters[0] / counters[1] entries copied over) continues the
+                                 walk. NULL: overflowing entries set flags[1] / flags[2]. */
+  int32_t start_generation;   /* 0: a new walk; g > 0: resume a paused walk at generation g (the
+                                 other device buffers as the paused call left them) */
+  int32_t pad1;
 } YggFlatWalkParams;
 
 int ygg_walk_flat(const YggFlatWalkParams* params, uint32_t* host_status,

@@ -120,12 +120,15 @@ def test_c5_learned_batch_256_matches_oracle_every_system(ygg):
     bt = batch.build_octree_batched(P, M, off, leaf_size=32)
     geom = batch.compute_tree_geometry_batched(bt)
     hint = model.capacity_hint_batched(bt, geom)
-    c0, h0 = FLAT_STATS["count_walks"], FLAT_STATS["hinted_walks"]
+    before = dict(FLAT_STATS)
     lists = batch.build_interactions_and_neighbors_batched(bt, geom, 0.6, mac_type="dehnen",
                                                            capacity_hint=hint)
     torch.cuda.synchronize()
-    assert FLAT_STATS["hinted_walks"] == h0 + 1
-    print(f"[configs] c5 hint {hint} counting walks {FLAT_STATS['count_walks'] - c0}", flush=True)
-    assert FLAT_STATS["count_walks"] == c0, "the learned hint fell back to counting"
+    assert FLAT_STATS["hinted_walks"] == before["hinted_walks"] + 1
+    print(f"[configs] c5 hint {hint} overflows "
+          f"{FLAT_STATS['hint_overflows'] - before['hint_overflows']} growths "
+          f"{FLAT_STATS['hint_growths'] - before['hint_growths']}", flush=True)
+    for k in ("count_walks", "plain_walks", "hint_overflows", "hint_growths"):
+        assert FLAT_STATS[k] == before[k], f"the learned hint did not hold ({k})"
     for k, got in enumerate(_batch_digests(ygg, bt, geom, lists, 256)):
         _check(got, fx, k)

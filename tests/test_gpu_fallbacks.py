@@ -1,8 +1,9 @@
 """Fallback paths that a capacity can force from the host, every kernel
-exiting normally on its own bounds checks: an undersized stream/queue hint
-(single system and batch) reruns the walk with counting, and a HIP-graph
-replay whose captured capacities or generation count are too small returns
-the eager pipeline's result. Outputs never depend on the capacity."""
+exiting normally on its own bounds checks: an undersized far or near hint
+(single system and batch) grows its stream during the walk, an undersized
+queue hint reruns the walk without the hint, and a HIP-graph replay whose
+captured capacities or generation count are too small returns the eager
+pipeline's result. Outputs never depend on the capacity."""
 
 from __future__ import annotations
 
@@ -44,9 +45,12 @@ def _assert_lists_equal(a, b):
         assert np.array_equal(k1, k2)
 
 
-@pytest.mark.parametrize("hint", [TINY_HINT, {"queue": 1 << 20, "far": 1, "near": 1 << 20},
-                                  {"queue": 8, "far": 1 << 22, "near": 1 << 22}])
-def test_undersized_hint_falls_back_single_system(ygg, hint):
+@pytest.mark.parametrize("hint,outcome", [
+    (TINY_HINT, "hint_overflows"),
+    ({"queue": 1 << 20, "far": 1, "near": 1 << 20}, "hint_growths"),
+    ({"queue": 1 << 20, "far": 1 << 22, "near": 1}, "hint_growths"),
+    ({"queue": 8, "far": 1 << 22, "near": 1 << 22}, "hint_overflows")])
+def test_undersized_hint_falls_back_single_system(ygg, hint, outcome):
     from yggdrax_amd.interactions import FLAT_STATS
     print(f"[fallback] single hint={hint}", flush=True)
     pos, mass = inputs.clustered(1 << 15, 7)
@@ -55,11 +59,17 @@ def test_undersized_hint_falls_back_single_system(ygg, hint):
     geom = ygg.compute_tree_geometry(tree, ps)
     kw = dict(theta=0.5, mac_type="dehnen")
     plain = ygg.build_interactions_and_neighbors(tree, geom, **kw)
-    h0, c0 = FLAT_STATS["hint_overflows"], FLAT_STATS["count_walks"]
+    before = dict(FLAT_STATS)
     hinted = ygg.build_interactions_and_neighbors(tree, geom, capacity_hint=hint, **kw)
     torch.cuda.synchronize()
-    assert FLAT_STATS["hint_overflows"] == h0 + 1
-    assert FLAT_STATS["count_walks"] == c0 + 1
+    other = "hint_growths" if outcome == "hint_overflows" else "hint_overflows"
+    assert FLAT_STATS[outcome] == before[outcome] + 1
+    assert FLAT_STATS[other] == before[other]
+    if outcome == "hint_growths":
+        assert FLAT_STATS["stream_growths"] > before["stream_growths"]
+        assert FLAT_STATS["plain_walks"] == before["plain_walks"]
+    else:  # the walk reran without the hint
+        assert FLAT_STATS["plain_walks"] == before["plain_walks"] + 1
     _assert_lists_equal(hinted, plain)
     parity.assert_neighbors_equal(hinted[1], {k: parity.np_(getattr(plain[1], k)) for k in
                                               plain[1]._fields}, dense=True)
@@ -78,20 +88,25 @@ def _batch(num, seed, nmax=4096):
             torch.from_numpy(np.concatenate(mass)).cuda(), offsets)
 
 
-def test_undersized_hint_falls_back_batched(ygg):
-    """A queue hint below the batch's root count is raised to it; the far and
-    near streams of one entry overflow; the batch reruns with counting."""
+@pytest.mark.parametrize("hint,outcome", [
+    (TINY_HINT, "hint_overflows"),
+    ({"queue": 1 << 22, "far": 1, "near": 1}, "hint_growths")])
+def test_undersized_hint_falls_back_batched(ygg, hint, outcome):
+    """A queue hint below the batch's root count is raised to it and then
+    overflows, so the batch reruns without the hint; one-entry far and near
+    hints with room in the queue grow during the walk."""
     from yggdrax_amd import batch
     from yggdrax_amd.interactions import FLAT_STATS
+    print(f"[fallback] batched hint={hint}", flush=True)
     P, M, offsets = _batch(24, 11)
     bt = batch.build_octree_batched(P, M, offsets, leaf_size=32)
     geom = batch.compute_tree_geometry_batched(bt)
     plain = batch.build_interactions_and_neighbors_batched(bt, geom, 0.6, mac_type="dehnen")
-    h0 = FLAT_STATS["hint_overflows"]
+    h0 = FLAT_STATS[outcome]
     hinted = batch.build_interactions_and_neighbors_batched(bt, geom, 0.6, mac_type="dehnen",
-                                                            capacity_hint=TINY_HINT)
+                                                            capacity_hint=hint)
     torch.cuda.synchronize()
-    assert FLAT_STATS["hint_overflows"] == h0 + 1
+    assert FLAT_STATS[outcome] == h0 + 1
     for k in range(24):
         _assert_lists_equal(hinted.system(k), plain.system(k))
 

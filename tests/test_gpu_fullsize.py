@@ -68,20 +68,21 @@ def test_clustered_2p24_every_field_matches_oracle(c3):
 
 
 def test_clustered_2p24_learned_capacity_matches_oracle(c3):
-    """The one-pass walk sized by the capacity model: no counting walk, and
-    every list field equal to the oracle's."""
+    """The one-pass walk sized by the capacity model: no counting walk, no
+    stream growth, and every list field equal to the oracle's."""
     import yggdrax_amd as ygg
     from yggdrax_amd import learned
     from yggdrax_amd.interactions import FLAT_STATS
     tree, geom, _, _, _ = c3
     model = learned.CapacityModel.load()
     hint = model.capacity_hint(tree, geom)
-    c0 = FLAT_STATS["count_walks"]
+    before = dict(FLAT_STATS)
     inter, nbrs = ygg.build_interactions_and_neighbors(
         tree, geom, theta=0.5, mac_type="dehnen", neighbor_positions=False,
         strict_int32=False, capacity_hint=hint)
     torch.cuda.synchronize()
-    assert FLAT_STATS["count_walks"] == c0, "the learned hint fell back to counting"
+    for k in ("count_walks", "plain_walks", "hint_overflows", "hint_growths"):
+        assert FLAT_STATS[k] == before[k], f"the learned hint did not hold ({k})"
     got = FS.gpu_system(tree, geom, inter, nbrs, octree=True)
     bad = FS.compare(got, FS.load(FIXTURE), 0)
     assert not bad, bad

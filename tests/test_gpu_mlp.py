@@ -77,9 +77,9 @@ def test_mlp_training_reduces_loss():
     reason="trained capacity model not present")
 def test_learned_capacity_path_matches_plain_path():
     """Held-out clustered system: the one-pass walk sized by the trained
-    network returns the same lists as the plain count-then-fill walk, runs
-    no counting walk (the plain path runs one), and sizes its streams well
-    below the reference's fixed per-node capacities."""
+    network returns the same lists as the plain walk, with no counting walk,
+    no rerun and no stream growth, and sizes its streams well below the
+    reference's fixed per-node capacities."""
     import time
     import yggdrax_amd as ygg
     from yggdrax_amd import inputs, interactions, learned
@@ -90,13 +90,14 @@ def test_learned_capacity_path_matches_plain_path():
     geom = ygg.compute_tree_geometry(tree, ps)
     kw = dict(theta=0.5, mac_type="dehnen", neighbor_positions=False, strict_int32=False)
     st = interactions.FLAT_STATS
-    c0 = st["count_walks"]
+    p0 = st["plain_walks"]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plain_i, plain_n = ygg.build_interactions_and_neighbors(tree, geom, **kw)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    assert st["count_walks"] == c0 + 1
+    assert st["plain_walks"] == p0 + 1
+    before = dict(st)
     t0 = time.perf_counter()
     hint = model.capacity_hint(tree, geom)
     li, ln = ygg.build_interactions_and_neighbors(tree, geom, capacity_hint=hint, **kw)
@@ -104,7 +105,8 @@ def test_learned_capacity_path_matches_plain_path():
     t_learned = time.perf_counter() - t0
     print(f"plain {t_plain * 1e3:.2f} ms, learned {t_learned * 1e3:.2f} ms, hint {hint}, "
           f"true far {plain_i.sources.numel()} near {plain_n.neighbors.numel()}")
-    assert st["count_walks"] == c0 + 1, "the learned hint overflowed and fell back to counting"
+    for k in ("count_walks", "plain_walks", "hint_overflows", "hint_growths"):
+        assert st[k] == before[k], f"the learned hint did not hold ({k})"
     nf, nn = plain_i.sources.numel(), plain_n.neighbors.numel()
     assert nf <= hint["far"] <= 3 * nf + (1 << 20)
     assert nn <= hint["near"] <= 3 * nn + (1 << 20)

@@ -65,7 +65,8 @@ class FlatWalkParams(ctypes.Structure):
         ("flags", _P), ("counters", _P), ("far_keys", _P), ("far_vals", _P),
         ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("xcd_groups", _I32),
         ("node_hist", _P), ("roots", _P), ("num_roots", _I64), ("spread_counters", _P),
-        ("spread_lanes", _I64), ("policy", _P), ("far_tags", _P),
+        ("spread_lanes", _I64), ("policy", _P), ("far_tags", _P), ("gen_fill", _P),
+        ("start_generation", _I32), ("pad1", _I32),
     ]
 
 

@@ -493,7 +493,8 @@ def build_interactions_and_neighbors_batched(trees: BatchedTrees, geom: BatchedG
     """build_interactions_and_neighbors (interactions.py:338-448) for every
     system of a batch, by one dual-tree walk over the forest. With
     `capacity_hint` ({"far", "near", "queue"} for the whole batch) the walk
-    runs once at those sizes, falling back to counting on overflow."""
+    starts at those sizes: far and near streams grow if they turn out too
+    small, and a pair queue that overflows reruns the walk without it."""
     lib = _lib.load()
     lay = trees.layout
     if dehnen_radius_scale <= 0:
@@ -720,11 +721,12 @@ class BatchedGraph:
         geom = compute_tree_geometry_batched(bt)
         self.hint = model.capacity_hint_batched(bt, geom) if model is not None else None
         from .interactions import FLAT_STATS
-        c0 = FLAT_STATS["count_walks"]
+        miss0 = FLAT_STATS["hint_overflows"] + FLAT_STATS["hint_growths"]
         lists = build_interactions_and_neighbors_batched(
             bt, geom, self.theta, mac_type=self.mac_type, dehnen_radius_scale=self.scale,
             capacity_hint=self.hint)
-        lists.stats["hinted"] = self.hint is not None and FLAT_STATS["count_walks"] == c0
+        lists.stats["hinted"] = (self.hint is not None and FLAT_STATS["hint_overflows"]
+                                 + FLAT_STATS["hint_growths"] == miss0)
         return lists
 
     def _pipeline(self) -> dict:

@@ -48,7 +48,21 @@ struct FlatArgs {
   int32_t spread_lanes;
   YggPolicyDesc policy;  // accept / near rules (the built-in MAC and leaf pairs by default)
   int32_t* far_tags;     // emission walk with tags: [far_cap]
+  // Resumable emission walk: one record of kGenRecWords words per
+  // generation, or null. Record g holds generation g's entry counts packed
+  // as far << 32 | near (word 0, the reservation counter) and the far and
+  // near stream offsets at which it starts (words 1 and 2, the totals of
+  // generations 0..g-1, written by workgroup 0 of generation g). Generation
+  // g runs only if its worst case, two entries per pair, fits the remaining
+  // capacity of both streams; otherwise it writes nothing and records g + 1
+  // in flags[3], and the host resumes from g with larger streams.
+  unsigned long long* gen_fill;
 };
+
+// Words per generation record of a resumable walk: one 128-byte line each,
+// so the reservations of generation g never share a line with the record
+// of generation g - 1 that every workgroup of g reads.
+constexpr int kGenRecWords = YGG_WALK_GEN_RECORD_WORDS;
 
 __device__ __forceinline__ unsigned long long wave_incl_u64(unsigned long long v) {
   const int lane = (int)lane_id();
@@ -65,7 +79,7 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
   static_assert(!TAGS || EMIT, "tags are written by the emission walk");
   constexpr int kFlatChunk = kFlatThreads * kFlatPer;
   if (atomic_load_agent(&a.flags[0]) | atomic_load_agent(&a.flags[1]) |
-      atomic_load_agent(&a.flags[2]))
+      atomic_load_agent(&a.flags[2]) | atomic_load_agent(&a.flags[3]))
     return;
   const int64_t raw = a.gen_sizes[g];
   const int64_t size = raw < a.queue_cap ? raw : a.queue_cap;
@@ -81,6 +95,36 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
     c_first = nchunks * g / X + blockIdx.x / X;
     c_end = nchunks * (g + 1) / X;
     c_step = gridDim.x / X;
+  }
+  // Resumable emission: this generation's stream offsets (the previous
+  // record's offsets plus its final counts, the same in every workgroup;
+  // workgroup 0 stores them in record g, also when it has no chunk) and the
+  // capacity check that pauses the walk before anything is written.
+  __shared__ unsigned long long s_gbase[2];
+  if (EMIT && a.gen_fill) {
+    if (c_first >= c_end && blockIdx.x != 0) return;
+    if (threadIdx.x == 0) {
+      unsigned long long bf = 0, bn = 0;
+      if (g > 0) {
+        const unsigned long long* r = a.gen_fill + (size_t)(g - 1) * kGenRecWords;
+        const unsigned long long c = r[0];
+        bf = r[1] + (c >> 32);
+        bn = r[2] + (c & 0xFFFFFFFFull);
+      }
+      s_gbase[0] = bf;
+      s_gbase[1] = bn;
+      if (blockIdx.x == 0) {
+        a.gen_fill[(size_t)g * kGenRecWords + 1] = bf;
+        a.gen_fill[(size_t)g * kGenRecWords + 2] = bn;
+      }
+    }
+    __syncthreads();
+    if (c_first >= c_end) return;
+    const int64_t worst = 2 * size;
+    if ((int64_t)s_gbase[0] + worst > a.far_cap || (int64_t)s_gbase[1] + worst > a.near_cap) {
+      if (threadIdx.x == 0) atomicMax(&a.flags[3], (uint32_t)(g + 1));
+      return;
+    }
   }
   if (c_first >= c_end) return;
   const uint64_t* __restrict__ wf_in = (g & 1) ? a.wf1 : a.wf0;
@@ -205,6 +249,14 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         acc_f += tf;
         acc_n += tn;
         s_base[1] = s_base[2] = 0;
+      } else if (EMIT && a.gen_fill) {
+        // this generation's far (high) and near (low 32 bits) counts in one
+        // reservation; they stay below 2 * queue_cap < 2^32
+        const unsigned long long add = (tf << 32) | tn;
+        const unsigned long long old =
+            add ? atomicAdd(&a.gen_fill[(size_t)g * kGenRecWords], add) : 0ull;
+        s_base[1] = s_gbase[0] + (old >> 32);
+        s_base[2] = s_gbase[1] + (old & 0xFFFFFFFFull);
       } else if (EMIT) {
         // far (high 32 bits) and near (low 32 bits) running totals in one
         // reservation; a field that would pass 2^32 marks the walk as
@@ -342,13 +394,36 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
 }
 
 // The emission walk's packed running total (far << 32 | near) in
-// counters[0], unpacked into counters[0] (far) and counters[1] (near).
-__global__ void k_unpack_totals(FlatArgs a) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+// counters[0], unpacked into counters[0] (far) and counters[1] (near); for a
+// resumable walk, the sums of the per-generation counts of generations
+// [0, G) (zero for generations that have not run). One wavefront.
+__global__ void k_unpack_totals(FlatArgs a, int G) {
+  if (a.gen_fill) {
+    unsigned long long f = 0, n = 0;
+    for (int j = (int)threadIdx.x; j < G; j += YGG_WAVE) {
+      const unsigned long long v = a.gen_fill[(size_t)j * kGenRecWords];
+      f += v >> 32;
+      n += v & 0xFFFFFFFFull;
+    }
+#pragma unroll
+    for (int off = YGG_WAVE / 2; off > 0; off >>= 1) {
+      f += __shfl_xor(f, off, YGG_WAVE);
+      n += __shfl_xor(n, off, YGG_WAVE);
+    }
+    if (threadIdx.x == 0) {
+      a.counters[0] = f;
+      a.counters[1] = n;
+    }
+  } else if (threadIdx.x == 0 && blockIdx.x == 0) {
     const unsigned long long c = a.counters[0];
     a.counters[0] = c >> 32;
     a.counters[1] = c & 0xFFFFFFFFull;
   }
+}
+
+// Resuming a paused walk: clear the pause mark.
+__global__ void k_flat_resume(FlatArgs a) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) a.flags[3] = 0;
 }
 
 // counters[0..1] = the sums of the spread lanes (one workgroup).
@@ -378,6 +453,8 @@ __global__ void k_flat_init(FlatArgs a, int root, int G) {
   for (int k = i; k <= G; k += gridDim.x * blockDim.x) a.gen_sizes[k] = 0;
   if (i < 4) a.flags[i] = 0;
   if (i < 2) a.counters[i] = 0;  // [0] far total, [1] near total
+  if (a.gen_fill)
+    for (int k = i; k < G * kGenRecWords; k += gridDim.x * blockDim.x) a.gen_fill[k] = 0;
   if (a.spread)
     for (int k = i; k < 2 * a.spread_lanes; k += gridDim.x * blockDim.x) a.spread[k] = 0;
   if (a.roots) {
@@ -690,12 +767,20 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
   a.spread_lanes = (int32_t)p->spread_lanes;
   if (int rc = set_policy(p, &a)) return rc;
   const int G = p->max_generations;
-  const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
-  hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
-                     G);
+  a.gen_fill = p->emit ? (unsigned long long*)p->gen_fill : nullptr;
+  const int g0 = p->start_generation;
+  if (g0 < 0 || g0 >= G || (g0 > 0 && !a.gen_fill)) return YGG_ERR_ARG;
+  if (g0 == 0) {
+    const int64_t init_n = (G + 1) > a.num_roots ? (G + 1) : a.num_roots;
+    hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a,
+                       p->root, G);
+  } else {
+    hipLaunchKernelGGL(k_flat_resume, dim3(1), dim3(64), 0, st, a);
+  }
   YGG_LAUNCH_CHECK();
   const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
-  int g = 0, batch = p->gen_hint > 0 ? p->gen_hint : 24;
+  int g = g0, batch = p->gen_hint > 0 ? p->gen_hint - g0 : 24;
+  if (batch < 2) batch = 2;
   while (true) {
     int end = g + batch < G ? g + batch : G;
     for (int k = g; k < end; ++k) {
@@ -716,6 +801,8 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
         done = k;
         break;
       }
+    // a paused walk ran generations [0, fl[3] - 1)
+    if (fl[3]) done = (int)fl[3] - 1;
     if ((fl[0] | fl[1] | fl[2]) || done >= 0) {
       *num_generations = done >= 0 ? done : end;
       if (a.spread && !p->emit) {
@@ -723,7 +810,7 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
         YGG_LAUNCH_CHECK();
       }
       if (p->emit) {
-        hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a, G);
         YGG_LAUNCH_CHECK();
       }
       if (gen_ms) {
@@ -750,8 +837,10 @@ int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
   if (p->emit && (!p->far_keys || !p->far_vals || !p->near_keys || !p->near_vals))
     return YGG_ERR_ARG;
   if (p->roots && (p->num_roots < 1 || p->num_roots > p->queue_cap)) return YGG_ERR_ARG;
+  if (p->gen_fill || p->start_generation) return YGG_ERR_ARG;  // no host to resume a pause
   hipStream_t st = (hipStream_t)stream;
   FlatArgs a;
+  a.gen_fill = nullptr;
   a.num_internal = p->num_internal;
   a.mac_type = p->mac_type;
   a.theta_sq = p->theta_sq;
@@ -785,7 +874,7 @@ int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
   const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
   for (int k = 0; k < G; ++k) launch_flat(a, k, p->emit != 0, p->pairs_per_thread, blocks, st);
   if (a.spread && !p->emit) hipLaunchKernelGGL(k_spread_sum, dim3(1), dim3(256), 0, st, a);
-  if (p->emit) hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a);
+  if (p->emit) hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a, G);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

@@ -28,16 +28,36 @@ from .tree import DeviceTopology, device_topology, resolve_tree_topology
 
 logger = logging.getLogger(__name__)
 
+_DEFAULT_MAX_INTERACTIONS = 512
 _DEFAULT_MAX_NEIGHBORS = 2048
 DEFAULT_PAIR_QUEUE_MULTIPLIER = 128
 _COUNT_PASS_FILL_INTERACTION_CAP = 4096
 _AUTO_INTERACTION_MAX = 16384
 MAX_GENERATIONS = 1024
+# 64-bit words per generation record of a resumable walk
+# (YGG_WALK_GEN_RECORD_WORDS of the C ABI)
+GEN_RECORD_WORDS = 16
 # Opt-in per-generation kernel timing (used by the benchmark's roofline leg).
 WALK_TIMING = {"enabled": False, "log": []}
-# Number of counting walks run by the flat path (a capacity hint that holds
-# runs none; a hint that overflows falls back to one).
-FLAT_STATS = {"count_walks": 0, "hinted_walks": 0, "hint_overflows": 0}
+# Walks run by the flat path: counting walks, emission walks without a
+# capacity hint, hinted walks, hinted walks whose pair queue overflowed
+# (rerun without the hint) or whose far/near streams had to grow, and
+# stream growths of any resumable walk.
+FLAT_STATS = {"count_walks": 0, "plain_walks": 0, "hinted_walks": 0, "hint_overflows": 0,
+              "hint_growths": 0, "stream_growths": 0}
+# Stream sizing of a walk without a capacity hint. "grow": one resumable
+# emission walk whose far and near streams start at the reference's default
+# per-row capacities in aggregate (max_interactions_per_node = 512 entries
+# per node, max_neighbors_per_leaf = 2048 per leaf;
+# _interactions_impl.py:47-48), at most STREAM_START_MAX_BYTES, and grow
+# when a generation might not fit. "count": a counting walk, then an
+# emission walk at the exact totals (the reference's count pass,
+# _interactions_impl.py:3989-4299). Both give the same lists.
+STREAM_SIZING = os.environ.get("YGGDRAX_STREAM_SIZING", "grow")
+STREAM_START_MAX_BYTES = 16 << 30
+# Sizing of the last flat walk: how its streams were sized ("hint", "grow"
+# or "count"), their final capacities and the number of growths.
+LAST_WALK: dict = {}
 # Walk chunks split into this many contiguous ranges, one per XCD (MI355X
 # has 8; 0 or 1 = plain grid-stride order).
 WALK_XCD_GROUPS = int(os.environ.get("YGGDRAX_WALK_XCD_GROUPS", "0"))
@@ -273,6 +293,28 @@ class WalkOutcome:
         return max([1] + list(self.gen_sizes))
 
 
+def _start_capacities(num_nodes: int, num_leaves: int) -> tuple:
+    """Starting far and near stream capacities of a walk without a hint:
+    the reference's default per-row capacities summed over the rows, scaled
+    down together to at most STREAM_START_MAX_BYTES (8 bytes per entry)."""
+    far = _DEFAULT_MAX_INTERACTIONS * max(1, num_nodes)
+    near = _DEFAULT_MAX_NEIGHBORS * max(1, num_leaves)
+    limit = STREAM_START_MAX_BYTES // 8
+    if far + near > limit:
+        far, near = far * limit // (far + near), near * limit // (far + near)
+    return max(1, far), max(1, near)
+
+
+def _grown(t: Optional[torch.Tensor], cap: int, keep: int) -> Optional[torch.Tensor]:
+    """A stream of `cap` entries holding the first `keep` entries of `t`."""
+    if t is None:
+        return None
+    out = torch.empty(max(1, cap), dtype=t.dtype, device=t.device)
+    if keep:
+        out[:keep].copy_(t[:keep])
+    return out
+
+
 class _WalkContext:
     """Per (tree, geometry, MAC) state shared by all attempts of one call."""
 
@@ -312,11 +354,16 @@ class _WalkContext:
 
     def run_flat(self, *, emit: bool, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
                  gen_hint: int = 0, hist_gen: int = -1, node_hist=None, grid_blocks: int = 0,
-                 policy=None):
+                 policy=None, grow: bool = False):
         """One flat-emission walk over the whole tree (ygg_walk_flat).
 
         ``policy`` (a _lib.PolicyDesc) replaces the built-in accept / near
-        rules; the emission walk then also writes a tag per far entry."""
+        rules; the emission walk then also writes a tag per far entry.
+        ``grow`` makes the emission walk resumable: a generation that might
+        not fit the far or near stream (two entries per pair) pauses the
+        walk before it runs, the streams are reallocated larger with their
+        entries copied over, and the walk continues from that generation,
+        so the capacities are starting sizes and never overflow."""
         dt = self.dt
         dev = self.dev
         roots = getattr(self, "roots", None)
@@ -359,11 +406,41 @@ class _WalkContext:
             p.far_tags = _lib.ptr(ftag)
         if roots is not None:
             p.roots, p.num_roots = roots.data_ptr(), int(roots.numel())
+        G = MAX_GENERATIONS
+        if emit and grow:
+            gen_fill = torch.empty(G * GEN_RECORD_WORDS, dtype=torch.int64, device=dev)
+            p.gen_fill = gen_fill.data_ptr()
         ngen = ctypes.c_int32(0)
         gen_ms = (ctypes.c_float * MAX_GENERATIONS)() if self.timing else None
-        _lib.check(self.lib.ygg_walk_flat(ctypes.byref(p), self.host_status.data_ptr(),
-                                          ctypes.byref(ngen), gen_ms, self.stream), "walk_flat")
-        G = MAX_GENERATIONS
+        growths = 0
+        while True:
+            _lib.check(self.lib.ygg_walk_flat(ctypes.byref(p), self.host_status.data_ptr(),
+                                              ctypes.byref(ngen), gen_ms, self.stream),
+                       "walk_flat")
+            paused = int(self.host_status[G + 4]) if emit and grow else 0
+            if not paused:
+                break
+            # generation g did not run: grow the streams to hold its worst
+            # case after the entries written so far, and continue from it
+            g = paused - 1
+            filled = counters.tolist()
+            size = min(int(self.host_status[g]) & 0xFFFFFFFF, Q)
+            need_f, need_n = filled[0] + 2 * size, filled[1] + 2 * size
+            if need_f <= p.far_cap and need_n <= p.near_cap:
+                raise RuntimeError(f"walk paused at generation {g} with room in both streams")
+            if need_f > p.far_cap:
+                cap = max(need_f, 3 * p.far_cap // 2)
+                fk, fv, ftag = (_grown(t, cap, filled[0]) for t in (fk, fv, ftag))
+                p.far_cap = cap
+                p.far_keys, p.far_vals, p.far_tags = _lib.ptr(fk), _lib.ptr(fv), _lib.ptr(ftag)
+            if need_n > p.near_cap:
+                cap = max(need_n, 3 * p.near_cap // 2)
+                nk, nv = (_grown(t, cap, filled[1]) for t in (nk, nv))
+                p.near_cap = cap
+                p.near_keys, p.near_vals = _lib.ptr(nk), _lib.ptr(nv)
+            p.start_generation = g
+            growths += 1
+            FLAT_STATS["stream_growths"] += 1
         hs = self.host_status[:G + 4].tolist()  # pinned words the walk wrote before returning
         raw_gens = [v & 0xFFFFFFFF for v in hs[:min(ngen.value + 2, G + 1)]]
         c = counters.tolist()
@@ -380,7 +457,9 @@ class _WalkContext:
                     gen_sizes=raw_gens, peak=max(raw_gens + [1]),
                     queue_overflow=bool(hs[G + 1]), far_overflow=bool(hs[G + 2]),
                     near_overflow=bool(hs[G + 3]), far_keys=fk, far_vals=fv,
-                    near_keys=nk, near_vals=nv, far_tags=ftag, queue_capacity=Q)
+                    near_keys=nk, near_vals=nv, far_tags=ftag, queue_capacity=Q,
+                    far_capacity=int(p.far_cap), near_capacity=int(p.near_cap),
+                    stream_growths=growths)
 
     def run(self, *, mode: int, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
             far_row_start=None, far_row_cap=None, near_row_start=None, near_row_cap=None,
@@ -748,8 +827,12 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                    sizes: Optional[dict] = None, strict_int32: bool = True,
                    need_max: bool = True, fused_lists: bool = False) -> Optional[_FlatResult]:
     """Full walk with flat emission. `sizes` (queue, far, near capacities)
-    skips the counting walk; on any stream overflow the counting walk runs
-    and the emission walk is repeated at exact sizes. The largest far and
+    are the starting sizes of one resumable emission walk: far and near
+    streams that turn out too small grow during the walk, and a pair queue
+    that overflows reruns the walk without the hint. Without `sizes` the
+    walk follows STREAM_SIZING: one resumable walk from the default
+    capacities, or a counting walk and an emission walk at the exact
+    totals (also the path after a queue overflow). The largest far and
     near row counts (one host read after the sorts) are taken when
     `need_max`, or when the int32 guard or the retry-ladder emulation needs
     them; otherwise stats hold None for them. With `fused_lists` the row
@@ -769,14 +852,26 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     ctx.pairs_per_thread = autotune.walk_tile(T, with_tile, ctx.pairs_per_thread)
     em = None
     t_stage = time.perf_counter()
-    c_walks = FLAT_STATS["count_walks"]
+    hinted = False
+    c0 = FLAT_STATS["count_walks"]
     if sizes is not None:
         FLAT_STATS["hinted_walks"] += 1
         em = ctx.run_flat(emit=True, queue_cap=sizes["queue"], far_cap=sizes["far"],
-                          near_cap=sizes["near"])
-        if em["queue_overflow"] or em["far_overflow"] or em["near_overflow"]:
+                          near_cap=sizes["near"], grow=True)
+        if em["queue_overflow"]:
             FLAT_STATS["hint_overflows"] += 1
             em = None
+        else:
+            hinted = True
+            if em["stream_growths"]:
+                FLAT_STATS["hint_growths"] += 1
+    if em is None:
+        FLAT_STATS["plain_walks"] += 1
+        if STREAM_SIZING == "grow":
+            f0, n0 = _start_capacities(T, L)
+            em = ctx.run_flat(emit=True, queue_cap=qbig, far_cap=f0, near_cap=n0, grow=True)
+            if em["queue_overflow"]:
+                em = None  # the counting walk below reports it
     if em is None:
         FLAT_STATS["count_walks"] += 1
         tot = ctx.run_flat(emit=False, queue_cap=qbig)
@@ -798,9 +893,18 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                           near_cap=tot["near_total"])
         assert not (em["queue_overflow"] or em["far_overflow"] or em["near_overflow"])
     nf, nn = em["far_total"], em["near_total"]
+    LAST_WALK.clear()
+    LAST_WALK.update(sizing="hint" if hinted else ("count" if FLAT_STATS["count_walks"] > c0
+                                                   else "grow"),
+                     far_capacity=em["far_capacity"], near_capacity=em["near_capacity"],
+                     queue_capacity=em["queue_capacity"], stream_growths=em["stream_growths"],
+                     far_total=nf, near_total=nn)
+    if strict_int32 and lad.use_count and max(nf, nn) > (1 << 31) - 1:
+        raise RuntimeError("Dual-tree traversal pair totals exceed signed int32 capacity; "
+                           "refine the tree or use a lower-memory traversal strategy.")
     _traversal_diag(f"compact fill sizing queue={em['queue_capacity']} "
-                    f"total_far_pairs={nf} total_near_pairs={nn} "
-                    f"hinted={sizes is not None and FLAT_STATS['count_walks'] == c_walks}")
+                    f"total_far_pairs={nf} total_near_pairs={nn} hinted={hinted} "
+                    f"stream_growths={em.get('stream_growths', 0)}")
     inter = near = None
     if fused_lists:
         inter, near, fk, far_off, near_off = _fused_lists(em, dt)

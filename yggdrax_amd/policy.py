@@ -146,21 +146,31 @@ def _policy_walk(ctx: _WalkContext, dt, pair_policy, policy_state, qcap, fcap, n
 
 
 def _descriptor_walk(ctx: _WalkContext, dt, desc: "PairPolicyDescriptor"):
-    """A descriptor policy evaluated inside the flat walk's own kernels: a
-    counting walk sizes the streams exactly, then the emission walk writes
-    far entries with their tags and near entries in one pass per
-    generation."""
-    from .interactions import _auto_pair_queue_candidates
+    """A descriptor policy evaluated inside the flat walk's own kernels: the
+    emission walk writes far entries with their tags and near entries in one
+    pass per generation, into streams sized as for the plain walk (one
+    resumable walk that grows them, or a counting walk first)."""
+    from . import interactions as I
     T, NI = dt.num_nodes, dt.num_internal
     c = desc.to_c()
-    qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
-    tot = ctx.run_flat(emit=False, queue_cap=qbig, policy=c)
-    if tot["queue_overflow"]:
-        raise RuntimeError("Count-pass traversal wavefront overflowed its conservative buffer.")
-    em = ctx.run_flat(emit=True, queue_cap=tot["peak"], far_cap=tot["far_total"],
-                      near_cap=tot["near_total"], policy=c)
+    qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, I._auto_pair_queue_candidates(T, NI)[-1])
+    em = None
+    if I.STREAM_SIZING == "grow":
+        f0, n0 = I._start_capacities(T, dt.num_leaves)
+        em = ctx.run_flat(emit=True, queue_cap=qbig, far_cap=f0, near_cap=n0, policy=c,
+                          grow=True)
+        if em["queue_overflow"]:
+            raise RuntimeError("Count-pass traversal wavefront overflowed its conservative "
+                               "buffer.")
+    else:
+        tot = ctx.run_flat(emit=False, queue_cap=qbig, policy=c)
+        if tot["queue_overflow"]:
+            raise RuntimeError("Count-pass traversal wavefront overflowed its conservative "
+                               "buffer.")
+        em = ctx.run_flat(emit=True, queue_cap=tot["peak"], far_cap=tot["far_total"],
+                          near_cap=tot["near_total"], policy=c)
     if em["queue_overflow"] or em["far_overflow"] or em["near_overflow"]:
-        raise RuntimeError("pair-policy emission walk overflowed its counted streams")
+        raise RuntimeError("pair-policy emission walk overflowed its streams")
     return dict(far_total=em["far_total"], near_total=em["near_total"],
                 gen_sizes=em["gen_sizes"], peak=em["peak"], far_keys=em["far_keys"],
                 far_vals=em["far_vals"], far_tags=em["far_tags"], near_keys=em["near_keys"],
