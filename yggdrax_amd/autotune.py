@@ -158,14 +158,14 @@ def walk_plan(num_nodes: int, grid_default: int, tile_default: int) -> tuple:
 
 def walk_tile(num_nodes: int, run_with: Callable[[int], None], default: int) -> int:
     """Queue tile of the flat walk (pairs per thread of a 256-thread chunk);
-    `run_with(per)` runs one counting walk with that tile."""
+    `run_with(per)` runs one walk with that tile."""
     key = f"walk/tile/n{_size_class(num_nodes)}"
     return choose(key, WALK_TILES, lambda t: _time_ms(lambda: run_with(t), reps=1), default)
 
 
 def walk_grid_blocks(num_nodes: int, run_with: Callable[[int], None], default: int) -> int:
     """Grid size of the flat walk's generation kernel; `run_with(blocks)`
-    runs one counting walk with that grid (used only while tuning)."""
+    runs one walk with that grid (used only while tuning)."""
     key = f"walk/grid/n{_size_class(num_nodes)}"
     return choose(key, WALK_GRID_BLOCKS, lambda b: _time_ms(lambda: run_with(b), reps=1), default)
 

@@ -723,6 +723,19 @@ static int set_policy(const YggFlatWalkParams* p, FlatArgs* a) {
   return YGG_OK;
 }
 
+// Workgroups for generation k: generation k holds at most roots * 4^k pairs
+// (a pair has at most four children) and never more than the queue, so the
+// first generations launch only the workgroups their chunks can use (a
+// launch's cost grows with its workgroup count even when most exit at once).
+static int gen_blocks(int k, int64_t roots, int64_t queue_cap, int per, int blocks) {
+  const int64_t chunk = 256 * (per >= 16 ? 16 : (per >= 8 ? 8 : 4));
+  int64_t bound = roots;
+  for (int i = 0; i < k && bound < queue_cap; ++i) bound *= 4;
+  if (bound > queue_cap) bound = queue_cap;
+  const int64_t nb = (bound + chunk - 1) / chunk;
+  return nb < blocks ? (int)(nb > 0 ? nb : 1) : blocks;
+}
+
 static hipEvent_t g_flat_events[2 * 1025];
 static int g_flat_events_ready = 0;
 
@@ -785,7 +798,8 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
     int end = g + batch < G ? g + batch : G;
     for (int k = g; k < end; ++k) {
       if (gen_ms) YGG_CHECK(hipEventRecord(g_flat_events[2 * k], st));
-      launch_flat(a, k, p->emit != 0, p->pairs_per_thread, blocks, st);
+      launch_flat(a, k, p->emit != 0, p->pairs_per_thread,
+                  gen_blocks(k, a.num_roots, a.queue_cap, p->pairs_per_thread, blocks), st);
       if (gen_ms) YGG_CHECK(hipEventRecord(g_flat_events[2 * k + 1], st));
     }
     YGG_LAUNCH_CHECK();
@@ -872,7 +886,9 @@ int ygg_walk_flat_fixed(const YggFlatWalkParams* p, void* stream) {
   hipLaunchKernelGGL(k_flat_init, dim3(grid_for(init_n, 256, 4096)), dim3(256), 0, st, a, p->root,
                      G);
   const int blocks = p->grid_blocks > 0 ? p->grid_blocks : 2048;
-  for (int k = 0; k < G; ++k) launch_flat(a, k, p->emit != 0, p->pairs_per_thread, blocks, st);
+  for (int k = 0; k < G; ++k)
+    launch_flat(a, k, p->emit != 0, p->pairs_per_thread,
+                gen_blocks(k, a.num_roots, a.queue_cap, p->pairs_per_thread, blocks), st);
   if (a.spread && !p->emit) hipLaunchKernelGGL(k_spread_sum, dim3(1), dim3(256), 0, st, a);
   if (p->emit) hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a, G);
   YGG_LAUNCH_CHECK();

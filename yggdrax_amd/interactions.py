@@ -840,13 +840,18 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
     qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
     from . import autotune
-    ctx.grid_blocks = autotune.walk_grid_blocks(
-        T, lambda b: ctx.run_flat(emit=False, queue_cap=qbig, grid_blocks=b), FLAT_GRID_BLOCKS)
+
+    def emission_walk(grid_blocks=0):
+        # the walk the plain path runs: resumable, from the default capacities
+        f0, n0 = _start_capacities(T, L)
+        ctx.run_flat(emit=True, queue_cap=qbig, far_cap=f0, near_cap=n0, grow=True,
+                     grid_blocks=grid_blocks)
+    ctx.grid_blocks = autotune.walk_grid_blocks(T, lambda b: emission_walk(b), FLAT_GRID_BLOCKS)
 
     def with_tile(per):
         old, ctx.pairs_per_thread = ctx.pairs_per_thread, per
         try:
-            ctx.run_flat(emit=False, queue_cap=qbig)
+            emission_walk()
         finally:
             ctx.pairs_per_thread = old
     ctx.pairs_per_thread = autotune.walk_tile(T, with_tile, ctx.pairs_per_thread)
@@ -892,6 +897,15 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
         em = ctx.run_flat(emit=True, queue_cap=tot["peak"], far_cap=tot["far_total"],
                           near_cap=tot["near_total"])
         assert not (em["queue_overflow"] or em["far_overflow"] or em["near_overflow"])
+    else:
+        # one walk gave the totals and the entries: it is the counting stage
+        # of the reference's timing and diagnostics, and the row sorts below
+        # are the fill
+        _traversal_diag(f"count pass far_total={em['far_total']} "
+                        f"near_total={em['near_total']} max_wf={em['peak']} "
+                        f"generations={len(em['gen_sizes'])}")
+        _record_stage("dual_split_shared_count", t_stage, ctx.dev)
+        t_stage = time.perf_counter()
     nf, nn = em["far_total"], em["near_total"]
     LAST_WALK.clear()
     LAST_WALK.update(sizing="hint" if hinted else ("count" if FLAT_STATS["count_walks"] > c0
