@@ -503,10 +503,18 @@ def build_interactions_and_neighbors_batched(trees: BatchedTrees, geom: BatchedG
     dt = ctx.dt
     lad = _ladder(dt, mac_type, max_interactions_per_node=None, max_neighbors_per_leaf=2048,
                   max_pair_queue=None, traversal_config=None)
-    fr = _run_walk_flat(ctx, dt, lad, sizes=capacity_hint, strict_int32=False, need_max=False)
-    out = _lists_from_streams(trees, _LayoutIndex(lay, geom.center.device), fr.far_keys,
-                              fr.far_vals, fr.near_vals, fr.far_off, fr.near_off,
-                              fr.stats["far_total"], fr.stats["near_total"], fr.stats)
+    li = _LayoutIndex(lay, geom.center.device)
+    stats = {}
+
+    def lists(em):
+        out, fk, far_off, near_off = _lists_from_streams(
+            trees, li, em["far_keys"], em["far_vals"], em["near_keys"], em["near_vals"],
+            em["far_total"], em["near_total"], stats)
+        return out, None, fk, far_off, near_off
+    fr = _run_walk_flat(ctx, dt, lad, sizes=capacity_hint, strict_int32=False, need_max=False,
+                        fused_lists=lists)
+    out = fr.inter
+    stats.update(fr.stats)
     out.set_bases()
     return out
 
@@ -523,62 +531,93 @@ class _LayoutIndex:
             torch.arange(S, device=dev), torch.from_numpy(np.diff(lay.node_off)).to(dev))
 
 
-def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_vals, near_vals,
-                        far_off, near_off, nf: int, nn: int, stats: dict) -> BatchedLists:
-    """Per-system far/near lists from the forest walk's key-sorted streams.
-    nf / nn may exceed the valid entry counts (capacity-sized streams whose
-    unused keys sort last): the kernels stop at the row offsets' totals."""
+def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_vals, near_keys,
+                        near_vals, nf: int, nn: int, stats: dict, far_count=None,
+                        near_count=None, concurrent: bool = True):
+    """Per-system far/near lists from the forest walk's emission streams.
+    The row-key sorts write the lists in their last pass (ygg_far_lists_sort
+    on a side stream, ygg_near_lists_sort on the current one) with
+    batch-global ids; the localize kernels then turn them into every
+    system's local ids and CSR offsets. `far_count` / `near_count` (int64
+    device scalars) hold the valid entry counts of streams allocated at
+    capacities nf / nn; without them all nf / nn entries are valid. With
+    `concurrent` False everything runs on the current stream (stream
+    capture).
+    Returns (BatchedLists, sorted far keys, far row offsets, near row
+    offsets)."""
+    from . import autotune
+    from .interactions import _side_stream
     lib = _lib.load()
     lay = trees.layout
     dev = far_keys.device
-    st = _lib.stream_ptr(dev)
     idx = INDEX_DTYPE
     i64 = int(idx == torch.int64)
     T, Lt, S = lay.num_nodes, lay.num_leaves, lay.num_systems
     nbl_global = (trees.rad["nodes_by_level"].long() + li.node_off[li.sys_of_node]).to(torch.int32)
-    sources = torch.empty(nf, dtype=idx, device=dev)
-    targets = torch.empty(nf, dtype=idx, device=dev)
-    levels = torch.empty(nf, dtype=idx, device=dev)
-    counts = torch.empty(T, dtype=idx, device=dev)
-    counts32 = torch.empty(T, dtype=torch.int32, device=dev)
-    node_off_glob = torch.empty(T + 1, dtype=idx, device=dev)
-    _lib.check(lib.ygg_far_lists_from_sorted(
-        T, far_keys.data_ptr(), far_vals.data_ptr(), nf, far_off.data_ptr(),
-        nbl_global.data_ptr(), trees.rad["node_level"].data_ptr(), i64, sources.data_ptr(),
-        targets.data_ptr(), levels.data_ptr(), counts.data_ptr(), node_off_glob.data_ptr(),
-        counts32.data_ptr(), st), "batch far lists")
-    far_offsets = torch.empty(T + S, dtype=idx, device=dev)
-    far_lvl = torch.empty((S, 65), dtype=idx, device=dev)
-    _lib.check(lib.ygg_batch_far_localize(
-        lay.ptr(), far_off.data_ptr(), nbl_global.data_ptr(),
-        trees.rad["level_offsets"].data_ptr(), nf, i64, sources.data_ptr(), targets.data_ptr(),
-        far_offsets.data_ptr(), far_lvl.data_ptr(), st), "batch far localize")
-    leaf_position = trees.octa["leaf_position"] if trees.octree else trees.rad["leaf_position"]
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev) if concurrent else main
+    if concurrent:
+        side.wait_stream(main)
+    with torch.cuda.stream(side):
+        fend = max(1, (T - 1).bit_length())
+        fbits = autotune.sort_digit_bits(4, fend, nf) if nf else 8
+        sources = torch.empty(nf, dtype=idx, device=dev)
+        targets = torch.empty(nf, dtype=idx, device=dev)
+        levels = torch.empty(nf, dtype=idx, device=dev)
+        counts = torch.empty(T, dtype=idx, device=dev)
+        counts32 = torch.empty(T, dtype=torch.int32, device=dev)
+        node_off_glob = torch.empty(T + 1, dtype=idx, device=dev)
+        far_off = torch.empty(T + 1, dtype=torch.int64, device=dev)
+        fsorted = torch.empty(max(1, nf), dtype=torch.int32, device=dev)
+        fws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(nf, fend, fbits), dev)
+        _lib.check(lib.ygg_far_lists_sort(
+            far_keys.data_ptr(), far_vals.data_ptr(), nf, _lib.ptr(far_count), T, fend, fbits,
+            nbl_global.data_ptr(), trees.rad["node_level"].data_ptr(), i64, fsorted.data_ptr(),
+            far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
+            counts.data_ptr(), node_off_glob.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
+            fws.numel(), _lib.stream_ptr(dev)), "batch far lists sort")
+        far_offsets = torch.empty(T + S, dtype=idx, device=dev)
+        far_lvl = torch.empty((S, 65), dtype=idx, device=dev)
+        _lib.check(lib.ygg_batch_far_localize(
+            lay.ptr(), far_off.data_ptr(), nbl_global.data_ptr(),
+            trees.rad["level_offsets"].data_ptr(), nf, i64, sources.data_ptr(),
+            targets.data_ptr(), far_offsets.data_ptr(), far_lvl.data_ptr(),
+            _lib.stream_ptr(dev)), "batch far localize")
+    nend = max(1, (Lt - 1).bit_length())
+    nbits = autotune.sort_digit_bits(4, nend, nn) if nn else 8
     neighbors = torch.empty(nn, dtype=idx, device=dev)
     ncounts = torch.empty(Lt, dtype=idx, device=dev)
     noff_glob = torch.empty(Lt + 1, dtype=idx, device=dev)
-    _lib.check(lib.ygg_near_lists_from_sorted(
-        Lt, near_off.data_ptr(), near_vals.data_ptr(), leaf_position.data_ptr(), 0, i64,
-        neighbors.data_ptr(), None, ncounts.data_ptr(), noff_glob.data_ptr(), st),
-        "batch near lists")
+    near_off = torch.empty(Lt + 1, dtype=torch.int64, device=dev)
+    nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
+    nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
+    st = _lib.stream_ptr(dev)
+    _lib.check(lib.ygg_near_lists_sort(
+        near_keys.data_ptr(), near_vals.data_ptr(), nn, _lib.ptr(near_count), Lt, nend, nbits,
+        i64, nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
+        noff_glob.data_ptr(), nws.data_ptr(), nws.numel(), st), "batch near lists sort")
     near_offsets = torch.empty(Lt + S, dtype=idx, device=dev)
     _lib.check(lib.ygg_batch_near_localize(lay.ptr(), near_off.data_ptr(), nn, i64,
                                            neighbors.data_ptr(), near_offsets.data_ptr(), st),
                "batch near localize")
+    if concurrent:
+        main.wait_stream(side)
+        for t in (sources, targets, levels, counts, counts32, node_off_glob, far_off, fsorted,
+                  fws, far_offsets, far_lvl, nbl_global):
+            t.record_stream(main)
     out = BatchedLists(lay, trees, sources, targets, levels, counts, far_offsets, far_lvl,
                        neighbors, ncounts, near_offsets, stats)
     out._base_idx = (far_off[li.node_off], near_off[li.leaf_off])
-    return out
+    return out, fsorted, far_off, near_off
 
 
 def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_type: str,
                 dehnen_radius_scale: float, caps: dict, generations: int, grid_blocks: int,
                 pairs_per_thread: int):
     """Forest walk with fixed stream capacities and generation count and no
-    host synchronisation (stream-capturable), followed by the row-key sorts
-    over the full capacities (unused stream slots hold key 0xFFFFFFFF and
-    sort last)."""
-    from .interactions import NODE_RECORD_BYTES, _sort_entries
+    host synchronisation (stream-capturable). Returns the raw emission
+    streams and the walk's device-side totals, which bound the row sorts."""
+    from .interactions import NODE_RECORD_BYTES
     lib = _lib.load()
     lay = trees.layout
     dev = trees.rad["parent"].device
@@ -602,9 +641,9 @@ def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_ty
     counters = torch.empty(2, dtype=torch.int64, device=dev)
     gen_sizes = torch.empty(G + 1, dtype=torch.int32, device=dev)
     flags = torch.empty(4, dtype=torch.int32, device=dev)
-    fk = torch.full((FC,), -1, dtype=torch.int32, device=dev)
+    fk = torch.empty(FC, dtype=torch.int32, device=dev)
     fv = torch.empty(FC, dtype=torch.int32, device=dev)
-    nk = torch.full((NC,), -1, dtype=torch.int32, device=dev)
+    nk = torch.empty(NC, dtype=torch.int32, device=dev)
     nv = torch.empty(NC, dtype=torch.int32, device=dev)
     p = _lib.FlatWalkParams()
     p.num_nodes, p.num_internal, p.num_leaves, p.root = T, max(1, lay.num_internal), Lt, 0
@@ -621,10 +660,8 @@ def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_ty
     p.xcd_groups = WALK_XCD_GROUPS
     p.pairs_per_thread = int(pairs_per_thread)
     _lib.check(lib.ygg_walk_flat_fixed(ctypes.byref(p), st), "graph walk")
-    fk2, fv2, far_off = _sort_entries(fk, fv, FC, T)
-    _, nv2, near_off = _sort_entries(nk, nv, NC, Lt)
-    return dict(far_keys=fk2, far_vals=fv2, near_vals=nv2, far_off=far_off, near_off=near_off,
-                counters=counters, gen_sizes=gen_sizes, flags=flags, nodes=nodes, wf=wf)
+    return dict(far_keys=fk, far_vals=fv, near_keys=nk, near_vals=nv, counters=counters,
+                gen_sizes=gen_sizes, flags=flags, nodes=nodes, wf=wf)
 
 
 class BatchedGraph:
@@ -672,8 +709,9 @@ class BatchedGraph:
         self.li = _LayoutIndex(self.lay, dev)
         # eager run (learned capacities when a model is given): exact stream
         # totals, generation count and a warm-up of every kernel. The graph's
-        # streams are sized from those totals times `slack`: the capacity-
-        # sized row sorts then do only `slack` times the necessary work.
+        # streams are sized from those totals times `slack`; the row sorts
+        # stop at the walk's device-side totals, so capacity costs memory,
+        # not sort work.
         was = autotune.enabled()
         if tune:
             autotune.enable(True)
@@ -694,11 +732,11 @@ class BatchedGraph:
                             else int(gens * slack) + 4)
         T, Lt = self.lay.num_nodes, self.lay.num_leaves
         if tune:
-            # the graph's row sorts run over the full stream capacities
+            # the graph looks its row-sort plans up by the stream capacities
             autotune.enable(True)
             try:
-                autotune.sort_digit_bits(4, max(1, T.bit_length()), self.caps["far"])
-                autotune.sort_digit_bits(4, max(1, Lt.bit_length()), self.caps["near"])
+                autotune.sort_digit_bits(4, max(1, (T - 1).bit_length()), self.caps["far"])
+                autotune.sort_digit_bits(4, max(1, (Lt - 1).bit_length()), self.caps["near"])
             finally:
                 autotune.enable(was)
         self.walk_grid, self.walk_tile = autotune.walk_plan(T, self.grid_blocks,
@@ -735,14 +773,16 @@ class BatchedGraph:
         geom = compute_tree_geometry_batched(bt)
         w = _graph_walk(bt, geom, self.theta, self.mac_type, self.scale, self.caps,
                         self.generations, self.walk_grid, self.walk_tile)
-        lists = _lists_from_streams(bt, self.li, w["far_keys"], w["far_vals"], w["near_vals"],
-                                    w["far_off"], w["near_off"], self.caps["far"],
-                                    self.caps["near"], {})
+        # the sorts and list kernels stop at the walk's device-side totals
+        lists, _, far_off, near_off = _lists_from_streams(
+            bt, self.li, w["far_keys"], w["far_vals"], w["near_keys"], w["near_vals"],
+            self.caps["far"], self.caps["near"], {}, far_count=w["counters"][0:1],
+            near_count=w["counters"][1:2], concurrent=False)
         lay = self.lay
         T, Lt = lay.num_nodes, lay.num_leaves
         status = torch.cat([
             w["flags"][:3].long(), w["gen_sizes"][self.generations:].long(), w["counters"],
-            w["far_off"][T:T + 1], w["near_off"][Lt:Lt + 1],
+            far_off[T:T + 1], near_off[Lt:Lt + 1],
             bt.rad["scalars"].long(), bt.octa["scalars"].long(),
             lists._base_idx[0], lists._base_idx[1]])
         return dict(trees=bt, geom=geom, lists=lists, walk=w, status=status)
@@ -762,7 +802,7 @@ class BatchedGraph:
         st = self.out["status"].cpu().numpy()
         flags, unfinished = st[:3], st[3]
         far_total, near_total = int(st[4]), int(st[5])     # stream totals of the walk
-        rows_far, rows_near = int(st[6]), int(st[7])       # entries with a valid row key
+        rows_far, rows_near = int(st[6]), int(st[7])       # entries the row sorts listed
         scal = st[8:8 + 8 * S].reshape(S, 8)
         osc = st[8 + 8 * S:8 + 12 * S].reshape(S, 4)
         fbase, nbase = st[8 + 12 * S:8 + 13 * S], st[8 + 13 * S:8 + 14 * S]

@@ -54,8 +54,10 @@ constexpr int passes_for(int range_bits, int bits) { return (range_bits + bits -
 template <typename K, int BITS>
 __global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, int64_t n,
                                                     int begin_bit, int end_bit, int passes,
-                                                    uint32_t* __restrict__ hist) {
+                                                    uint32_t* __restrict__ hist,
+                                                    const int64_t* __restrict__ n_dev) {
   constexpr int kBins = 1 << BITS;
+  if (n_dev && *n_dev < n) n = *n_dev > 0 ? *n_dev : 0;
   constexpr int kMaxPasses = passes_for(8 * (int)sizeof(K), BITS);
   __shared__ uint32_t h[kMaxPasses * kBins];
   for (int i = threadIdx.x; i < passes * kBins; i += blockDim.x) h[i] = 0;
@@ -160,7 +162,8 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys
                                                   Emit em, int64_t n,
                                                   int shift, const uint32_t* __restrict__ hist_scan,
                                                   uint32_t* __restrict__ lookback,
-                                                  uint32_t* __restrict__ tile_counter) {
+                                                  uint32_t* __restrict__ tile_counter,
+                                                  const int64_t* __restrict__ n_dev) {
   constexpr int kBins = 1 << BITS;
   constexpr unsigned kMask = kBins - 1;
   // thread t < kBins / D owns digits [t * D, t * D + D)
@@ -195,6 +198,11 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys
   __syncthreads();
   const int64_t tile = s_tile;
   const int64_t base = tile * kTile;
+  // A device-side count (n_dev) shortens the sort to its first *n_dev keys:
+  // tiles past it exit at once (tiles are claimed in order, and a tile looks
+  // back only at earlier ones).
+  if (n_dev && *n_dev < n) n = *n_dev > 0 ? *n_dev : 0;
+  if (base >= n) return;
   const int64_t wbase = base + (int64_t)w * kWaveKeys;
 
   K key[ITEMS];
@@ -381,7 +389,7 @@ template <typename K, int BITS, typename Last = EmitKV<K>>
 static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_out,
                            uint32_t* vals_out, int64_t n, int begin_bit, int end_bit, void* ws,
                            size_t ws_bytes, hipStream_t st, const uint32_t* hist_pre,
-                           const Last* last = nullptr) {
+                           const Last* last = nullptr, const int64_t* n_dev = nullptr) {
   constexpr int kBins = 1 << BITS;
   constexpr int kThreads = sort_threads<K, BITS>();
   constexpr int kItems = sort_items<K, BITS>();
@@ -404,7 +412,7 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
     rc = fill_u32(hist, 0u, (int64_t)passes * kBins, st);
     if (rc) return rc;
     hipLaunchKernelGGL((k_digit_hist<K, BITS>), dim3(grid_for(n, 256, 2048)), dim3(256), 0, st,
-                       keys_in, n, begin_bit, end_bit, passes, hist);
+                       keys_in, n, begin_bit, end_bit, passes, hist, n_dev);
     YGG_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_hist_scan<BITS>, dim3(passes), dim3(256), 0, st, hist);
@@ -418,12 +426,12 @@ static int sort_pairs_bits(const K* keys_in, const uint32_t* vals_in, K* keys_ou
     if (last && p == passes - 1) {
       hipLaunchKernelGGL((k_onesweep<K, BITS, kItems, kThreads, Last>), dim3((unsigned)tiles),
                          dim3(kThreads), 0, st, src_k, src_v, *last, n, begin_bit + BITS * p,
-                         hist + p * kBins, lookback + (int64_t)p * tiles * kBins, ctr + p);
+                         hist + p * kBins, lookback + (int64_t)p * tiles * kBins, ctr + p, n_dev);
     } else {
       hipLaunchKernelGGL((k_onesweep<K, BITS, kItems, kThreads, EmitKV<K>>), dim3((unsigned)tiles),
                          dim3(kThreads), 0, st, src_k, src_v, EmitKV<K>{dk, dv}, n,
                          begin_bit + BITS * p, hist + p * kBins,
-                         lookback + (int64_t)p * tiles * kBins, ctr + p);
+                         lookback + (int64_t)p * tiles * kBins, ctr + p, n_dev);
     }
     YGG_LAUNCH_CHECK();
     src_k = dk;
@@ -443,7 +451,7 @@ size_t sort_rows_fused_workspace_bytes(int64_t n, int end_bit, int digit_bits) {
 template <typename Last>
 static int sort_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                            int64_t n, int end_bit, int digit_bits, void* ws, size_t ws_bytes,
-                           hipStream_t st, const Last& last) {
+                           hipStream_t st, const Last& last, const int64_t* n_dev) {
   if (n <= 0) return YGG_OK;
   if (n > (int64_t)kCountMask || end_bit < 1 || end_bit > 32 || keys_in == keys_out)
     return YGG_ERR_ARG;
@@ -456,43 +464,46 @@ static int sort_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uin
   switch (digit_bits) {
     case 8:
       return sort_pairs_bits<uint32_t, 8, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
-                                                end_bit, sws, rest, st, nullptr, &last);
+                                                end_bit, sws, rest, st, nullptr, &last, n_dev);
     case 9:
       return sort_pairs_bits<uint32_t, 9, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
-                                                end_bit, sws, rest, st, nullptr, &last);
+                                                end_bit, sws, rest, st, nullptr, &last, n_dev);
     case 10:
       return sort_pairs_bits<uint32_t, 10, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
-                                                 end_bit, sws, rest, st, nullptr, &last);
+                                                 end_bit, sws, rest, st, nullptr, &last, n_dev);
     case 11:
       return sort_pairs_bits<uint32_t, 11, Last>(keys_in, vals_in, keys_out, vals_tmp, n, 0,
-                                                 end_bit, sws, rest, st, nullptr, &last);
+                                                 end_bit, sws, rest, st, nullptr, &last, n_dev);
     default:
       return YGG_ERR_ARG;
   }
 }
 
 int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                        int64_t n, int end_bit, int digit_bits, int out_i64, void* sources,
-                        void* ws, size_t ws_bytes, hipStream_t st) {
+                        int64_t n, const int64_t* n_dev, int end_bit, int digit_bits, int out_i64,
+                        void* sources, void* ws, size_t ws_bytes, hipStream_t st) {
   if (out_i64) {
     EmitFarList<int64_t> e{keys_out, (int64_t*)sources};
     return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st,
-                           e);
+                           e, n_dev);
   }
   EmitFarList<int32_t> e{keys_out, (int32_t*)sources};
-  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e);
+  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e,
+                         n_dev);
 }
 
 int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                         int64_t n, int end_bit, int digit_bits, int out_i64, void* neighbors,
-                         void* ws, size_t ws_bytes, hipStream_t st) {
+                         int64_t n, const int64_t* n_dev, int end_bit, int digit_bits,
+                         int out_i64, void* neighbors, void* ws, size_t ws_bytes,
+                         hipStream_t st) {
   if (out_i64) {
     EmitNearList<int64_t> e{keys_out, (int64_t*)neighbors};
     return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st,
-                           e);
+                           e, n_dev);
   }
   EmitNearList<int32_t> e{keys_out, (int32_t*)neighbors};
-  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e);
+  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e,
+                         n_dev);
 }
 
 template <typename K>

@@ -469,11 +469,13 @@ __global__ void k_flat_init(FlatArgs a, int root, int G) {
   }
 }
 
-// offsets[r] = lower_bound(keys, r) for r in [0, rows], keys sorted ascending.
+// offsets[r] = lower_bound(keys, r) for r in [0, rows], keys sorted ascending
+// (the first min(n, *n_dev) keys when a device-side count is given).
 __global__ void k_row_offsets(const uint32_t* __restrict__ keys, int64_t n, int rows,
-                              int64_t* __restrict__ off) {
+                              int64_t* __restrict__ off, const int64_t* __restrict__ n_dev) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r > rows) return;
+  if (n_dev && *n_dev < n) n = *n_dev > 0 ? *n_dev : 0;
   int64_t lo = 0, hi = n;
   while (lo < hi) {
     int64_t mid = (lo + hi) >> 1;
@@ -567,7 +569,9 @@ __global__ __launch_bounds__(256) void k_far_targets(const uint32_t* __restrict_
                                                      const int32_t* __restrict__ nbl,
                                                      const int32_t* __restrict__ node_level,
                                                      OutT* __restrict__ targets,
-                                                     OutT* __restrict__ levels) {
+                                                     OutT* __restrict__ levels,
+                                                     const int64_t* __restrict__ n_dev) {
+  if (n_dev && *n_dev < n) n = *n_dev > 0 ? *n_dev : 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t key = keys[i];
@@ -899,7 +903,7 @@ int ygg_row_offsets(const uint32_t* sorted_keys, int64_t n, int64_t rows, int64_
                     void* stream) {
   if (rows < 0 || n < 0) return YGG_ERR_ARG;
   hipLaunchKernelGGL(k_row_offsets, dim3(grid_for(rows + 1, 256)), dim3(256), 0,
-                     (hipStream_t)stream, sorted_keys, n, (int)rows, offsets);
+                     (hipStream_t)stream, sorted_keys, n, (int)rows, offsets, nullptr);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
@@ -936,31 +940,33 @@ size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits
   return ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits);
 }
 
-int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n, int64_t num_nodes,
-                       int end_bit, int digit_bits, const int32_t* nodes_by_level,
-                       const int32_t* node_level, int out_i64, uint32_t* sorted_keys,
-                       int64_t* lm_offsets, void* sources, void* targets, void* target_levels,
-                       void* counts, void* node_offsets, int32_t* counts32, void* ws,
-                       size_t ws_bytes, void* stream) {
+int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
+                       const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,
+                       const int32_t* nodes_by_level, const int32_t* node_level, int out_i64,
+                       uint32_t* sorted_keys, int64_t* lm_offsets, void* sources, void* targets,
+                       void* target_levels, void* counts, void* node_offsets, int32_t* counts32,
+                       void* ws, size_t ws_bytes, void* stream) {
   if (num_nodes < 1 || n < 0 || !lm_offsets || !sorted_keys) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int T = (int)num_nodes;
-  int rc = ygg::sort_far_rows_fused(keys, vals, sorted_keys, n, end_bit, digit_bits, out_i64,
-                                    sources, ws, ws_bytes, st);
+  int rc = ygg::sort_far_rows_fused(keys, vals, sorted_keys, n, count, end_bit, digit_bits,
+                                    out_i64, sources, ws, ws_bytes, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_row_offsets, dim3(grid_for((int64_t)T + 1, 256)), dim3(256), 0, st,
-                     sorted_keys, n, T, lm_offsets);
+                     sorted_keys, n, T, lm_offsets, count);
   dim3 gr(grid_for(T, 256)), ge(grid_for(n > 0 ? n : 1, 256, 16384));
   if (out_i64) {
     if (n > 0)
       hipLaunchKernelGGL(k_far_targets<int64_t>, ge, dim3(256), 0, st, sorted_keys, n, T,
-                         nodes_by_level, node_level, (int64_t*)targets, (int64_t*)target_levels);
+                         nodes_by_level, node_level, (int64_t*)targets, (int64_t*)target_levels,
+                         count);
     hipLaunchKernelGGL(k_far_rows<int64_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int64_t*)counts, (int64_t*)node_offsets, counts32);
   } else {
     if (n > 0)
       hipLaunchKernelGGL(k_far_targets<int32_t>, ge, dim3(256), 0, st, sorted_keys, n, T,
-                         nodes_by_level, node_level, (int32_t*)targets, (int32_t*)target_levels);
+                         nodes_by_level, node_level, (int32_t*)targets, (int32_t*)target_levels,
+                         count);
     hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int32_t*)counts, (int32_t*)node_offsets, counts32);
   }
@@ -972,18 +978,19 @@ size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bit
   return ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits);
 }
 
-int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n, int64_t num_leaves,
-                        int end_bit, int digit_bits, int out_i64, uint32_t* sorted_keys,
-                        int64_t* row_offsets, void* neighbors, void* counts, void* offsets,
-                        void* ws, size_t ws_bytes, void* stream) {
+int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
+                        const int64_t* count, int64_t num_leaves, int end_bit, int digit_bits,
+                        int out_i64, uint32_t* sorted_keys, int64_t* row_offsets,
+                        void* neighbors, void* counts, void* offsets, void* ws, size_t ws_bytes,
+                        void* stream) {
   if (num_leaves < 1 || n < 0 || !row_offsets || !sorted_keys) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int L = (int)num_leaves;
-  int rc = ygg::sort_near_rows_fused(keys, vals, sorted_keys, n, end_bit, digit_bits, out_i64,
-                                     neighbors, ws, ws_bytes, st);
+  int rc = ygg::sort_near_rows_fused(keys, vals, sorted_keys, n, count, end_bit, digit_bits,
+                                     out_i64, neighbors, ws, ws_bytes, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_row_offsets, dim3(grid_for((int64_t)L + 1, 256)), dim3(256), 0, st,
-                     sorted_keys, n, L, row_offsets);
+                     sorted_keys, n, L, row_offsets, count);
   dim3 gr(grid_for((int64_t)L + 1, 256));
   if (out_i64)
     hipLaunchKernelGGL(k_near_rows<int64_t>, gr, dim3(256), 0, st, L, row_offsets,

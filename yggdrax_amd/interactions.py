@@ -706,7 +706,7 @@ def _fused_lists(em: dict, dt: DeviceTopology):
         fsorted = torch.empty(max(1, nf), dtype=torch.int32, device=dev)
         fws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(nf, fend, fbits), dev)
         _lib.check(lib.ygg_far_lists_sort(
-            em["far_keys"].data_ptr(), em["far_vals"].data_ptr(), nf, T, fend, fbits,
+            em["far_keys"].data_ptr(), em["far_vals"].data_ptr(), nf, None, T, fend, fbits,
             dt.nodes_by_level.data_ptr(), dt.node_level.data_ptr(), i64, fsorted.data_ptr(),
             far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
             counts.data_ptr(), node_off.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
@@ -720,7 +720,7 @@ def _fused_lists(em: dict, dt: DeviceTopology):
     nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
     nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
     _lib.check(lib.ygg_near_lists_sort(
-        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, L, nend, nbits, i64,
+        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, None, L, nend, nbits, i64,
         nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
         noff.data_ptr(), nws.data_ptr(), nws.numel(), _lib.stream_ptr(dev)), "near lists sort")
     main.wait_stream(side)
@@ -836,7 +836,10 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     near row counts (one host read after the sorts) are taken when
     `need_max`, or when the int32 guard or the retry-ladder emulation needs
     them; otherwise stats hold None for them. With `fused_lists` the row
-    sorts write the list outputs themselves (result fields inter, near)."""
+    sorts write the list outputs themselves (result fields inter, near);
+    a callable `fused_lists(em)` builds them from the emission streams
+    instead and returns (inter, near, sorted far keys, far row offsets,
+    near row offsets)."""
     T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
     qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
     from . import autotune
@@ -920,7 +923,10 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                     f"total_far_pairs={nf} total_near_pairs={nn} hinted={hinted} "
                     f"stream_growths={em.get('stream_growths', 0)}")
     inter = near = None
-    if fused_lists:
+    if callable(fused_lists):  # the caller's own list layout (batched mode)
+        inter, near, fk, far_off, near_off = fused_lists(em)
+        fv = nv = None
+    elif fused_lists:
         inter, near, fk, far_off, near_off = _fused_lists(em, dt)
         fv = nv = None
     else:
