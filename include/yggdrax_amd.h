@@ -535,27 +535,30 @@ int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
                                void* stream);
 
 /* The row-key sorts fused with the list outputs: the last radix pass writes
- * the level-major far list (sources, targets, target levels) or the near
- * neighbours directly, instead of sorted values that a second kernel
- * converts (same outputs as ygg_sort_pairs_u32 + ygg_row_offsets +
- * ygg_far_lists_from_sorted / ygg_near_lists_from_sorted). Keys are row keys
- * over [0, end_bit) (keys >= rows are unused capacity and sort last);
- * sorted_keys[n] receives the sorted keys and lm_offsets[T + 1] /
- * row_offsets[L + 1] the int64 row offsets. keys and vals are not modified.
- * `count` (optional, device memory) holds the number of valid entries of
- * streams allocated at capacity n: only the first min(n, *count) are
- * sorted and listed, with no host read (for graph capture). */
+ * the far sources or the near neighbours directly and counts each row's
+ * entries from its runs of equal keys; row offsets (int64 lm_offsets[T + 1]
+ * / row_offsets[L + 1]) are the scan of those counts, and the far targets
+ * and target levels are filled row by row. Same outputs as
+ * ygg_sort_pairs_u32 + ygg_row_offsets + ygg_far_lists_from_sorted /
+ * ygg_near_lists_from_sorted. Keys are row keys below 2^end_bit, with rows
+ * (T or L) <= 2^end_bit; keys >= rows are unused capacity, sort last and
+ * are listed nowhere. key_scratch[n] is scratch for the passes before the
+ * last (its contents afterwards are unspecified). keys and vals are not
+ * modified. `count` (optional, device memory) holds the number of valid
+ * entries of streams allocated at capacity n: only the first
+ * min(n, *count) are sorted and listed, with no host read (for graph
+ * capture). */
 size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                        const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,
                        const int32_t* nodes_by_level, const int32_t* node_level, int out_i64,
-                       uint32_t* sorted_keys, int64_t* lm_offsets, void* sources, void* targets,
+                       uint32_t* key_scratch, int64_t* lm_offsets, void* sources, void* targets,
                        void* target_levels, void* counts, void* node_offsets, int32_t* counts32,
                        void* ws, size_t ws_bytes, void* stream);
 size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                         const int64_t* count, int64_t num_leaves, int end_bit, int digit_bits,
-                        int out_i64, uint32_t* sorted_keys, int64_t* row_offsets,
+                        int out_i64, uint32_t* key_scratch, int64_t* row_offsets,
                         void* neighbors, void* counts, void* offsets, void* ws, size_t ws_bytes,
                         void* stream);
 /* The dense -1 padded [L][max_count] neighbour leaf rows from a near list

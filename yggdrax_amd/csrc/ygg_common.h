@@ -282,16 +282,19 @@ int sort_pairs_u32(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* k
                    size_t ws_bytes, hipStream_t st, int digit_bits = 0);
 
 size_t scan_workspace_bytes(int64_t n);
-// Row-key sorts whose last pass writes the far sources / near neighbours;
-// n_dev (optional, device) shortens them to the first *n_dev entries.
+// Row-key sorts whose last pass writes the far sources / near neighbours
+// and adds the per-row entry counts into row_counts[rows] (zero before the
+// call); key_scratch[n] is scratch for the passes before the last. n_dev
+// (optional, device) shortens them to the first *n_dev entries.
 size_t sort_rows_fused_workspace_bytes(int64_t n, int end_bit, int digit_bits);
-int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* key_scratch,
                         int64_t n, const int64_t* n_dev, int end_bit, int digit_bits, int out_i64,
-                        void* sources, void* ws, size_t ws_bytes, hipStream_t st);
-int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                         int64_t n, const int64_t* n_dev, int end_bit, int digit_bits,
-                         int out_i64, void* neighbors, void* ws, size_t ws_bytes,
-                         hipStream_t st);
+                        void* sources, uint32_t* row_counts, int64_t rows, void* ws,
+                        size_t ws_bytes, hipStream_t st);
+int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in,
+                         uint32_t* key_scratch, int64_t n, const int64_t* n_dev, int end_bit,
+                         int digit_bits, int out_i64, void* neighbors, uint32_t* row_counts,
+                         int64_t rows, void* ws, size_t ws_bytes, hipStream_t st);
 // Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:
 // out[n] = total). `gather`, if non-null, reads counts[gather[i]].
 int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int64_t* out,

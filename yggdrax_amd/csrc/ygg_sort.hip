@@ -126,35 +126,62 @@ __global__ __launch_bounds__(256) void k_hist_scan(uint32_t* hist) {
 // keys and values themselves.
 template <typename K>
 struct EmitKV {
+  static constexpr bool kRowCounts = false;
   K* __restrict__ keys_out;
   uint32_t* __restrict__ vals_out;
+  uint32_t* counts = nullptr;
+  uint32_t rows = 0;
   __device__ __forceinline__ void key(int64_t g, K k) const { keys_out[g] = k; }
   __device__ __forceinline__ void val(int64_t g, uint32_t v) const { vals_out[g] = v; }
 };
 
 // Last pass of the far row-key sort fused with the far list's sources
 // (_result_to_interactions, _interactions_impl.py:3561-3637): the key is the
-// target's level-major rank, the value the source node. Writes the sorted
-// key (targets, levels and row offsets follow from it in kernels with full
-// occupancy: dependent gathers here would stall the low-occupancy sort) and
-// the source.
+// target's level-major rank, the value the source node. Writes the source
+// and adds each tile's runs of equal keys into the per-row counts (row
+// offsets, targets and levels follow from the counts in kernels with full
+// occupancy: dependent gathers here would stall the low-occupancy sort);
+// the sorted keys themselves are not written.
 template <typename OutT>
 struct EmitFarList {
-  uint32_t* __restrict__ keys_out;
+  static constexpr bool kRowCounts = true;
   OutT* __restrict__ sources;
-  __device__ __forceinline__ void key(int64_t g, uint32_t k) const { keys_out[g] = k; }
+  uint32_t* __restrict__ counts;  // [rows], zero before the pass
+  uint32_t rows;
+  __device__ __forceinline__ void key(int64_t, uint32_t) const {}
   __device__ __forceinline__ void val(int64_t g, uint32_t v) const { sources[g] = (OutT)v; }
 };
 
 // Last pass of the near row-key sort fused with the near list: the key is
-// the native leaf row, the value the neighbour leaf node.
+// the native leaf row, the value the neighbour leaf node; row counts as for
+// the far list.
 template <typename OutT>
 struct EmitNearList {
-  uint32_t* __restrict__ keys_out;
+  static constexpr bool kRowCounts = true;
   OutT* __restrict__ neighbors;
-  __device__ __forceinline__ void key(int64_t g, uint32_t k) const { keys_out[g] = k; }
+  uint32_t* __restrict__ counts;
+  uint32_t rows;
+  __device__ __forceinline__ void key(int64_t, uint32_t) const {}
   __device__ __forceinline__ void val(int64_t g, uint32_t v) const { neighbors[g] = (OutT)v; }
 };
+
+// Row counts of a last pass: the tile's keys sit in s_buf fully sorted (the
+// pass orders by the top digit, stably, keys already ordered by the lower
+// ones), so each run of equal keys [f, l] adds l + 1 - f to its row with
+// two atomics: -f from its first element and l + 1 from its last (32-bit
+// wrap-around cancels). Keys at or above `rows` (unused capacity) count
+// nowhere.
+template <typename K, typename Emit>
+__device__ __forceinline__ void add_row_runs(const K* s_buf, int64_t valid, const Emit& em,
+                                             int tid, int threads) {
+  for (int p = tid; p < valid; p += threads) {
+    const uint32_t kk = (uint32_t)s_buf[p];
+    if (kk >= em.rows) continue;
+    if (p == 0 || (uint32_t)s_buf[p - 1] != kk) atomicSub(&em.counts[kk], (uint32_t)p);
+    if (p == valid - 1 || (uint32_t)s_buf[p + 1] != kk)
+      atomicAdd(&em.counts[kk], (uint32_t)(p + 1));
+  }
+}
 
 template <typename K, int BITS, int ITEMS, int THREADS, typename Emit>
 __global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys_in,
@@ -297,6 +324,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(const K* __restrict__ keys
   }
   __syncthreads();
   const int64_t valid = (n - base) < kTile ? (n - base) : kTile;
+  if constexpr (Emit::kRowCounts) add_row_runs(s_buf, valid, em, tid, THREADS);
   if (!kTwoPhase) {
     for (int p = tid; p < valid; p += THREADS) {
       const K kk = s_buf[p];
@@ -479,31 +507,32 @@ static int sort_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uin
   }
 }
 
-int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+int sort_far_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* key_scratch,
                         int64_t n, const int64_t* n_dev, int end_bit, int digit_bits, int out_i64,
-                        void* sources, void* ws, size_t ws_bytes, hipStream_t st) {
+                        void* sources, uint32_t* row_counts, int64_t rows, void* ws,
+                        size_t ws_bytes, hipStream_t st) {
   if (out_i64) {
-    EmitFarList<int64_t> e{keys_out, (int64_t*)sources};
-    return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st,
-                           e, n_dev);
+    EmitFarList<int64_t> e{(int64_t*)sources, row_counts, (uint32_t)rows};
+    return sort_rows_fused(keys_in, vals_in, key_scratch, n, end_bit, digit_bits, ws, ws_bytes,
+                           st, e, n_dev);
   }
-  EmitFarList<int32_t> e{keys_out, (int32_t*)sources};
-  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e,
-                         n_dev);
+  EmitFarList<int32_t> e{(int32_t*)sources, row_counts, (uint32_t)rows};
+  return sort_rows_fused(keys_in, vals_in, key_scratch, n, end_bit, digit_bits, ws, ws_bytes, st,
+                         e, n_dev);
 }
 
-int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                         int64_t n, const int64_t* n_dev, int end_bit, int digit_bits,
-                         int out_i64, void* neighbors, void* ws, size_t ws_bytes,
-                         hipStream_t st) {
+int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in,
+                         uint32_t* key_scratch, int64_t n, const int64_t* n_dev, int end_bit,
+                         int digit_bits, int out_i64, void* neighbors, uint32_t* row_counts,
+                         int64_t rows, void* ws, size_t ws_bytes, hipStream_t st) {
   if (out_i64) {
-    EmitNearList<int64_t> e{keys_out, (int64_t*)neighbors};
-    return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st,
-                           e, n_dev);
+    EmitNearList<int64_t> e{(int64_t*)neighbors, row_counts, (uint32_t)rows};
+    return sort_rows_fused(keys_in, vals_in, key_scratch, n, end_bit, digit_bits, ws, ws_bytes,
+                           st, e, n_dev);
   }
-  EmitNearList<int32_t> e{keys_out, (int32_t*)neighbors};
-  return sort_rows_fused(keys_in, vals_in, keys_out, n, end_bit, digit_bits, ws, ws_bytes, st, e,
-                         n_dev);
+  EmitNearList<int32_t> e{(int32_t*)neighbors, row_counts, (uint32_t)rows};
+  return sort_rows_fused(keys_in, vals_in, key_scratch, n, end_bit, digit_bits, ws, ws_bytes, st,
+                         e, n_dev);
 }
 
 template <typename K>

@@ -561,24 +561,28 @@ __global__ __launch_bounds__(256) void k_near_dense(int L, const int64_t* __rest
   }
 }
 
-// Targets and their levels of the level-major far list from its sorted row
-// keys (level-major ranks); keys >= rows are unused stream capacity.
+// Far targets and target levels row by row: one wavefront fills row r's
+// range [off[r], off[r + 1]) with its node and that node's level
+// (coalesced stores; no per-entry key read).
 template <typename OutT>
-__global__ __launch_bounds__(256) void k_far_targets(const uint32_t* __restrict__ keys,
-                                                     int64_t n, int rows,
-                                                     const int32_t* __restrict__ nbl,
-                                                     const int32_t* __restrict__ node_level,
-                                                     OutT* __restrict__ targets,
-                                                     OutT* __restrict__ levels,
-                                                     const int64_t* __restrict__ n_dev) {
-  if (n_dev && *n_dev < n) n = *n_dev > 0 ? *n_dev : 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t key = keys[i];
-    if (key >= (uint32_t)rows) continue;
-    const int32_t node = nbl[key];
-    targets[i] = (OutT)node;
-    levels[i] = (OutT)node_level[node];
+__global__ __launch_bounds__(256) void k_far_targets_rows(const int64_t* __restrict__ off,
+                                                          int rows,
+                                                          const int32_t* __restrict__ nbl,
+                                                          const int32_t* __restrict__ node_level,
+                                                          OutT* __restrict__ targets,
+                                                          OutT* __restrict__ levels) {
+  const unsigned lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / YGG_WAVE);
+  for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / YGG_WAVE) + threadIdx.x / YGG_WAVE;
+       r < rows; r += nw) {
+    const int64_t b = off[r], e = off[r + 1];
+    if (b >= e) continue;
+    const int32_t node = nbl[r];
+    const OutT t = (OutT)node, lv = (OutT)node_level[node];
+    for (int64_t i = b + lane; i < e; i += YGG_WAVE) {
+      targets[i] = t;
+      levels[i] = lv;
+    }
   }
 }
 
@@ -936,37 +940,63 @@ int ygg_far_lists_from_sorted(int64_t num_nodes, const uint32_t* keys, const uin
   return YGG_OK;
 }
 
+// Workspace of the fused row sorts: the sort's own, then the per-row
+// counts (2^end_bit rows at most) and their scan.
+static size_t lists_sort_ws(int64_t n, int end_bit, int digit_bits, int64_t* rows_max) {
+  const int64_t rmax = end_bit >= 1 && end_bit <= 31 ? ((int64_t)1 << end_bit) : 1;
+  if (rows_max) *rows_max = rmax;
+  return align_up(ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits)) +
+         align_up(sizeof(uint32_t) * (size_t)rmax) + ygg::scan_workspace_bytes(rmax);
+}
+
+// Splits the fused-sort workspace and zeroes the row counts.
+static int lists_sort_split(int64_t n, int end_bit, int digit_bits, int64_t rows, void* ws,
+                            size_t ws_bytes, hipStream_t st, size_t* sort_bytes,
+                            uint32_t** row_counts, void** scan_ws, size_t* scan_bytes) {
+  int64_t rmax = 0;
+  if (ws_bytes < lists_sort_ws(n, end_bit, digit_bits, &rmax)) return YGG_ERR_WORKSPACE;
+  if (rows > rmax) return YGG_ERR_ARG;  // row keys need end_bit bits
+  *sort_bytes = align_up(ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits));
+  *row_counts = (uint32_t*)((char*)ws + *sort_bytes);
+  *scan_ws = (char*)ws + *sort_bytes + align_up(sizeof(uint32_t) * (size_t)rmax);
+  *scan_bytes = ygg::scan_workspace_bytes(rmax);
+  return ygg::fill_u32(*row_counts, 0u, rows, st);
+}
+
 size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits) {
-  return ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits);
+  return lists_sort_ws(n, end_bit, digit_bits, nullptr);
 }
 
 int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                        const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,
                        const int32_t* nodes_by_level, const int32_t* node_level, int out_i64,
-                       uint32_t* sorted_keys, int64_t* lm_offsets, void* sources, void* targets,
+                       uint32_t* key_scratch, int64_t* lm_offsets, void* sources, void* targets,
                        void* target_levels, void* counts, void* node_offsets, int32_t* counts32,
                        void* ws, size_t ws_bytes, void* stream) {
-  if (num_nodes < 1 || n < 0 || !lm_offsets || !sorted_keys) return YGG_ERR_ARG;
+  if (num_nodes < 1 || n < 0 || !lm_offsets || !key_scratch || !ws) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int T = (int)num_nodes;
-  int rc = ygg::sort_far_rows_fused(keys, vals, sorted_keys, n, count, end_bit, digit_bits,
-                                    out_i64, sources, ws, ws_bytes, st);
+  size_t sort_bytes = 0, scan_bytes = 0;
+  uint32_t* row_counts = nullptr;
+  void* scan_ws = nullptr;
+  int rc = lists_sort_split(n, end_bit, digit_bits, T, ws, ws_bytes, st, &sort_bytes,
+                            &row_counts, &scan_ws, &scan_bytes);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_row_offsets, dim3(grid_for((int64_t)T + 1, 256)), dim3(256), 0, st,
-                     sorted_keys, n, T, lm_offsets, count);
-  dim3 gr(grid_for(T, 256)), ge(grid_for(n > 0 ? n : 1, 256, 16384));
+  rc = ygg::sort_far_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits, out_i64,
+                                sources, row_counts, T, ws, sort_bytes, st);
+  if (rc) return rc;
+  rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, lm_offsets, T,
+                                      scan_ws, scan_bytes, st);
+  if (rc) return rc;
+  dim3 gr(grid_for(T, 256)), gw(grid_for((int64_t)T, 256 / YGG_WAVE, 16384));
   if (out_i64) {
-    if (n > 0)
-      hipLaunchKernelGGL(k_far_targets<int64_t>, ge, dim3(256), 0, st, sorted_keys, n, T,
-                         nodes_by_level, node_level, (int64_t*)targets, (int64_t*)target_levels,
-                         count);
+    hipLaunchKernelGGL(k_far_targets_rows<int64_t>, gw, dim3(256), 0, st, lm_offsets, T,
+                       nodes_by_level, node_level, (int64_t*)targets, (int64_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int64_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int64_t*)counts, (int64_t*)node_offsets, counts32);
   } else {
-    if (n > 0)
-      hipLaunchKernelGGL(k_far_targets<int32_t>, ge, dim3(256), 0, st, sorted_keys, n, T,
-                         nodes_by_level, node_level, (int32_t*)targets, (int32_t*)target_levels,
-                         count);
+    hipLaunchKernelGGL(k_far_targets_rows<int32_t>, gw, dim3(256), 0, st, lm_offsets, T,
+                       nodes_by_level, node_level, (int32_t*)targets, (int32_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int32_t*)counts, (int32_t*)node_offsets, counts32);
   }
@@ -975,22 +1005,29 @@ int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
 }
 
 size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits) {
-  return ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits);
+  return lists_sort_ws(n, end_bit, digit_bits, nullptr);
 }
 
 int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                         const int64_t* count, int64_t num_leaves, int end_bit, int digit_bits,
-                        int out_i64, uint32_t* sorted_keys, int64_t* row_offsets,
+                        int out_i64, uint32_t* key_scratch, int64_t* row_offsets,
                         void* neighbors, void* counts, void* offsets, void* ws, size_t ws_bytes,
                         void* stream) {
-  if (num_leaves < 1 || n < 0 || !row_offsets || !sorted_keys) return YGG_ERR_ARG;
+  if (num_leaves < 1 || n < 0 || !row_offsets || !key_scratch || !ws) return YGG_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int L = (int)num_leaves;
-  int rc = ygg::sort_near_rows_fused(keys, vals, sorted_keys, n, count, end_bit, digit_bits,
-                                     out_i64, neighbors, ws, ws_bytes, st);
+  size_t sort_bytes = 0, scan_bytes = 0;
+  uint32_t* row_counts = nullptr;
+  void* scan_ws = nullptr;
+  int rc = lists_sort_split(n, end_bit, digit_bits, L, ws, ws_bytes, st, &sort_bytes,
+                            &row_counts, &scan_ws, &scan_bytes);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_row_offsets, dim3(grid_for((int64_t)L + 1, 256)), dim3(256), 0, st,
-                     sorted_keys, n, L, row_offsets, count);
+  rc = ygg::sort_near_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits, out_i64,
+                                 neighbors, row_counts, L, ws, sort_bytes, st);
+  if (rc) return rc;
+  rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, row_offsets, L,
+                                      scan_ws, scan_bytes, st);
+  if (rc) return rc;
   dim3 gr(grid_for((int64_t)L + 1, 256));
   if (out_i64)
     hipLaunchKernelGGL(k_near_rows<int64_t>, gr, dim3(256), 0, st, L, row_offsets,

@@ -666,7 +666,8 @@ def _run_walk_raw(tree, geometry: TreeGeometry, theta: float, mac_type: str, *,
 # ---- flat-emission path -----------------------------------------------------
 
 class _FlatResult(NamedTuple):
-    far_keys: torch.Tensor     # sorted level-major row keys (uint32 as int32)
+    far_keys: torch.Tensor     # sorted level-major row keys (uint32 as int32); the sort's
+                               # key scratch when the sorts wrote the lists themselves
     far_vals: Optional[torch.Tensor]
     near_vals: Optional[torch.Tensor]    # sorted by native leaf row
     far_off: torch.Tensor      # int64 [T + 1] level-major row offsets
