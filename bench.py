@@ -132,11 +132,20 @@ def walk_roofline(ygg, torch, pos, mass, leaf, theta):
 PAIR_BYTES, PUSH_BYTES, ENTRY_BYTES = 2 * 28 + 8, 8, 4
 
 
-# Compulsory bytes per point of sort + build + geometry with 8-bit radix
-# digits: bounds 12, encode 20, sort 8 passes x 24 + 8 = 200, reorder 40,
-# Karras + levels ~0.6 and octree metadata ~1 (per point at leaf 64),
-# geometry 13.
-BUILD_BYTES_PER_POINT = 287.0
+# Compulsory bytes per point of sort + build + geometry at radix digit width
+# d: the sort's P x (8 + 4) x 2 + 8 with P = ceil(63 / d) passes, plus 87 for
+# the rest (bounds 12, encode 20, reorder 40, Karras + levels ~0.6 and
+# octree metadata ~1 per point at leaf 64, geometry ~13; 287 in all at
+# d = 8). The Morton sort uses 9-bit digits (7 passes): 263 B/pt.
+MORTON_DIGIT_BITS = 9
+
+
+def build_bytes_per_point(digit_bits: int = MORTON_DIGIT_BITS) -> float:
+    passes = -(-63 // digit_bits)
+    return float(passes * (8 + 4) * 2 + 8 + 87)
+
+
+BUILD_BYTES_PER_POINT = build_bytes_per_point()
 
 
 def build_roofline(n, stages):
@@ -144,6 +153,7 @@ def build_roofline(n, stages):
     achieved = BUILD_BYTES_PER_POINT * n / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), bytes_per_point=BUILD_BYTES_PER_POINT,
+                morton_digit_bits=MORTON_DIGIT_BITS,
                 stages="build_octree + compute_tree_geometry", ms=round(ms, 4))
 
 
