@@ -450,8 +450,11 @@ typedef struct YggPolicyDesc {
  * size of every generation; the accept / near / refine decision counts
  * follow as far/2, near/2 and the remaining evaluated pairs). A stream that would exceed its capacity sets its
  * flag (queue, far, near) and the walk stops after that generation.
- * host_status and gen_ms (optional per-generation kernel times from HIP
- * events on `stream`) are as for ygg_walk_bfs. */
+ * host_status (pinned host memory of max_generations + 9 words) receives the
+ * generation sizes and the 4 flags as for ygg_walk_bfs, then the far and
+ * near totals as two 64-bit words (also left in counters[0..1]); gen_ms
+ * (optional per-generation kernel times from HIP events on `stream`) is as
+ * for ygg_walk_bfs. */
 #define YGG_WALK_GEN_RECORD_WORDS 16
 typedef struct YggFlatWalkParams {
   int32_t num_nodes, num_internal, num_leaves, root;

@@ -411,7 +411,7 @@ class _ForestContext(_WalkContext):
             leaf_position.data_ptr(), trees.rad["nodes_by_level"].data_ptr(), self.mac,
             ctypes.c_float(float(dehnen_radius_scale)).value, self.nodes.data_ptr(),
             self.roots.data_ptr(), self.stream), "batch walk nodes")
-        self.host_status = torch.zeros(MAX_GENERATIONS + 8, dtype=torch.int32, pin_memory=True)
+        self.host_status = torch.zeros(MAX_GENERATIONS + 16, dtype=torch.int32, pin_memory=True)
         self.gen_sizes = torch.empty(MAX_GENERATIONS + 1, dtype=torch.int32, device=self.dev)
         self.flags = torch.empty(4, dtype=torch.int32, device=self.dev)
         self.decisions = torch.empty(3, dtype=torch.int64, device=self.dev)

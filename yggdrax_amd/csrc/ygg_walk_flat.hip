@@ -837,11 +837,18 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
                   gen_blocks(k, a.num_roots, a.queue_cap, p->pairs_per_thread, blocks), st);
       if (gen_ms) YGG_CHECK(hipEventRecord(g_flat_events[2 * k + 1], st));
     }
+    // A resumable walk's totals are sums over its generation records, which
+    // the launched generations never modify afterwards: summing them after
+    // every batch lets the batch's one copy carry them to the host.
+    if (a.gen_fill) hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a, G);
     YGG_LAUNCH_CHECK();
     YGG_CHECK(hipMemcpyAsync(host_status, p->gen_sizes, sizeof(uint32_t) * (G + 1),
                              hipMemcpyDeviceToHost, st));
     YGG_CHECK(hipMemcpyAsync(host_status + G + 1, p->flags, sizeof(uint32_t) * 4,
                              hipMemcpyDeviceToHost, st));
+    if (a.gen_fill)
+      YGG_CHECK(hipMemcpyAsync(host_status + G + 5, a.counters, sizeof(uint64_t) * 2,
+                               hipMemcpyDeviceToHost, st));
     YGG_CHECK(hipStreamSynchronize(st));
     const uint32_t* fl = host_status + G + 1;
     int done = -1;
@@ -858,9 +865,14 @@ int ygg_walk_flat(const YggFlatWalkParams* p, uint32_t* host_status, int32_t* nu
         hipLaunchKernelGGL(k_spread_sum, dim3(1), dim3(256), 0, st, a);
         YGG_LAUNCH_CHECK();
       }
-      if (p->emit) {
+      if (p->emit && !a.gen_fill) {
         hipLaunchKernelGGL(k_unpack_totals, dim3(1), dim3(64), 0, st, a, G);
         YGG_LAUNCH_CHECK();
+      }
+      if (!a.gen_fill) {  // the totals to the host (already there for a resumable walk)
+        YGG_CHECK(hipMemcpyAsync(host_status + G + 5, a.counters, sizeof(uint64_t) * 2,
+                                 hipMemcpyDeviceToHost, st));
+        YGG_CHECK(hipStreamSynchronize(st));
       }
       if (gen_ms) {
         for (int k = 0; k < G; ++k) gen_ms[k] = 0.f;

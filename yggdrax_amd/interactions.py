@@ -305,6 +305,13 @@ def _start_capacities(num_nodes: int, num_leaves: int) -> tuple:
     return max(1, far), max(1, near)
 
 
+def _host_totals(hs: list, G: int) -> tuple:
+    """Far and near totals of a flat walk from its host status words
+    (two 64-bit words after the generation sizes and the four flags)."""
+    w = [v & 0xFFFFFFFF for v in hs[G + 5:G + 9]]
+    return w[0] | (w[1] << 32), w[2] | (w[3] << 32)
+
+
 def _grown(t: Optional[torch.Tensor], cap: int, keep: int) -> Optional[torch.Tensor]:
     """A stream of `cap` entries holding the first `keep` entries of `t`."""
     if t is None:
@@ -342,7 +349,7 @@ class _WalkContext:
             dt.leaf_position.data_ptr(), dt.nodes_by_level.data_ptr(), dt.num_nodes, self.mac,
             ctypes.c_float(float(dehnen_radius_scale)).value, self.nodes.data_ptr(),
             self.stream), "walk_prepare_nodes")
-        self.host_status = torch.zeros(MAX_GENERATIONS + 8, dtype=torch.int32, pin_memory=True)
+        self.host_status = torch.zeros(MAX_GENERATIONS + 16, dtype=torch.int32, pin_memory=True)
         self.gen_sizes = torch.empty(MAX_GENERATIONS + 1, dtype=torch.int32, device=self.dev)
         self.flags = torch.empty(4, dtype=torch.int32, device=self.dev)
         self.decisions = torch.empty(3, dtype=torch.int64, device=self.dev)
@@ -417,14 +424,15 @@ class _WalkContext:
             _lib.check(self.lib.ygg_walk_flat(ctypes.byref(p), self.host_status.data_ptr(),
                                               ctypes.byref(ngen), gen_ms, self.stream),
                        "walk_flat")
-            paused = int(self.host_status[G + 4]) if emit and grow else 0
+            hs = self.host_status[:G + 9].tolist()  # pinned words the walk wrote
+            paused = hs[G + 4] if emit and grow else 0
             if not paused:
                 break
             # generation g did not run: grow the streams to hold its worst
             # case after the entries written so far, and continue from it
             g = paused - 1
-            filled = counters.tolist()
-            size = min(int(self.host_status[g]) & 0xFFFFFFFF, Q)
+            filled = _host_totals(hs, G)
+            size = min(hs[g] & 0xFFFFFFFF, Q)
             need_f, need_n = filled[0] + 2 * size, filled[1] + 2 * size
             if need_f <= p.far_cap and need_n <= p.near_cap:
                 raise RuntimeError(f"walk paused at generation {g} with room in both streams")
@@ -441,9 +449,8 @@ class _WalkContext:
             p.start_generation = g
             growths += 1
             FLAT_STATS["stream_growths"] += 1
-        hs = self.host_status[:G + 4].tolist()  # pinned words the walk wrote before returning
         raw_gens = [v & 0xFFFFFFFF for v in hs[:min(ngen.value + 2, G + 1)]]
-        c = counters.tolist()
+        c = _host_totals(hs, G)
         # every evaluated pair is exactly one of accept (2 far entries),
         # near (2 near entries) or refine
         pairs = sum(min(v, Q) for v in raw_gens)
@@ -680,10 +687,11 @@ class _FlatResult(NamedTuple):
 
 def _fused_lists(em: dict, dt: DeviceTopology):
     """Far and near row-key sorts whose last pass writes the reference's
-    list layouts directly (ygg_far_lists_sort / ygg_near_lists_sort): the far
-    sort on a side stream, the near one on the current stream. Returns the
-    NodeInteractionList, the near (neighbors, counts, offsets) and both int64
-    row offset arrays."""
+    list layouts directly (ygg_far_lists_sort / ygg_near_lists_sort). The
+    near sort, the longer of the two, is launched first on the current
+    stream; the far sort follows on a side stream, ordered after the walk
+    only. Returns the NodeInteractionList, the near (neighbors, counts,
+    offsets) and both int64 row offset arrays."""
     from . import autotune
     lib = _lib.load()
     dev = dt.parent.device
@@ -692,8 +700,22 @@ def _fused_lists(em: dict, dt: DeviceTopology):
     T, L = dt.num_nodes, dt.num_leaves
     nf, nn = em["far_total"], em["near_total"]
     main = torch.cuda.current_stream(dev)
+    walked = torch.cuda.Event()
+    walked.record(main)
+    nend = max(1, (L - 1).bit_length())
+    nbits = autotune.sort_digit_bits(4, nend, nn) if nn else 8
+    neighbors = torch.empty(nn, dtype=idx, device=dev)
+    ncounts = torch.empty(L, dtype=idx, device=dev)
+    noff = torch.empty(L + 1, dtype=idx, device=dev)
+    near_off = torch.empty(L + 1, dtype=torch.int64, device=dev)
+    nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
+    nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
+    _lib.check(lib.ygg_near_lists_sort(
+        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, None, L, nend, nbits, i64,
+        nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
+        noff.data_ptr(), nws.data_ptr(), nws.numel(), _lib.stream_ptr(dev)), "near lists sort")
     side = _side_stream(dev)
-    side.wait_stream(main)
+    side.wait_event(walked)
     with torch.cuda.stream(side):
         fend = max(1, (T - 1).bit_length())
         fbits = autotune.sort_digit_bits(4, fend, nf) if nf else 8
@@ -712,18 +734,9 @@ def _fused_lists(em: dict, dt: DeviceTopology):
             far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
             counts.data_ptr(), node_off.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
             fws.numel(), _lib.stream_ptr(dev)), "far lists sort")
-    nend = max(1, (L - 1).bit_length())
-    nbits = autotune.sort_digit_bits(4, nend, nn) if nn else 8
-    neighbors = torch.empty(nn, dtype=idx, device=dev)
-    ncounts = torch.empty(L, dtype=idx, device=dev)
-    noff = torch.empty(L + 1, dtype=idx, device=dev)
-    near_off = torch.empty(L + 1, dtype=torch.int64, device=dev)
-    nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
-    nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
-    _lib.check(lib.ygg_near_lists_sort(
-        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, None, L, nend, nbits, i64,
-        nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
-        noff.data_ptr(), nws.data_ptr(), nws.numel(), _lib.stream_ptr(dev)), "near lists sort")
+    # the walk's streams are read on the side stream too
+    for t in (em["far_keys"], em["far_vals"]):
+        t.record_stream(side)
     main.wait_stream(side)
     for t in (sources, targets, levels, counts, counts32, node_off, far_off, fsorted, fws):
         t.record_stream(main)
