@@ -5,8 +5,8 @@
 // wavefront with this node as target, each divided by a per-output scale.
 // Trained on the Poisson negative log-likelihood, the rates are unbiased, so
 // their sums estimate the walk's stream sizes in one pass. A misprediction
-// only costs a fallback to counting: the walk detects every stream overflow
-// exactly.
+// only costs a stream growth during the walk (or, for the pair queue, a
+// rerun without the prediction): lists never depend on the model.
 //
 // Parameter layout (P = 9091 floats): W1[64][8] b1[64] W2[64][64] b2[64]
 // W3[64][64] b3[64] W4[3][64] b4[3]; W[o][i] multiplies input i into output o.
@@ -175,48 +175,132 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ 
   }
 }
 
-// ---- training: layer kernels over a batch (row-major activations).
-// y[b][o] = act(sum_i W[o][i] x[b][i] + bias[o])
-__global__ void k_lin_fwd(const uint16_t* __restrict__ W, const uint16_t* __restrict__ bias,
-                          const float* __restrict__ x, int B, int nin, int nout, int relu,
-                          float* __restrict__ y) {
-  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)B * nout) return;
-  int b = (int)(e / nout), o = (int)(e % nout);
-  float acc = bf16_to_f32(bias[o]);
-  for (int i = 0; i < nin; ++i) acc += bf16_to_f32(W[o * nin + i]) * x[(int64_t)b * nin + i];
-  y[e] = relu ? fmaxf(acc, 0.0f) : acc;
+// ---- training: layer kernels over a batch (row-major activations) -------
+// On the matrix cores with v_mfma_f32_16x16x4_f32 (fp32 operands, fp32
+// accumulation; the sums run in a different order than a scalar loop, the
+// products are the same). Operand layouts on wave64 (lane l):
+//   A[m][k]: lane l holds A[l % 16][l / 16]
+//   B[k][n]: lane l holds B[l / 16][l % 16]
+//   D[m][n]: lane l holds D[4 (l / 16) + j][l % 16], j = 0..3
+// (the D layout of the 16x16x16 bf16 form above). Operands past a matrix's
+// edge are zero, so any size works with whole 16 x 16 tiles.
+__device__ __forceinline__ f32x4_t mfma4(float a, float b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// dx[b][i] = sum_o W[o][i] dy[b][o], times relu'(x) when the input came from a ReLU.
-__global__ void k_lin_bwd_data(const uint16_t* __restrict__ W, const float* __restrict__ dy,
-                               const float* __restrict__ xin, int B, int nin, int nout,
-                               int relu_in, float* __restrict__ dx) {
-  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)B * nin) return;
-  int b = (int)(e / nin), i = (int)(e % nin);
-  float acc = 0.0f;
-  for (int o = 0; o < nout; ++o) acc += bf16_to_f32(W[o * nin + i]) * dy[(int64_t)b * nout + o];
-  if (relu_in && !(xin[e] > 0.0f)) acc = 0.0f;
-  dx[e] = acc;
-}
-
-// dW[o][i] = sum_b dy[b][o] x[b][i]; db[o] = sum_b dy[b][o] (one thread per weight).
-__global__ void k_lin_bwd_weight(const float* __restrict__ dy, const float* __restrict__ x,
-                                 int B, int nin, int nout, float* __restrict__ dW,
-                                 float* __restrict__ db) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < nout * nin) {
-    int o = e / nin, i = e % nin;
-    float acc = 0.0f;
-    for (int b = 0; b < B; ++b) acc += dy[(int64_t)b * nout + o] * x[(int64_t)b * nin + i];
-    dW[e] = acc;
-  } else if (e < nout * nin + nout) {
-    int o = e - nout * nin;
-    float acc = 0.0f;
-    for (int b = 0; b < B; ++b) acc += dy[(int64_t)b * nout + o];
-    db[o] = acc;
+// y[b][o] = act(sum_i W[o][i] x[b][i] + bias[o]): one wavefront per 16
+// batch rows, every 16-output tile (m = row, n = output, k = input).
+__global__ __launch_bounds__(256) void k_lin_fwd(const uint16_t* __restrict__ W,
+                                                 const uint16_t* __restrict__ bias,
+                                                 const float* __restrict__ x, int B, int nin,
+                                                 int nout, int relu, float* __restrict__ y) {
+  const int lane = (int)lane_id(), r = lane % 16, q = lane / 16;
+  const int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE * 16;
+  if (b0 >= B) return;
+  for (int n0 = 0; n0 < nout; n0 += 16) {
+    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < nin; k0 += 4) {
+      const int64_t b = b0 + r;
+      const int k = k0 + q, o = n0 + r;
+      const float av = (b < B && k < nin) ? x[b * nin + k] : 0.0f;
+      const float bv = (o < nout && k < nin) ? bf16_to_f32(W[o * nin + k]) : 0.0f;
+      acc = mfma4(av, bv, acc);
+    }
+    const int o = n0 + r;
+    if (o < nout) {  // every lane reaches the next tile's MFMAs
+      const float bo = bf16_to_f32(bias[o]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t b = b0 + 4 * q + j;
+        if (b < B) {
+          const float v = acc[j] + bo;
+          y[b * nout + o] = relu ? fmaxf(v, 0.0f) : v;
+        }
+      }
+    }
   }
+}
+
+// dx[b][i] = sum_o W[o][i] dy[b][o], times relu'(x) when the input came from
+// a ReLU (m = row, n = input, k = output).
+__global__ __launch_bounds__(256) void k_lin_bwd_data(const uint16_t* __restrict__ W,
+                                                      const float* __restrict__ dy,
+                                                      const float* __restrict__ xin, int B,
+                                                      int nin, int nout, int relu_in,
+                                                      float* __restrict__ dx) {
+  const int lane = (int)lane_id(), r = lane % 16, q = lane / 16;
+  const int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE * 16;
+  if (b0 >= B) return;
+  for (int n0 = 0; n0 < nin; n0 += 16) {
+    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < nout; k0 += 4) {
+      const int64_t b = b0 + r;
+      const int o = k0 + q, i = n0 + r;
+      const float av = (b < B && o < nout) ? dy[b * nout + o] : 0.0f;
+      const float bv = (o < nout && i < nin) ? bf16_to_f32(W[o * nin + i]) : 0.0f;
+      acc = mfma4(av, bv, acc);
+    }
+    const int i = n0 + r;
+    if (i < nin) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t b = b0 + 4 * q + j;
+        if (b < B) {
+          const int64_t e = b * nin + i;
+          dx[e] = (relu_in && !(xin[e] > 0.0f)) ? 0.0f : acc[j];
+        }
+      }
+    }
+  }
+}
+
+// Weight gradients, split over the batch: slice s of kGradSlice rows adds
+// into partial[s][m][n] = sum_b dy[b][m] xa[b][n] with xa = [x | 1] (the
+// column n = nin gives the bias gradient); m = output, n = input, k = row.
+// One wavefront per (16 x 16 tile, slice); Mp, Np are nout and nin + 1
+// rounded up to 16. The slices are summed in a fixed order afterwards, so
+// the gradients are deterministic.
+constexpr int kGradSlice = 256;
+
+__global__ __launch_bounds__(256) void k_lin_bwd_weight(const float* __restrict__ dy,
+                                                        const float* __restrict__ x, int B,
+                                                        int nin, int nout, int Mp, int Np,
+                                                        float* __restrict__ partial) {
+  const int lane = (int)lane_id(), r = lane % 16, q = lane / 16;
+  const int wave = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / YGG_WAVE);
+  const int tiles_n = Np / 16, tiles = (Mp / 16) * tiles_n;
+  const int slices = (B + kGradSlice - 1) / kGradSlice;
+  if (wave >= tiles * slices) return;
+  const int s = wave / tiles, t = wave % tiles;
+  const int m0 = (t / tiles_n) * 16, n0 = (t % tiles_n) * 16;
+  const int kb = s * kGradSlice, ke = min(B, kb + kGradSlice);
+  f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int b0 = kb; b0 < ke; b0 += 4) {
+    const int b = b0 + q;
+    const int o = m0 + r, i = n0 + r;
+    const float av = (b < ke && o < nout) ? dy[(int64_t)b * nout + o] : 0.0f;
+    const float bv = b < ke ? (i < nin ? x[(int64_t)b * nin + i] : (i == nin ? 1.0f : 0.0f))
+                            : 0.0f;
+    acc = mfma4(av, bv, acc);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    partial[((int64_t)s * Mp + m0 + 4 * q + j) * Np + n0 + r] = acc[j];
+}
+
+// dW[o][i] and db[o] from the slices' partial sums, in slice order.
+__global__ void k_lin_grad_reduce(const float* __restrict__ partial, int slices, int Mp, int Np,
+                                  int nin, int nout, float* __restrict__ dW,
+                                  float* __restrict__ db) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nout * (nin + 1)) return;
+  const int o = e / (nin + 1), i = e % (nin + 1);
+  float acc = 0.0f;
+  for (int s = 0; s < slices; ++s) acc += partial[((int64_t)s * Mp + o) * Np + i];
+  if (i < nin)
+    dW[o * nin + i] = acc;
+  else
+    db[o] = acc;
 }
 
 // Masked mean squared error: dy = 2 (y - t) mask / B, loss accumulated.
@@ -424,7 +508,8 @@ int ygg_mlp_forward(const uint16_t* params_bf16, const float* features, int64_t 
 // One training step on a batch: forward, masked MSE, backward, Adam.
 // ws: float workspace of ygg_mlp_train_workspace_floats(B) entries.
 int64_t ygg_mlp_train_workspace_floats(int64_t B) {
-  return (int64_t)B * (3 * kH + kOut) * 2 + kParams + 16;
+  const int64_t slices = (B + kGradSlice - 1) / kGradSlice;
+  return (int64_t)B * (3 * kH + kOut) * 2 + kParams + 16 + slices * kH * (kH + 16);
 }
 
 int ygg_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, float* adam_v,
@@ -442,37 +527,44 @@ int ygg_mlp_train_step(float* params, uint16_t* params_bf16, float* adam_m, floa
   float* d2 = d1 + (int64_t)B * kH;
   float* d3 = d2 + (int64_t)B * kH;
   float* dyo = d3 + (int64_t)B * kH;
+  float* partial = dyo + (int64_t)B * kOut;  // weight-gradient slices
   const uint16_t* W = params_bf16;
   auto g = [](int64_t n) { return dim3(grid_for(n, 256)); };
-  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW1, W + kOffB1, x,
-                     B, kIn, kH, 1, a1);
-  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW2, W + kOffB2, a1,
-                     B, kH, kH, 1, a2);
-  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW3, W + kOffB3, a2,
-                     B, kH, kH, 1, a3);
-  hipLaunchKernelGGL(k_lin_fwd, g((int64_t)B * kOut), dim3(256), 0, st, W + kOffW4, W + kOffB4,
-                     a3, B, kH, kOut, 0, yo);
+  // one wavefront per 16 batch rows
+  const dim3 gr(grid_for(((int64_t)B + 15) / 16 * YGG_WAVE, 256));
+  hipLaunchKernelGGL(k_lin_fwd, gr, dim3(256), 0, st, W + kOffW1, W + kOffB1, x, B, kIn, kH, 1,
+                     a1);
+  hipLaunchKernelGGL(k_lin_fwd, gr, dim3(256), 0, st, W + kOffW2, W + kOffB2, a1, B, kH, kH, 1,
+                     a2);
+  hipLaunchKernelGGL(k_lin_fwd, gr, dim3(256), 0, st, W + kOffW3, W + kOffB3, a2, B, kH, kH, 1,
+                     a3);
+  hipLaunchKernelGGL(k_lin_fwd, gr, dim3(256), 0, st, W + kOffW4, W + kOffB4, a3, B, kH, kOut, 0,
+                     yo);
   if (loss_out) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)loss_out, 1);
   hipLaunchKernelGGL(k_poisson, g((int64_t)B * kOut), dim3(256), 0, st, yo, target, mask, B,
                      dyo, loss_out);
+  const int slices = (B + kGradSlice - 1) / kGradSlice;
+  auto weight_grads = [&](const float* dy, const float* xin, int nin, int nout, float* dW,
+                          float* db) {
+    const int Mp = (nout + 15) / 16 * 16, Np = (nin + 1 + 15) / 16 * 16;
+    const int64_t waves = (int64_t)(Mp / 16) * (Np / 16) * slices;
+    hipLaunchKernelGGL(k_lin_bwd_weight, dim3(grid_for(waves * YGG_WAVE, 256)), dim3(256), 0, st,
+                       dy, xin, B, nin, nout, Mp, Np, partial);
+    hipLaunchKernelGGL(k_lin_grad_reduce, g((int64_t)nout * (nin + 1)), dim3(256), 0, st,
+                       partial, slices, Mp, Np, nin, nout, dW, db);
+  };
   // layer 4
-  hipLaunchKernelGGL(k_lin_bwd_weight, g(kOut * kH + kOut), dim3(256), 0, st, dyo, a3, B, kH,
-                     kOut, grads + kOffW4, grads + kOffB4);
-  hipLaunchKernelGGL(k_lin_bwd_data, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW4, dyo, a3,
-                     B, kH, kOut, 1, d3);
+  weight_grads(dyo, a3, kH, kOut, grads + kOffW4, grads + kOffB4);
+  hipLaunchKernelGGL(k_lin_bwd_data, gr, dim3(256), 0, st, W + kOffW4, dyo, a3, B, kH, kOut, 1,
+                     d3);
   // layer 3
-  hipLaunchKernelGGL(k_lin_bwd_weight, g(kH * kH + kH), dim3(256), 0, st, d3, a2, B, kH, kH,
-                     grads + kOffW3, grads + kOffB3);
-  hipLaunchKernelGGL(k_lin_bwd_data, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW3, d3, a2, B,
-                     kH, kH, 1, d2);
+  weight_grads(d3, a2, kH, kH, grads + kOffW3, grads + kOffB3);
+  hipLaunchKernelGGL(k_lin_bwd_data, gr, dim3(256), 0, st, W + kOffW3, d3, a2, B, kH, kH, 1, d2);
   // layer 2
-  hipLaunchKernelGGL(k_lin_bwd_weight, g(kH * kH + kH), dim3(256), 0, st, d2, a1, B, kH, kH,
-                     grads + kOffW2, grads + kOffB2);
-  hipLaunchKernelGGL(k_lin_bwd_data, g((int64_t)B * kH), dim3(256), 0, st, W + kOffW2, d2, a1, B,
-                     kH, kH, 1, d1);
+  weight_grads(d2, a1, kH, kH, grads + kOffW2, grads + kOffB2);
+  hipLaunchKernelGGL(k_lin_bwd_data, gr, dim3(256), 0, st, W + kOffW2, d2, a1, B, kH, kH, 1, d1);
   // layer 1
-  hipLaunchKernelGGL(k_lin_bwd_weight, g(kH * kIn + kH), dim3(256), 0, st, d1, x, B, kIn, kH,
-                     grads + kOffW1, grads + kOffB1);
+  weight_grads(d1, x, kIn, kH, grads + kOffW1, grads + kOffB1);
   YGG_LAUNCH_CHECK();
   if (lr > 0.0f) {
     const float b1 = 0.9f, b2 = 0.999f;
