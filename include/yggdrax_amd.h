@@ -537,20 +537,26 @@ int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
                                void* neighbor_leaf_positions, void* counts, void* offsets,
                                void* stream);
 
-/* The row-key sorts fused with the list outputs: the last radix pass writes
- * the far sources or the near neighbours directly and counts each row's
- * entries from its runs of equal keys; row offsets (int64 lm_offsets[T + 1]
- * / row_offsets[L + 1]) are the scan of those counts, and the far targets
- * and target levels are filled row by row. Same outputs as
- * ygg_sort_pairs_u32 + ygg_row_offsets + ygg_far_lists_from_sorted /
- * ygg_near_lists_from_sorted. Keys are row keys below 2^end_bit, with rows
- * (T or L) <= 2^end_bit; keys >= rows are unused capacity, sort last and
- * are listed nowhere. key_scratch[n] is scratch for the passes before the
- * last (its contents afterwards are unspecified). keys and vals are not
+/* The far and near lists grouped by row straight from the walk's streams:
+ * every row's values contiguous, in row order, each row's count from the
+ * grouping; row offsets (int64 lm_offsets[T + 1] / row_offsets[L + 1]) are
+ * the scan of those counts, and the far targets and target levels are
+ * filled row by row. Same outputs as ygg_sort_pairs_u32 + ygg_row_offsets +
+ * ygg_far_lists_from_sorted / ygg_near_lists_from_sorted, except the order
+ * inside a row, which is not specified (the walk emits in atomic order).
+ * digit_bits 0 with end_bit <= 22 groups by bucketing: a split by the row
+ * key's high digit, then per-bucket counts and a scatter by the low digit,
+ * ranked with LDS atomics. digit_bits 8..11 (or end_bit > 22) uses the
+ * stable radix sort with that digit width (0: its default), whose last pass
+ * writes the lists and counts runs of equal keys. Keys are row keys below
+ * 2^end_bit, with rows (T or L) <= 2^end_bit; keys >= rows are unused
+ * capacity and are listed nowhere. Values are node indices: below
+ * num_nodes for the far lists and below 2 * num_leaves for the near lists
+ * (the bucketing packs them with the low digit). key_scratch[n] is scratch
+ * (its contents afterwards are unspecified). keys and vals are not
  * modified. `count` (optional, device memory) holds the number of valid
  * entries of streams allocated at capacity n: only the first
- * min(n, *count) are sorted and listed, with no host read (for graph
- * capture). */
+ * min(n, *count) are listed, with no host read (for graph capture). */
 size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                        const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,

@@ -41,8 +41,13 @@ def _lists_equal(a, b):
                               np.sort(rows << 32 | parity.np_(x2).astype(np.int64)))
 
 
-def test_tuned_plans_are_the_argmin_and_keep_the_lists(tuner):
+@pytest.mark.parametrize("row_lists", ["bucket", "sort"])
+def test_tuned_plans_are_the_argmin_and_keep_the_lists(tuner, monkeypatch, row_lists):
+    """The walk's grid and tile are tuned in both row-list modes; the sort's
+    digit width only where the lists come from the stable sort."""
     import yggdrax_amd as ygg
+    from yggdrax_amd import interactions
+    monkeypatch.setattr(interactions, "ROW_LISTS", row_lists)
     pos, mass = inputs.clustered(1 << 18, 21)
     tree, ps, _, _ = ygg.build_octree(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
                                       leaf_size=32, return_reordered=True)
@@ -56,8 +61,9 @@ def test_tuned_plans_are_the_argmin_and_keep_the_lists(tuner):
     torch.cuda.synchronize()
     plans, times = tuner.plans(), tuner.timings()
     print("[autotune] plans", plans, flush=True)
-    assert {k.split("/")[0] + "/" + k.split("/")[1] for k in plans} >= {"walk/grid", "walk/tile",
-                                                                          "sort/k4"}
+    kinds = {k.split("/")[0] + "/" + k.split("/")[1] for k in plans}
+    assert kinds >= {"walk/grid", "walk/tile"}
+    assert ("sort/k4" in kinds) == (row_lists == "sort")
     for key, plan in plans.items():
         assert key in times and len(times[key]) >= 3
         assert plan == min(times[key], key=times[key].get)

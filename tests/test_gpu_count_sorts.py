@@ -70,9 +70,9 @@ def test_fused_sorts_stop_at_the_device_count(lib, cap, valid):
     dev = torch.device("cuda")
     # valid keys are row keys; the tail holds garbage keys inside the row range
     fk = torch.from_numpy(rng.integers(0, T, cap).astype(np.int32)).to(dev)
-    fv = torch.from_numpy(rng.integers(0, 1 << 20, cap).astype(np.int32)).to(dev)
+    fv = torch.from_numpy(rng.integers(0, T, cap).astype(np.int32)).to(dev)
     nk = torch.from_numpy(rng.integers(0, L, cap).astype(np.int32)).to(dev)
-    nv = torch.from_numpy(rng.integers(0, 1 << 20, cap).astype(np.int32)).to(dev)
+    nv = torch.from_numpy(rng.integers(0, 2 * L, cap).astype(np.int32)).to(dev)
     nbl = torch.from_numpy(rng.permutation(T).astype(np.int32)).to(dev)
     lvl = torch.from_numpy(rng.integers(0, 30, T).astype(np.int32)).to(dev)
     count = torch.tensor([valid], dtype=torch.int64, device=dev)

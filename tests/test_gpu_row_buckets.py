@@ -1,0 +1,213 @@
+"""The bucketing row lists (ygg_far_lists_sort / ygg_near_lists_sort with
+digit_bits 0): entries grouped by row with no stable sort. Values are node
+indices (below 2 * rows for the near lists, below rows for the far lists),
+packed with the low digit where they fit. Against numpy:
+row counts and offsets exactly, each row's values as a multiset (the order
+inside a row is unspecified, as the walk's emission order is), and nothing
+written past the valid entries. Cases cover one bucket (row keys of at most
+11 bits), every split width up to 22 bits, keys at or above the row count
+(unused capacity, listed nowhere), a device-side entry count with garbage
+behind it, no entries, all entries in one row (one LDS counter takes every
+atomic) and one bucket holding most entries (many tiles per bucket). The
+far targets and levels equal the stable sort's."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from yggdrax_amd import _lib
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return _lib.load()
+
+
+def _near(lib, keys, vals, n, count, L, bits, out_i64=1):
+    from yggdrax_amd import _lib
+    dev = keys.device
+    nend = max(1, (L - 1).bit_length())
+    dt = torch.int64 if out_i64 else torch.int32
+    out = dict(neighbors=torch.full((max(1, n),), -7, dtype=dt, device=dev),
+               counts=torch.empty(L, dtype=dt, device=dev),
+               offsets=torch.empty(L + 1, dtype=dt, device=dev),
+               rows=torch.empty(L + 1, dtype=torch.int64, device=dev),
+               scratch=torch.empty(max(1, n), dtype=torch.int32, device=dev))
+    ws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(n, nend, bits), dev)
+    _lib.check(lib.ygg_near_lists_sort(
+        keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), L, nend, bits, out_i64,
+        out["scratch"].data_ptr(), out["rows"].data_ptr(), out["neighbors"].data_ptr(),
+        out["counts"].data_ptr(), out["offsets"].data_ptr(), ws.data_ptr(), ws.numel(),
+        _lib.stream_ptr(dev)), "near lists")
+    return out
+
+
+def _far(lib, keys, vals, n, count, T, nbl, lvl, bits):
+    from yggdrax_amd import _lib
+    dev = keys.device
+    fend = max(1, (T - 1).bit_length())
+    out = dict(sources=torch.full((max(1, n),), -7, dtype=torch.int64, device=dev),
+               targets=torch.full((max(1, n),), -7, dtype=torch.int64, device=dev),
+               levels=torch.full((max(1, n),), -7, dtype=torch.int64, device=dev),
+               counts=torch.empty(T, dtype=torch.int64, device=dev),
+               node_off=torch.empty(T + 1, dtype=torch.int64, device=dev),
+               counts32=torch.empty(T, dtype=torch.int32, device=dev),
+               lm=torch.empty(T + 1, dtype=torch.int64, device=dev),
+               scratch=torch.empty(max(1, n), dtype=torch.int32, device=dev))
+    ws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(n, fend, bits), dev)
+    _lib.check(lib.ygg_far_lists_sort(
+        keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), T, fend, bits, nbl.data_ptr(),
+        lvl.data_ptr(), 1, out["scratch"].data_ptr(), out["lm"].data_ptr(),
+        out["sources"].data_ptr(), out["targets"].data_ptr(), out["levels"].data_ptr(),
+        out["counts"].data_ptr(), out["node_off"].data_ptr(), out["counts32"].data_ptr(),
+        ws.data_ptr(), ws.numel(), _lib.stream_ptr(dev)), "far lists")
+    return out
+
+
+def _expected(keys: np.ndarray, vals: np.ndarray, rows: int):
+    """Row counts and the values ordered by (row, value) of keys < rows."""
+    keep = keys < rows
+    k, v = keys[keep].astype(np.int64), vals[keep].astype(np.int64)
+    counts = np.bincount(k, minlength=rows)
+    order = np.lexsort((v, k))
+    return counts, v[order]
+
+
+def _row_sorted(values: np.ndarray, counts: np.ndarray) -> np.ndarray:
+    """values with each row's segment sorted (rows laid out by counts)."""
+    rows = np.repeat(np.arange(len(counts)), counts)
+    return values[np.lexsort((values, rows))]
+
+
+def _keys(rng, kind: str, n: int, rows: int) -> np.ndarray:
+    if kind == "uniform":
+        return rng.integers(0, rows, n)
+    if kind == "one_row":
+        return np.full(n, rows // 3)
+    if kind == "skewed":  # most entries in one bucket, runs of equal rows
+        k = rng.integers(0, rows, n)
+        hot = rng.random(n) < 0.8
+        k[hot] = rng.integers(0, min(rows, 300), int(hot.sum()))
+        return np.sort(k[: n // 2]).tolist() + k[n // 2:].tolist()
+    if kind == "spill":  # 5 % unused capacity (keys >= rows)
+        k = rng.integers(0, rows, n)
+        k[rng.random(n) < 0.05] = rows + 7
+        return k
+    raise ValueError(kind)
+
+
+CASES = [  # (rows, n, key kind)
+    (1000, 200_000, "uniform"),          # one bucket, 10-bit keys
+    (2048, 100_000, "spill"),            # one bucket, 11-bit keys
+    (5000, 300_000, "uniform"),          # 13 bits: 6 + 7
+    (5000, 300_000, "one_row"),
+    (200_000, 1_000_000, "skewed"),      # 18 bits: 9 + 9
+    (300_000, 2_000_000, "spill"),       # 19 bits: 9 + 10
+    (1 << 20, 3_000_000, "uniform"),     # 20 bits: 10 + 10, values packed with the low digit
+    (1_500_000, 2_000_000, "spill"),     # 21 bits: 10 + 11, values too wide to pack
+    ((1 << 22) - 5, 2_500_000, "uniform"),  # 22 bits: 11 + 11, values too wide to pack
+    (70_000, 0, "uniform"),              # no entries
+]
+
+
+@pytest.mark.parametrize("rows,n,kind", CASES)
+def test_near_buckets_match_numpy(lib, rows, n, kind):
+    print(f"[row buckets] near rows={rows} n={n} {kind}", flush=True)
+    rng = np.random.default_rng(rows + n)
+    keys = np.asarray(_keys(rng, kind, n, rows), dtype=np.int64).astype(np.int32)
+    # near values are node indices below 2 * rows, down to the last one
+    vals = rng.integers(0, 2 * rows, n).astype(np.int32)
+    if n:
+        vals[n // 2] = 2 * rows - 1
+    dev = torch.device("cuda")
+    kt, vt = torch.from_numpy(keys).to(dev), torch.from_numpy(vals).to(dev)
+    for out_i64 in (1, 0):
+        got = _near(lib, kt, vt, n, None, rows, 0, out_i64)
+        torch.cuda.synchronize()
+        counts, vals_sorted = _expected(keys, vals, rows)
+        valid = int(counts.sum())
+        assert np.array_equal(got["counts"].cpu().numpy(), counts)
+        off = np.concatenate([[0], np.cumsum(counts)])
+        assert np.array_equal(got["offsets"].cpu().numpy(), off)
+        assert np.array_equal(got["rows"].cpu().numpy(), off)
+        nb = got["neighbors"].cpu().numpy().astype(np.int64)
+        assert np.array_equal(_row_sorted(nb[:valid], counts), vals_sorted)
+        assert bool((nb[valid:n] == -7).all()), "written past the valid entries"
+
+
+@pytest.mark.parametrize("rows,n,kind", [(5000, 300_000, "uniform"),
+                                         (300_000, 2_000_000, "skewed")])
+def test_far_buckets_match_the_stable_sort(lib, rows, n, kind):
+    print(f"[row buckets] far rows={rows} n={n} {kind}", flush=True)
+    rng = np.random.default_rng(rows * 3 + n)
+    keys = np.asarray(_keys(rng, kind, n, rows), dtype=np.int64).astype(np.int32)
+    vals = rng.integers(0, rows, n).astype(np.int32)  # source nodes
+    vals[0] = rows - 1
+    dev = torch.device("cuda")
+    kt, vt = torch.from_numpy(keys).to(dev), torch.from_numpy(vals).to(dev)
+    nbl = torch.from_numpy(rng.permutation(rows).astype(np.int32)).to(dev)
+    lvl = torch.from_numpy(rng.integers(0, 30, rows).astype(np.int32)).to(dev)
+    got = _far(lib, kt, vt, n, None, rows, nbl, lvl, 0)
+    ref = _far(lib, kt, vt, n, None, rows, nbl, lvl, 10)
+    torch.cuda.synchronize()
+    for k in ("counts", "node_off", "counts32", "lm", "targets", "levels"):
+        assert torch.equal(got[k], ref[k]), k
+    counts = np.bincount(keys.astype(np.int64), minlength=rows)
+    src = got["sources"].cpu().numpy()
+    assert np.array_equal(_row_sorted(src, counts), _row_sorted(ref["sources"].cpu().numpy(),
+                                                                counts))
+
+
+@pytest.mark.parametrize("cap,valid", [(300_000, 123_457), (1 << 20, (1 << 20) - 1),
+                                       (50_000, 0)])
+def test_buckets_stop_at_the_device_count(lib, cap, valid):
+    print(f"[row buckets] device count cap={cap} valid={valid}", flush=True)
+    rng = np.random.default_rng(cap + 3 * valid)
+    L = 3000 if cap < (1 << 20) else 150_000
+    dev = torch.device("cuda")
+    keys = rng.integers(0, L, cap).astype(np.int32)
+    vals = rng.integers(0, 2 * L, cap).astype(np.int32)
+    kt, vt = torch.from_numpy(keys).to(dev), torch.from_numpy(vals).to(dev)
+    count = torch.tensor([valid], dtype=torch.int64, device=dev)
+    got = _near(lib, kt, vt, cap, count, L, 0)
+    torch.cuda.synchronize()
+    counts, vals_sorted = _expected(keys[:valid], vals[:valid], L)
+    assert np.array_equal(got["counts"].cpu().numpy(), counts)
+    nb = got["neighbors"].cpu().numpy()
+    assert np.array_equal(_row_sorted(nb[:valid], counts), vals_sorted)
+    assert bool((nb[valid:] == -7).all()), "written past the device count"
+
+
+def test_buckets_repeat_with_a_smaller_input(lib):
+    """Two calls in a row on one workspace, the second smaller: the look-back
+    flags, tile counter and bucket tables are reset by each call."""
+    print("[row buckets] repeat", flush=True)
+    rng = np.random.default_rng(5)
+    L = 100_000
+    dev = torch.device("cuda")
+    from yggdrax_amd import _lib
+    big = 1_500_000
+    keys = rng.integers(0, L, big).astype(np.int32)
+    vals = rng.integers(0, 2 * L, big).astype(np.int32)
+    kt, vt = torch.from_numpy(keys).to(dev), torch.from_numpy(vals).to(dev)
+    nend = (L - 1).bit_length()
+    ws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(big, nend, 0), dev)
+    for n in (big, 400_000):
+        nb = torch.full((big,), -7, dtype=torch.int64, device=dev)
+        counts = torch.empty(L, dtype=torch.int64, device=dev)
+        offs = torch.empty(L + 1, dtype=torch.int64, device=dev)
+        rows = torch.empty(L + 1, dtype=torch.int64, device=dev)
+        scratch = torch.empty(big, dtype=torch.int32, device=dev)
+        _lib.check(lib.ygg_near_lists_sort(
+            kt.data_ptr(), vt.data_ptr(), n, None, L, nend, 0, 1, scratch.data_ptr(),
+            rows.data_ptr(), nb.data_ptr(), counts.data_ptr(), offs.data_ptr(), ws.data_ptr(),
+            ws.numel(), _lib.stream_ptr(dev)), "near lists")
+        torch.cuda.synchronize()
+        c, v = _expected(keys[:n], vals[:n], L)
+        assert np.array_equal(counts.cpu().numpy(), c)
+        assert np.array_equal(_row_sorted(nb.cpu().numpy()[:n], c), v)

@@ -8,7 +8,9 @@ Plans can be saved to and loaded from a JSON file (``YGGDRAX_AUTOTUNE_CACHE``).
 
 Tuned parameters:
   * radix digit width (8..11 bits) of the row-key sorts that turn the
-    walk's emission streams into CSR lists, and of the Morton-code sort;
+    walk's emission streams into CSR lists (when the lists come from the
+    stable sort; the bucketing path has no digit width to tune), and of
+    the Morton-code sort;
   * the flat walk's grid size (persistent blocks per generation launch:
     the analogue of the reference's process_block) and its queue tile
     (pairs per thread of one chunk, i.e. 1024-4096-pair chunks, each

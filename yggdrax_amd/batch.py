@@ -535,18 +535,17 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
                         near_vals, nf: int, nn: int, stats: dict, far_count=None,
                         near_count=None, concurrent: bool = True):
     """Per-system far/near lists from the forest walk's emission streams.
-    The row-key sorts write the lists in their last pass (ygg_far_lists_sort
-    on a side stream, ygg_near_lists_sort on the current one) with
-    batch-global ids; the localize kernels then turn them into every
+    ygg_far_lists_sort (on a side stream) and ygg_near_lists_sort (on the
+    current one) group the entries by row into the lists, with batch-global
+    ids; the localize kernels then turn them into every
     system's local ids and CSR offsets. `far_count` / `near_count` (int64
     device scalars) hold the valid entry counts of streams allocated at
     capacities nf / nn; without them all nf / nn entries are valid. With
     `concurrent` False everything runs on the current stream (stream
     capture).
-    Returns (BatchedLists, sorted far keys, far row offsets, near row
-    offsets)."""
-    from . import autotune
-    from .interactions import _side_stream
+    Returns (BatchedLists, the far lists' key scratch (contents
+    unspecified), far row offsets, near row offsets)."""
+    from .interactions import _side_stream, row_lists_digit_bits
     lib = _lib.load()
     lay = trees.layout
     dev = far_keys.device
@@ -560,7 +559,7 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
         side.wait_stream(main)
     with torch.cuda.stream(side):
         fend = max(1, (T - 1).bit_length())
-        fbits = autotune.sort_digit_bits(4, fend, nf) if nf else 8
+        fbits = row_lists_digit_bits(fend, nf)
         sources = torch.empty(nf, dtype=idx, device=dev)
         targets = torch.empty(nf, dtype=idx, device=dev)
         levels = torch.empty(nf, dtype=idx, device=dev)
@@ -584,7 +583,7 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
             targets.data_ptr(), far_offsets.data_ptr(), far_lvl.data_ptr(),
             _lib.stream_ptr(dev)), "batch far localize")
     nend = max(1, (Lt - 1).bit_length())
-    nbits = autotune.sort_digit_bits(4, nend, nn) if nn else 8
+    nbits = row_lists_digit_bits(nend, nn)
     neighbors = torch.empty(nn, dtype=idx, device=dev)
     ncounts = torch.empty(Lt, dtype=idx, device=dev)
     noff_glob = torch.empty(Lt + 1, dtype=idx, device=dev)
@@ -696,7 +695,7 @@ class BatchedGraph:
                  capacities: Optional[dict] = None, generations: Optional[int] = None,
                  tune: bool = False):
         from . import autotune
-        from .interactions import WALK_PAIRS_PER_THREAD
+        from .interactions import WALK_PAIRS_PER_THREAD, row_lists_digit_bits
         _lib.require_device(positions)
         dev = positions.device
         self.theta, self.mac_type, self.scale = float(theta), mac_type, float(dehnen_radius_scale)
@@ -735,8 +734,8 @@ class BatchedGraph:
             # the graph looks its row-sort plans up by the stream capacities
             autotune.enable(True)
             try:
-                autotune.sort_digit_bits(4, max(1, (T - 1).bit_length()), self.caps["far"])
-                autotune.sort_digit_bits(4, max(1, (Lt - 1).bit_length()), self.caps["near"])
+                row_lists_digit_bits(max(1, (T - 1).bit_length()), self.caps["far"])
+                row_lists_digit_bits(max(1, (Lt - 1).bit_length()), self.caps["near"])
             finally:
                 autotune.enable(was)
         self.walk_grid, self.walk_tile = autotune.walk_plan(T, self.grid_blocks,

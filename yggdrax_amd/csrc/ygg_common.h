@@ -295,6 +295,35 @@ int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in,
                          uint32_t* key_scratch, int64_t n, const int64_t* n_dev, int end_bit,
                          int digit_bits, int out_i64, void* neighbors, uint32_t* row_counts,
                          int64_t rows, void* ws, size_t ws_bytes, hipStream_t st);
+// Row lists by bucketing (ygg_rowlists.hip): entries grouped by row key
+// with no stable sort (order inside a row unspecified), for row keys of at
+// most 22 bits (row_bucket_split false beyond). row_bucket_count adds each
+// row's entries into row_counts[rows] (zero before the call) and fills
+// `state`; after the caller's scan of the counts into row_offsets,
+// row_bucket_scatter writes every value at its row's place. Keys >= rows
+// are dropped. vals_scratch[n] holds the split's values.
+struct RowBucketState {
+  int64_t n;  // entries (capacity)
+  const uint32_t* keys;
+  const uint16_t* lo16;
+  const uint32_t* vals;
+  bool packed;  // vals hold value << lo_bits | low digit
+  uint32_t rows;
+  int lo_bits;
+  int buckets;
+  unsigned grid;
+  const uint32_t* bstart;
+  const uint32_t* tstart;
+  const uint32_t* tile_bases;
+};
+bool row_bucket_split(int end_bit, int* hi_bits, int* lo_bits);
+size_t row_bucket_workspace_bytes(int64_t n, int end_bit);
+int row_bucket_count(const uint32_t* keys, const uint32_t* vals, int64_t n, const int64_t* n_dev,
+                     int end_bit, int64_t rows, int64_t val_bound, uint32_t* vals_scratch,
+                     uint32_t* row_counts, void* ws, size_t ws_bytes, hipStream_t st,
+                     RowBucketState* state);
+int row_bucket_scatter(const RowBucketState& s, const int64_t* row_offsets, int out_i64,
+                       void* out, hipStream_t st);
 // Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:
 // out[n] = total). `gather`, if non-null, reads counts[gather[i]].
 int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int64_t* out,

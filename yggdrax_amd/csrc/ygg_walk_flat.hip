@@ -979,23 +979,37 @@ int ygg_far_lists_from_sorted(int64_t num_nodes, const uint32_t* keys, const uin
   return YGG_OK;
 }
 
-// Workspace of the fused row sorts: the sort's own, then the per-row
-// counts (2^end_bit rows at most) and their scan.
+// The row lists are grouped by bucketing (ygg::row_bucket_*) when
+// digit_bits is 0 and the keys span at most 22 bits, else by the stable
+// radix sort with that digit width (0: its default).
+static bool lists_bucketed(int end_bit, int digit_bits) {
+  int hi = 0, lo = 0;
+  return digit_bits == 0 && ygg::row_bucket_split(end_bit, &hi, &lo);
+}
+
+static size_t lists_group_ws(int64_t n, int end_bit, int digit_bits) {
+  return lists_bucketed(end_bit, digit_bits)
+             ? ygg::row_bucket_workspace_bytes(n, end_bit)
+             : ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits);
+}
+
+// Workspace of the row lists: the grouping's own, then the per-row counts
+// (2^end_bit rows at most) and their scan.
 static size_t lists_sort_ws(int64_t n, int end_bit, int digit_bits, int64_t* rows_max) {
   const int64_t rmax = end_bit >= 1 && end_bit <= 31 ? ((int64_t)1 << end_bit) : 1;
   if (rows_max) *rows_max = rmax;
-  return align_up(ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits)) +
+  return align_up(lists_group_ws(n, end_bit, digit_bits)) +
          align_up(sizeof(uint32_t) * (size_t)rmax) + ygg::scan_workspace_bytes(rmax);
 }
 
-// Splits the fused-sort workspace and zeroes the row counts.
+// Splits the row-list workspace and zeroes the row counts.
 static int lists_sort_split(int64_t n, int end_bit, int digit_bits, int64_t rows, void* ws,
                             size_t ws_bytes, hipStream_t st, size_t* sort_bytes,
                             uint32_t** row_counts, void** scan_ws, size_t* scan_bytes) {
   int64_t rmax = 0;
   if (ws_bytes < lists_sort_ws(n, end_bit, digit_bits, &rmax)) return YGG_ERR_WORKSPACE;
   if (rows > rmax) return YGG_ERR_ARG;  // row keys need end_bit bits
-  *sort_bytes = align_up(ygg::sort_rows_fused_workspace_bytes(n, end_bit, digit_bits));
+  *sort_bytes = align_up(lists_group_ws(n, end_bit, digit_bits));
   *row_counts = (uint32_t*)((char*)ws + *sort_bytes);
   *scan_ws = (char*)ws + *sort_bytes + align_up(sizeof(uint32_t) * (size_t)rmax);
   *scan_bytes = ygg::scan_workspace_bytes(rmax);
@@ -1021,11 +1035,22 @@ int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
   int rc = lists_sort_split(n, end_bit, digit_bits, T, ws, ws_bytes, st, &sort_bytes,
                             &row_counts, &scan_ws, &scan_bytes);
   if (rc) return rc;
-  rc = ygg::sort_far_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits, out_i64,
-                                sources, row_counts, T, ws, sort_bytes, st);
-  if (rc) return rc;
-  rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, lm_offsets, T,
-                                      scan_ws, scan_bytes, st);
+  if (lists_bucketed(end_bit, digit_bits)) {
+    ygg::RowBucketState bs;
+    rc = ygg::row_bucket_count(keys, vals, n, count, end_bit, T, T, key_scratch, row_counts, ws,
+                               sort_bytes, st, &bs);
+    if (rc) return rc;
+    rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, lm_offsets, T,
+                                        scan_ws, scan_bytes, st);
+    if (rc) return rc;
+    rc = ygg::row_bucket_scatter(bs, lm_offsets, out_i64, sources, st);
+  } else {
+    rc = ygg::sort_far_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits, out_i64,
+                                  sources, row_counts, T, ws, sort_bytes, st);
+    if (rc) return rc;
+    rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, lm_offsets, T,
+                                        scan_ws, scan_bytes, st);
+  }
   if (rc) return rc;
   dim3 gr(grid_for(T, 256)), gw(grid_for((int64_t)T, 256 / YGG_WAVE, 16384));
   if (out_i64) {
@@ -1061,11 +1086,22 @@ int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
   int rc = lists_sort_split(n, end_bit, digit_bits, L, ws, ws_bytes, st, &sort_bytes,
                             &row_counts, &scan_ws, &scan_bytes);
   if (rc) return rc;
-  rc = ygg::sort_near_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits, out_i64,
-                                 neighbors, row_counts, L, ws, sort_bytes, st);
-  if (rc) return rc;
-  rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, row_offsets, L,
-                                      scan_ws, scan_bytes, st);
+  if (lists_bucketed(end_bit, digit_bits)) {
+    ygg::RowBucketState bs;
+    rc = ygg::row_bucket_count(keys, vals, n, count, end_bit, L, 2 * (int64_t)L, key_scratch,
+                               row_counts, ws, sort_bytes, st, &bs);
+    if (rc) return rc;
+    rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, row_offsets, L,
+                                        scan_ws, scan_bytes, st);
+    if (rc) return rc;
+    rc = ygg::row_bucket_scatter(bs, row_offsets, out_i64, neighbors, st);
+  } else {
+    rc = ygg::sort_near_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits,
+                                   out_i64, neighbors, row_counts, L, ws, sort_bytes, st);
+    if (rc) return rc;
+    rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, row_offsets, L,
+                                        scan_ws, scan_bytes, st);
+  }
   if (rc) return rc;
   dim3 gr(grid_for((int64_t)L + 1, 256));
   if (out_i64)

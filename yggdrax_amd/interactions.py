@@ -55,6 +55,14 @@ FLAT_STATS = {"count_walks": 0, "plain_walks": 0, "hinted_walks": 0, "hint_overf
 # _interactions_impl.py:3989-4299). Both give the same lists.
 STREAM_SIZING = os.environ.get("YGGDRAX_STREAM_SIZING", "grow")
 STREAM_START_MAX_BYTES = 16 << 30
+# How the far and near row lists are grouped by row. "bucket": a split by
+# the row key's high digit, then per-bucket counts and a scatter by the low
+# digit, ranked with LDS atomics (row keys of at most 22 bits). "sort": the
+# stable radix sort with the autotuned digit width, its last pass writing
+# the lists. The order inside a row is the walk's emission order in both,
+# which is not specified; the rows are the same.
+ROW_LISTS = os.environ.get("YGGDRAX_ROW_LISTS", "bucket")
+ROW_BUCKET_MAX_BITS = 22
 # Sizing of the last flat walk: how its streams were sized ("hint", "grow"
 # or "count"), their final capacities and the number of growths.
 LAST_WALK: dict = {}
@@ -685,14 +693,24 @@ class _FlatResult(NamedTuple):
     near: Optional[tuple] = None       # (neighbors, counts, offsets)
 
 
-def _fused_lists(em: dict, dt: DeviceTopology):
-    """Far and near row-key sorts whose last pass writes the reference's
-    list layouts directly (ygg_far_lists_sort / ygg_near_lists_sort). The
-    near sort, the longer of the two, is launched first on the current
-    stream; the far sort follows on a side stream, ordered after the walk
-    only. Returns the NodeInteractionList, the near (neighbors, counts,
-    offsets) and both int64 row offset arrays."""
+def row_lists_digit_bits(end_bit: int, n: int) -> int:
+    """digit_bits argument of ygg_far_lists_sort / ygg_near_lists_sort for
+    n entries with end_bit-bit row keys: 0 selects the bucketing kernels,
+    8..11 the stable sort with that digit width."""
+    if ROW_LISTS == "bucket" and end_bit <= ROW_BUCKET_MAX_BITS:
+        return 0
     from . import autotune
+    return autotune.sort_digit_bits(4, end_bit, n) if n else 8
+
+
+def _fused_lists(em: dict, dt: DeviceTopology):
+    """Far and near row lists in the reference's layouts, grouped by row
+    by ygg_far_lists_sort / ygg_near_lists_sort (bucketing kernels, or the
+    row-key sort whose last pass writes the lists). The near lists, the
+    longer of the two, are launched first on the current stream; the far
+    lists follow on a side stream, ordered after the walk only. Returns the
+    NodeInteractionList, the near (neighbors, counts, offsets) and both
+    int64 row offset arrays."""
     lib = _lib.load()
     dev = dt.parent.device
     idx = INDEX_DTYPE
@@ -703,7 +721,7 @@ def _fused_lists(em: dict, dt: DeviceTopology):
     walked = torch.cuda.Event()
     walked.record(main)
     nend = max(1, (L - 1).bit_length())
-    nbits = autotune.sort_digit_bits(4, nend, nn) if nn else 8
+    nbits = row_lists_digit_bits(nend, nn)
     neighbors = torch.empty(nn, dtype=idx, device=dev)
     ncounts = torch.empty(L, dtype=idx, device=dev)
     noff = torch.empty(L + 1, dtype=idx, device=dev)
@@ -718,7 +736,7 @@ def _fused_lists(em: dict, dt: DeviceTopology):
     side.wait_event(walked)
     with torch.cuda.stream(side):
         fend = max(1, (T - 1).bit_length())
-        fbits = autotune.sort_digit_bits(4, fend, nf) if nf else 8
+        fbits = row_lists_digit_bits(fend, nf)
         sources = torch.empty(nf, dtype=idx, device=dev)
         targets = torch.empty(nf, dtype=idx, device=dev)
         levels = torch.empty(nf, dtype=idx, device=dev)

@@ -268,7 +268,7 @@ class _Scratch:
         self.buf = torch.zeros(16, dtype=torch.int32, pin_memory=True)
 
 
-def _alloc_radix(L: int, device):
+def _alloc_radix(L: int, device, scalars: Optional[torch.Tensor] = None):
     T = 2 * L - 1
     NI = L - 1
     i32 = dict(dtype=torch.int32, device=device)
@@ -283,7 +283,7 @@ def _alloc_radix(L: int, device):
         level_offsets=torch.empty(65, **i32),
         nodes_by_level=torch.empty(T, **i32),
         leaf_codes=torch.empty(L, dtype=torch.int64, device=device),
-        scalars=torch.zeros(8, **i32),
+        scalars=torch.zeros(8, **i32) if scalars is None else scalars,
     )
 
 
@@ -295,7 +295,7 @@ def _ptrs(struct_cls, arrays: dict):
     return s
 
 
-def _alloc_octree(L: int, device):
+def _alloc_octree(L: int, device, scalars: Optional[torch.Tensor] = None):
     M = max(1, 2 * L - 1)
     T = 2 * L - 1
     i32 = dict(dtype=torch.int32, device=device)
@@ -309,7 +309,8 @@ def _alloc_octree(L: int, device):
         oct_nodes_by_level=torch.empty(M, **i32), oct_level_offsets=torch.empty(23, **i32),
         oct_leaf_mask=torch.empty(M, **u8), oct_leaf_nodes=torch.empty(M, **i32),
         radix_node_to_oct=torch.empty(T, **i32), leaf_indices=torch.empty(L, **i32),
-        leaf_position=torch.empty(T, **i32), scalars=torch.zeros(4, **i32),
+        leaf_position=torch.empty(T, **i32),
+        scalars=torch.zeros(4, **i32) if scalars is None else scalars,
     )
 
 
@@ -357,13 +358,15 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
     L = (n + leaf_size - 1) // leaf_size
     T = 2 * L - 1
     NI = L - 1
-    rad = _alloc_radix(L, dev)
+    # both builds' scalar words in one buffer: one host copy reads them
+    scalars = torch.zeros(12, dtype=torch.int32, device=dev)
+    rad = _alloc_radix(L, dev, scalars[:8])
     rptr = _ptrs(_lib.RadixTopologyPtrs, rad)
     rws = _lib.workspace(lib.ygg_radix_build_workspace_bytes(n, leaf_size), dev)
     _lib.check(lib.ygg_radix_build(sorted_codes.data_ptr(), n, leaf_size, ctypes.byref(rptr),
                                    rws.data_ptr(), rws.numel(), st), "radix_build")
     if octree:
-        octa = _alloc_octree(L, dev)
+        octa = _alloc_octree(L, dev, scalars[8:])
         optr = _ptrs(_lib.OctreeMetaPtrs, octa)
         ows = _lib.workspace(lib.ygg_octree_meta_workspace_bytes(L), dev)
         _lib.check(lib.ygg_octree_meta(sorted_codes.data_ptr(), rad["node_ranges"].data_ptr(), L,
@@ -384,13 +387,15 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
         _lib.check(reorder(pos.data_ptr(), _lib.ptr(m), perm.data_ptr(), n,
                            pos_sorted.data_ptr(), _lib.ptr(mass_sorted), inv.data_ptr(), st),
                    "reorder")
+    # One host read of the build's scalar words (level count, depth flags and
+    # the octree level count), queued before the index widening so that the
+    # host waits only for the build; the widening runs meanwhile.
+    main = torch.cuda.current_stream(dev)
+    host_scalars = _read_scalars_async(rad["scalars"], octa["scalars"] if octree else None)
     if reorder_side is not None:  # reordered particles and inverse are ready after this
-        torch.cuda.current_stream(dev).wait_stream(reorder_side)
-    # The int64 index arrays are widened before the host waits for the
-    # build, so that launch is queued while the scalars are read.
+        main.wait_stream(reorder_side)
     wide = _widen_fields(rad, octa if octree else None, perm, inv)
-    # One host read: level count, depth flags and the octree level count.
-    scal, oscal = _read_scalars(rad["scalars"], octa["scalars"] if octree else None)
+    scal, oscal = host_scalars()
     if scal[2] != 0:  # levels beyond the fast walk: exact pointer doubling
         _lib.check(lib.ygg_levels_pointer_doubling(ctypes.byref(rptr), T, rws.data_ptr(),
                                                    rws.numel(), st), "levels")
@@ -412,10 +417,33 @@ def _build(positions, masses, bounds, *, leaf_size: int, return_reordered: bool,
 
 def _read_scalars(radix_scalars: torch.Tensor, octree_scalars: Optional[torch.Tensor]):
     """Both builds' int32 scalar words in one device-to-host copy."""
-    parts = [radix_scalars] + ([octree_scalars] if octree_scalars is not None else [])
-    host = torch.cat(parts).tolist() if len(parts) > 1 else radix_scalars.tolist()
+    return _read_scalars_async(radix_scalars, octree_scalars)()
+
+
+def _read_scalars_async(radix_scalars: torch.Tensor, octree_scalars: Optional[torch.Tensor]):
+    """Queues the copy of both builds' int32 scalar words to pinned host
+    memory (one copy when they are adjacent in one buffer) and returns a
+    function that waits for it and gives (radix words, octree words)."""
     nr = radix_scalars.numel()
-    return host[:nr], host[nr:]
+    no = octree_scalars.numel() if octree_scalars is not None else 0
+    host = torch.empty(nr + no, dtype=torch.int32, pin_memory=True)
+    adjacent = (no and octree_scalars.untyped_storage().data_ptr()
+                == radix_scalars.untyped_storage().data_ptr()
+                and octree_scalars.data_ptr() == radix_scalars.data_ptr() + 4 * nr)
+    if adjacent:
+        host.copy_(torch.as_strided(radix_scalars, (nr + no,), (1,)), non_blocking=True)
+    else:
+        host[:nr].copy_(radix_scalars, non_blocking=True)
+        if no:
+            host[nr:].copy_(octree_scalars, non_blocking=True)
+    done = torch.cuda.Event()
+    done.record(torch.cuda.current_stream(radix_scalars.device))
+
+    def wait():
+        done.synchronize()
+        words = host.tolist()
+        return words[:nr], words[nr:]
+    return wait
 
 
 def _widen_index_arrays(arrays: dict) -> dict:
@@ -423,14 +451,25 @@ def _widen_index_arrays(arrays: dict) -> dict:
     tensors are widened to int64 together in one kernel launch
     (ygg_widen_i32_to_i64); anything else goes through .to()."""
     idx = INDEX_DTYPE
-    out, items = {}, []
+    out, items, fast = {}, [], []
     for name, t in arrays.items():
         if (idx == torch.int64 and t.dtype == torch.int32 and t.is_cuda and t.is_contiguous()
-                and len(items) < _lib.WIDEN_MAX_ITEMS):
-            out[name] = torch.empty(t.shape, dtype=torch.int64, device=t.device)
-            items.append((t, out[name]))
+                and len(fast) < _lib.WIDEN_MAX_ITEMS):
+            fast.append((name, t))
         else:
             out[name] = t.to(idx)
+    if fast:
+        # one allocation for all of them, each output a view of its own
+        # 256-byte aligned range
+        def span(t):
+            return (t.numel() + 31) // 32 * 32
+        buf = torch.empty(sum(span(t) for _, t in fast), dtype=torch.int64,
+                          device=fast[0][1].device)
+        at = 0
+        for name, t in fast:
+            out[name] = buf[at:at + t.numel()].view(t.shape)
+            items.append((t, out[name]))
+            at += span(t)
     if items:
         arr = (_lib.WidenItem * len(items))()
         for j, (src, dst) in enumerate(items):
@@ -492,8 +531,8 @@ def _assemble_tree(rad, octa, perm, sorted_codes, b6, n, leaf_size, num_levels, 
         level_offsets=wide["level_offsets"],
         nodes_by_level=wide["nodes_by_level"],
         num_levels=num_levels,
-        bounds_min=b6[:3].clone(),
-        bounds_max=b6[3:].clone(),
+        bounds_min=b6[:3],
+        bounds_max=b6[3:],
         leaf_codes=rad["leaf_codes"].view(torch.uint64),
         leaf_depths=torch.full((L,), -1, dtype=idx, device=dev),
         use_morton_geometry=torch.zeros((), dtype=torch.bool, device=dev),
