@@ -502,6 +502,13 @@ typedef struct YggFlatWalkParams {
   int32_t start_generation;   /* 0: a new walk; g > 0: resume a paused walk at generation g (the
                                  other device buffers as the paused call left them) */
   int32_t pad1;
+  const uint32_t* gen_pairs_hint; /* optional, host memory: [max_generations] pairs expected in
+                                 each generation (e.g. a previous walk's sizes on a similar
+                                 tree). It sizes each generation's grid (1.5x the hinted chunks,
+                                 at least 256 workgroups, at most grid_blocks) and so only the
+                                 launch cost: any grid covers a generation, its workgroups
+                                 stepping over the chunks. 0xFFFFFFFF for a generation, or
+                                 NULL for all: grids from the bound roots * 4^g alone. */
 } YggFlatWalkParams;
 
 int ygg_walk_flat(const YggFlatWalkParams* params, uint32_t* host_status,

@@ -66,7 +66,7 @@ class FlatWalkParams(ctypes.Structure):
         ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("xcd_groups", _I32),
         ("node_hist", _P), ("roots", _P), ("num_roots", _I64), ("spread_counters", _P),
         ("spread_lanes", _I64), ("policy", _P), ("far_tags", _P), ("gen_fill", _P),
-        ("start_generation", _I32), ("pad1", _I32),
+        ("start_generation", _I32), ("pad1", _I32), ("gen_pairs_hint", _P),
     ]
 
 

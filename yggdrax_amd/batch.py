@@ -424,6 +424,8 @@ class _ForestContext(_WalkContext):
         self.kernel_log = WALK_TIMING["log"]
         self.dt = SimpleNamespace(num_nodes=T, num_internal=max(1, lay.num_internal),
                                   num_leaves=lay.num_leaves, num_levels=max_levels)
+        self.profile_key = (T, lay.num_internal, lay.num_leaves, lay.num_systems, self.mac,
+                            self.theta_sq)
 
 
 @dataclass

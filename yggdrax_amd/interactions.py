@@ -19,6 +19,7 @@ from collections import OrderedDict
 from dataclasses import dataclass
 from typing import Callable, List, NamedTuple, Optional
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -66,6 +67,16 @@ ROW_BUCKET_MAX_BITS = 22
 # Sizing of the last flat walk: how its streams were sized ("hint", "grow"
 # or "count"), their final capacities and the number of growths.
 LAST_WALK: dict = {}
+# Launch shapes of the flat walk from the last walk on a tree of the same
+# shape (node and leaf counts, roots, MAC, opening angle): its pairs per
+# generation size each generation's grid (ygg_walk_flat's gen_pairs_hint)
+# and the first batch of generation launches, so the tail generations do
+# not dispatch thousands of workgroups that exit at once and the walk needs
+# no second host round trip. Only launch costs depend on it: any grid covers
+# a generation.
+WALK_GEN_PROFILE = os.environ.get("YGGDRAX_WALK_GEN_PROFILE", "1") != "0"
+_GEN_PROFILES: "OrderedDict[tuple, list]" = OrderedDict()
+_GEN_PROFILES_MAX = 32
 # Walk chunks split into this many contiguous ranges, one per XCD (MI355X
 # has 8; 0 or 1 = plain grid-stride order).
 WALK_XCD_GROUPS = int(os.environ.get("YGGDRAX_WALK_XCD_GROUPS", "0"))
@@ -345,7 +356,7 @@ class _WalkContext:
         th = ctypes.c_float(float(theta)).value
         th_sq = ctypes.c_float(ctypes.c_float(th * th).value).value
         self.theta_sq = th_sq
-        self.theta_root = float(torch.sqrt(torch.tensor(th_sq, dtype=torch.float32)))
+        self.theta_root = float(np.sqrt(np.float32(th_sq)))  # float32 square root
         self.nodes = torch.empty(dt.num_nodes * NODE_RECORD_BYTES, dtype=torch.uint8,
                                  device=self.dev)
         g = geometry
@@ -363,6 +374,8 @@ class _WalkContext:
         self.decisions = torch.empty(3, dtype=torch.int64, device=self.dev)
         self.timing = WALK_TIMING["enabled"]
         self.gen_hint = int(1.5 * max(1, dt.num_levels)) + 4
+        self.profile_key = (dt.num_nodes, dt.num_internal, dt.num_leaves, self.mac,
+                            self.theta_sq)
         self.grid_blocks = FLAT_GRID_BLOCKS
         self.pairs_per_thread = WALK_PAIRS_PER_THREAD
         self.kernel_log = WALK_TIMING["log"]
@@ -425,6 +438,13 @@ class _WalkContext:
         if emit and grow:
             gen_fill = torch.empty(G * GEN_RECORD_WORDS, dtype=torch.int64, device=dev)
             p.gen_fill = gen_fill.data_ptr()
+        pkey = (self.profile_key, 0 if roots is None else int(roots.numel()), policy is None)
+        prof = _GEN_PROFILES.get(pkey) if WALK_GEN_PROFILE else None
+        if prof:
+            # (its last generation is the empty one that ended the walk)
+            p.gen_pairs_hint = ctypes.cast(prof[1], ctypes.c_void_p)
+            if not gen_hint:  # one batch: every generation of the profile
+                p.gen_hint = max(int(p.gen_hint), min(len(prof[0]), G))
         ngen = ctypes.c_int32(0)
         gen_ms = (ctypes.c_float * MAX_GENERATIONS)() if self.timing else None
         growths = 0
@@ -458,6 +478,17 @@ class _WalkContext:
             growths += 1
             FLAT_STATS["stream_growths"] += 1
         raw_gens = [v & 0xFFFFFFFF for v in hs[:min(ngen.value + 2, G + 1)]]
+        if WALK_GEN_PROFILE and not (hs[G + 1] or hs[G + 2] or hs[G + 3]):
+            sizes = [min(v, Q) for v in raw_gens]
+            if not prof or prof[0] != sizes:
+                # generations past the profile are unknown: their grids from the bound
+                arr = (ctypes.c_uint32 * G)(*sizes[:G])
+                for k in range(len(sizes), G):
+                    arr[k] = 0xFFFFFFFF
+                _GEN_PROFILES[pkey] = (sizes, arr)
+            _GEN_PROFILES.move_to_end(pkey)
+            while len(_GEN_PROFILES) > _GEN_PROFILES_MAX:
+                _GEN_PROFILES.popitem(last=False)
         c = _host_totals(hs, G)
         # every evaluated pair is exactly one of accept (2 far entries),
         # near (2 near entries) or refine

@@ -449,37 +449,37 @@ def _read_scalars_async(radix_scalars: torch.Tensor, octree_scalars: Optional[to
 def _widen_index_arrays(arrays: dict) -> dict:
     """INDEX_DTYPE versions of int32 device arrays: contiguous int32 GPU
     tensors are widened to int64 together in one kernel launch
-    (ygg_widen_i32_to_i64); anything else goes through .to()."""
+    (ygg_widen_i32_to_i64) into one allocation, each output a view of its
+    own 256-byte aligned range; anything else goes through .to(). The
+    launch goes out before the views are made, so the kernel runs while
+    the host builds them."""
     idx = INDEX_DTYPE
-    out, items, fast = {}, [], []
+    out, fast = {}, []
     for name, t in arrays.items():
         if (idx == torch.int64 and t.dtype == torch.int32 and t.is_cuda and t.is_contiguous()
                 and len(fast) < _lib.WIDEN_MAX_ITEMS):
-            fast.append((name, t))
+            fast.append((name, t, t.numel()))
         else:
             out[name] = t.to(idx)
-    if fast:
-        # one allocation for all of them, each output a view of its own
-        # 256-byte aligned range
-        def span(t):
-            return (t.numel() + 31) // 32 * 32
-        buf = torch.empty(sum(span(t) for _, t in fast), dtype=torch.int64,
-                          device=fast[0][1].device)
-        at = 0
-        for name, t in fast:
-            out[name] = buf[at:at + t.numel()].view(t.shape)
-            items.append((t, out[name]))
-            at += span(t)
-    if items:
-        arr = (_lib.WidenItem * len(items))()
-        for j, (src, dst) in enumerate(items):
-            n = src.numel()
-            arr[j].src = src.data_ptr() if n else None
-            arr[j].dst = dst.data_ptr() if n else None
-            arr[j].n = n
-        dev = items[0][0].device
-        _lib.check(_lib.load().ygg_widen_i32_to_i64(arr, len(items), _lib.stream_ptr(dev)),
-                   "widen index arrays")
+    if not fast:
+        return out
+    spans = [(k + 31) // 32 * 32 for _, _, k in fast]
+    dev = fast[0][1].device
+    buf = torch.empty(sum(spans), dtype=torch.int64, device=dev)
+    base = buf.data_ptr()
+    arr = (_lib.WidenItem * len(fast))()
+    at = 0
+    for j, ((_, t, k), sp) in enumerate(zip(fast, spans)):
+        arr[j].src = t.data_ptr() if k else None
+        arr[j].dst = base + 8 * at if k else None
+        arr[j].n = k
+        at += sp
+    _lib.check(_lib.load().ygg_widen_i32_to_i64(arr, len(fast), _lib.stream_ptr(dev)),
+               "widen index arrays")
+    at = 0
+    for (name, t, k), sp in zip(fast, spans):
+        out[name] = buf[at:at + k].view(t.shape)
+        at += sp
     return out
 
 
