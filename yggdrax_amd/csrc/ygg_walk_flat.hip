@@ -161,12 +161,14 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         x3[kFlatPer];
     unsigned nc = 0, nf = 0, nn = 0;
     // Memory first, decisions after: every pair word of the chunk, then the
-    // node records of a group of two pairs, are loaded before any of them
+    // node records of a group of four pairs, are loaded before any of them
     // is used, so their latencies overlap instead of adding up pair by pair
     // (a lane past the wavefront's end loads node 0 and decides nothing).
-    // Measured on the 2^24 clustered walk: 7.8 ms with the loads issued
-    // pair by pair, 6.9 ms with groups of one or two at 8 pairs per thread,
-    // 7.1 ms with groups of four (more registers, fewer waves).
+    // Measured on the 2^24 clustered walk at 8 pairs per thread: 7.8 ms with
+    // the loads issued pair by pair, 6.9 ms with groups of two when the
+    // decisions branched per pair; with branch-free decisions 6.20 ms with
+    // groups of two, 6.13 ms with four (151 VGPRs, still 3 waves per SIMD)
+    // and 7.38 ms with eight (2 waves per SIMD).
     constexpr uint64_t kNoPair = ~0ull;
     uint64_t prw[kFlatPer];
 #pragma unroll
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
       const int64_t idx = chunk + k * kFlatThreads + threadIdx.x;
       prw[k] = idx < size ? wf_in[idx] : kNoPair;
     }
-    constexpr int kGroup = kFlatPer < 2 ? kFlatPer : 2;
+    constexpr int kGroup = kFlatPer < 4 ? kFlatPer : 4;
 #pragma unroll
     for (int k0 = 0; k0 < kFlatPer; k0 += kGroup) {
     NodeRec rt[kGroup], rs[kGroup];
