@@ -563,20 +563,25 @@ int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
  * (its contents afterwards are unspecified). keys and vals are not
  * modified. `count` (optional, device memory) holds the number of valid
  * entries of streams allocated at capacity n: only the first
- * min(n, *count) are listed, with no host read (for graph capture). */
+ * min(n, *count) are listed, with no host read (for graph capture).
+ * `row_sub` (optional, device memory, [rows]) is taken off every value of
+ * row r, and off the far target of row r: ids local to a block of nodes,
+ * as for the systems of a batched forest (bucketing only: with the stable
+ * sort it returns YGG_ERR_UNSUPPORTED). */
 size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                        const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,
-                       const int32_t* nodes_by_level, const int32_t* node_level, int out_i64,
-                       uint32_t* key_scratch, int64_t* lm_offsets, void* sources, void* targets,
-                       void* target_levels, void* counts, void* node_offsets, int32_t* counts32,
-                       void* ws, size_t ws_bytes, void* stream);
+                       const int32_t* nodes_by_level, const int32_t* node_level,
+                       const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
+                       int64_t* lm_offsets, void* sources, void* targets, void* target_levels,
+                       void* counts, void* node_offsets, int32_t* counts32, void* ws,
+                       size_t ws_bytes, void* stream);
 size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                         const int64_t* count, int64_t num_leaves, int end_bit, int digit_bits,
-                        int out_i64, uint32_t* key_scratch, int64_t* row_offsets,
-                        void* neighbors, void* counts, void* offsets, void* ws, size_t ws_bytes,
-                        void* stream);
+                        const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
+                        int64_t* row_offsets, void* neighbors, void* counts, void* offsets,
+                        void* ws, size_t ws_bytes, void* stream);
 /* The dense -1 padded [L][max_count] neighbour leaf rows from a near list
  * (neighbors int64 if out_i64, else int32). */
 int ygg_near_dense_from_list(int64_t num_leaves, const int64_t* row_offsets,
@@ -687,7 +692,10 @@ int ygg_batch_walk_prepare_nodes(const YggBatch* batch, const float* center, con
  * [65 s + l] from the system's radix level_offsets. Near: neighbors (global
  * leaf node ids, in place) become local; near_offsets_out[leaf_offsets[s] +
  * s + j] (L_s + 1 per system). row_offsets are the sorted streams' offsets
- * over batch-global rows (level-major ranks / leaf rows). */
+ * over batch-global rows (level-major ranks / leaf rows). num_entries 0
+ * leaves the entries as they are: lists grouped with row_sub (each row's
+ * system node offset) are local already, and only the offsets are
+ * written. */
 int ygg_batch_far_localize(const YggBatch* batch, const int64_t* row_offsets,
                            const int32_t* nodes_by_level_global, const int32_t* level_offsets,
                            int64_t num_entries, int out_i64, void* sources, void* targets,

@@ -40,7 +40,7 @@ def _near(lib, keys, vals, n, count, L, bits, out_i64=1):
                scratch=torch.empty(max(1, n), dtype=torch.int32, device=dev))
     ws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(n, nend, bits), dev)
     _lib.check(lib.ygg_near_lists_sort(
-        keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), L, nend, bits, out_i64,
+        keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), L, nend, bits, None, out_i64,
         out["scratch"].data_ptr(), out["rows"].data_ptr(), out["neighbors"].data_ptr(),
         out["counts"].data_ptr(), out["offsets"].data_ptr(), ws.data_ptr(), ws.numel(),
         _lib.stream_ptr(dev)), "near lists")
@@ -62,7 +62,7 @@ def _far(lib, keys, vals, n, count, T, nbl, lvl, bits):
     ws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(n, fend, bits), dev)
     _lib.check(lib.ygg_far_lists_sort(
         keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), T, fend, bits, nbl.data_ptr(),
-        lvl.data_ptr(), 1, out["scratch"].data_ptr(), out["lm"].data_ptr(),
+        lvl.data_ptr(), None, 1, out["scratch"].data_ptr(), out["lm"].data_ptr(),
         out["sources"].data_ptr(), out["targets"].data_ptr(), out["levels"].data_ptr(),
         out["counts"].data_ptr(), out["node_off"].data_ptr(), out["counts32"].data_ptr(),
         ws.data_ptr(), ws.numel(), _lib.stream_ptr(dev)), "far lists")
@@ -204,10 +204,76 @@ def test_buckets_repeat_with_a_smaller_input(lib):
         rows = torch.empty(L + 1, dtype=torch.int64, device=dev)
         scratch = torch.empty(big, dtype=torch.int32, device=dev)
         _lib.check(lib.ygg_near_lists_sort(
-            kt.data_ptr(), vt.data_ptr(), n, None, L, nend, 0, 1, scratch.data_ptr(),
+            kt.data_ptr(), vt.data_ptr(), n, None, L, nend, 0, None, 1, scratch.data_ptr(),
             rows.data_ptr(), nb.data_ptr(), counts.data_ptr(), offs.data_ptr(), ws.data_ptr(),
             ws.numel(), _lib.stream_ptr(dev)), "near lists")
         torch.cuda.synchronize()
         c, v = _expected(keys[:n], vals[:n], L)
         assert np.array_equal(counts.cpu().numpy(), c)
         assert np.array_equal(_row_sorted(nb.cpu().numpy()[:n], c), v)
+
+
+def test_row_sub_makes_values_local(lib):
+    """row_sub[r] is taken off every value of row r (and off the far target
+    of row r): the batched forest's system-local ids. The stable sort does
+    not take it."""
+    from yggdrax_amd import _lib
+    print("[row buckets] row_sub", flush=True)
+    rng = np.random.default_rng(17)
+    dev = torch.device("cuda")
+    rows, n = 200_000, 1_500_000
+    keys = rng.integers(0, rows, n).astype(np.int32)
+    sub = rng.integers(0, 1000, rows).astype(np.int64)
+    vals = (sub[keys] + rng.integers(0, 2 * rows - 1000, n)).astype(np.int32)
+    kt, vt = torch.from_numpy(keys).to(dev), torch.from_numpy(vals).to(dev)
+    st = torch.from_numpy(sub).to(dev)
+    nend = (rows - 1).bit_length()
+    out = torch.full((n,), -7, dtype=torch.int64, device=dev)
+    cnt = torch.empty(rows, dtype=torch.int64, device=dev)
+    off = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    roff = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    scr = torch.empty(n, dtype=torch.int32, device=dev)
+    ws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(n, nend, 0), dev)
+    _lib.check(lib.ygg_near_lists_sort(
+        kt.data_ptr(), vt.data_ptr(), n, None, rows, nend, 0, st.data_ptr(), 1, scr.data_ptr(),
+        roff.data_ptr(), out.data_ptr(), cnt.data_ptr(), off.data_ptr(), ws.data_ptr(),
+        ws.numel(), _lib.stream_ptr(dev)), "near lists")
+    torch.cuda.synchronize()
+    counts, expect = _expected(keys, (vals - sub[keys]).astype(np.int64), rows)
+    assert np.array_equal(cnt.cpu().numpy(), counts)
+    assert np.array_equal(_row_sorted(out.cpu().numpy(), counts), expect)
+    # far: sources (node ids below T) and targets both local
+    T = rows
+    vt = torch.from_numpy((sub[keys] + rng.integers(0, T - 1000, n)).astype(np.int32)).to(dev)
+    nbl = torch.from_numpy(rng.permutation(T).astype(np.int32)).to(dev)
+    lvl = torch.from_numpy(rng.integers(0, 30, T).astype(np.int32)).to(dev)
+    got = _far(lib, kt, vt, n, None, T, nbl, lvl, 0)
+    ref_t = got["targets"].clone()
+    src = torch.full((n,), -7, dtype=torch.int64, device=dev)
+    tgt = torch.full((n,), -7, dtype=torch.int64, device=dev)
+    levels = torch.full((n,), -7, dtype=torch.int64, device=dev)
+    c64 = torch.empty(T, dtype=torch.int64, device=dev)
+    noff = torch.empty(T + 1, dtype=torch.int64, device=dev)
+    c32 = torch.empty(T, dtype=torch.int32, device=dev)
+    lm = torch.empty(T + 1, dtype=torch.int64, device=dev)
+    ws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(n, nend, 0), dev)
+    _lib.check(lib.ygg_far_lists_sort(
+        kt.data_ptr(), vt.data_ptr(), n, None, T, nend, 0, nbl.data_ptr(), lvl.data_ptr(),
+        st.data_ptr(), 1, scr.data_ptr(), lm.data_ptr(), src.data_ptr(), tgt.data_ptr(),
+        levels.data_ptr(), c64.data_ptr(), noff.data_ptr(), c32.data_ptr(), ws.data_ptr(),
+        ws.numel(), _lib.stream_ptr(dev)), "far lists")
+    torch.cuda.synchronize()
+    assert torch.equal(lm, got["lm"]) and torch.equal(levels, got["levels"])
+    per_rank = (lm[1:] - lm[:-1])  # entries lie in level-major rank order
+    rows_of = torch.repeat_interleave(torch.arange(T, device=dev), per_rank)
+    assert torch.equal(tgt, ref_t - st[rows_of])
+    pr = per_rank.cpu().numpy()
+    assert np.array_equal(_row_sorted(src.cpu().numpy(), pr),
+                          _row_sorted((got["sources"] - st[rows_of]).cpu().numpy(), pr))
+    # the stable sort refuses it
+    ws9 = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(n, nend, 9), dev)
+    rc = lib.ygg_near_lists_sort(
+        kt.data_ptr(), vt.data_ptr(), n, None, rows, nend, 9, st.data_ptr(), 1, scr.data_ptr(),
+        roff.data_ptr(), out.data_ptr(), cnt.data_ptr(), off.data_ptr(), ws9.data_ptr(),
+        ws9.numel(), _lib.stream_ptr(dev))
+    assert rc != 0

@@ -531,6 +531,11 @@ class _LayoutIndex:
         self.leaf_off = torch.from_numpy(lay.leaf_off[:-1].astype(np.int64)).to(dev)
         self.sys_of_node = torch.repeat_interleave(
             torch.arange(S, device=dev), torch.from_numpy(np.diff(lay.node_off)).to(dev))
+        # each level-major rank's and each leaf row's system node offset: what
+        # turns batch-global node ids into the system's own
+        self.node_sub = self.node_off[self.sys_of_node].contiguous()
+        self.leaf_sub = torch.repeat_interleave(
+            self.node_off, torch.from_numpy(np.diff(lay.leaf_off)).to(dev)).contiguous()
 
 
 def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_vals, near_keys,
@@ -538,9 +543,11 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
                         near_count=None, concurrent: bool = True):
     """Per-system far/near lists from the forest walk's emission streams.
     ygg_far_lists_sort (on a side stream) and ygg_near_lists_sort (on the
-    current one) group the entries by row into the lists, with batch-global
-    ids; the localize kernels then turn them into every
-    system's local ids and CSR offsets. `far_count` / `near_count` (int64
+    current one) group the entries by row into the lists; the bucketing
+    writes every system's local ids itself (each row's system node offset
+    taken off), the stable sort writes batch-global ids that the localize
+    kernels rewrite, and the localize kernels write every system's CSR
+    offsets. `far_count` / `near_count` (int64
     device scalars) hold the valid entry counts of streams allocated at
     capacities nf / nn; without them all nf / nn entries are valid. With
     `concurrent` False everything runs on the current stream (stream
@@ -554,7 +561,7 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
     idx = INDEX_DTYPE
     i64 = int(idx == torch.int64)
     T, Lt, S = lay.num_nodes, lay.num_leaves, lay.num_systems
-    nbl_global = (trees.rad["nodes_by_level"].long() + li.node_off[li.sys_of_node]).to(torch.int32)
+    nbl_global = (trees.rad["nodes_by_level"].long() + li.node_sub).to(torch.int32)
     main = torch.cuda.current_stream(dev)
     side = _side_stream(dev) if concurrent else main
     if concurrent:
@@ -571,9 +578,12 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
         far_off = torch.empty(T + 1, dtype=torch.int64, device=dev)
         fsorted = torch.empty(max(1, nf), dtype=torch.int32, device=dev)
         fws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(nf, fend, fbits), dev)
+        # the bucketing writes system-local sources and targets itself
+        far_local = fbits == 0
         _lib.check(lib.ygg_far_lists_sort(
             far_keys.data_ptr(), far_vals.data_ptr(), nf, _lib.ptr(far_count), T, fend, fbits,
-            nbl_global.data_ptr(), trees.rad["node_level"].data_ptr(), i64, fsorted.data_ptr(),
+            nbl_global.data_ptr(), trees.rad["node_level"].data_ptr(),
+            li.node_sub.data_ptr() if far_local else None, i64, fsorted.data_ptr(),
             far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
             counts.data_ptr(), node_off_glob.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
             fws.numel(), _lib.stream_ptr(dev)), "batch far lists sort")
@@ -581,8 +591,8 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
         far_lvl = torch.empty((S, 65), dtype=idx, device=dev)
         _lib.check(lib.ygg_batch_far_localize(
             lay.ptr(), far_off.data_ptr(), nbl_global.data_ptr(),
-            trees.rad["level_offsets"].data_ptr(), nf, i64, sources.data_ptr(),
-            targets.data_ptr(), far_offsets.data_ptr(), far_lvl.data_ptr(),
+            trees.rad["level_offsets"].data_ptr(), 0 if far_local else nf, i64,
+            sources.data_ptr(), targets.data_ptr(), far_offsets.data_ptr(), far_lvl.data_ptr(),
             _lib.stream_ptr(dev)), "batch far localize")
     nend = max(1, (Lt - 1).bit_length())
     nbits = row_lists_digit_bits(nend, nn)
@@ -593,14 +603,16 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
     nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
     nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
     st = _lib.stream_ptr(dev)
+    near_local = nbits == 0  # system-local neighbours from the bucketing itself
     _lib.check(lib.ygg_near_lists_sort(
         near_keys.data_ptr(), near_vals.data_ptr(), nn, _lib.ptr(near_count), Lt, nend, nbits,
-        i64, nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
-        noff_glob.data_ptr(), nws.data_ptr(), nws.numel(), st), "batch near lists sort")
+        li.leaf_sub.data_ptr() if near_local else None, i64, nsorted.data_ptr(),
+        near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(), noff_glob.data_ptr(),
+        nws.data_ptr(), nws.numel(), st), "batch near lists sort")
     near_offsets = torch.empty(Lt + S, dtype=idx, device=dev)
-    _lib.check(lib.ygg_batch_near_localize(lay.ptr(), near_off.data_ptr(), nn, i64,
-                                           neighbors.data_ptr(), near_offsets.data_ptr(), st),
-               "batch near localize")
+    _lib.check(lib.ygg_batch_near_localize(lay.ptr(), near_off.data_ptr(),
+                                           0 if near_local else nn, i64, neighbors.data_ptr(),
+                                           near_offsets.data_ptr(), st), "batch near localize")
     if concurrent:
         main.wait_stream(side)
         for t in (sources, targets, levels, counts, counts32, node_off_glob, far_off, fsorted,

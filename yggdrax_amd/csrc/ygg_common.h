@@ -315,6 +315,7 @@ struct RowBucketState {
   const uint32_t* bstart;
   const uint32_t* tstart;
   const uint32_t* tile_bases;
+  const int64_t* row_sub;  // taken off each row's values by the scatter (may be null)
 };
 bool row_bucket_split(int end_bit, int* hi_bits, int* lo_bits);
 size_t row_bucket_workspace_bytes(int64_t n, int end_bit);
@@ -322,8 +323,8 @@ int row_bucket_count(const uint32_t* keys, const uint32_t* vals, int64_t n, cons
                      int end_bit, int64_t rows, int64_t val_bound, uint32_t* vals_scratch,
                      uint32_t* row_counts, void* ws, size_t ws_bytes, hipStream_t st,
                      RowBucketState* state);
-int row_bucket_scatter(const RowBucketState& s, const int64_t* row_offsets, int out_i64,
-                       void* out, hipStream_t st);
+int row_bucket_scatter(const RowBucketState& s, const int64_t* row_offsets,
+                       const int64_t* row_sub, int out_i64, void* out, hipStream_t st);
 // Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:
 // out[n] = total). `gather`, if non-null, reads counts[gather[i]].
 int exclusive_scan_i32_to_i64(const int32_t* counts, const int32_t* gather, int64_t* out,

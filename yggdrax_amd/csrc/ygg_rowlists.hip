@@ -361,10 +361,11 @@ __global__ __launch_bounds__(kBucketThreads) void k_row_scatter(
     const uint32_t* __restrict__ vals, uint32_t rows, int lo_bits, int nb,
     const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ tstart,
     const uint32_t* __restrict__ tile_bases, const int64_t* __restrict__ row_offsets,
-    OutT* __restrict__ out) {
+    const int64_t* __restrict__ row_sub, OutT* __restrict__ out) {
   constexpr int kPer = BINS / kBucketThreads;
   __shared__ uint32_t s_cnt[BINS];  // counts, then each digit's first slot
   __shared__ int64_t s_g[BINS];     // output index of a digit's first slot, minus it
+  __shared__ int64_t s_sub[BINS];   // what row_sub takes off the row's values
   __shared__ uint32_t s_wsum[kBucketThreads / YGG_WAVE];
   __shared__ uint32_t s_val[kBucketTile];
   __shared__ uint16_t s_dig[kBucketTile];
@@ -389,16 +390,18 @@ __global__ __launch_bounds__(kBucketThreads) void k_row_scatter(
   // the entries (a digit absent from the tile leaves its slot unwritten and
   // unused; rows past the last read the final offset).
   const int64_t tb = (int64_t)blockIdx.x * nl;
-  int64_t roff[kPer];
+  int64_t roff[kPer], rsub[kPer];
   uint32_t tbase[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int d = tid * kPer + j;
     roff[j] = 0;
+    rsub[j] = 0;
     tbase[j] = 0;
     if (d < nl) {
       const uint32_t row = ((uint32_t)rt.bucket << lo_bits) | (uint32_t)d;
       roff[j] = row_offsets[row < rows ? row : rows];
+      if (row_sub) rsub[j] = row_sub[row < rows ? row : rows - 1];
       tbase[j] = tile_bases[tb + d];
     }
   }
@@ -424,7 +427,10 @@ __global__ __launch_bounds__(kBucketThreads) void k_row_scatter(
   for (int j = 0; j < kPer; ++j) {
     const int d = tid * kPer + j;
     if (d < nl) {
-      if (c[j]) s_g[d] = roff[j] + (int64_t)tbase[j] - (int64_t)run;
+      if (c[j]) {
+        s_g[d] = roff[j] + (int64_t)tbase[j] - (int64_t)run;
+        s_sub[d] = rsub[j];
+      }
       s_cnt[d] = run;
     }
     run += c[j];
@@ -438,8 +444,15 @@ __global__ __launch_bounds__(kBucketThreads) void k_row_scatter(
     s_dig[slot] = (uint16_t)dig[k];
   }
   __syncthreads();
-  for (int p = tid; p < (int)valid; p += kBucketThreads)
-    out[s_g[s_dig[p]] + p] = (OutT)s_val[p];
+  if (row_sub) {
+    for (int p = tid; p < (int)valid; p += kBucketThreads) {
+      const int d = s_dig[p];
+      out[s_g[d] + p] = (OutT)((int64_t)s_val[p] - s_sub[d]);
+    }
+  } else {
+    for (int p = tid; p < (int)valid; p += kBucketThreads)
+      out[s_g[s_dig[p]] + p] = (OutT)s_val[p];
+  }
 }
 
 template <int BINS, int kMode>
@@ -453,11 +466,11 @@ void launch_row_bucket(const RowBucketState& s, uint32_t* row_counts, const int6
   else if (out_i64)
     hipLaunchKernelGGL((k_row_scatter<BINS, kMode, int64_t>), g, b, 0, st, s.keys, s.lo16,
                        s.vals, s.rows, s.lo_bits, s.buckets, s.bstart, s.tstart, s.tile_bases,
-                       row_offsets, (int64_t*)out);
+                       row_offsets, s.row_sub, (int64_t*)out);
   else
     hipLaunchKernelGGL((k_row_scatter<BINS, kMode, int32_t>), g, b, 0, st, s.keys, s.lo16,
                        s.vals, s.rows, s.lo_bits, s.buckets, s.bstart, s.tstart, s.tile_bases,
-                       row_offsets, (int32_t*)out);
+                       row_offsets, s.row_sub, (int32_t*)out);
 }
 
 // Reserve (row_counts given) or scatter, for the state's layout.
@@ -582,19 +595,23 @@ int row_bucket_count(const uint32_t* keys, const uint32_t* vals, int64_t n,
   state->bstart = bstart;
   state->tstart = tstart;
   state->tile_bases = bases;
+  state->row_sub = nullptr;
   launch_row_bucket_any(*state, row_counts, nullptr, 0, nullptr, st);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }
 
-// Step 6: every valid entry's value, as OutT, at its row's offset
-// (row_offsets: exclusive scan of the counts) plus its slot in the row.
-// With no entries there is nothing to write (and `out` may be null).
-int row_bucket_scatter(const RowBucketState& s, const int64_t* row_offsets, int out_i64,
-                       void* out, hipStream_t st) {
+// Step 6: every valid entry's value, less row_sub[row] when row_sub is
+// given, as OutT, at its row's offset (row_offsets: exclusive scan of the
+// counts) plus its slot in the row. With no entries there is nothing to
+// write (and `out` may be null).
+int row_bucket_scatter(const RowBucketState& s, const int64_t* row_offsets,
+                       const int64_t* row_sub, int out_i64, void* out, hipStream_t st) {
   if (s.n == 0) return YGG_OK;
   if (!row_offsets || !out) return YGG_ERR_ARG;
-  launch_row_bucket_any(s, nullptr, row_offsets, out_i64, out, st);
+  RowBucketState sub = s;
+  sub.row_sub = row_sub;
+  launch_row_bucket_any(sub, nullptr, row_offsets, out_i64, out, st);
   YGG_LAUNCH_CHECK();
   return YGG_OK;
 }

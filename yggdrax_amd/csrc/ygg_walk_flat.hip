@@ -630,6 +630,7 @@ __global__ __launch_bounds__(256) void k_far_targets_rows(const int64_t* __restr
                                                           int rows,
                                                           const int32_t* __restrict__ nbl,
                                                           const int32_t* __restrict__ node_level,
+                                                          const int64_t* __restrict__ row_sub,
                                                           OutT* __restrict__ targets,
                                                           OutT* __restrict__ levels) {
   const unsigned lane = lane_id();
@@ -639,7 +640,8 @@ __global__ __launch_bounds__(256) void k_far_targets_rows(const int64_t* __restr
     const int64_t b = off[r], e = off[r + 1];
     if (b >= e) continue;
     const int32_t node = nbl[r];
-    const OutT t = (OutT)node, lv = (OutT)node_level[node];
+    const OutT t = (OutT)((int64_t)node - (row_sub ? row_sub[r] : 0)),
+               lv = (OutT)node_level[node];
     for (int64_t i = b + lane; i < e; i += YGG_WAVE) {
       targets[i] = t;
       levels[i] = lv;
@@ -1074,11 +1076,13 @@ size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits
 
 int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                        const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,
-                       const int32_t* nodes_by_level, const int32_t* node_level, int out_i64,
-                       uint32_t* key_scratch, int64_t* lm_offsets, void* sources, void* targets,
-                       void* target_levels, void* counts, void* node_offsets, int32_t* counts32,
-                       void* ws, size_t ws_bytes, void* stream) {
+                       const int32_t* nodes_by_level, const int32_t* node_level,
+                       const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
+                       int64_t* lm_offsets, void* sources, void* targets, void* target_levels,
+                       void* counts, void* node_offsets, int32_t* counts32, void* ws,
+                       size_t ws_bytes, void* stream) {
   if (num_nodes < 1 || n < 0 || !lm_offsets || !key_scratch || !ws) return YGG_ERR_ARG;
+  if (row_sub && !lists_bucketed(end_bit, digit_bits)) return YGG_ERR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
   const int T = (int)num_nodes;
   size_t sort_bytes = 0, scan_bytes = 0;
@@ -1095,7 +1099,7 @@ int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
     rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, lm_offsets, T,
                                         scan_ws, scan_bytes, st);
     if (rc) return rc;
-    rc = ygg::row_bucket_scatter(bs, lm_offsets, out_i64, sources, st);
+    rc = ygg::row_bucket_scatter(bs, lm_offsets, row_sub, out_i64, sources, st);
   } else {
     rc = ygg::sort_far_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits, out_i64,
                                   sources, row_counts, T, ws, sort_bytes, st);
@@ -1107,12 +1111,14 @@ int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
   dim3 gr(grid_for(T, 256)), gw(grid_for((int64_t)T, 256 / YGG_WAVE, 16384));
   if (out_i64) {
     hipLaunchKernelGGL(k_far_targets_rows<int64_t>, gw, dim3(256), 0, st, lm_offsets, T,
-                       nodes_by_level, node_level, (int64_t*)targets, (int64_t*)target_levels);
+                       nodes_by_level, node_level, row_sub, (int64_t*)targets,
+                       (int64_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int64_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int64_t*)counts, (int64_t*)node_offsets, counts32);
   } else {
     hipLaunchKernelGGL(k_far_targets_rows<int32_t>, gw, dim3(256), 0, st, lm_offsets, T,
-                       nodes_by_level, node_level, (int32_t*)targets, (int32_t*)target_levels);
+                       nodes_by_level, node_level, row_sub, (int32_t*)targets,
+                       (int32_t*)target_levels);
     hipLaunchKernelGGL(k_far_rows<int32_t>, gr, dim3(256), 0, st, lm_offsets, T, nodes_by_level,
                        (int32_t*)counts, (int32_t*)node_offsets, counts32);
   }
@@ -1126,10 +1132,11 @@ size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bit
 
 int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                         const int64_t* count, int64_t num_leaves, int end_bit, int digit_bits,
-                        int out_i64, uint32_t* key_scratch, int64_t* row_offsets,
-                        void* neighbors, void* counts, void* offsets, void* ws, size_t ws_bytes,
-                        void* stream) {
+                        const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
+                        int64_t* row_offsets, void* neighbors, void* counts, void* offsets,
+                        void* ws, size_t ws_bytes, void* stream) {
   if (num_leaves < 1 || n < 0 || !row_offsets || !key_scratch || !ws) return YGG_ERR_ARG;
+  if (row_sub && !lists_bucketed(end_bit, digit_bits)) return YGG_ERR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
   const int L = (int)num_leaves;
   size_t sort_bytes = 0, scan_bytes = 0;
@@ -1146,7 +1153,7 @@ int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
     rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, row_offsets, L,
                                         scan_ws, scan_bytes, st);
     if (rc) return rc;
-    rc = ygg::row_bucket_scatter(bs, row_offsets, out_i64, neighbors, st);
+    rc = ygg::row_bucket_scatter(bs, row_offsets, row_sub, out_i64, neighbors, st);
   } else {
     rc = ygg::sort_near_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits,
                                    out_i64, neighbors, row_counts, L, ws, sort_bytes, st);

@@ -760,7 +760,7 @@ def _fused_lists(em: dict, dt: DeviceTopology):
     nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
     nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
     _lib.check(lib.ygg_near_lists_sort(
-        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, None, L, nend, nbits, i64,
+        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, None, L, nend, nbits, None, i64,
         nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
         noff.data_ptr(), nws.data_ptr(), nws.numel(), _lib.stream_ptr(dev)), "near lists sort")
     side = _side_stream(dev)
@@ -779,7 +779,7 @@ def _fused_lists(em: dict, dt: DeviceTopology):
         fws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(nf, fend, fbits), dev)
         _lib.check(lib.ygg_far_lists_sort(
             em["far_keys"].data_ptr(), em["far_vals"].data_ptr(), nf, None, T, fend, fbits,
-            dt.nodes_by_level.data_ptr(), dt.node_level.data_ptr(), i64, fsorted.data_ptr(),
+            dt.nodes_by_level.data_ptr(), dt.node_level.data_ptr(), None, i64, fsorted.data_ptr(),
             far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
             counts.data_ptr(), node_off.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
             fws.numel(), _lib.stream_ptr(dev)), "far lists sort")
