@@ -227,7 +227,9 @@ def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
     # slot memory the fixed-capacity method would need on this tree: one row
     # of the largest count per node (far) and per leaf (near), int32 entries
     max_far, max_near = int(pi.counts.max()), int(pn.counts.max())
-    # far and near stream bytes (32-bit key + 32-bit value per entry)
+    # far and near stream bytes: a 32-bit key and value per entry, or per
+    # pair record (two entries) when the walk writes records
+    bpe = plain_walk["stream_bytes_per_entry"]
     return dict(plain_walk_ms=round(t_plain, 3), count_pass_walk_ms=round(t_count, 3),
                 learned_walk_ms=round(t_learn, 3),
                 max_far_per_node=max_far, max_near_per_leaf=max_near,
@@ -238,10 +240,10 @@ def learned_leg(ygg, torch, pos, mass, leaf, theta, steps):
                 fallback_walks=fallbacks, hint=hint, true_far=nf, true_near=nn,
                 far_overalloc=round(hint["far"] / max(1, nf), 3),
                 near_overalloc=round(hint["near"] / max(1, nn), 3),
-                plain_stream_bytes=8 * (plain_walk["far_capacity"] + plain_walk["near_capacity"]),
+                plain_stream_bytes=plain_walk["stream_bytes"],
                 plain_stream_growths=plain_walk["stream_growths"],
-                learned_stream_bytes=8 * (hint["far"] + hint["near"]),
-                count_pass_stream_bytes=8 * (nf + nn))
+                learned_stream_bytes=bpe * (hint["far"] + hint["near"]),
+                count_pass_stream_bytes=bpe * (nf + nn))
 
 
 _MIX_C = 0x9E3779B97F4A7C15 - (1 << 64)
@@ -455,10 +457,10 @@ def batched_leg(torch, steps, warmup):
                 learned_fallback_walks=learned_fallbacks,
                 learned_speedup_vs_batched_plain=round(t_b / t_l, 3),
                 learned_speedup_vs_batched_count_pass=round(t_c / t_l, 3),
-                plain_stream_bytes=8 * (plain_walk["far_capacity"]
-                                        + plain_walk["near_capacity"]),
+                plain_stream_bytes=plain_walk["stream_bytes"],
                 plain_stream_growths=plain_walk["stream_growths"],
-                learned_stream_bytes=8 * (hint["far"] + hint["near"]),
+                learned_stream_bytes=plain_walk["stream_bytes_per_entry"] * (hint["far"]
+                                                                             + hint["near"]),
                 graph_replay_ms=round(t_g, 3), graph_replay_mpts_s=round(n / t_g / 1e3, 3),
                 graph_replay_positions="new positions every step (N(0, 1e-4) displacement)",
                 graph_capacities=dict(graph.caps, generations=graph.generations,

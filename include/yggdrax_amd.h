@@ -501,7 +501,12 @@ typedef struct YggFlatWalkParams {
                                  walk. NULL: overflowing entries set flags[1] / flags[2]. */
   int32_t start_generation;   /* 0: a new walk; g > 0: resume a paused walk at generation g (the
                                  other device buffers as the paused call left them) */
-  int32_t pad1;
+  int32_t pair_records;       /* emission walk without a policy: nonzero writes one record per
+                                 accepted or near pair, the node ids (t, s), in place of the
+                                 pair's two entries (t's row key, s) and (s's row key, t).
+                                 far_cap / near_cap, the counters, the gen_fill counts and the
+                                 overflow flags then count records. ygg_far_lists_sort /
+                                 ygg_near_lists_sort list them (pair_records / leaf_nodes). */
   const uint32_t* gen_pairs_hint; /* optional, host memory: [max_generations] pairs expected in
                                  each generation (e.g. a previous walk's sizes on a similar
                                  tree). It sizes each generation's grid (1.5x the hinted chunks,
@@ -567,21 +572,34 @@ int ygg_near_lists_from_sorted(int64_t num_leaves, const int64_t* row_offsets,
  * `row_sub` (optional, device memory, [rows]) is taken off every value of
  * row r, and off the far target of row r: ids local to a block of nodes,
  * as for the systems of a batched forest (bucketing only: with the stable
- * sort it returns YGG_ERR_UNSUPPORTED). */
+ * sort it returns YGG_ERR_UNSUPPORTED).
+ * Pair records (bucketing only): with pair_records nonzero (far) or
+ * leaf_nodes given (near, device memory, [num_leaves]: the node of each
+ * leaf row, i.e. leaf_indices), keys and vals hold n records (t, s) of
+ * node ids from the walk's pair_records mode, each listing s in t's row
+ * and t in s's row. Far rows are taken in level-major order through
+ * nodes_by_level, near rows in leaf order through leaf_nodes; the near
+ * keys then span end_bit = the node ids' bits. `count` counts records, the
+ * outputs and key_scratch hold 2n entries, the workspace is
+ * ygg_*_lists_sort_workspace_bytes(2n, ...), and row_sub is indexed by
+ * node id, over the whole key range (num_nodes entries for the far lists,
+ * 2^end_bit for the near lists). The far targets still take row_sub[rank]:
+ * a batched forest's per-system offsets are the same array, each system's
+ * nodes and ranks sharing one index range. */
 size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                        const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,
                        const int32_t* nodes_by_level, const int32_t* node_level,
-                       const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
-                       int64_t* lm_offsets, void* sources, void* targets, void* target_levels,
-                       void* counts, void* node_offsets, int32_t* counts32, void* ws,
-                       size_t ws_bytes, void* stream);
+                       const int64_t* row_sub, int pair_records, int out_i64,
+                       uint32_t* key_scratch, int64_t* lm_offsets, void* sources, void* targets,
+                       void* target_levels, void* counts, void* node_offsets, int32_t* counts32,
+                       void* ws, size_t ws_bytes, void* stream);
 size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits);
 int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                         const int64_t* count, int64_t num_leaves, int end_bit, int digit_bits,
-                        const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
-                        int64_t* row_offsets, void* neighbors, void* counts, void* offsets,
-                        void* ws, size_t ws_bytes, void* stream);
+                        const int64_t* row_sub, const int32_t* leaf_nodes, int out_i64,
+                        uint32_t* key_scratch, int64_t* row_offsets, void* neighbors,
+                        void* counts, void* offsets, void* ws, size_t ws_bytes, void* stream);
 /* The dense -1 padded [L][max_count] neighbour leaf rows from a near list
  * (neighbors int64 if out_i64, else int32). */
 int ygg_near_dense_from_list(int64_t num_leaves, const int64_t* row_offsets,

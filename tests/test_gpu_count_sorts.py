@@ -36,7 +36,7 @@ def _far(lib, keys, vals, n, count, T, nbl, lvl):
     ws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(n, fend, 10), dev)
     _lib.check(lib.ygg_far_lists_sort(
         keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), T, fend, 10, nbl.data_ptr(),
-        lvl.data_ptr(), None, 1, out["sorted"].data_ptr(), out["lm"].data_ptr(),
+        lvl.data_ptr(), None, 0, 1, out["sorted"].data_ptr(), out["lm"].data_ptr(),
         out["sources"].data_ptr(), out["targets"].data_ptr(), out["levels"].data_ptr(),
         out["counts"].data_ptr(), out["node_off"].data_ptr(), out["counts32"].data_ptr(),
         ws.data_ptr(), ws.numel(), _lib.stream_ptr(dev)), "far lists sort")
@@ -54,7 +54,7 @@ def _near(lib, keys, vals, n, count, L):
                sorted=torch.empty(max(1, n), dtype=torch.int32, device=dev))
     ws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(n, nend, 9), dev)
     _lib.check(lib.ygg_near_lists_sort(
-        keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), L, nend, 9, None, 1,
+        keys.data_ptr(), vals.data_ptr(), n, _lib.ptr(count), L, nend, 9, None, None, 1,
         out["sorted"].data_ptr(), out["rows"].data_ptr(), out["neighbors"].data_ptr(),
         out["counts"].data_ptr(), out["offsets"].data_ptr(), ws.data_ptr(), ws.numel(),
         _lib.stream_ptr(dev)), "near lists sort")

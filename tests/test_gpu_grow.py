@@ -2,8 +2,10 @@
 run from its default capacities and from starting capacities small enough
 that the walk pauses and grows its far stream, its near stream or both many
 times; the reference's count pass (a counting walk, then the emission walk
-at the exact totals) is run beside them. Every variant gives the oracle's
-lists, single system and batch, and only the resumable walks grow."""
+at the exact totals) is run beside them. Each runs with the walk writing
+pair records (one per accepted or near pair, the default) and writing the
+two entries of each pair. Every variant gives the oracle's lists, single
+system and batch, and only the resumable walks grow."""
 
 from __future__ import annotations
 
@@ -62,13 +64,15 @@ def _sizing(monkeypatch, start):
             monkeypatch.setattr(I, "_start_capacities", lambda T, L: s)
 
 
+@pytest.mark.parametrize("records", [True, False], ids=["records", "entries"])
 @pytest.mark.parametrize("start", list(STARTS))
 @pytest.mark.parametrize("name", list(CASES))
-def test_stream_sizing_matches_oracle(ygg, refs, monkeypatch, name, start):
+def test_stream_sizing_matches_oracle(ygg, refs, monkeypatch, name, start, records):
     from yggdrax_amd import interactions as I
     _, kind, leaf, theta, mac = CASES[name]
     pos, mass, rint, rnb = refs[name]
-    print(f"[grow] case={name} start={start}", flush=True)
+    print(f"[grow] case={name} start={start} records={records}", flush=True)
+    monkeypatch.setattr(I, "PAIR_RECORDS", records)
     gbuild = ygg.build_octree if kind == "octree" else ygg.build_tree
     tree, ps, _, _ = gbuild(torch.from_numpy(pos).cuda(), torch.from_numpy(mass).cuda(),
                             leaf_size=leaf, return_reordered=True)
@@ -80,6 +84,7 @@ def test_stream_sizing_matches_oracle(ygg, refs, monkeypatch, name, start):
     parity.assert_interactions_equal(inter, rint)
     parity.assert_neighbors_equal(nbrs, rnb)
     walk = dict(I.LAST_WALK)
+    assert walk["stream_bytes_per_entry"] == (4 if records else 8)
     growths = I.FLAT_STATS["stream_growths"] - before["stream_growths"]
     counts = I.FLAT_STATS["count_walks"] - before["count_walks"]
     print(f"[grow] {walk} growths={growths} count_walks={counts}", flush=True)
@@ -124,9 +129,11 @@ def test_batched_stream_sizings_agree(ygg, monkeypatch):
     bt = batch.build_octree_batched(P, M, offsets, leaf_size=16)
     geom = batch.compute_tree_geometry_batched(bt)
     outs = {}
-    for start in ("default", "both_tiny", "count_pass"):
+    for start in ("default", "both_tiny", "count_pass", "entries"):
         with monkeypatch.context() as m:
-            _sizing(m, start)
+            _sizing(m, "default" if start == "entries" else start)
+            # pair records unless the walk is asked for entries
+            m.setattr(I, "PAIR_RECORDS", start != "entries")
             g0 = I.FLAT_STATS["stream_growths"]
             outs[start] = batch.build_interactions_and_neighbors_batched(bt, geom, 0.6,
                                                                          mac_type="dehnen")
@@ -135,7 +142,7 @@ def test_batched_stream_sizings_agree(ygg, monkeypatch):
             assert (grew >= 2) if start == "both_tiny" else grew == 0
     for k in range(40):
         ref = outs["count_pass"].system(k)
-        for start in ("default", "both_tiny"):
+        for start in ("default", "both_tiny", "entries"):
             got = outs[start].system(k)
             for a, b in zip(ref, got):
                 for f in ("counts", "offsets", "targets", "level_offsets", "leaf_indices"):

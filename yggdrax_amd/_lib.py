@@ -66,7 +66,7 @@ class FlatWalkParams(ctypes.Structure):
         ("near_keys", _P), ("near_vals", _P), ("hist_gen", _I32), ("xcd_groups", _I32),
         ("node_hist", _P), ("roots", _P), ("num_roots", _I64), ("spread_counters", _P),
         ("spread_lanes", _I64), ("policy", _P), ("far_tags", _P), ("gen_fill", _P),
-        ("start_generation", _I32), ("pad1", _I32), ("gen_pairs_hint", _P),
+        ("start_generation", _I32), ("pair_records", _I32), ("gen_pairs_hint", _P),
     ]
 
 
@@ -125,11 +125,11 @@ _SIGS = {
     "ygg_morton_sort": (_INT, [_P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "ygg_morton_sort_noalloc": (_INT, [_P, _I64, _P, _P, _P, _P]),
     "ygg_far_lists_sort_workspace_bytes": (_SZ, [_I64, _INT, _INT]),
-    "ygg_far_lists_sort": (_INT, [_P, _P, _I64, _P, _I64, _INT, _INT, _P, _P, _P, _INT, _P, _P,
-                                  _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "ygg_far_lists_sort": (_INT, [_P, _P, _I64, _P, _I64, _INT, _INT, _P, _P, _P, _INT, _INT, _P,
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
     "ygg_near_lists_sort_workspace_bytes": (_SZ, [_I64, _INT, _INT]),
-    "ygg_near_lists_sort": (_INT, [_P, _P, _I64, _P, _I64, _INT, _INT, _P, _INT, _P, _P, _P,
-                                   _P, _P, _P, _SZ, _P]),
+    "ygg_near_lists_sort": (_INT, [_P, _P, _I64, _P, _I64, _INT, _INT, _P, _P, _INT, _P, _P,
+                                   _P, _P, _P, _P, _SZ, _P]),
     "ygg_near_dense_from_list": (_INT, [_I64, _P, _P, _P, _I64, _INT, _P, _P]),
     "ygg_target_block_offsets_workspace_bytes": (_SZ, [_I64]),
     "ygg_target_block_offsets": (_INT, [_I64, _P, _I64, _P, _P, _SZ, _P]),

@@ -511,7 +511,8 @@ def build_interactions_and_neighbors_batched(trees: BatchedTrees, geom: BatchedG
     def lists(em):
         out, fk, far_off, near_off = _lists_from_streams(
             trees, li, em["far_keys"], em["far_vals"], em["near_keys"], em["near_vals"],
-            em["far_total"], em["near_total"], stats)
+            em["far_total"], em["near_total"], stats,
+            pair_records=bool(em.get("pair_records")))
         return out, None, fk, far_off, near_off
     fr = _run_walk_flat(ctx, dt, lad, sizes=capacity_hint, strict_int32=False, need_max=False,
                         fused_lists=lists)
@@ -536,11 +537,17 @@ class _LayoutIndex:
         self.node_sub = self.node_off[self.sys_of_node].contiguous()
         self.leaf_sub = torch.repeat_interleave(
             self.node_off, torch.from_numpy(np.diff(lay.leaf_off)).to(dev)).contiguous()
+        # node_sub over the whole key range of node ids (the near lists of
+        # pair records are keyed by node id; ids past the last node are no
+        # key and take 0)
+        keys = 1 << max(1, (lay.num_nodes - 1).bit_length())
+        self.node_sub_keys = torch.cat([self.node_sub, torch.zeros(
+            keys - self.node_sub.numel(), dtype=self.node_sub.dtype, device=dev)])
 
 
 def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_vals, near_keys,
                         near_vals, nf: int, nn: int, stats: dict, far_count=None,
-                        near_count=None, concurrent: bool = True):
+                        near_count=None, concurrent: bool = True, pair_records: bool = False):
     """Per-system far/near lists from the forest walk's emission streams.
     ygg_far_lists_sort (on a side stream) and ygg_near_lists_sort (on the
     current one) group the entries by row into the lists; the bucketing
@@ -551,7 +558,10 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
     device scalars) hold the valid entry counts of streams allocated at
     capacities nf / nn; without them all nf / nn entries are valid. With
     `concurrent` False everything runs on the current stream (stream
-    capture).
+    capture). With `pair_records` the streams hold the walk's pair records
+    (ceil(nf / 2) and ceil(nn / 2) slots, the counts counting records),
+    expanded by the bucketing through the batch-global row key -> node
+    tables.
     Returns (BatchedLists, the far lists' key scratch (contents
     unspecified), far row offsets, near row offsets)."""
     from .interactions import _side_stream, row_lists_digit_bits
@@ -562,6 +572,17 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
     i64 = int(idx == torch.int64)
     T, Lt, S = lay.num_nodes, lay.num_leaves, lay.num_systems
     nbl_global = (trees.rad["nodes_by_level"].long() + li.node_sub).to(torch.int32)
+    rf, rn = nf, nn  # stream slots
+    leaf_nodes = None
+    if pair_records:
+        # record slots, room for both entries of every record, and each
+        # batch leaf row's global node (the near rows' order); records are
+        # keyed by global node id, so the near values' system offsets are
+        # taken by node
+        rf, rn = -(-nf // 2), -(-nn // 2)
+        nf, nn = 2 * rf, 2 * rn
+        leaf_indices = (trees.octa if trees.octree else trees.rad)["leaf_indices"]
+        leaf_nodes = (leaf_indices.long() + li.leaf_sub).to(torch.int32)
     main = torch.cuda.current_stream(dev)
     side = _side_stream(dev) if concurrent else main
     if concurrent:
@@ -581,9 +602,10 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
         # the bucketing writes system-local sources and targets itself
         far_local = fbits == 0
         _lib.check(lib.ygg_far_lists_sort(
-            far_keys.data_ptr(), far_vals.data_ptr(), nf, _lib.ptr(far_count), T, fend, fbits,
+            far_keys.data_ptr(), far_vals.data_ptr(), rf, _lib.ptr(far_count), T, fend, fbits,
             nbl_global.data_ptr(), trees.rad["node_level"].data_ptr(),
-            li.node_sub.data_ptr() if far_local else None, i64, fsorted.data_ptr(),
+            li.node_sub.data_ptr() if far_local else None, int(pair_records), i64,
+            fsorted.data_ptr(),
             far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
             counts.data_ptr(), node_off_glob.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
             fws.numel(), _lib.stream_ptr(dev)), "batch far lists sort")
@@ -594,7 +616,7 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
             trees.rad["level_offsets"].data_ptr(), 0 if far_local else nf, i64,
             sources.data_ptr(), targets.data_ptr(), far_offsets.data_ptr(), far_lvl.data_ptr(),
             _lib.stream_ptr(dev)), "batch far localize")
-    nend = max(1, (Lt - 1).bit_length())
+    nend = max(1, ((T if pair_records else Lt) - 1).bit_length())
     nbits = row_lists_digit_bits(nend, nn)
     neighbors = torch.empty(nn, dtype=idx, device=dev)
     ncounts = torch.empty(Lt, dtype=idx, device=dev)
@@ -605,8 +627,10 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
     st = _lib.stream_ptr(dev)
     near_local = nbits == 0  # system-local neighbours from the bucketing itself
     _lib.check(lib.ygg_near_lists_sort(
-        near_keys.data_ptr(), near_vals.data_ptr(), nn, _lib.ptr(near_count), Lt, nend, nbits,
-        li.leaf_sub.data_ptr() if near_local else None, i64, nsorted.data_ptr(),
+        near_keys.data_ptr(), near_vals.data_ptr(), rn, _lib.ptr(near_count), Lt, nend, nbits,
+        (li.node_sub_keys if pair_records else li.leaf_sub).data_ptr() if near_local else None,
+        _lib.ptr(leaf_nodes), i64,
+        nsorted.data_ptr(),
         near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(), noff_glob.data_ptr(),
         nws.data_ptr(), nws.numel(), st), "batch near lists sort")
     near_offsets = torch.empty(Lt + S, dtype=idx, device=dev)
@@ -626,10 +650,12 @@ def _lists_from_streams(trees: BatchedTrees, li: _LayoutIndex, far_keys, far_val
 
 def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_type: str,
                 dehnen_radius_scale: float, caps: dict, generations: int, grid_blocks: int,
-                pairs_per_thread: int):
+                pairs_per_thread: int, pair_records: bool = False):
     """Forest walk with fixed stream capacities and generation count and no
     host synchronisation (stream-capturable). Returns the raw emission
-    streams and the walk's device-side totals, which bound the row sorts."""
+    streams and the walk's device-side totals, which bound the row sorts.
+    With `pair_records` the streams hold one record per pair (capacities
+    caps / 2, rounded up; the totals count records)."""
     from .interactions import NODE_RECORD_BYTES
     lib = _lib.load()
     lay = trees.layout
@@ -650,6 +676,8 @@ def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_ty
         ctypes.c_float(float(dehnen_radius_scale)).value, nodes.data_ptr(), roots.data_ptr(), st),
         "graph walk nodes")
     Q, FC, NC, G = int(caps["queue"]), int(caps["far"]), int(caps["near"]), int(generations)
+    if pair_records:
+        FC, NC = -(-FC // 2), -(-NC // 2)
     wf = torch.empty((2, Q), dtype=torch.int64, device=dev)
     counters = torch.empty(2, dtype=torch.int64, device=dev)
     gen_sizes = torch.empty(G + 1, dtype=torch.int32, device=dev)
@@ -672,6 +700,7 @@ def _graph_walk(trees: BatchedTrees, geom: BatchedGeometry, theta: float, mac_ty
     from .interactions import WALK_XCD_GROUPS
     p.xcd_groups = WALK_XCD_GROUPS
     p.pairs_per_thread = int(pairs_per_thread)
+    p.pair_records = int(pair_records)
     _lib.check(lib.ygg_walk_flat_fixed(ctypes.byref(p), st), "graph walk")
     return dict(far_keys=fk, far_vals=fv, near_keys=nk, near_vals=nv, counters=counters,
                 gen_sizes=gen_sizes, flags=flags, nodes=nodes, wf=wf)
@@ -709,7 +738,7 @@ class BatchedGraph:
                  capacities: Optional[dict] = None, generations: Optional[int] = None,
                  tune: bool = False):
         from . import autotune
-        from .interactions import WALK_PAIRS_PER_THREAD, row_lists_digit_bits
+        from .interactions import WALK_PAIRS_PER_THREAD, pair_records_ok, row_lists_digit_bits
         _lib.require_device(positions)
         dev = positions.device
         self.theta, self.mac_type, self.scale = float(theta), mac_type, float(dehnen_radius_scale)
@@ -744,6 +773,7 @@ class BatchedGraph:
         self.generations = (max(1, int(generations)) if generations is not None
                             else int(gens * slack) + 4)
         T, Lt = self.lay.num_nodes, self.lay.num_leaves
+        self.pair_records = pair_records_ok(T)
         if tune:
             # the graph looks its row-sort plans up by the stream capacities
             autotune.enable(True)
@@ -785,12 +815,12 @@ class BatchedGraph:
                             return_reordered=True, layout=self.lay, sync=False)
         geom = compute_tree_geometry_batched(bt)
         w = _graph_walk(bt, geom, self.theta, self.mac_type, self.scale, self.caps,
-                        self.generations, self.walk_grid, self.walk_tile)
+                        self.generations, self.walk_grid, self.walk_tile, self.pair_records)
         # the sorts and list kernels stop at the walk's device-side totals
         lists, _, far_off, near_off = _lists_from_streams(
             bt, self.li, w["far_keys"], w["far_vals"], w["near_keys"], w["near_vals"],
             self.caps["far"], self.caps["near"], {}, far_count=w["counters"][0:1],
-            near_count=w["counters"][1:2], concurrent=False)
+            near_count=w["counters"][1:2], concurrent=False, pair_records=self.pair_records)
         lay = self.lay
         T, Lt = lay.num_nodes, lay.num_leaves
         status = torch.cat([
@@ -815,6 +845,8 @@ class BatchedGraph:
         st = self.out["status"].cpu().numpy()
         flags, unfinished = st[:3], st[3]
         far_total, near_total = int(st[4]), int(st[5])     # stream totals of the walk
+        if self.pair_records:  # records: two entries each
+            far_total, near_total = 2 * far_total, 2 * near_total
         rows_far, rows_near = int(st[6]), int(st[7])       # entries the row sorts listed
         scal = st[8:8 + 8 * S].reshape(S, 8)
         osc = st[8 + 8 * S:8 + 12 * S].reshape(S, 4)

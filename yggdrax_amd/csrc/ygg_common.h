@@ -301,7 +301,9 @@ int sort_near_rows_fused(const uint32_t* keys_in, const uint32_t* vals_in,
 // row's entries into row_counts[rows] (zero before the call) and fills
 // `state`; after the caller's scan of the counts into row_offsets,
 // row_bucket_scatter writes every value at its row's place. Keys >= rows
-// are dropped. vals_scratch[n] holds the split's values.
+// are dropped. vals_scratch[entries] holds the split's values. With
+// `records` the input is n pair records (a, b), each standing for the
+// entries (a, b) and (b, a); the workspace is then sized for 2n entries.
 struct RowBucketState {
   int64_t n;  // entries (capacity)
   const uint32_t* keys;
@@ -319,10 +321,10 @@ struct RowBucketState {
 };
 bool row_bucket_split(int end_bit, int* hi_bits, int* lo_bits);
 size_t row_bucket_workspace_bytes(int64_t n, int end_bit);
-int row_bucket_count(const uint32_t* keys, const uint32_t* vals, int64_t n, const int64_t* n_dev,
-                     int end_bit, int64_t rows, int64_t val_bound, uint32_t* vals_scratch,
-                     uint32_t* row_counts, void* ws, size_t ws_bytes, hipStream_t st,
-                     RowBucketState* state);
+int row_bucket_count(const uint32_t* keys, const uint32_t* vals, bool records, int64_t n,
+                     const int64_t* n_dev, int end_bit, int64_t rows, int64_t val_bound,
+                     uint32_t* vals_scratch, uint32_t* row_counts, void* ws, size_t ws_bytes,
+                     hipStream_t st, RowBucketState* state);
 int row_bucket_scatter(const RowBucketState& s, const int64_t* row_offsets,
                        const int64_t* row_sub, int out_i64, void* out, hipStream_t st);
 // Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:

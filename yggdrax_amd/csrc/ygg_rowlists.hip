@@ -32,6 +32,12 @@
 //                      at row offset + tile slot.
 //
 // Row keys of at most 11 bits skip 1 and 3: the input itself is one bucket.
+//
+// Pair records: the input may hold n records (a, b), each standing for the
+// two entries (a, b) and (b, a) (the walk writes one record per accepted or
+// near pair: the two node ids). The histogram counts both keys and the
+// split writes both entries, so every later step sees 2n entries; the
+// split runs (one bucket) also for keys of at most 11 bits.
 // Traffic per entry with packed values: 4 B (histogram) + 8 B read and 4 B
 // written (split) + 4 B (reserve) + 4 B read and 4 or 8 B written
 // (scatter); 32 B with int64 lists, against 36 B for two stable passes
@@ -99,7 +105,9 @@ __device__ __forceinline__ int tile_bucket(const uint32_t* tstart, int nb, uint3
   return lo;
 }
 
-__global__ __launch_bounds__(256) void k_row_hist(const uint32_t* __restrict__ keys, int64_t n,
+// keys2: the second keys of pair records (counted too), or null.
+__global__ __launch_bounds__(256) void k_row_hist(const uint32_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ keys2, int64_t n,
                                                   const int64_t* __restrict__ n_dev,
                                                   uint32_t rows, int lo_bits, int nb,
                                                   uint32_t* __restrict__ hist) {
@@ -111,15 +119,18 @@ __global__ __launch_bounds__(256) void k_row_hist(const uint32_t* __restrict__ k
   constexpr int kUnroll = 4;
   const int64_t step = (int64_t)gridDim.x * blockDim.x * kUnroll;
   for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x * kUnroll; b0 < n; b0 += step) {
-    uint32_t k[kUnroll];
+    uint32_t k[kUnroll], k2[kUnroll];
 #pragma unroll
     for (int j = 0; j < kUnroll; ++j) {
       const int64_t i = b0 + (int64_t)j * blockDim.x + threadIdx.x;
       k[j] = i < n ? keys[i] : rows;
+      k2[j] = keys2 && i < n ? keys2[i] : rows;
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j)
+    for (int j = 0; j < kUnroll; ++j) {
       if (k[j] < rows) atomicAdd(&h[k[j] >> lo_bits], 1u);
+      if (k2[j] < rows) atomicAdd(&h[k2[j] >> lo_bits], 1u);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += blockDim.x)
@@ -178,7 +189,9 @@ enum RowMode { kRowDirect = 0, kRowLo16 = 1, kRowPacked = 2 };
 // has staged its entries in LDS, so the atomics' round trip overlaps that
 // work. Ranges are disjoint and fill each bucket; their order is the order
 // the tiles reach the cursor, which the lists do not depend on.
-template <int THREADS, int BINS, bool kPacked>
+// kPairs: the input is n pair records, half a tile of records per
+// workgroup.
+template <int THREADS, int BINS, bool kPacked, bool kPairs>
 __global__ __launch_bounds__(THREADS) void k_row_split(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
     const int64_t* __restrict__ n_dev, uint32_t rows, int lo_bits, int nb,
@@ -195,19 +208,34 @@ __global__ __launch_bounds__(THREADS) void k_row_split(
   __shared__ uint16_t s_lo[kPacked ? 1 : kTile];
   __shared__ uint16_t s_dig[kTile];
   const int tid = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t base = (int64_t)blockIdx.x * (kPairs ? kTile / 2 : kTile);
   n = effective_count(n, n_dev);
   if (base >= n) return;  // the same for every thread
   for (int i = tid; i < nb; i += THREADS) s_cnt[i] = 0;
   __syncthreads();
   const uint32_t lmask = (1u << lo_bits) - 1u;
   uint32_t key[kRowItems], val[kRowItems], rank[kRowItems];
+  if (kPairs) {
 #pragma unroll
-  for (int k = 0; k < kRowItems; ++k) {
-    const int64_t idx = base + (int64_t)k * THREADS + tid;
-    const bool ok = idx < n;
-    key[k] = ok ? keys[idx] : rows;
-    val[k] = ok ? vals[idx] : 0u;
+    for (int k = 0; k < kRowItems / 2; ++k) {
+      const int64_t idx = base + (int64_t)k * THREADS + tid;
+      const bool ok = idx < n;
+      key[2 * k] = ok ? keys[idx] : rows;
+      key[2 * k + 1] = ok ? vals[idx] : rows;
+    }
+#pragma unroll
+    for (int k = 0; k < kRowItems / 2; ++k) {
+      val[2 * k] = key[2 * k + 1];
+      val[2 * k + 1] = key[2 * k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRowItems; ++k) {
+      const int64_t idx = base + (int64_t)k * THREADS + tid;
+      const bool ok = idx < n;
+      key[k] = ok ? keys[idx] : rows;
+      val[k] = ok ? vals[idx] : 0u;
+    }
   }
 #pragma unroll
   for (int k = 0; k < kRowItems; ++k)
@@ -494,6 +522,26 @@ void launch_row_bucket_any(const RowBucketState& s, uint32_t* row_counts,
   }
 }
 
+template <bool kPairs>
+void launch_row_split(int nb, bool packed, unsigned tiles, const uint32_t* keys,
+                      const uint32_t* vals, int64_t n, const int64_t* n_dev, uint32_t rows,
+                      int lo, const uint32_t* bstart, uint32_t* cursor, uint16_t* lo16,
+                      uint32_t* vals_out, hipStream_t st) {
+  dim3 sg(tiles), sb(kSplitThreads);
+  if (nb <= 1024 && packed)
+    hipLaunchKernelGGL((k_row_split<kSplitThreads, 1024, true, kPairs>), sg, sb, 0, st, keys,
+                       vals, n, n_dev, rows, lo, nb, bstart, cursor, lo16, vals_out);
+  else if (nb <= 1024)
+    hipLaunchKernelGGL((k_row_split<kSplitThreads, 1024, false, kPairs>), sg, sb, 0, st, keys,
+                       vals, n, n_dev, rows, lo, nb, bstart, cursor, lo16, vals_out);
+  else if (packed)
+    hipLaunchKernelGGL((k_row_split<kSplitThreads, 2048, true, kPairs>), sg, sb, 0, st, keys,
+                       vals, n, n_dev, rows, lo, nb, bstart, cursor, lo16, vals_out);
+  else
+    hipLaunchKernelGGL((k_row_split<kSplitThreads, 2048, false, kPairs>), sg, sb, 0, st, keys,
+                       vals, n, n_dev, rows, lo, nb, bstart, cursor, lo16, vals_out);
+}
+
 }  // namespace
 
 // (high, low) digit widths of a row-key range of end_bit bits; false when
@@ -516,10 +564,10 @@ size_t row_bucket_workspace_bytes(int64_t n, int end_bit) {
   const int64_t nb = (int64_t)1 << hi;
   const int64_t tiles2 = (n + kBucketTile - 1) / kBucketTile + nb;
   Arena a(nullptr, 0);
-  if (hi > 0) {
-    a.take<uint16_t>((size_t)n);                        // low digits after the split
-    a.take<uint32_t>((size_t)nb);                       // bucket counts, then cursors
-  }
+  // the split's scratch, also for keys of at most 11 bits (pair records
+  // are split into one bucket)
+  a.take<uint16_t>((size_t)n);                          // low digits after the split
+  a.take<uint32_t>((size_t)nb);                         // bucket counts, then cursors
   a.take<uint32_t>((size_t)nb + 1);                     // bucket starts
   a.take<uint32_t>((size_t)nb + 1);                     // tile starts
   a.take<uint32_t>((size_t)(tiles2 << lo));             // tiles' first slots per row
@@ -527,71 +575,66 @@ size_t row_bucket_workspace_bytes(int64_t n, int end_bit) {
 }
 
 // Steps 1-4 (the row counts, added into row_counts[rows], zero before the
-// call). The split's values go to vals_scratch[n], packed with the low
-// digits when every value is below val_bound <= 2^(32 - lo_bits)
-// (val_bound 0: no bound known). `state` carries what the scatter needs.
-int row_bucket_count(const uint32_t* keys, const uint32_t* vals, int64_t n,
+// call). The split's values go to vals_scratch[entries], packed with the
+// low digits when every value is below val_bound <= 2^(32 - lo_bits)
+// (val_bound 0: no bound known). With `records` the input is n pair
+// records (2n entries; n_dev counts records), else n entries. The
+// workspace is row_bucket_workspace_bytes of the entry count. `state`
+// carries what the scatter needs.
+int row_bucket_count(const uint32_t* keys, const uint32_t* vals, bool records, int64_t n,
                      const int64_t* n_dev, int end_bit, int64_t rows, int64_t val_bound,
                      uint32_t* vals_scratch, uint32_t* row_counts, void* ws, size_t ws_bytes,
                      hipStream_t st, RowBucketState* state) {
   int hi = 0, lo = 0;
+  const int64_t ne = records ? 2 * n : n;  // entries
   if (!row_bucket_split(end_bit, &hi, &lo) || rows < 1 || rows > ((int64_t)1 << end_bit) ||
-      n < 0 || n > kRowMaxEntries || val_bound < 0 || !state)
+      n < 0 || ne > kRowMaxEntries || val_bound < 0 || !state)
     return YGG_ERR_ARG;
+  const bool split = hi > 0 || records;
   const int64_t nbmax = (int64_t)1 << hi;
-  const int64_t tiles1 = (n + kSplitTile - 1) / kSplitTile;
-  const int64_t tiles2 = (n + kBucketTile - 1) / kBucketTile + nbmax;
+  const int64_t tiles1 = (ne + kSplitTile - 1) / kSplitTile;
+  const int64_t tiles2 = (ne + kBucketTile - 1) / kBucketTile + nbmax;
   Arena a(ws, ws_bytes);
-  uint16_t* lo16 = nullptr;
-  uint32_t* hist = nullptr;
-  if (hi > 0) {
-    lo16 = a.take<uint16_t>((size_t)n);
-    hist = a.take<uint32_t>((size_t)nbmax);
-    if (!lo16 || !hist || !vals_scratch) return YGG_ERR_WORKSPACE;
-  }
+  uint16_t* lo16 = a.take<uint16_t>((size_t)ne);
+  uint32_t* hist = a.take<uint32_t>((size_t)nbmax);
+  if (!lo16 || !hist || (split && !vals_scratch)) return YGG_ERR_WORKSPACE;
   uint32_t* bstart = a.take<uint32_t>((size_t)nbmax + 1);
   uint32_t* tstart = a.take<uint32_t>((size_t)nbmax + 1);
   uint32_t* bases = a.take<uint32_t>((size_t)(tiles2 << lo));
   if (!bstart || !tstart || !bases) return YGG_ERR_WORKSPACE;
-  const bool packed = hi > 0 && val_bound > 0 && val_bound <= ((int64_t)1 << (32 - lo));
+  const bool packed = split && val_bound > 0 && val_bound <= ((int64_t)1 << (32 - lo));
   // buckets the row range reaches
   const int nb = hi > 0 ? (int)(((rows - 1) >> lo) + 1) : 1;
   const uint32_t r32 = (uint32_t)rows;
   int rc;
-  if (hi > 0) {
+  if (split) {
     if ((rc = fill_u32(hist, 0u, nb, st))) return rc;
     hipLaunchKernelGGL(k_row_hist, dim3(grid_for(n > 0 ? n : 1, 1024, 2048)), dim3(256), 0, st,
-                       keys, n, n_dev, r32, lo, nb, hist);
+                       keys, records ? vals : nullptr, n, n_dev, r32, lo, nb, hist);
     YGG_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_row_tiles, dim3(1), dim3(1024), 0, st, hist, nb, n, n_dev, bstart, tstart);
+  hipLaunchKernelGGL(k_row_tiles, dim3(1), dim3(1024), 0, st, split ? hist : nullptr, nb, n,
+                     n_dev, bstart, tstart);
   YGG_LAUNCH_CHECK();
-  if (hi > 0 && tiles1 > 0) {
-    dim3 sg((unsigned)tiles1), sb(kSplitThreads);
-    if (nb <= 1024 && packed)
-      hipLaunchKernelGGL((k_row_split<kSplitThreads, 1024, true>), sg, sb, 0, st, keys, vals, n,
-                         n_dev, r32, lo, nb, bstart, hist, lo16, vals_scratch);
-    else if (nb <= 1024)
-      hipLaunchKernelGGL((k_row_split<kSplitThreads, 1024, false>), sg, sb, 0, st, keys, vals, n,
-                         n_dev, r32, lo, nb, bstart, hist, lo16, vals_scratch);
-    else if (packed)
-      hipLaunchKernelGGL((k_row_split<kSplitThreads, 2048, true>), sg, sb, 0, st, keys, vals, n,
-                         n_dev, r32, lo, nb, bstart, hist, lo16, vals_scratch);
+  if (split && tiles1 > 0) {
+    if (records)
+      launch_row_split<true>(nb, packed, (unsigned)tiles1, keys, vals, n, n_dev, r32, lo,
+                             bstart, hist, lo16, vals_scratch, st);
     else
-      hipLaunchKernelGGL((k_row_split<kSplitThreads, 2048, false>), sg, sb, 0, st, keys, vals, n,
-                         n_dev, r32, lo, nb, bstart, hist, lo16, vals_scratch);
+      launch_row_split<false>(nb, packed, (unsigned)tiles1, keys, vals, n, n_dev, r32, lo,
+                              bstart, hist, lo16, vals_scratch, st);
     YGG_LAUNCH_CHECK();
   }
-  state->n = n;
-  state->keys = hi > 0 ? nullptr : keys;
-  state->lo16 = hi > 0 && !packed ? lo16 : nullptr;
-  state->vals = hi > 0 ? vals_scratch : vals;
+  state->n = ne;
+  state->keys = split ? nullptr : keys;
+  state->lo16 = split && !packed ? lo16 : nullptr;
+  state->vals = split ? vals_scratch : vals;
   state->packed = packed;
   state->rows = r32;
   state->lo_bits = lo;
   state->buckets = nb;
   // at most one partial tile per bucket beyond the full ones
-  state->grid = (unsigned)((n + kBucketTile - 1) / kBucketTile + nb);
+  state->grid = (unsigned)((ne + kBucketTile - 1) / kBucketTile + nb);
   state->bstart = bstart;
   state->tstart = tstart;
   state->tile_bases = bases;

@@ -44,6 +44,10 @@ struct FlatArgs {
   const int32_t* roots; // forest seeds (r, r), or null for the single root
   int64_t num_roots;
   int32_t xcd_groups;   // > 1: chunk ranges per XCD (see k_walk_flat)
+  // Emission walk without tags: one record per accepted or near pair, the
+  // two node ids (t, s), instead of its two entries (t's row key, s) and
+  // (s's row key, t); the caps and counts are then in records.
+  int32_t pair_records;
   unsigned long long* spread;  // counting walk: [2 * spread_lanes] far/near totals, or null
   int32_t spread_lanes;
   YggPolicyDesc policy;  // accept / near rules (the built-in MAC and leaf pairs by default)
@@ -120,7 +124,7 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
     }
     __syncthreads();
     if (c_first >= c_end) return;
-    const int64_t worst = 2 * size;
+    const int64_t worst = a.pair_records ? size : 2 * size;
     if ((int64_t)s_gbase[0] + worst > a.far_cap || (int64_t)s_gbase[1] + worst > a.near_cap) {
       if (threadIdx.x == 0) atomicMax(&a.flags[3], (uint32_t)(g + 1));
       return;
@@ -228,6 +232,7 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
       else if (a.policy.accept == YGG_ACCEPT_NEVER)
         acc = false;
       const bool near_rule = a.policy.near == YGG_NEAR_LEAF_PAIRS;
+      const unsigned per_pair = !TAGS && a.pair_records ? 1u : 2u;  // stream slots
       if (!TAGS) {
         // Every field from selects, with no per-pair branch: accept (1),
         // near (2) or a split (3 both of the same node, 4 both, 5 t only,
@@ -246,8 +251,8 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         x1[k] = (uint32_t)(fa ? ns.pad : (fn ? ns.leafpos : nt.right));
         x2[k] = (uint32_t)ns.left;
         x3[k] = (uint32_t)ns.right;
-        nf += fa ? 2u : 0u;
-        nn += fn ? 2u : 0u;
+        nf += fa ? per_pair : 0u;
+        nn += fn ? per_pair : 0u;
         nc += m == 3 ? 3u : (m == 4 ? 4u : (m >= 5 ? 2u : 0u));
         continue;
       }
@@ -338,14 +343,18 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
     unsigned lf = (unsigned)((lex >> 21) & mask);
     unsigned ln = wave_nf + (unsigned)((lex >> 42) & mask);
     bool qover = false, fover = false, nover = false;
+    const bool records = !TAGS && a.pair_records;
+    const unsigned per_pair = records ? 1u : 2u;
 #pragma unroll
     for (int k = 0; k < kFlatPer; ++k) {
       const int m = mode[k];
       const bool isf = m == 1, isn = m == 2;
       if (EMIT && !TAGS) {
+        // entries (t's row key, s) and (s's row key, t), or the record
+        // (t, s) with the second store going to the spare slot
         const unsigned dst = isf ? lf : (isn ? ln : kDump);
-        s_stage[w][dst] = make_uint2(x0[k], ps[k]);
-        s_stage[w][dst + 1] = make_uint2(x1[k], pt[k]);
+        s_stage[w][dst] = make_uint2(records ? pt[k] : x0[k], ps[k]);
+        s_stage[w][records ? kDump + 1 : dst + 1] = make_uint2(x1[k], pt[k]);
       } else if (EMIT) {
         if (isf) {
           s_stage[w][lf] = make_uint2(x0[k], ps[k]);
@@ -357,8 +366,8 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
           s_stage[w][ln + 1] = make_uint2(x1[k], pt[k]);
         }
       }
-      lf += isf ? 2u : 0u;
-      ln += isn ? 2u : 0u;
+      lf += isf ? per_pair : 0u;
+      ln += isn ? per_pair : 0u;
     }
     if (EMIT) {
       __syncthreads();  // staged entries visible to the whole wavefront
@@ -778,8 +787,11 @@ static void launch_flat(const FlatArgs& a, int k, bool emit, int per, int blocks
 }
 
 // Accept / near rules and tag stream of the walk: a policy descriptor's, or
-// the built-in MAC with near leaf pairs and no tags.
+// the built-in MAC with near leaf pairs and no tags; and the stream format
+// (pair records only for an emission walk without a policy).
 static int set_policy(const YggFlatWalkParams* p, FlatArgs* a) {
+  if (p->pair_records && (!p->emit || p->policy)) return YGG_ERR_ARG;
+  a->pair_records = p->pair_records ? 1 : 0;
   YggPolicyDesc d = {};
   d.accept = YGG_ACCEPT_MAC;
   d.near = YGG_NEAR_LEAF_PAIRS;
@@ -1048,18 +1060,22 @@ static size_t lists_group_ws(int64_t n, int end_bit, int digit_bits) {
 }
 
 // Workspace of the row lists: the grouping's own, then the per-row counts
-// (2^end_bit rows at most) and their scan.
+// (2^end_bit rows at most), their scan and, for the bucketing, the row
+// offsets by node of pair records.
 static size_t lists_sort_ws(int64_t n, int end_bit, int digit_bits, int64_t* rows_max) {
   const int64_t rmax = end_bit >= 1 && end_bit <= 31 ? ((int64_t)1 << end_bit) : 1;
   if (rows_max) *rows_max = rmax;
-  return align_up(lists_group_ws(n, end_bit, digit_bits)) +
-         align_up(sizeof(uint32_t) * (size_t)rmax) + ygg::scan_workspace_bytes(rmax);
+  size_t b = align_up(lists_group_ws(n, end_bit, digit_bits)) +
+             align_up(sizeof(uint32_t) * (size_t)rmax) + align_up(ygg::scan_workspace_bytes(rmax));
+  if (lists_bucketed(end_bit, digit_bits)) b += sizeof(int64_t) * (size_t)(rmax + 1);
+  return b;
 }
 
 // Splits the row-list workspace and zeroes the row counts.
 static int lists_sort_split(int64_t n, int end_bit, int digit_bits, int64_t rows, void* ws,
                             size_t ws_bytes, hipStream_t st, size_t* sort_bytes,
-                            uint32_t** row_counts, void** scan_ws, size_t* scan_bytes) {
+                            uint32_t** row_counts, void** scan_ws, size_t* scan_bytes,
+                            int64_t** key_offsets) {
   int64_t rmax = 0;
   if (ws_bytes < lists_sort_ws(n, end_bit, digit_bits, &rmax)) return YGG_ERR_WORKSPACE;
   if (rows > rmax) return YGG_ERR_ARG;  // row keys need end_bit bits
@@ -1067,7 +1083,52 @@ static int lists_sort_split(int64_t n, int end_bit, int digit_bits, int64_t rows
   *row_counts = (uint32_t*)((char*)ws + *sort_bytes);
   *scan_ws = (char*)ws + *sort_bytes + align_up(sizeof(uint32_t) * (size_t)rmax);
   *scan_bytes = ygg::scan_workspace_bytes(rmax);
+  *key_offsets = lists_bucketed(end_bit, digit_bits)
+                     ? (int64_t*)((char*)*scan_ws + align_up(*scan_bytes))
+                     : nullptr;
   return ygg::fill_u32(*row_counts, 0u, rows, st);
+}
+
+// Row offsets by node key for pair records: every key first gets the end
+// of the lists (off[rows]), then row r's node gets off[r]. A key that is
+// no row's node (never in the walk's records) so lists its entries inside
+// the output, past every row's.
+__global__ __launch_bounds__(256) void k_key_offsets_fill(int64_t* __restrict__ by_key,
+                                                          int64_t keys_n,
+                                                          const int64_t* __restrict__ off,
+                                                          int rows) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i <= keys_n) by_key[i] = off[rows];
+}
+__global__ __launch_bounds__(256) void k_key_offsets_rows(int64_t* __restrict__ by_key,
+                                                          int64_t keys_n,
+                                                          const int64_t* __restrict__ off,
+                                                          const int32_t* __restrict__ row_nodes,
+                                                          int rows) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const int32_t k = row_nodes[i];
+  if (k >= 0 && k < keys_n) by_key[k] = off[i];
+}
+
+// The lists of the bucketing's counts: row offsets (int64 off[rows + 1],
+// the scan of the counts, through row_nodes for pair records), then the
+// scatter of every value (by node key for pair records).
+static int lists_offsets_scatter(const ygg::RowBucketState& bs, const uint32_t* row_counts,
+                                 const int32_t* row_nodes, int rows, int64_t keys_n,
+                                 int64_t* off, int64_t* key_offsets, void* scan_ws,
+                                 size_t scan_bytes, const int64_t* row_sub, int out_i64,
+                                 void* out, hipStream_t st) {
+  int rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, row_nodes, off, rows,
+                                          scan_ws, scan_bytes, st);
+  if (rc) return rc;
+  if (!row_nodes) return ygg::row_bucket_scatter(bs, off, row_sub, out_i64, out, st);
+  hipLaunchKernelGGL(k_key_offsets_fill, dim3(grid_for(keys_n + 1, 256)), dim3(256), 0, st,
+                     key_offsets, keys_n, off, rows);
+  hipLaunchKernelGGL(k_key_offsets_rows, dim3(grid_for(rows, 256)), dim3(256), 0, st,
+                     key_offsets, keys_n, off, row_nodes, rows);
+  YGG_LAUNCH_CHECK();
+  return ygg::row_bucket_scatter(bs, key_offsets, row_sub, out_i64, out, st);
 }
 
 size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits) {
@@ -1077,29 +1138,33 @@ size_t ygg_far_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bits
 int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                        const int64_t* count, int64_t num_nodes, int end_bit, int digit_bits,
                        const int32_t* nodes_by_level, const int32_t* node_level,
-                       const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
-                       int64_t* lm_offsets, void* sources, void* targets, void* target_levels,
-                       void* counts, void* node_offsets, int32_t* counts32, void* ws,
-                       size_t ws_bytes, void* stream) {
+                       const int64_t* row_sub, int pair_records, int out_i64,
+                       uint32_t* key_scratch, int64_t* lm_offsets, void* sources, void* targets,
+                       void* target_levels, void* counts, void* node_offsets, int32_t* counts32,
+                       void* ws, size_t ws_bytes, void* stream) {
   if (num_nodes < 1 || n < 0 || !lm_offsets || !key_scratch || !ws) return YGG_ERR_ARG;
-  if (row_sub && !lists_bucketed(end_bit, digit_bits)) return YGG_ERR_UNSUPPORTED;
+  if ((row_sub || pair_records) && !lists_bucketed(end_bit, digit_bits))
+    return YGG_ERR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
   const int T = (int)num_nodes;
+  const int64_t entries = pair_records ? 2 * n : n;
   size_t sort_bytes = 0, scan_bytes = 0;
   uint32_t* row_counts = nullptr;
   void* scan_ws = nullptr;
-  int rc = lists_sort_split(n, end_bit, digit_bits, T, ws, ws_bytes, st, &sort_bytes,
-                            &row_counts, &scan_ws, &scan_bytes);
+  int64_t* key_offsets = nullptr;
+  int rc = lists_sort_split(entries, end_bit, digit_bits, T, ws, ws_bytes, st, &sort_bytes,
+                            &row_counts, &scan_ws, &scan_bytes, &key_offsets);
   if (rc) return rc;
   if (lists_bucketed(end_bit, digit_bits)) {
+    // pair records hold node ids: counts by node, offsets in level-major
+    // order through nodes_by_level
     ygg::RowBucketState bs;
-    rc = ygg::row_bucket_count(keys, vals, n, count, end_bit, T, T, key_scratch, row_counts, ws,
-                               sort_bytes, st, &bs);
+    rc = ygg::row_bucket_count(keys, vals, pair_records != 0, n, count, end_bit, T, T,
+                               key_scratch, row_counts, ws, sort_bytes, st, &bs);
     if (rc) return rc;
-    rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, lm_offsets, T,
-                                        scan_ws, scan_bytes, st);
-    if (rc) return rc;
-    rc = ygg::row_bucket_scatter(bs, lm_offsets, row_sub, out_i64, sources, st);
+    rc = lists_offsets_scatter(bs, row_counts, pair_records ? nodes_by_level : nullptr, T, T,
+                               lm_offsets, key_offsets, scan_ws, scan_bytes, row_sub, out_i64,
+                               sources, st);
   } else {
     rc = ygg::sort_far_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits, out_i64,
                                   sources, row_counts, T, ws, sort_bytes, st);
@@ -1132,28 +1197,33 @@ size_t ygg_near_lists_sort_workspace_bytes(int64_t n, int end_bit, int digit_bit
 
 int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
                         const int64_t* count, int64_t num_leaves, int end_bit, int digit_bits,
-                        const int64_t* row_sub, int out_i64, uint32_t* key_scratch,
-                        int64_t* row_offsets, void* neighbors, void* counts, void* offsets,
-                        void* ws, size_t ws_bytes, void* stream) {
+                        const int64_t* row_sub, const int32_t* leaf_nodes, int out_i64,
+                        uint32_t* key_scratch, int64_t* row_offsets, void* neighbors,
+                        void* counts, void* offsets, void* ws, size_t ws_bytes, void* stream) {
   if (num_leaves < 1 || n < 0 || !row_offsets || !key_scratch || !ws) return YGG_ERR_ARG;
-  if (row_sub && !lists_bucketed(end_bit, digit_bits)) return YGG_ERR_UNSUPPORTED;
+  if ((row_sub || leaf_nodes) && !lists_bucketed(end_bit, digit_bits))
+    return YGG_ERR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
   const int L = (int)num_leaves;
+  const int64_t entries = leaf_nodes ? 2 * n : n;
+  // pair records are keyed by node id: the whole 2^end_bit key range
+  const int64_t keys_n = leaf_nodes ? ((int64_t)1 << end_bit) : L;
+  if (leaf_nodes && (end_bit < 1 || end_bit > 31)) return YGG_ERR_ARG;
   size_t sort_bytes = 0, scan_bytes = 0;
   uint32_t* row_counts = nullptr;
   void* scan_ws = nullptr;
-  int rc = lists_sort_split(n, end_bit, digit_bits, L, ws, ws_bytes, st, &sort_bytes,
-                            &row_counts, &scan_ws, &scan_bytes);
+  int64_t* key_offsets = nullptr;
+  int rc = lists_sort_split(entries, end_bit, digit_bits, keys_n, ws, ws_bytes, st, &sort_bytes,
+                            &row_counts, &scan_ws, &scan_bytes, &key_offsets);
   if (rc) return rc;
   if (lists_bucketed(end_bit, digit_bits)) {
     ygg::RowBucketState bs;
-    rc = ygg::row_bucket_count(keys, vals, n, count, end_bit, L, 2 * (int64_t)L, key_scratch,
-                               row_counts, ws, sort_bytes, st, &bs);
+    rc = ygg::row_bucket_count(keys, vals, leaf_nodes != nullptr, n, count, end_bit, keys_n,
+                               leaf_nodes ? keys_n : 2 * (int64_t)L, key_scratch, row_counts, ws,
+                               sort_bytes, st, &bs);
     if (rc) return rc;
-    rc = ygg::exclusive_scan_i32_to_i64((const int32_t*)row_counts, nullptr, row_offsets, L,
-                                        scan_ws, scan_bytes, st);
-    if (rc) return rc;
-    rc = ygg::row_bucket_scatter(bs, row_offsets, row_sub, out_i64, neighbors, st);
+    rc = lists_offsets_scatter(bs, row_counts, leaf_nodes, L, keys_n, row_offsets, key_offsets,
+                               scan_ws, scan_bytes, row_sub, out_i64, neighbors, st);
   } else {
     rc = ygg::sort_near_rows_fused(keys, vals, key_scratch, n, count, end_bit, digit_bits,
                                    out_i64, neighbors, row_counts, L, ws, sort_bytes, st);

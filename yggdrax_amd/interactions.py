@@ -64,6 +64,20 @@ STREAM_START_MAX_BYTES = 16 << 30
 # which is not specified; the rows are the same.
 ROW_LISTS = os.environ.get("YGGDRAX_ROW_LISTS", "bucket")
 ROW_BUCKET_MAX_BITS = 22
+# Stream format of the emission walk when the bucketing groups the lists:
+# one record per accepted or near pair (both row keys), which the row-list
+# split expands into the pair's two entries through a row key -> node table,
+# instead of the two entries themselves. The walk writes, and the split
+# reads, half the bytes; the lists are the same.
+PAIR_RECORDS = os.environ.get("YGGDRAX_PAIR_RECORDS", "1") != "0"
+
+
+def pair_records_ok(num_nodes: int) -> bool:
+    """Whether a walk whose lists are grouped by ygg_far_lists_sort /
+    ygg_near_lists_sort may emit pair records: only the bucketing lists
+    them, keyed by node id (at most ROW_BUCKET_MAX_BITS bits)."""
+    return (PAIR_RECORDS and ROW_LISTS == "bucket"
+            and max(1, (num_nodes - 1).bit_length()) <= ROW_BUCKET_MAX_BITS)
 # Sizing of the last flat walk: how its streams were sized ("hint", "grow"
 # or "count"), their final capacities and the number of growths.
 LAST_WALK: dict = {}
@@ -391,7 +405,11 @@ class _WalkContext:
         not fit the far or near stream (two entries per pair) pauses the
         walk before it runs, the streams are reallocated larger with their
         entries copied over, and the walk continues from that generation,
-        so the capacities are starting sizes and never overflow."""
+        so the capacities are starting sizes and never overflow.
+        With the context's ``pair_records`` set (and no policy) the
+        emission walk writes one record per pair instead of two entries:
+        capacities, totals and decisions stay in entries, the streams hold
+        records (``pair_records`` True in the result)."""
         dt = self.dt
         dev = self.dev
         roots = getattr(self, "roots", None)
@@ -403,6 +421,10 @@ class _WalkContext:
         # reserving per chunk on two contended addresses
         spread = None if emit else torch.empty(2 * SPREAD_LANES, dtype=torch.int64, device=dev)
         fk = fv = nk = nv = ftag = None
+        pairs = bool(emit and policy is None and getattr(self, "pair_records", False))
+        per = 1 if pairs else 2  # stream slots per accepted or near pair
+        if pairs:  # entry capacities to record capacities
+            far_cap, near_cap = -(-int(far_cap) // 2), -(-int(near_cap) // 2)
         if emit:
             fk = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
             fv = torch.empty(max(1, far_cap), dtype=torch.int32, device=dev)
@@ -427,6 +449,7 @@ class _WalkContext:
         p.hist_gen, p.node_hist = int(hist_gen), _lib.ptr(node_hist)
         p.xcd_groups = WALK_XCD_GROUPS
         p.pairs_per_thread = int(self.pairs_per_thread)
+        p.pair_records = int(pairs)
         if spread is not None:
             p.spread_counters, p.spread_lanes = spread.data_ptr(), SPREAD_LANES
         if policy is not None:
@@ -461,7 +484,7 @@ class _WalkContext:
             g = paused - 1
             filled = _host_totals(hs, G)
             size = min(hs[g] & 0xFFFFFFFF, Q)
-            need_f, need_n = filled[0] + 2 * size, filled[1] + 2 * size
+            need_f, need_n = filled[0] + per * size, filled[1] + per * size
             if need_f <= p.far_cap and need_n <= p.near_cap:
                 raise RuntimeError(f"walk paused at generation {g} with room in both streams")
             if need_f > p.far_cap:
@@ -489,11 +512,12 @@ class _WalkContext:
             _GEN_PROFILES.move_to_end(pkey)
             while len(_GEN_PROFILES) > _GEN_PROFILES_MAX:
                 _GEN_PROFILES.popitem(last=False)
-        c = _host_totals(hs, G)
-        # every evaluated pair is exactly one of accept (2 far entries),
-        # near (2 near entries) or refine
-        pairs = sum(min(v, Q) for v in raw_gens)
-        dec = (c[0] // 2, c[1] // 2, pairs - c[0] // 2 - c[1] // 2)
+        slots = _host_totals(hs, G)
+        # every evaluated pair is exactly one of accept (2 far entries, or
+        # one record), near (2 near entries, or one record) or refine
+        c = (2 * slots[0], 2 * slots[1]) if pairs else slots
+        evaluated = sum(min(v, Q) for v in raw_gens)
+        dec = (c[0] // 2, c[1] // 2, evaluated - c[0] // 2 - c[1] // 2)
         self.gen_hint = ngen.value + 1
         if gen_ms is not None:
             self.kernel_log.append(dict(kernel="k_walk_flat", emit=bool(emit),
@@ -504,8 +528,11 @@ class _WalkContext:
                     queue_overflow=bool(hs[G + 1]), far_overflow=bool(hs[G + 2]),
                     near_overflow=bool(hs[G + 3]), far_keys=fk, far_vals=fv,
                     near_keys=nk, near_vals=nv, far_tags=ftag, queue_capacity=Q,
-                    far_capacity=int(p.far_cap), near_capacity=int(p.near_cap),
-                    stream_growths=growths)
+                    far_capacity=int(p.far_cap) * (2 // per),
+                    near_capacity=int(p.near_cap) * (2 // per),
+                    stream_bytes=(8 * (int(p.far_cap) + int(p.near_cap))
+                                  + (4 * int(p.far_cap) if ftag is not None else 0)) if emit else 0,
+                    stream_growths=growths, pair_records=pairs)
 
     def run(self, *, mode: int, queue_cap: int, far_cap: int = 0, near_cap: int = 0,
             far_row_start=None, far_row_cap=None, near_row_start=None, near_row_cap=None,
@@ -748,10 +775,15 @@ def _fused_lists(em: dict, dt: DeviceTopology):
     i64 = int(idx == torch.int64)
     T, L = dt.num_nodes, dt.num_leaves
     nf, nn = em["far_total"], em["near_total"]
+    # pair records: half as many stream slots, keyed by node id; the near
+    # rows follow leaf_indices (leaf order -> node)
+    pairs = bool(em.get("pair_records"))
+    n_near, n_far = (nn // 2, nf // 2) if pairs else (nn, nf)
+    leaf_nodes = dt.leaf_indices.data_ptr() if pairs else None
     main = torch.cuda.current_stream(dev)
     walked = torch.cuda.Event()
     walked.record(main)
-    nend = max(1, (L - 1).bit_length())
+    nend = max(1, ((T if pairs else L) - 1).bit_length())
     nbits = row_lists_digit_bits(nend, nn)
     neighbors = torch.empty(nn, dtype=idx, device=dev)
     ncounts = torch.empty(L, dtype=idx, device=dev)
@@ -760,8 +792,9 @@ def _fused_lists(em: dict, dt: DeviceTopology):
     nsorted = torch.empty(max(1, nn), dtype=torch.int32, device=dev)
     nws = _lib.workspace(lib.ygg_near_lists_sort_workspace_bytes(nn, nend, nbits), dev)
     _lib.check(lib.ygg_near_lists_sort(
-        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), nn, None, L, nend, nbits, None, i64,
-        nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(), ncounts.data_ptr(),
+        em["near_keys"].data_ptr(), em["near_vals"].data_ptr(), n_near, None, L, nend, nbits,
+        None, leaf_nodes, i64, nsorted.data_ptr(), near_off.data_ptr(), neighbors.data_ptr(),
+        ncounts.data_ptr(),
         noff.data_ptr(), nws.data_ptr(), nws.numel(), _lib.stream_ptr(dev)), "near lists sort")
     side = _side_stream(dev)
     side.wait_event(walked)
@@ -778,8 +811,9 @@ def _fused_lists(em: dict, dt: DeviceTopology):
         fsorted = torch.empty(max(1, nf), dtype=torch.int32, device=dev)
         fws = _lib.workspace(lib.ygg_far_lists_sort_workspace_bytes(nf, fend, fbits), dev)
         _lib.check(lib.ygg_far_lists_sort(
-            em["far_keys"].data_ptr(), em["far_vals"].data_ptr(), nf, None, T, fend, fbits,
-            dt.nodes_by_level.data_ptr(), dt.node_level.data_ptr(), None, i64, fsorted.data_ptr(),
+            em["far_keys"].data_ptr(), em["far_vals"].data_ptr(), n_far, None, T, fend, fbits,
+            dt.nodes_by_level.data_ptr(), dt.node_level.data_ptr(), None, int(pairs), i64,
+            fsorted.data_ptr(),
             far_off.data_ptr(), sources.data_ptr(), targets.data_ptr(), levels.data_ptr(),
             counts.data_ptr(), node_off.data_ptr(), counts32.data_ptr(), fws.data_ptr(),
             fws.numel(), _lib.stream_ptr(dev)), "far lists sort")
@@ -906,6 +940,9 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
     T, NI, L = dt.num_nodes, dt.num_internal, dt.num_leaves
     qbig = max(T * DEFAULT_PAIR_QUEUE_MULTIPLIER, _auto_pair_queue_candidates(T, NI)[-1])
     from . import autotune
+    # the fused list builders expand pair records; the host sort path and
+    # the raw result read entries
+    ctx.pair_records = bool(fused_lists) and pair_records_ok(T)
 
     def emission_walk(grid_blocks=0):
         # the walk the plain path runs: resumable, from the default capacities
@@ -978,7 +1015,8 @@ def _run_walk_flat(ctx: _WalkContext, dt: DeviceTopology, lad: _Ladder,
                                                    else "grow"),
                      far_capacity=em["far_capacity"], near_capacity=em["near_capacity"],
                      queue_capacity=em["queue_capacity"], stream_growths=em["stream_growths"],
-                     far_total=nf, near_total=nn)
+                     far_total=nf, near_total=nn, stream_bytes=em["stream_bytes"],
+                     stream_bytes_per_entry=4 if em["pair_records"] else 8)
     if strict_int32 and lad.use_count and max(nf, nn) > (1 << 31) - 1:
         raise RuntimeError("Dual-tree traversal pair totals exceed signed int32 capacity; "
                            "refine the tree or use a lower-memory traversal strategy.")
