@@ -322,9 +322,9 @@ struct RowBucketState {
 bool row_bucket_split(int end_bit, int* hi_bits, int* lo_bits);
 size_t row_bucket_workspace_bytes(int64_t n, int end_bit);
 int row_bucket_count(const uint32_t* keys, const uint32_t* vals, bool records, int64_t n,
-                     const int64_t* n_dev, int end_bit, int64_t rows, int64_t val_bound,
-                     uint32_t* vals_scratch, uint32_t* row_counts, void* ws, size_t ws_bytes,
-                     hipStream_t st, RowBucketState* state);
+                     const int64_t* n_dev, int end_bit, int64_t rows, int64_t list_rows,
+                     int64_t val_bound, uint32_t* vals_scratch, uint32_t* row_counts, void* ws,
+                     size_t ws_bytes, hipStream_t st, RowBucketState* state);
 int row_bucket_scatter(const RowBucketState& s, const int64_t* row_offsets,
                        const int64_t* row_sub, int out_i64, void* out, hipStream_t st);
 // Exclusive prefix sum of int32 counts into int64 offsets (n+1 entries:

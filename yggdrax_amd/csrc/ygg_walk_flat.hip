@@ -1159,7 +1159,7 @@ int ygg_far_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
     // pair records hold node ids: counts by node, offsets in level-major
     // order through nodes_by_level
     ygg::RowBucketState bs;
-    rc = ygg::row_bucket_count(keys, vals, pair_records != 0, n, count, end_bit, T, T,
+    rc = ygg::row_bucket_count(keys, vals, pair_records != 0, n, count, end_bit, T, T, T,
                                key_scratch, row_counts, ws, sort_bytes, st, &bs);
     if (rc) return rc;
     rc = lists_offsets_scatter(bs, row_counts, pair_records ? nodes_by_level : nullptr, T, T,
@@ -1219,6 +1219,7 @@ int ygg_near_lists_sort(const uint32_t* keys, const uint32_t* vals, int64_t n,
   if (lists_bucketed(end_bit, digit_bits)) {
     ygg::RowBucketState bs;
     rc = ygg::row_bucket_count(keys, vals, leaf_nodes != nullptr, n, count, end_bit, keys_n,
+                               leaf_nodes ? (L < keys_n ? L : keys_n) : L,
                                leaf_nodes ? keys_n : 2 * (int64_t)L, key_scratch, row_counts, ws,
                                sort_bytes, st, &bs);
     if (rc) return rc;
