@@ -3,14 +3,17 @@
 // Same pair semantics as the per-row-slot walk (MAC, accept / near / refine,
 // split rule and child order of _dual_tree_walk_impl,
 // _interactions_impl.py:1379-1454), but instead of one device-scope atomic
-// per list entry, each 256-thread workgroup evaluates a chunk of 1024 pairs,
-// prefix-sums its outputs in LDS and reserves space in three append-only
-// streams (next wavefront, far entries, near entries) with one atomic each.
-// A far entry is (level-major rank of its target row, source node); a near
-// entry is (native leaf row, neighbour leaf node). Rows are formed afterwards
-// by a stable radix sort on the row key, so no per-row counters are needed.
-// Wavefront sizes per generation are exact, which lets the host reproduce
-// the reference's pair-queue capacity decisions.
+// per list entry, each 256-thread workgroup evaluates a chunk of pairs
+// (pairs per thread x 256), prefix-sums its outputs in LDS and reserves
+// space in three append-only streams (next wavefront, far, near) with one
+// atomic each. A far entry is (level-major rank of its target row, source
+// node); a near entry is (native leaf row, neighbour leaf node); with pair
+// records the streams hold one (t, s) of node ids per accepted or near pair
+// instead of its two entries. Rows are formed afterwards by grouping on the
+// row key (the bucketing of ygg_rowlists.hip, or a stable radix sort), so
+// no per-row counters are needed. Wavefront sizes per generation are exact,
+// which lets the host reproduce the reference's pair-queue capacity
+// decisions.
 #include "ygg_common.h"
 #include "ygg_walk.h"
 
