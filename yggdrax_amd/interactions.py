@@ -64,6 +64,7 @@ STREAM_START_MAX_BYTES = 16 << 30
 # which is not specified; the rows are the same.
 ROW_LISTS = os.environ.get("YGGDRAX_ROW_LISTS", "bucket")
 ROW_BUCKET_MAX_BITS = 22
+ROW_BUCKET_DIRECT_BITS = 11  # row keys of at most this many bits are one bucket
 # Stream format of the emission walk when the bucketing groups the lists:
 # one record per accepted or near pair (both row keys), which the row-list
 # split expands into the pair's two entries through a row key -> node table,
@@ -75,9 +76,13 @@ PAIR_RECORDS = os.environ.get("YGGDRAX_PAIR_RECORDS", "1") != "0"
 def pair_records_ok(num_nodes: int) -> bool:
     """Whether a walk whose lists are grouped by ygg_far_lists_sort /
     ygg_near_lists_sort may emit pair records: only the bucketing lists
-    them, keyed by node id (at most ROW_BUCKET_MAX_BITS bits)."""
+    them, keyed by node id (at most ROW_BUCKET_MAX_BITS bits). Trees of
+    at most 2^ROW_BUCKET_DIRECT_BITS nodes keep entries: their row keys
+    are one bucket, which the bucketing reads in place, while records
+    would add the histogram and split passes."""
+    bits = max(1, (num_nodes - 1).bit_length())
     return (PAIR_RECORDS and ROW_LISTS == "bucket"
-            and max(1, (num_nodes - 1).bit_length()) <= ROW_BUCKET_MAX_BITS)
+            and ROW_BUCKET_DIRECT_BITS < bits <= ROW_BUCKET_MAX_BITS)
 # Sizing of the last flat walk: how its streams were sized ("hint", "grow"
 # or "count"), their final capacities and the number of growths.
 LAST_WALK: dict = {}
