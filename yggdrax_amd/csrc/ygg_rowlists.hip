@@ -41,7 +41,8 @@
 // Traffic per entry with packed values: 4 B (histogram) + 8 B read and 4 B
 // written (split) + 4 B (reserve) + 4 B read and 4 or 8 B written
 // (scatter); 32 B with int64 lists, against 36 B for two stable passes
-// whose last writes the list.
+// whose last writes the list. Pair records read 4 B per entry in the
+// split (one 8-byte record per two entries): 28 B.
 #include "ygg_common.h"
 
 namespace ygg {
