@@ -112,11 +112,16 @@ def _assert_batches_equal(a, b, systems):
             assert np.array_equal(k1, k2), k
 
 
-def test_batched_graph_replays_match_eager():
+@pytest.mark.parametrize("records", [True, False], ids=["records", "entries"])
+def test_batched_graph_replays_match_eager(monkeypatch, records):
     """The HIP-graph capture of the batched pipeline, replayed several times
     on new positions of the same layout (eager work and synchronisation in
-    between), gives every replay the eager pipeline's lists."""
+    between), gives every replay the eager pipeline's lists; with the walk
+    writing pair records (the default) and writing entries."""
     from yggdrax_amd import batch
+    from yggdrax_amd import interactions as I
+    monkeypatch.setattr(I, "PAIR_RECORDS", records)
+    print(f"[batch] graph replays records={records}", flush=True)
     rng = np.random.default_rng(6)
     sizes = np.round(np.exp(rng.uniform(np.log(256), np.log(8192), 48))).astype(np.int64)
     pos, mass, offsets = _systems(sizes, 6)
@@ -124,6 +129,7 @@ def test_batched_graph_replays_match_eager():
     M = torch.from_numpy(np.concatenate(mass)).cuda()
     g = batch.BatchedGraph(P, M, offsets, leaf_size=32, theta=0.6, mac_type="dehnen",
                            slack=1.3)
+    assert g.pair_records == records
     for step in range(3):
         Pn = P + torch.from_numpy(rng.normal(0, 1e-3, tuple(P.shape)).astype(np.float32)).cuda()
         bt, geom, lists = g.replay(Pn, M)
