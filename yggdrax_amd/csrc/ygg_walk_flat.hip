@@ -378,13 +378,17 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
       const unsigned long long wex = wbase;  // block-exclusive prefix of this wavefront
       const int64_t fbase = (int64_t)s_base[1] + (int64_t)((wex >> 21) & mask);
       const int64_t nbase = (int64_t)s_base[2] + (int64_t)((wex >> 42) & mask);
+      // The streams are read only after the walk, by the row lists: written
+      // non-temporally they leave L2 and the Infinity Cache to the node
+      // records every generation gathers (2^24 clustered emission walk:
+      // 5.69 -> 5.56 ms).
       for (unsigned j = lane; j < wave_nf; j += YGG_WAVE) {
         const int64_t slot = fbase + j;
         if (slot < a.far_cap) {
           const uint2 e = s_stage[w][j];
-          a.far_keys[slot] = e.x;
-          a.far_vals[slot] = e.y;
-          if (TAGS) a.far_tags[slot] = s_tag[w][j];
+          __builtin_nontemporal_store(e.x, &a.far_keys[slot]);
+          __builtin_nontemporal_store(e.y, &a.far_vals[slot]);
+          if (TAGS) __builtin_nontemporal_store(s_tag[w][j], &a.far_tags[slot]);
         } else {
           fover = true;
         }
@@ -393,8 +397,8 @@ __global__ __launch_bounds__(256) void k_walk_flat(FlatArgs a, int g) {
         const int64_t slot = nbase + j;
         if (slot < a.near_cap) {
           const uint2 e = s_stage[w][wave_nf + j];
-          a.near_keys[slot] = e.x;
-          a.near_vals[slot] = e.y;
+          __builtin_nontemporal_store(e.x, &a.near_keys[slot]);
+          __builtin_nontemporal_store(e.y, &a.near_vals[slot]);
         } else {
           nover = true;
         }
